@@ -1,0 +1,195 @@
+"""ctypes wrapper of the C oracle (``salmon_oracle.c``).  Test infrastructure only."""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "libsalmon_oracle.so")
+
+_c_int_p = ctypes.POINTER(ctypes.c_int)
+_c_dbl_p = ctypes.POINTER(ctypes.c_double)
+
+
+class OrSystem(ctypes.Structure):
+    _fields_ = [
+        ("NLx", ctypes.c_int), ("NLy", ctypes.c_int), ("NLz", ctypes.c_int), ("NL", ctypes.c_int),
+        ("Hx", ctypes.c_double), ("Hy", ctypes.c_double), ("Hz", ctypes.c_double), ("Hxyz", ctypes.c_double),
+        ("aLx", ctypes.c_double), ("aLy", ctypes.c_double), ("aLz", ctypes.c_double), ("aLxyz", ctypes.c_double),
+        ("lapx", ctypes.c_double * 5), ("lapy", ctypes.c_double * 5), ("lapz", ctypes.c_double * 5),
+        ("nabx", ctypes.c_double * 5), ("naby", ctypes.c_double * 5), ("nabz", ctypes.c_double * 5),
+        ("vloc", _c_dbl_p),
+        ("NI", ctypes.c_int), ("Nps", ctypes.c_int), ("Nlma", ctypes.c_int),
+        ("Mps", _c_int_p), ("Jxyz", _c_int_p), ("Jxx", _c_int_p), ("Jyy", _c_int_p), ("Jzz", _c_int_p),
+        ("a_tbl", _c_int_p), ("uV", _c_dbl_p), ("iuV", _c_int_p),
+        ("NK", ctypes.c_int), ("NB", ctypes.c_int), ("occ", _c_dbl_p),
+    ]
+
+
+def build(force=False):
+    if force or not os.path.exists(LIB) or os.path.getmtime(LIB) < os.path.getmtime(os.path.join(HERE, "salmon_oracle.c")):
+        subprocess.check_call(["gcc", "-O2", "-fPIC", "-fopenmp", "-shared", "-o", LIB,
+                               os.path.join(HERE, "salmon_oracle.c"), "-lm"])
+    return LIB
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = ctypes.CDLL(LIB)
+        P = ctypes.POINTER(OrSystem)
+        vp = ctypes.c_void_p
+        L.or_update_projector.argtypes = [P, vp, vp]
+        L.or_hpsi.argtypes = [P, vp, vp, vp, vp, vp]
+        L.or_hamiltonian.argtypes = [P, vp, ctypes.c_double, vp]
+        L.or_current.argtypes = [P, vp, vp, vp]
+        L.or_density.argtypes = [P, vp, vp]
+        L.or_energy.argtypes = [P, vp, vp, vp, vp, vp]
+        L.or_norm.argtypes = [P, vp]
+        L.or_norm.restype = ctypes.c_double
+        L.or_gram_schmidt.argtypes = [P, vp]
+        L.or_sp_energy.argtypes = [P, vp, vp, vp]
+        L.or_cg.argtypes = [P, vp, vp, ctypes.c_int, ctypes.c_int, vp, vp]
+        L.or_rt_run.argtypes = [P, vp, vp, ctypes.c_int, ctypes.c_double, vp, vp, vp, vp, vp, ctypes.c_int]
+        L.or_num_threads.restype = ctypes.c_int
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+class Oracle:
+    """Holds one synthetic system (``salmon_amd.synth.System``) for the C oracle."""
+
+    def __init__(self, sys):
+        self.sys = sys
+        self._keep = []
+
+        def ia(x):
+            a = np.ascontiguousarray(x, dtype=np.int32)
+            self._keep.append(a)
+            return a.ctypes.data_as(_c_int_p)
+
+        def da(x):
+            a = np.ascontiguousarray(x, dtype=np.float64)
+            self._keep.append(a)
+            return a.ctypes.data_as(_c_dbl_p)
+
+        s = OrSystem()
+        s.NLx, s.NLy, s.NLz, s.NL = sys.NLx, sys.NLy, sys.NLz, sys.NL
+        s.Hx, s.Hy, s.Hz, s.Hxyz = sys.Hx, sys.Hy, sys.Hz, sys.Hxyz
+        s.aLx, s.aLy, s.aLz, s.aLxyz = sys.aLx, sys.aLy, sys.aLz, sys.aLxyz
+        for name in ("lapx", "lapy", "lapz", "nabx", "naby", "nabz"):
+            setattr(s, name, (ctypes.c_double * 5)(*[float(v) for v in getattr(sys, name)]))
+        s.vloc = da(sys.vloc)
+        s.NI, s.Nps, s.Nlma = sys.NI, sys.Nps, sys.Nlma
+        s.Mps = ia(sys.Mps)
+        s.Jxyz, s.Jxx, s.Jyy, s.Jzz = ia(sys.Jxyz), ia(sys.Jxx), ia(sys.Jyy), ia(sys.Jzz)
+        s.a_tbl, s.uV, s.iuV = ia(sys.a_tbl), da(sys.uV), ia(sys.iuV)
+        s.NK, s.NB = sys.NK, sys.NB
+        s.occ = da(sys.occ)
+        self.c = s
+        self.cp = ctypes.byref(s)
+
+    # --- single orbital -------------------------------------------------
+    def zproj(self, k):
+        out = np.zeros((self.sys.Nlma, self.sys.Nps), np.complex128)
+        k = np.ascontiguousarray(k, dtype=np.float64)
+        lib().or_update_projector(self.cp, _p(k), _p(out))
+        return out
+
+    def hpsi(self, k, psi, with_tt=False):
+        k = np.ascontiguousarray(k, dtype=np.float64)
+        zp = self.zproj(k)
+        psi = np.ascontiguousarray(psi, dtype=np.complex128)
+        out = np.zeros_like(psi)
+        tt = np.zeros_like(psi) if with_tt else None
+        lib().or_hpsi(self.cp, _p(k), _p(zp), _p(psi), _p(out), _p(tt) if with_tt else None)
+        return (out, tt) if with_tt else out
+
+    # --- all orbitals ---------------------------------------------------
+    def hamiltonian(self, kAc, zu, dt):
+        """Taylor-4 step in place on a copy; kAc is (3, NK)."""
+        zu = np.array(zu, dtype=np.complex128, order="C", copy=True)
+        kAc = np.ascontiguousarray(kAc, dtype=np.float64)
+        lib().or_hamiltonian(self.cp, _p(kAc), float(dt), _p(zu))
+        return zu
+
+    def current(self, kAc, zu):
+        jav = np.zeros(3)
+        kAc = np.ascontiguousarray(kAc, dtype=np.float64)
+        zu = np.ascontiguousarray(zu, dtype=np.complex128)
+        lib().or_current(self.cp, _p(kAc), _p(zu), _p(jav))
+        return jav
+
+    def density(self, zu):
+        rho = np.zeros(self.sys.NL)
+        zu = np.ascontiguousarray(zu, dtype=np.complex128)
+        lib().or_density(self.cp, _p(zu), _p(rho))
+        return rho
+
+    def energy(self, kAc, zu):
+        ek, en, el = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        kAc = np.ascontiguousarray(kAc, dtype=np.float64)
+        zu = np.ascontiguousarray(zu, dtype=np.complex128)
+        lib().or_energy(self.cp, _p(kAc), _p(zu), ctypes.byref(ek), ctypes.byref(en), ctypes.byref(el))
+        return ek.value, en.value, el.value
+
+    def norm(self, zu):
+        zu = np.ascontiguousarray(zu, dtype=np.complex128)
+        return lib().or_norm(self.cp, _p(zu))
+
+    def gram_schmidt(self, zu):
+        zu = np.array(zu, dtype=np.complex128, order="C", copy=True)
+        lib().or_gram_schmidt(self.cp, _p(zu))
+        return zu
+
+    def sp_energy(self, kAc, zu):
+        esp = np.zeros((self.sys.NK, self.sys.NB))
+        kAc = np.ascontiguousarray(kAc, dtype=np.float64)
+        zu = np.ascontiguousarray(zu, dtype=np.complex128)
+        lib().or_sp_energy(self.cp, _p(kAc), _p(zu), _p(esp))
+        return esp
+
+    def cg(self, kAc, zu, ncg, precond=0):
+        zu = np.array(zu, dtype=np.complex128, order="C", copy=True)
+        kAc = np.ascontiguousarray(kAc, dtype=np.float64)
+        ev = np.zeros((self.sys.NK, self.sys.NB))
+        it = np.zeros((self.sys.NK, self.sys.NB), np.int32)
+        lib().or_cg(self.cp, _p(kAc), _p(zu), int(ncg), int(precond), _p(ev), _p(it))
+        return zu, ev, it
+
+    def rt_run(self, ac, nsteps, zu=None, energy_every=1, want_rho=False):
+        s = self.sys
+        zu = np.array(s.zu if zu is None else zu, dtype=np.complex128, order="C", copy=True)
+        ac = np.ascontiguousarray(ac[: nsteps + 1], dtype=np.float64)
+        jt = np.zeros((nsteps, 3))
+        et = np.zeros(nsteps)
+        nt = np.zeros(nsteps)
+        rho = np.zeros(s.NL) if want_rho else None
+        kAc0 = np.ascontiguousarray(s.kAc0, dtype=np.float64)
+        lib().or_rt_run(self.cp, _p(kAc0), _p(ac), int(nsteps), float(s.dt), _p(zu), _p(jt), _p(et), _p(nt),
+                        _p(rho) if want_rho else None, int(energy_every))
+        out = dict(zu=zu, J=jt, E=et, norm=nt)
+        if want_rho:
+            out["rho"] = rho
+        return out
+
+
+def midpoint_kac(sys, ac, n):
+    """kAc = kAc0 + (A_n + A_{n+1})/2 as (3, NK) (dt_evolve.f90:47)."""
+    return sys.kAc0 + 0.5 * (ac[n] + ac[n + 1])[:, None]
+
+
+def kac_at(sys, A):
+    return sys.kAc0 + np.asarray(A, dtype=np.float64)[:, None]
