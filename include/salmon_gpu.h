@@ -1,0 +1,139 @@
+/*
+ * salmon_gpu.h -- C-ABI of the MI355X (gfx950) implementation of ARTED's
+ * real-time electron-dynamics hot path.
+ *
+ * Conventions (those of the reference's Fortran module globals):
+ *  - arrays are Fortran column-major; complex values are interleaved (re,im)
+ *    doubles; the grid index is i = ix*NLy*NLz + iy*NLz + iz (z fastest),
+ *    ARTED/preparation/init.f90:28-37;
+ *  - orbitals zu(NL,NB,NK) complex(8), ARTED/modules/global_variables.f90 (zu_t);
+ *  - kAc(NK,3) and kAc0(NK,3), occ(NB,NK), Jxyz/Jxx/Jyy/Jzz(Nps,NI) with Jxyz
+ *    1-based, uV(Nps,Nlma), iuV(Nlma), a_tbl(Nlma) 1-based;
+ *  - all pointers are host pointers.  Orbitals stay resident on the GPU
+ *    between calls (the reference's OpenACC build does the same with
+ *    "!$acc enter data", ARTED/control/control_sc.f90:133-149).
+ *
+ * Every entry returns 0 on success or a SALMON_ERR_* code; the message of the
+ * last failure is salmon_gpu_last_error(ctx).  The reference has no status
+ * codes and aborts through err_finalize (ARTED/main.f90:21-33); its Fortran
+ * callers keep that behaviour by calling err_finalize on a non-zero status.
+ *
+ * A context owns one HIP stream on the current device.  Calls are ordered on
+ * that stream; those that return host values synchronise it.  A context is
+ * used by one host thread at a time.
+ */
+#ifndef SALMON_GPU_H
+#define SALMON_GPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  SALMON_OK = 0,
+  SALMON_ERR_ARG = 1,         /* invalid argument or size */
+  SALMON_ERR_HIP = 2,         /* a HIP runtime call failed */
+  SALMON_ERR_STATE = 3,       /* required set-up call missing */
+  SALMON_ERR_NOMEM = 4,       /* device allocation failed */
+  SALMON_ERR_UNSUPPORTED = 5  /* option not available for this shape */
+};
+
+enum {
+  SALMON_FP64 = 0,   /* complex128 orbitals, FP64 arithmetic */
+  SALMON_MIXED = 1   /* complex64 stencil/KB/Taylor with FP32 accumulation;
+                        FP64 norm, current, energy and density sums */
+};
+
+typedef struct salmon_ctx salmon_ctx;
+
+/* Grid and finite-difference coefficients: lapx..lapz = lap(0:4)/H^2 and
+ * nabx..nabz = nab(0:4)/H as produced by fd_coef
+ * (ARTED/preparation/fd_coef.f90:75-80); nab*[0] is unused. */
+typedef struct {
+  int NLx, NLy, NLz;
+  double aLx, aLy, aLz;
+  double lapx[5], lapy[5], lapz[5];
+  double nabx[5], naby[5], nabz[5];
+} salmon_grid_t;
+
+/* Creates a context.  Replaces the device side of the reference's
+ * set-up (opt_vars_initialize_p2, ARTED/modules/opt_variables.f90:105-179). */
+int salmon_gpu_init(const salmon_grid_t *grid, int precision, salmon_ctx **ctx);
+
+/* Vloc(NL): the local potential read by hpsi1_RT_stencil as B
+ * (ARTED/common/hpsi.f90:127; ARTED/RT/dt_evolve.f90:205-207). */
+int salmon_gpu_set_potential(salmon_ctx *ctx, const double *vloc);
+
+/* Kleinman-Bylander projector tables (ARTED/preparation/prep_ps.f90:81-208,
+ * atom/pp/prep_pp.f90:263-347): Mps(NI), Jxyz/Jxx/Jyy/Jzz(Nps,NI),
+ * a_tbl(Nlma), uV(Nps,Nlma), iuV(Nlma).  Projectors of one atom must be
+ * consecutive in a_tbl (as idx_lma assumes, opt_variables.f90:241-250). */
+int salmon_gpu_set_projectors(salmon_ctx *ctx, int NI, int Nps, const int *Mps, const int *Jxyz,
+                              const int *Jxx, const int *Jyy, const int *Jzz, int Nlma,
+                              const int *a_tbl, const double *uV, const int *iuV);
+
+/* k points kAc0(NK,3) and occupations occ(NB,NK); NB = NBoccmax
+ * (ARTED/preparation/init.f90:92-109, 283-331). */
+int salmon_gpu_set_kpoints(salmon_ctx *ctx, int NK, int NB, const double *kAc0, const double *occ);
+
+/* zu(NL,NB,NK) complex(8) host <-> device ("!$acc enter data" / "update self",
+ * ARTED/control/control_sc.f90:133-149,215).  In SALMON_MIXED the device copy
+ * is complex64. */
+int salmon_gpu_put_orbitals(salmon_ctx *ctx, const double *zu);
+int salmon_gpu_get_orbitals(salmon_ctx *ctx, double *zu);
+
+/* H*psi for one orbital at wave vector kAc(3): hpsi_omp_KB_GS / _RT
+ * (ARTED/common/hpsi.f90:33-96 -> hpsi_omp_KB_base :98-207).  in/out are
+ * complex(8) (NL).  ttpsi (nullable) receives H*psi - Vloc*psi without the
+ * nonlocal part (hpsi.f90:131-133). */
+int salmon_gpu_hpsi(salmon_ctx *ctx, const double kAc[3], const double *in, double *out, double *ttpsi);
+
+/* One 4th-order Taylor step of every resident orbital at kAc(NK,3)
+ * (the midpoint k+A): subroutine hamiltonian(zu, flag_current),
+ * ARTED/RT/hamiltonian.f90:27-123. */
+int salmon_gpu_hamiltonian(salmon_ctx *ctx, const double *kAc, double dt);
+
+/* Current jav(3) at kAc(NK,3): current_RT -> current('RT'),
+ * ARTED/RT/current.f90:65-292, including the summation (:276-291). */
+int salmon_gpu_current(salmon_ctx *ctx, const double *kAc, double jav[3]);
+
+/* rho(NL) = sum occ |zu|^2: psi_rho_RT with Sym=1, ARTED/common/psi_rho.f90:40-206. */
+int salmon_gpu_density(salmon_ctx *ctx, double *rho);
+
+/* Wavefunction energies at kAc(NK,3): Ekin and Enl of Total_Energy_omp
+ * (ARTED/common/total_energy.f90:113,172-203); eloc = sum V*rho*Hxyz of the
+ * frozen local potential. */
+int salmon_gpu_energy(salmon_ctx *ctx, const double *kAc, double *ekin, double *enl, double *eloc);
+
+/* Total norm sum occ |zu|^2 Hxyz. */
+int salmon_gpu_norm(salmon_ctx *ctx, double *norm);
+
+/* Gram-Schmidt of the resident orbitals of every k, in FP64:
+ * Gram_Schmidt_ompk, ARTED/GS/Gram_Schmidt.f90:32-55. */
+int salmon_gpu_gram_schmidt(salmon_ctx *ctx);
+
+/* esp(NB,NK) = <psi|H|psi> Hxyz at kAc(NK,3): sp_energy_omp, ARTED/GS/sp_energy.f90:20-60. */
+int salmon_gpu_sp_energy(salmon_ctx *ctx, const double *kAc, double *esp);
+
+/* Device-resident RT loop: the time loop of tddft_sc
+ * (ARTED/control/control_sc.f90:150-222) with the middlepoint propagator
+ * (ARTED/RT/dt_evolve.f90:43-51,106-216) and a frozen potential.  ac holds
+ * A(t_n) for n = 0..nsteps as (nsteps+1) rows of 3.  Step n propagates at
+ * kAc0 + (A_n+A_{n+1})/2, computes the density, then current and energy at
+ * kAc0 + A_{n+1}.  Outputs (nullable): jt (nsteps rows of 3), et(nsteps) =
+ * Ekin+Enl+Eloc, norm(nsteps).  reortho_every > 0 applies an FP64
+ * Gram-Schmidt after every reortho_every steps. */
+int salmon_gpu_rt_run(salmon_ctx *ctx, const double *ac, int nsteps, double dt, int reortho_every,
+                      double *jt, double *et, double *norm);
+
+/* Waits for all work of the context. */
+int salmon_gpu_synchronize(salmon_ctx *ctx);
+
+const char *salmon_gpu_errstr(int code);
+const char *salmon_gpu_last_error(const salmon_ctx *ctx);
+int salmon_gpu_finalize(salmon_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

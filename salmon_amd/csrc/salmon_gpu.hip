@@ -1,0 +1,706 @@
+// C-ABI of the gfx950 implementation of ARTED's real-time hot path.
+// The entry points are declared (with the reference routine each replaces)
+// in include/salmon_gpu.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/salmon_gpu.h"
+#include "common.hpp"
+#include "streamed.hpp"
+
+using namespace salmon;
+
+struct salmon_ctx {
+  int precision = SALMON_FP64;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  Grid g{};
+  std::string err;
+  // potential
+  double* d_vloc = nullptr;
+  float* d_vloc_f = nullptr;
+  bool have_v = false;
+  // projectors
+  bool have_kb = false;
+  KBTables kb{};
+  std::vector<void*> kb_bufs;
+  // k points / orbitals
+  int NK = 0, NB = 0, NKB = 0;
+  double* d_kac0 = nullptr;   // (3*NK) as kAc0(NK,3)
+  double* d_kac = nullptr;    // (3*NK) caller-given kAc
+  double* d_occ = nullptr;    // (NKB)
+  void* d_zu = nullptr;       // (NKB*NL) cplx<T>
+  bool have_zu = false;
+  // streamed work
+  int chunk = 0;
+  void* d_work[4] = {nullptr, nullptr, nullptr, nullptr};
+  void* d_uvpsi = nullptr;
+  double* d_rows = nullptr;   // (NKB*8)
+  double* d_sum = nullptr;    // (8)
+  double* d_rho = nullptr;    // (NL)
+  double* d_rho_part = nullptr;
+  int rho_chunks = 0;
+  // single-orbital staging
+  double* d_kone = nullptr;   // (3)
+  double* d_oneocc = nullptr; // (1)
+  void* d_stage[3] = {nullptr, nullptr, nullptr};
+  cplx<double>* d_f64 = nullptr;  // FP64 scratch copy of the orbitals (mixed Gram-Schmidt)
+  // fused step kernel configuration (0 = streamed path)
+  int fused = -1;
+  void* fz = nullptr;  // state owned by the fused path
+};
+
+namespace {
+
+int fail(salmon_ctx* c, int code, const char* fmt, ...) {
+  if (c) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    c->err = buf;
+  }
+  return code;
+}
+
+#define HIPCHK(c, x)                                                                                 \
+  do {                                                                                               \
+    hipError_t e_ = (x);                                                                             \
+    if (e_ != hipSuccess) return fail((c), SALMON_ERR_HIP, "%s: %s", #x, hipGetErrorString(e_));     \
+  } while (0)
+
+#define LAUNCHCHK(c)                                                                                 \
+  do {                                                                                               \
+    hipError_t e_ = hipGetLastError();                                                               \
+    if (e_ != hipSuccess) return fail((c), SALMON_ERR_HIP, "launch: %s", hipGetErrorString(e_));    \
+  } while (0)
+
+template <typename X>
+int dalloc(salmon_ctx* c, X** p, size_t n) {
+  if (*p) {
+    (void)hipFree(*p);
+    *p = nullptr;
+  }
+  if (n == 0) n = 1;
+  hipError_t e = hipMalloc((void**)p, n * sizeof(X));
+  if (e != hipSuccess) return fail(c, SALMON_ERR_NOMEM, "hipMalloc(%zu bytes): %s", n * sizeof(X), hipGetErrorString(e));
+  return SALMON_OK;
+}
+
+void dfree(void** p) {
+  if (*p) (void)hipFree(*p);
+  *p = nullptr;
+}
+
+size_t elem_size(const salmon_ctx* c) { return c->precision == SALMON_MIXED ? sizeof(cplx<float>) : sizeof(cplx<double>); }
+
+int ensure_work(salmon_ctx* c) {
+  if (c->d_work[0]) return SALMON_OK;
+  // work buffers hold at most ~1 GiB each
+  size_t per = (size_t)c->g.NL * elem_size(c);
+  size_t cap = std::max<size_t>(1, ((size_t)1 << 30) / per);
+  c->chunk = (int)std::min<size_t>((size_t)c->NKB, cap);
+  for (int i = 0; i < 4; ++i) {
+    hipError_t e = hipMalloc(&c->d_work[i], (size_t)c->chunk * per);
+    if (e != hipSuccess) return fail(c, SALMON_ERR_NOMEM, "work buffer: %s", hipGetErrorString(e));
+  }
+  int r = dalloc(c, (char**)&c->d_uvpsi, (size_t)c->chunk * std::max(1, c->kb.Nlma) * sizeof(cplx<double>));
+  if (r) return r;
+  return SALMON_OK;
+}
+
+int check_ready(salmon_ctx* c, bool need_zu) {
+  if (!c) return SALMON_ERR_ARG;
+  if (!c->have_v) return fail(c, SALMON_ERR_STATE, "salmon_gpu_set_potential not called");
+  if (!c->have_kb) return fail(c, SALMON_ERR_STATE, "salmon_gpu_set_projectors not called");
+  if (c->NKB == 0) return fail(c, SALMON_ERR_STATE, "salmon_gpu_set_kpoints not called");
+  if (need_zu && !c->have_zu) return fail(c, SALMON_ERR_STATE, "salmon_gpu_put_orbitals not called");
+  return SALMON_OK;
+}
+
+template <typename T>
+const T* vloc_ptr(const salmon_ctx* c);
+template <>
+const double* vloc_ptr<double>(const salmon_ctx* c) { return c->d_vloc; }
+template <>
+const float* vloc_ptr<float>(const salmon_ctx* c) { return c->d_vloc_f; }
+
+// H applied to nOrb orbitals (rows of NL) from `in` to `out` (optionally tt).
+template <typename T>
+int apply_h(salmon_ctx* c, const cplx<T>* in, cplx<T>* out, cplx<T>* tt, int orb0, int nOrb, int NB, int NK,
+            const double* kbase, double ax, double ay, double az) {
+  cplx<T>* uv = (cplx<T>*)c->d_uvpsi;
+  dim3 gp((c->kb.Nlma + 3) / 4, nOrb);
+  if (c->kb.Nlma > 0) {
+    hipLaunchKernelGGL(k_kb_project<T>, gp, dim3(256), 0, c->stream, in, orb0, NB, NK, kbase, ax, ay, az, c->g,
+                       c->kb, uv);
+    LAUNCHCHK(c);
+  }
+  dim3 gh((c->g.NL + 255) / 256, nOrb);
+  hipLaunchKernelGGL(k_hpsi<T>, gh, dim3(256), 0, c->stream, in, out, tt, orb0, NB, NK, kbase, ax, ay, az,
+                     vloc_ptr<T>(c), c->g, c->kb, (const cplx<T>*)uv);
+  LAUNCHCHK(c);
+  return SALMON_OK;
+}
+
+template <typename T>
+int taylor_streamed(salmon_ctx* c, const double* kbase, double ax, double ay, double az, double dt) {
+  int r = ensure_work(c);
+  if (r) return r;
+  // zfac(n) = (-i dt)^n / n!  (hamiltonian.f90:46-49)
+  const double z1i = -dt, z2r = -dt * dt / 2.0, z3i = dt * dt * dt / 6.0, z4r = dt * dt * dt * dt / 24.0;
+  const size_t NL = c->g.NL;
+  for (int o0 = 0; o0 < c->NKB; o0 += c->chunk) {
+    const int n = std::min(c->chunk, c->NKB - o0);
+    cplx<T>* zu = (cplx<T>*)c->d_zu + (size_t)o0 * NL;
+    cplx<T>* h[4];
+    for (int i = 0; i < 4; ++i) h[i] = (cplx<T>*)c->d_work[i];
+    if ((r = apply_h<T>(c, zu, h[0], nullptr, o0, n, c->NB, c->NK, kbase, ax, ay, az))) return r;
+    if ((r = apply_h<T>(c, h[0], h[1], nullptr, o0, n, c->NB, c->NK, kbase, ax, ay, az))) return r;
+    if ((r = apply_h<T>(c, h[1], h[2], nullptr, o0, n, c->NB, c->NK, kbase, ax, ay, az))) return r;
+    if ((r = apply_h<T>(c, h[2], h[3], nullptr, o0, n, c->NB, c->NK, kbase, ax, ay, az))) return r;
+    const size_t tot = (size_t)n * NL;
+    hipLaunchKernelGGL(k_taylor_fold<T>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, c->stream, zu, h[0], h[1],
+                       h[2], h[3], tot, (T)z1i, (T)z2r, (T)z3i, (T)z4r);
+    LAUNCHCHK(c);
+  }
+  return SALMON_OK;
+}
+
+template <typename T>
+int observe_rows(salmon_ctx* c, const double* kbase, double ax, double ay, double az) {
+  hipLaunchKernelGGL(k_observe<T>, dim3(c->NKB), dim3(256), 0, c->stream, (const cplx<T>*)c->d_zu, 0, c->NB, c->NK,
+                     kbase, ax, ay, az, c->d_occ, c->d_vloc, c->g, c->kb, c->d_rows);
+  LAUNCHCHK(c);
+  return SALMON_OK;
+}
+
+template <typename T>
+int density_dev(salmon_ctx* c) {
+  const int NL = c->g.NL;
+  if (!c->d_rho_part) {
+    c->rho_chunks = std::max(1, std::min(c->NKB, std::max(1, 4096 * 1024 / std::max(1, NL))));
+    int r = dalloc(c, &c->d_rho_part, (size_t)c->rho_chunks * NL);
+    if (r) return r;
+  }
+  const int per = (c->NKB + c->rho_chunks - 1) / c->rho_chunks;
+  const int nch = (c->NKB + per - 1) / per;
+  hipLaunchKernelGGL(k_density_part<T>, dim3((NL + 255) / 256, nch), dim3(256), 0, c->stream,
+                     (const cplx<T>*)c->d_zu, c->NKB, NL, per, c->d_occ, c->d_rho_part);
+  LAUNCHCHK(c);
+  hipLaunchKernelGGL(k_density_final, dim3((NL + 255) / 256), dim3(256), 0, c->stream, c->d_rho_part, nch, NL,
+                     c->d_rho);
+  LAUNCHCHK(c);
+  return SALMON_OK;
+}
+
+int gram_schmidt_dev(salmon_ctx* c) {
+  const size_t n = (size_t)c->NKB * c->g.NL;
+  if (c->precision == SALMON_FP64) {
+    hipLaunchKernelGGL(k_gram_schmidt, dim3(c->NK), dim3(256), 0, c->stream, (cplx<double>*)c->d_zu, c->NB, c->g.NL,
+                       c->g.Hxyz);
+    LAUNCHCHK(c);
+    return SALMON_OK;
+  }
+  if (!c->d_f64) {
+    int r = dalloc(c, &c->d_f64, n);
+    if (r) return r;
+  }
+  const unsigned nb = (unsigned)((n + 255) / 256);
+  hipLaunchKernelGGL((k_convert<float, double>), dim3(nb), dim3(256), 0, c->stream, (const cplx<float>*)c->d_zu,
+                     c->d_f64, n);
+  LAUNCHCHK(c);
+  hipLaunchKernelGGL(k_gram_schmidt, dim3(c->NK), dim3(256), 0, c->stream, c->d_f64, c->NB, c->g.NL, c->g.Hxyz);
+  LAUNCHCHK(c);
+  hipLaunchKernelGGL((k_convert<double, float>), dim3(nb), dim3(256), 0, c->stream, (const cplx<double>*)c->d_f64,
+                     (cplx<float>*)c->d_zu, n);
+  LAUNCHCHK(c);
+  return SALMON_OK;
+}
+
+int upload_kac(salmon_ctx* c, const double* kAc) {
+  if (!kAc) return fail(c, SALMON_ERR_ARG, "kAc is NULL");
+  HIPCHK(c, hipMemcpyAsync(c->d_kac, kAc, sizeof(double) * 3 * c->NK, hipMemcpyHostToDevice, c->stream));
+  return SALMON_OK;
+}
+
+int read_sums(salmon_ctx* c, double out[8]) {
+  hipLaunchKernelGGL(k_sum_rows, dim3(1), dim3(256), 0, c->stream, c->d_rows, c->NKB, c->d_sum);
+  LAUNCHCHK(c);
+  HIPCHK(c, hipMemcpyAsync(out, c->d_sum, sizeof(double) * 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return SALMON_OK;
+}
+
+}  // namespace
+
+#include "fused.hpp"
+
+namespace {
+
+// One RT step on resident orbitals: Taylor-4 at kAc0 + amid, density, then
+// observables at kAc0 + anew into res[0..7].
+template <typename T>
+int rt_step(salmon_ctx* c, const double amid[3], const double anew[3], double dt, double* res_dev) {
+  int r;
+  if (c->fused > 0) {
+    r = fused_step<T>(c, amid, anew, dt, res_dev);
+    return r;
+  }
+  if ((r = taylor_streamed<T>(c, c->d_kac0, amid[0], amid[1], amid[2], dt))) return r;
+  if ((r = density_dev<T>(c))) return r;
+  if ((r = observe_rows<T>(c, c->d_kac0, anew[0], anew[1], anew[2]))) return r;
+  hipLaunchKernelGGL(k_sum_rows, dim3(1), dim3(256), 0, c->stream, c->d_rows, c->NKB, res_dev);
+  LAUNCHCHK(c);
+  return SALMON_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* salmon_gpu_errstr(int code) {
+  switch (code) {
+    case SALMON_OK: return "ok";
+    case SALMON_ERR_ARG: return "invalid argument";
+    case SALMON_ERR_HIP: return "HIP runtime error";
+    case SALMON_ERR_STATE: return "missing set-up call";
+    case SALMON_ERR_NOMEM: return "device memory allocation failed";
+    case SALMON_ERR_UNSUPPORTED: return "unsupported option";
+    default: return "unknown error";
+  }
+}
+
+const char* salmon_gpu_last_error(const salmon_ctx* c) { return c ? c->err.c_str() : "no context"; }
+
+int salmon_gpu_init(const salmon_grid_t* grid, int precision, salmon_ctx** out) {
+  if (!grid || !out) return SALMON_ERR_ARG;
+  *out = nullptr;
+  if (grid->NLx < 9 || grid->NLy < 9 || grid->NLz < 9) return SALMON_ERR_ARG;  // Nd=4 stencil needs > 2*Nd points
+  if (precision != SALMON_FP64 && precision != SALMON_MIXED) return SALMON_ERR_ARG;
+  if (!(grid->aLx > 0 && grid->aLy > 0 && grid->aLz > 0)) return SALMON_ERR_ARG;
+  salmon_ctx* c = new salmon_ctx();
+  c->precision = precision;
+  if (hipGetDevice(&c->device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess) {
+    delete c;
+    return SALMON_ERR_HIP;
+  }
+  Grid& g = c->g;
+  g.NLx = grid->NLx; g.NLy = grid->NLy; g.NLz = grid->NLz;
+  g.NL = g.NLx * g.NLy * g.NLz;
+  g.aLx = grid->aLx; g.aLy = grid->aLy; g.aLz = grid->aLz;
+  g.aLxyz = g.aLx * g.aLy * g.aLz;
+  g.Hx = g.aLx / g.NLx; g.Hy = g.aLy / g.NLy; g.Hz = g.aLz / g.NLz;
+  g.Hxyz = g.Hx * g.Hy * g.Hz;
+  for (int n = 0; n < 5; ++n) {
+    g.lap[0][n] = grid->lapx[n]; g.lap[1][n] = grid->lapy[n]; g.lap[2][n] = grid->lapz[n];
+    g.nab[0][n] = grid->nabx[n]; g.nab[1][n] = grid->naby[n]; g.nab[2][n] = grid->nabz[n];
+  }
+  int r;
+  if ((r = dalloc(c, &c->d_vloc, g.NL)) || (r = dalloc(c, &c->d_vloc_f, g.NL)) || (r = dalloc(c, &c->d_rho, g.NL)) ||
+      (r = dalloc(c, &c->d_kone, 3)) || (r = dalloc(c, &c->d_oneocc, 1)) || (r = dalloc(c, &c->d_sum, 8))) {
+    salmon_gpu_finalize(c);
+    return r;
+  }
+  for (int i = 0; i < 3; ++i) {
+    if (hipMalloc(&c->d_stage[i], (size_t)g.NL * sizeof(cplx<double>)) != hipSuccess) {
+      salmon_gpu_finalize(c);
+      return SALMON_ERR_NOMEM;
+    }
+  }
+  const double one = 1.0;
+  if (hipMemcpy(c->d_oneocc, &one, sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
+    salmon_gpu_finalize(c);
+    return SALMON_ERR_HIP;
+  }
+  c->fused = fused_config(c);
+  *out = c;
+  return SALMON_OK;
+}
+
+int salmon_gpu_set_potential(salmon_ctx* c, const double* vloc) {
+  if (!c) return SALMON_ERR_ARG;
+  if (!vloc) return fail(c, SALMON_ERR_ARG, "vloc is NULL");
+  const int NL = c->g.NL;
+  std::vector<float> vf(NL);
+  for (int i = 0; i < NL; ++i) vf[i] = (float)vloc[i];
+  HIPCHK(c, hipMemcpy(c->d_vloc, vloc, sizeof(double) * NL, hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(c->d_vloc_f, vf.data(), sizeof(float) * NL, hipMemcpyHostToDevice));
+  c->have_v = true;
+  return SALMON_OK;
+}
+
+int salmon_gpu_set_projectors(salmon_ctx* c, int NI, int Nps, const int* Mps, const int* Jxyz, const int* Jxx,
+                              const int* Jyy, const int* Jzz, int Nlma, const int* a_tbl, const double* uV,
+                              const int* iuV) {
+  if (!c) return SALMON_ERR_ARG;
+  if (NI < 0 || Nps < 0 || Nlma < 0) return fail(c, SALMON_ERR_ARG, "negative table size");
+  if ((NI > 0 && (!Mps || !Jxyz || !Jxx || !Jyy || !Jzz)) || (Nlma > 0 && (!a_tbl || !uV || !iuV)))
+    return fail(c, SALMON_ERR_ARG, "projector table pointer is NULL");
+  const Grid& g = c->g;
+  std::vector<int> start(NI + 1, 0);
+  for (int a = 0; a < NI; ++a) {
+    if (Mps[a] < 0 || Mps[a] > Nps) return fail(c, SALMON_ERR_ARG, "Mps(%d)=%d outside 0..Nps=%d", a + 1, Mps[a], Nps);
+    start[a + 1] = start[a] + Mps[a];
+  }
+  const int NPI = start[NI];
+  std::vector<int> pidx(NPI);
+  std::vector<double> pos(3 * (size_t)NPI);
+  for (int a = 0; a < NI; ++a)
+    for (int j = 0; j < Mps[a]; ++j) {
+      const int i = Jxyz[(size_t)a * Nps + j];
+      if (i < 1 || i > g.NL) return fail(c, SALMON_ERR_ARG, "Jxyz(%d,%d)=%d outside 1..NL", j + 1, a + 1, i);
+      const int i0 = i - 1, jp = start[a] + j;
+      const int lx = i0 / (g.NLy * g.NLz), ly = (i0 / g.NLz) % g.NLy, lz = i0 % g.NLz;
+      pidx[jp] = i0;
+      pos[3 * jp + 0] = lx * g.Hx - Jxx[(size_t)a * Nps + j] * g.aLx;
+      pos[3 * jp + 1] = ly * g.Hy - Jyy[(size_t)a * Nps + j] * g.aLy;
+      pos[3 * jp + 2] = lz * g.Hz - Jzz[(size_t)a * Nps + j] * g.aLz;
+    }
+  std::vector<int> lma_atom(Nlma);
+  std::vector<double> sign(Nlma);
+  for (int l = 0; l < Nlma; ++l) {
+    const int a = a_tbl[l] - 1;
+    if (a < 0 || a >= NI) return fail(c, SALMON_ERR_ARG, "a_tbl(%d)=%d outside 1..NI", l + 1, a_tbl[l]);
+    if (l > 0 && a < lma_atom[l - 1]) return fail(c, SALMON_ERR_ARG, "a_tbl must group projectors by atom in order");
+    lma_atom[l] = a;
+    sign[l] = (double)iuV[l];
+  }
+  // transposed table: per touched grid point, its (projector, packed point) entries
+  std::vector<std::vector<std::pair<int, int>>> per(g.NL);
+  for (int l = 0; l < Nlma; ++l) {
+    const int a = lma_atom[l];
+    for (int j = 0; j < Mps[a]; ++j) per[pidx[start[a] + j]].push_back({start[a] + j, l});
+  }
+  std::vector<int> pt2t(g.NL, -1), tstart(1, 0), elma, ejp;
+  int NT = 0;
+  for (int p = 0; p < g.NL; ++p) {
+    if (per[p].empty()) continue;
+    std::sort(per[p].begin(), per[p].end());
+    pt2t[p] = NT++;
+    for (auto& e : per[p]) {
+      ejp.push_back(e.first);
+      elma.push_back(e.second);
+    }
+    tstart.push_back((int)ejp.size());
+  }
+  for (void* p : c->kb_bufs) (void)hipFree(p);
+  c->kb_bufs.clear();
+  auto up = [&](const void* src, size_t bytes, void** dst) -> int {
+    void* d = nullptr;
+    if (hipMalloc(&d, bytes ? bytes : 4) != hipSuccess) return fail(c, SALMON_ERR_NOMEM, "projector tables");
+    c->kb_bufs.push_back(d);
+    if (bytes && hipMemcpy(d, src, bytes, hipMemcpyHostToDevice) != hipSuccess)
+      return fail(c, SALMON_ERR_HIP, "projector table upload");
+    *dst = d;
+    return SALMON_OK;
+  };
+  KBTables kb{};
+  kb.NI = NI; kb.Nps = Nps; kb.Nlma = Nlma; kb.NPI = NPI; kb.NT = NT; kb.NE = (int)ejp.size();
+  int r;
+  if ((r = up(start.data(), sizeof(int) * start.size(), (void**)&kb.atom_start)) ||
+      (r = up(pidx.data(), sizeof(int) * pidx.size(), (void**)&kb.pidx)) ||
+      (r = up(pos.data(), sizeof(double) * pos.size(), (void**)&kb.pos)) ||
+      (r = up(lma_atom.data(), sizeof(int) * lma_atom.size(), (void**)&kb.lma_atom)) ||
+      (r = up(sign.data(), sizeof(double) * sign.size(), (void**)&kb.lma_sign)) ||
+      (r = up(uV, sizeof(double) * (size_t)Nps * Nlma, (void**)&kb.uv)) ||
+      (r = up(pt2t.data(), sizeof(int) * pt2t.size(), (void**)&kb.pt2t)) ||
+      (r = up(tstart.data(), sizeof(int) * tstart.size(), (void**)&kb.t_start)) ||
+      (r = up(elma.data(), sizeof(int) * elma.size(), (void**)&kb.e_lma)) ||
+      (r = up(ejp.data(), sizeof(int) * ejp.size(), (void**)&kb.e_jp)))
+    return r;
+  c->kb = kb;
+  c->have_kb = true;
+  dfree(&c->d_uvpsi);
+  for (int i = 0; i < 4; ++i) dfree(&c->d_work[i]);
+  int fr = fused_set_tables(c);
+  if (fr) return fr;
+  return SALMON_OK;
+}
+
+int salmon_gpu_set_kpoints(salmon_ctx* c, int NK, int NB, const double* kAc0, const double* occ) {
+  if (!c) return SALMON_ERR_ARG;
+  if (NK < 1 || NB < 1 || !kAc0 || !occ) return fail(c, SALMON_ERR_ARG, "NK, NB must be >= 1 with kAc0, occ given");
+  if ((size_t)NK * NB * c->g.NL > ((size_t)1 << 40)) return fail(c, SALMON_ERR_ARG, "orbital set too large");
+  c->NK = NK; c->NB = NB; c->NKB = NK * NB;
+  int r;
+  if ((r = dalloc(c, &c->d_kac0, 3 * (size_t)NK)) || (r = dalloc(c, &c->d_kac, 3 * (size_t)NK)) ||
+      (r = dalloc(c, &c->d_occ, (size_t)c->NKB)) || (r = dalloc(c, &c->d_rows, 8 * (size_t)c->NKB)))
+    return r;
+  HIPCHK(c, hipMemcpy(c->d_kac0, kAc0, sizeof(double) * 3 * NK, hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(c->d_occ, occ, sizeof(double) * c->NKB, hipMemcpyHostToDevice));
+  dfree(&c->d_zu);
+  dfree((void**)&c->d_f64);
+  dfree((void**)&c->d_rho_part);
+  dfree(&c->d_uvpsi);
+  for (int i = 0; i < 4; ++i) dfree(&c->d_work[i]);
+  c->have_zu = false;
+  void* z = nullptr;
+  HIPCHK(c, hipMalloc(&z, (size_t)c->NKB * c->g.NL * elem_size(c)));
+  c->d_zu = z;
+  return SALMON_OK;
+}
+
+int salmon_gpu_put_orbitals(salmon_ctx* c, const double* zu) {
+  if (!c) return SALMON_ERR_ARG;
+  if (!c->d_zu) return fail(c, SALMON_ERR_STATE, "salmon_gpu_set_kpoints not called");
+  if (!zu) return fail(c, SALMON_ERR_ARG, "zu is NULL");
+  const size_t n = (size_t)c->NKB * c->g.NL;
+  if (c->precision == SALMON_FP64) {
+    HIPCHK(c, hipMemcpy(c->d_zu, zu, n * sizeof(cplx<double>), hipMemcpyHostToDevice));
+  } else {
+    // complex(8) rows are converted to complex64 on the device, a block of rows at a time
+    const size_t blk = (size_t)c->g.NL, rows = std::max<size_t>(1, std::min<size_t>(c->NKB, (64u << 20) / (blk * 16)));
+    cplx<double>* st = nullptr;
+    HIPCHK(c, hipMalloc((void**)&st, rows * blk * sizeof(cplx<double>)));
+    int r = SALMON_OK;
+    for (size_t o = 0; o < (size_t)c->NKB && r == SALMON_OK; o += rows) {
+      const size_t m = std::min(rows, (size_t)c->NKB - o) * blk;
+      hipError_t e = hipMemcpyAsync(st, zu + 2 * o * blk, m * sizeof(cplx<double>), hipMemcpyHostToDevice, c->stream);
+      if (e == hipSuccess) {
+        hipLaunchKernelGGL((k_convert<double, float>), dim3((unsigned)((m + 255) / 256)), dim3(256), 0, c->stream, st,
+                           (cplx<float>*)c->d_zu + o * blk, m);
+        e = hipGetLastError();
+      }
+      if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+      if (e != hipSuccess) r = fail(c, SALMON_ERR_HIP, "put_orbitals: %s", hipGetErrorString(e));
+    }
+    (void)hipFree(st);
+    if (r) return r;
+  }
+  c->have_zu = true;
+  return SALMON_OK;
+}
+
+int salmon_gpu_get_orbitals(salmon_ctx* c, double* zu) {
+  if (!c) return SALMON_ERR_ARG;
+  if (!c->have_zu) return fail(c, SALMON_ERR_STATE, "no resident orbitals");
+  if (!zu) return fail(c, SALMON_ERR_ARG, "zu is NULL");
+  const size_t n = (size_t)c->NKB * c->g.NL;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (c->precision == SALMON_FP64) {
+    HIPCHK(c, hipMemcpy(zu, c->d_zu, n * sizeof(cplx<double>), hipMemcpyDeviceToHost));
+  } else {
+    const size_t blk = (size_t)c->g.NL, rows = std::max<size_t>(1, std::min<size_t>(c->NKB, (64u << 20) / (blk * 16)));
+    cplx<double>* st = nullptr;
+    HIPCHK(c, hipMalloc((void**)&st, rows * blk * sizeof(cplx<double>)));
+    int r = SALMON_OK;
+    for (size_t o = 0; o < (size_t)c->NKB && r == SALMON_OK; o += rows) {
+      const size_t m = std::min(rows, (size_t)c->NKB - o) * blk;
+      hipLaunchKernelGGL((k_convert<float, double>), dim3((unsigned)((m + 255) / 256)), dim3(256), 0, c->stream,
+                         (const cplx<float>*)c->d_zu + o * blk, st, m);
+      hipError_t e = hipGetLastError();
+      if (e == hipSuccess) e = hipMemcpyAsync(zu + 2 * o * blk, st, m * sizeof(cplx<double>), hipMemcpyDeviceToHost, c->stream);
+      if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+      if (e != hipSuccess) r = fail(c, SALMON_ERR_HIP, "get_orbitals: %s", hipGetErrorString(e));
+    }
+    (void)hipFree(st);
+    if (r) return r;
+  }
+  return SALMON_OK;
+}
+
+int salmon_gpu_hpsi(salmon_ctx* c, const double kAc[3], const double* in, double* out, double* ttpsi) {
+  int r = check_ready(c, false);
+  if (r) return r;
+  if (!kAc || !in || !out) return fail(c, SALMON_ERR_ARG, "NULL argument");
+  if ((r = ensure_work(c))) return r;
+  const size_t NL = c->g.NL;
+  HIPCHK(c, hipMemcpyAsync(c->d_kone, kAc, 3 * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->d_stage[0], in, NL * sizeof(cplx<double>), hipMemcpyHostToDevice, c->stream));
+  if (c->precision == SALMON_FP64) {
+    if ((r = apply_h<double>(c, (const cplx<double>*)c->d_stage[0], (cplx<double>*)c->d_stage[1],
+                             ttpsi ? (cplx<double>*)c->d_stage[2] : nullptr, 0, 1, 1, 1, c->d_kone, 0, 0, 0)))
+      return r;
+  } else {
+    cplx<float>* w0 = (cplx<float>*)c->d_work[0];
+    cplx<float>* w1 = (cplx<float>*)c->d_work[1];
+    cplx<float>* w2 = (cplx<float>*)c->d_work[2];
+    const unsigned nb = (unsigned)((NL + 255) / 256);
+    hipLaunchKernelGGL((k_convert<double, float>), dim3(nb), dim3(256), 0, c->stream,
+                       (const cplx<double>*)c->d_stage[0], w0, NL);
+    LAUNCHCHK(c);
+    if ((r = apply_h<float>(c, w0, w1, ttpsi ? w2 : nullptr, 0, 1, 1, 1, c->d_kone, 0, 0, 0))) return r;
+    hipLaunchKernelGGL((k_convert<float, double>), dim3(nb), dim3(256), 0, c->stream, w1,
+                       (cplx<double>*)c->d_stage[1], NL);
+    LAUNCHCHK(c);
+    if (ttpsi) {
+      hipLaunchKernelGGL((k_convert<float, double>), dim3(nb), dim3(256), 0, c->stream, w2,
+                         (cplx<double>*)c->d_stage[2], NL);
+      LAUNCHCHK(c);
+    }
+  }
+  HIPCHK(c, hipMemcpyAsync(out, c->d_stage[1], NL * sizeof(cplx<double>), hipMemcpyDeviceToHost, c->stream));
+  if (ttpsi)
+    HIPCHK(c, hipMemcpyAsync(ttpsi, c->d_stage[2], NL * sizeof(cplx<double>), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return SALMON_OK;
+}
+
+int salmon_gpu_hamiltonian(salmon_ctx* c, const double* kAc, double dt) {
+  int r = check_ready(c, true);
+  if (r) return r;
+  if ((r = upload_kac(c, kAc))) return r;
+  if (c->precision == SALMON_FP64) return taylor_streamed<double>(c, c->d_kac, 0, 0, 0, dt);
+  return taylor_streamed<float>(c, c->d_kac, 0, 0, 0, dt);
+}
+
+int salmon_gpu_current(salmon_ctx* c, const double* kAc, double jav[3]) {
+  int r = check_ready(c, true);
+  if (r) return r;
+  if (!jav) return fail(c, SALMON_ERR_ARG, "jav is NULL");
+  if ((r = upload_kac(c, kAc))) return r;
+  r = c->precision == SALMON_FP64 ? observe_rows<double>(c, c->d_kac, 0, 0, 0) : observe_rows<float>(c, c->d_kac, 0, 0, 0);
+  if (r) return r;
+  double s[8];
+  if ((r = read_sums(c, s))) return r;
+  jav[0] = s[0]; jav[1] = s[1]; jav[2] = s[2];
+  return SALMON_OK;
+}
+
+int salmon_gpu_energy(salmon_ctx* c, const double* kAc, double* ekin, double* enl, double* eloc) {
+  int r = check_ready(c, true);
+  if (r) return r;
+  if ((r = upload_kac(c, kAc))) return r;
+  r = c->precision == SALMON_FP64 ? observe_rows<double>(c, c->d_kac, 0, 0, 0) : observe_rows<float>(c, c->d_kac, 0, 0, 0);
+  if (r) return r;
+  double s[8];
+  if ((r = read_sums(c, s))) return r;
+  if (ekin) *ekin = s[3];
+  if (enl) *enl = s[4];
+  if (eloc) *eloc = s[5];
+  return SALMON_OK;
+}
+
+int salmon_gpu_norm(salmon_ctx* c, double* norm) {
+  int r = check_ready(c, true);
+  if (r) return r;
+  if (!norm) return fail(c, SALMON_ERR_ARG, "norm is NULL");
+  r = c->precision == SALMON_FP64 ? observe_rows<double>(c, c->d_kac0, 0, 0, 0)
+                                  : observe_rows<float>(c, c->d_kac0, 0, 0, 0);
+  if (r) return r;
+  double s[8];
+  if ((r = read_sums(c, s))) return r;
+  *norm = s[6];
+  return SALMON_OK;
+}
+
+int salmon_gpu_density(salmon_ctx* c, double* rho) {
+  int r = check_ready(c, true);
+  if (r) return r;
+  if (!rho) return fail(c, SALMON_ERR_ARG, "rho is NULL");
+  r = c->precision == SALMON_FP64 ? density_dev<double>(c) : density_dev<float>(c);
+  if (r) return r;
+  HIPCHK(c, hipMemcpyAsync(rho, c->d_rho, sizeof(double) * c->g.NL, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return SALMON_OK;
+}
+
+int salmon_gpu_gram_schmidt(salmon_ctx* c) {
+  int r = check_ready(c, true);
+  if (r) return r;
+  return gram_schmidt_dev(c);
+}
+
+int salmon_gpu_sp_energy(salmon_ctx* c, const double* kAc, double* esp) {
+  int r = check_ready(c, true);
+  if (r) return r;
+  if (!esp) return fail(c, SALMON_ERR_ARG, "esp is NULL");
+  if ((r = upload_kac(c, kAc))) return r;
+  if ((r = ensure_work(c))) return r;
+  const size_t NL = c->g.NL;
+  std::vector<double> tmp;
+  double* d_dots = c->d_rows;  // (NKB*2) fits in NKB*8
+  for (int o0 = 0; o0 < c->NKB; o0 += c->chunk) {
+    const int n = std::min(c->chunk, c->NKB - o0);
+    if (c->precision == SALMON_FP64) {
+      const cplx<double>* zu = (const cplx<double>*)c->d_zu + (size_t)o0 * NL;
+      if ((r = apply_h<double>(c, zu, (cplx<double>*)c->d_work[0], nullptr, o0, n, c->NB, c->NK, c->d_kac, 0, 0, 0)))
+        return r;
+      hipLaunchKernelGGL(k_dot_rows<double>, dim3(n), dim3(256), 0, c->stream, zu, (const cplx<double>*)c->d_work[0],
+                         c->g.NL, c->g.Hxyz, d_dots + 2 * (size_t)o0);
+    } else {
+      const cplx<float>* zu = (const cplx<float>*)c->d_zu + (size_t)o0 * NL;
+      if ((r = apply_h<float>(c, zu, (cplx<float>*)c->d_work[0], nullptr, o0, n, c->NB, c->NK, c->d_kac, 0, 0, 0)))
+        return r;
+      hipLaunchKernelGGL(k_dot_rows<float>, dim3(n), dim3(256), 0, c->stream, zu, (const cplx<float>*)c->d_work[0],
+                         c->g.NL, c->g.Hxyz, d_dots + 2 * (size_t)o0);
+    }
+    LAUNCHCHK(c);
+  }
+  tmp.resize(2 * (size_t)c->NKB);
+  HIPCHK(c, hipMemcpyAsync(tmp.data(), d_dots, sizeof(double) * tmp.size(), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (int o = 0; o < c->NKB; ++o) esp[o] = tmp[2 * (size_t)o];
+  return SALMON_OK;
+}
+
+int salmon_gpu_rt_run(salmon_ctx* c, const double* ac, int nsteps, double dt, int reortho_every, double* jt, double* et,
+                      double* norm) {
+  int r = check_ready(c, true);
+  if (r) return r;
+  if (!ac || nsteps < 0) return fail(c, SALMON_ERR_ARG, "ac is NULL or nsteps < 0");
+  if (nsteps == 0) return SALMON_OK;
+  double* d_res = nullptr;
+  if ((r = dalloc(c, &d_res, 8 * (size_t)nsteps))) return r;
+  for (int n = 0; n < nsteps && r == SALMON_OK; ++n) {
+    double amid[3], anew[3];
+    for (int d = 0; d < 3; ++d) {
+      amid[d] = 0.5 * (ac[3 * n + d] + ac[3 * (n + 1) + d]);
+      anew[d] = ac[3 * (n + 1) + d];
+    }
+    r = c->precision == SALMON_FP64 ? rt_step<double>(c, amid, anew, dt, d_res + 8 * (size_t)n)
+                                    : rt_step<float>(c, amid, anew, dt, d_res + 8 * (size_t)n);
+    if (r == SALMON_OK && reortho_every > 0 && (n + 1) % reortho_every == 0) r = gram_schmidt_dev(c);
+  }
+  if (r == SALMON_OK) {
+    std::vector<double> res(8 * (size_t)nsteps);
+    hipError_t e = hipMemcpyAsync(res.data(), d_res, sizeof(double) * res.size(), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) {
+      r = fail(c, SALMON_ERR_HIP, "rt_run results: %s", hipGetErrorString(e));
+    } else {
+      for (int n = 0; n < nsteps; ++n) {
+        const double* q = &res[8 * (size_t)n];
+        if (jt) { jt[3 * n] = q[0]; jt[3 * n + 1] = q[1]; jt[3 * n + 2] = q[2]; }
+        if (et) et[n] = q[3] + q[4] + q[5];
+        if (norm) norm[n] = q[6];
+      }
+    }
+  }
+  (void)hipStreamSynchronize(c->stream);
+  (void)hipFree(d_res);
+  return r;
+}
+
+int salmon_gpu_synchronize(salmon_ctx* c) {
+  if (!c) return SALMON_ERR_ARG;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return SALMON_OK;
+}
+
+int salmon_gpu_finalize(salmon_ctx* c) {
+  if (!c) return SALMON_ERR_ARG;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  fused_free(c);
+  dfree((void**)&c->d_vloc); dfree((void**)&c->d_vloc_f); dfree((void**)&c->d_rho); dfree((void**)&c->d_rho_part);
+  dfree((void**)&c->d_kac0); dfree((void**)&c->d_kac); dfree((void**)&c->d_occ); dfree((void**)&c->d_rows);
+  dfree((void**)&c->d_sum); dfree((void**)&c->d_kone); dfree((void**)&c->d_oneocc); dfree(&c->d_zu);
+  dfree((void**)&c->d_f64); dfree(&c->d_uvpsi);
+  for (int i = 0; i < 4; ++i) dfree(&c->d_work[i]);
+  for (int i = 0; i < 3; ++i) dfree(&c->d_stage[i]);
+  for (void* p : c->kb_bufs) (void)hipFree(p);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return SALMON_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,376 @@
+// Streamed kernels: one pass over HBM per operator application, any grid.
+//
+// These serve every grid shape (the LDS-resident step kernel covers the
+// grids whose orbital fits in LDS), the single-orbital hpsi entry and the
+// ground-state routines.
+#pragma once
+#include "common.hpp"
+
+namespace salmon {
+
+// wave vector of orbital ikb: kAc0(ik,:) + a (a = A shift of the step)
+__device__ __forceinline__ void orbital_k(const double* kac0, int NK, int ik, double ax, double ay, double az,
+                                          double& kx, double& ky, double& kz) {
+  kx = kac0[ik] + ax;
+  ky = kac0[NK + ik] + ay;
+  kz = kac0[2 * NK + ik] + az;
+}
+
+// ---------------------------------------------------------------------------
+// KB projection, phase 1 of pseudo_pt (ARTED/common/hpsi.f90:183-193):
+//   uVpsi(ilma) = Hxyz * iuV * sum_j conj(zproj(j,ilma)) * psi(Jxyz(j))
+// with conj(zproj) = exp(i k.r_j) uV(j,ilma) (projector.f90:62,72) formed on
+// the fly.  One wave per (projector, orbital); block = 4 waves.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(256) k_kb_project(const cplx<T>* __restrict__ psi, int orb0, int NB, int NK,
+                                                    const double* __restrict__ kac0, double ax, double ay,
+                                                    double az, Grid g, KBTables kb, cplx<T>* __restrict__ uvpsi) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int ilma = blockIdx.x * 4 + (threadIdx.x / kWave);
+  const int o = blockIdx.y;
+  if (ilma >= kb.Nlma) return;
+  const int ikb = orb0 + o;
+  double kx, ky, kz;
+  orbital_k(kac0, NK, ikb / NB, ax, ay, az, kx, ky, kz);
+  const int a = kb.lma_atom[ilma];
+  const int j0 = kb.atom_start[a], m = kb.atom_start[a + 1] - j0;
+  const cplx<T>* E = psi + (size_t)o * g.NL;
+  T sr = 0, si = 0;
+  for (int j = lane; j < m; j += kWave) {
+    const int jp = j0 + j;
+    const double kr = kx * kb.pos[3 * jp] + ky * kb.pos[3 * jp + 1] + kz * kb.pos[3 * jp + 2];
+    T s, c;
+    sincos_t((T)kr, &s, &c);
+    const T u = (T)kb.uv[(size_t)ilma * kb.Nps + j];
+    const cplx<T> e = E[kb.pidx[jp]];
+    // (u c + i u s) * (e.re + i e.im)
+    sr += u * (c * e.re - s * e.im);
+    si += u * (c * e.im + s * e.re);
+  }
+  sr = wave_sum(sr);
+  si = wave_sum(si);
+  if (lane == 0) {
+    const T f = (T)(g.Hxyz * kb.lma_sign[ilma]);
+    uvpsi[(size_t)o * kb.Nlma + ilma] = cplx<T>{sr * f, si * f};
+  }
+}
+
+// ---------------------------------------------------------------------------
+// hpsi: stencil (hpsi1_RT_stencil, ARTED/stencil/F90/hpsi.f90:88-190) + local
+// potential + KB apply (pseudo_pt phase 2, hpsi.f90:195-206, gathered per
+// touched point through the transposed table as the OpenACC build does,
+// common/acc/hpsi.f90:95-113).  One thread per grid point.
+//   F = (V + A) E - 1/2 sum C_n (E[+n] + E[-n]) - i sum D_n (E[+n] - E[-n])
+// with A = (|k|^2 - sum lap0)/2, C = lap/H^2, D = k.nab/H.
+// tt (nullable) gets F - V E before the KB term (hpsi.f90:131-133).
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(256) k_hpsi(const cplx<T>* __restrict__ in, cplx<T>* __restrict__ out,
+                                              cplx<T>* __restrict__ tt, int orb0, int NB, int NK,
+                                              const double* __restrict__ kac0, double ax, double ay, double az,
+                                              const T* __restrict__ vloc, Grid g, KBTables kb,
+                                              const cplx<T>* __restrict__ uvpsi) {
+  const int p = blockIdx.x * 256 + threadIdx.x;
+  const int o = blockIdx.y;
+  if (p >= g.NL) return;
+  const int ikb = orb0 + o;
+  double kx, ky, kz;
+  orbital_k(kac0, NK, ikb / NB, ax, ay, az, kx, ky, kz);
+  const T A = (T)((kx * kx + ky * ky + kz * kz - (g.lap[0][0] + g.lap[1][0] + g.lap[2][0])) * 0.5);
+  const int NLx = g.NLx, NLy = g.NLy, NLz = g.NLz;
+  const int iz = p % NLz, iy = (p / NLz) % NLy, ix = p / (NLz * NLy);
+  const cplx<T>* E = in + (size_t)o * g.NL;
+  const double kk[3] = {kx, ky, kz};
+  T vr = 0, vi = 0, wr = 0, wi = 0;
+  // z, y, x in the reference's order
+#pragma unroll
+  for (int ax3 = 2; ax3 >= 0; --ax3) {
+    const int n_ax = ax3 == 2 ? NLz : (ax3 == 1 ? NLy : NLx);
+    const int c_ax = ax3 == 2 ? iz : (ax3 == 1 ? iy : ix);
+    const int stride = ax3 == 2 ? 1 : (ax3 == 1 ? NLz : NLz * NLy);
+    const int base = p - c_ax * stride;
+#pragma unroll
+    for (int n = 1; n <= 4; ++n) {
+      int cp = c_ax + n;
+      if (cp >= n_ax) cp -= n_ax;
+      int cm = c_ax - n;
+      if (cm < 0) cm += n_ax;
+      const cplx<T> ep = E[base + cp * stride], em = E[base + cm * stride];
+      const T C = (T)g.lap[ax3][n], D = (T)(kk[ax3] * g.nab[ax3][n]);
+      vr += C * (ep.re + em.re);
+      vi += C * (ep.im + em.im);
+      wr += D * (ep.re - em.re);
+      wi += D * (ep.im - em.im);
+    }
+  }
+  const cplx<T> e = E[p];
+  const T V = vloc[p];
+  cplx<T> f;
+  f.re = (V + A) * e.re - (T)0.5 * vr + wi;
+  f.im = (V + A) * e.im - (T)0.5 * vi - wr;
+  if (tt) tt[(size_t)o * g.NL + p] = cplx<T>{f.re - V * e.re, f.im - V * e.im};
+  const int t = kb.pt2t[p];
+  if (t >= 0) {
+    const cplx<T>* U = uvpsi + (size_t)o * kb.Nlma;
+    for (int q = kb.t_start[t]; q < kb.t_start[t + 1]; ++q) {
+      const int il = kb.e_lma[q], jp = kb.e_jp[q];
+      const int a = kb.lma_atom[il];
+      const double kr = kx * kb.pos[3 * jp] + ky * kb.pos[3 * jp + 1] + kz * kb.pos[3 * jp + 2];
+      T s, c;
+      sincos_t((T)kr, &s, &c);
+      const T u = (T)kb.uv[(size_t)il * kb.Nps + (jp - kb.atom_start[a])];
+      // zproj * uVpsi = u (c - i s) (U.re + i U.im)
+      const cplx<T> w = U[il];
+      f.re += u * (c * w.re + s * w.im);
+      f.im += u * (c * w.im - s * w.re);
+    }
+  }
+  out[(size_t)o * g.NL + p] = f;
+}
+
+// zu += z1 h1 + z2 h2 + z3 h3 + z4 h4 (update, ARTED/RT/hamiltonian.f90:99-122)
+template <typename T>
+__global__ void __launch_bounds__(256) k_taylor_fold(cplx<T>* __restrict__ zu, const cplx<T>* __restrict__ h1,
+                                                     const cplx<T>* __restrict__ h2, const cplx<T>* __restrict__ h3,
+                                                     const cplx<T>* __restrict__ h4, size_t n, T z1i, T z2r, T z3i,
+                                                     T z4r) {
+  // zfac = (-i dt)^n/n!: z1 = (0, z1i), z2 = (z2r, 0), z3 = (0, z3i), z4 = (z4r, 0)
+  const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  cplx<T> z = zu[i];
+  const cplx<T> a = h1[i], b = h2[i], c = h3[i], d = h4[i];
+  z.re += -z1i * a.im;
+  z.im += z1i * a.re;
+  z.re += z2r * b.re;
+  z.im += z2r * b.im;
+  z.re += -z3i * c.im;
+  z.im += z3i * c.re;
+  z.re += z4r * d.re;
+  z.im += z4r * d.im;
+  zu[i] = z;
+}
+
+// ---------------------------------------------------------------------------
+// Observables of one orbital, all in FP64 (one 256-thread block per orbital):
+//  current (RT/current.f90:172-273 + stencil/F90/current.f90:82-114):
+//    J = (occ k sum|psi|^2 + occ 2 sum Im(conj psi sum_n nab_n psi[+n])) Hxyz/Omega
+//        + occ/Omega 2 Im(conj(<x p|psi>) <p|psi> iuV) Hxyz^2
+//  kinetic energy (total_energy.f90:183 + stencil/F90/total_energy.f90:93-180):
+//    Ekin = occ Re sum conj(psi)(A0 psi - 1/2 sum C(psi+ + psi-) - i sum D(psi+ - psi-)) Hxyz
+//           + occ |k|^2/2
+//  nonlocal energy (total_energy.f90:191-199): Enl = occ sum iuV |Hxyz <p|psi>|^2
+//  local energy of the frozen potential and the norm.
+// out(ikb, 0..7) = Jx, Jy, Jz, Ekin, Enl, Eloc, Norm, 0
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(256) k_observe(const cplx<T>* __restrict__ zu, int orb0, int NB, int NK,
+                                                 const double* __restrict__ kac0, double ax, double ay, double az,
+                                                 const double* __restrict__ occv, const double* __restrict__ vloc,
+                                                 Grid g, KBTables kb, double* __restrict__ out) {
+  __shared__ double smem[4 * 8];
+  __shared__ double nl[4][4];
+  const int o = blockIdx.x;
+  const int ikb = orb0 + o;
+  const int ik = ikb / NB;
+  double kx, ky, kz;
+  orbital_k(kac0, NK, ik, ax, ay, az, kx, ky, kz);
+  const double occ = occv[ikb];
+  const double kk[3] = {kx, ky, kz};
+  const double A0 = -(g.lap[0][0] + g.lap[1][0] + g.lap[2][0]) * 0.5;
+  const cplx<T>* E = zu + (size_t)o * g.NL;
+  const int NLx = g.NLx, NLy = g.NLy, NLz = g.NLz;
+  double n2 = 0, vs = 0, te = 0, jc[3] = {0, 0, 0};
+  for (int p = threadIdx.x; p < g.NL; p += 256) {
+    const int iz = p % NLz, iy = (p / NLz) % NLy, ix = p / (NLz * NLy);
+    const double er = (double)E[p].re, ei = (double)E[p].im;
+    const double a2 = er * er + ei * ei;
+    n2 += a2;
+    vs += vloc[p] * a2;
+    double vr = 0, vi = 0, wr = 0, wi = 0;
+#pragma unroll
+    for (int ax3 = 2; ax3 >= 0; --ax3) {
+      const int n_ax = ax3 == 2 ? NLz : (ax3 == 1 ? NLy : NLx);
+      const int c_ax = ax3 == 2 ? iz : (ax3 == 1 ? iy : ix);
+      const int stride = ax3 == 2 ? 1 : (ax3 == 1 ? NLz : NLz * NLy);
+      const int base = p - c_ax * stride;
+      double fr = 0, fi = 0;
+#pragma unroll
+      for (int n = 1; n <= 4; ++n) {
+        int cp = c_ax + n;
+        if (cp >= n_ax) cp -= n_ax;
+        int cm = c_ax - n;
+        if (cm < 0) cm += n_ax;
+        const double pr = (double)E[base + cp * stride].re, pi = (double)E[base + cp * stride].im;
+        const double mr = (double)E[base + cm * stride].re, mi = (double)E[base + cm * stride].im;
+        const double nb = g.nab[ax3][n], C = g.lap[ax3][n], D = kk[ax3] * nb;
+        fr += nb * pr;
+        fi += nb * pi;
+        vr += C * (pr + mr);
+        vi += C * (pi + mi);
+        wr += D * (pr - mr);
+        wi += D * (pi - mi);
+      }
+      jc[ax3] += er * fi - ei * fr;  // Im(conj(psi) * f)
+    }
+    // z = A0 psi - 0.5 v - i w ; Re(conj(psi) z)
+    const double zr = A0 * er - 0.5 * vr + wi, zi = A0 * ei - 0.5 * vi - wr;
+    te += er * zr + ei * zi;
+  }
+  double v[6] = {n2, vs, te, jc[0], jc[1], jc[2]};
+  block_sum256<6>(v, smem);
+  // nonlocal: wave w takes projectors w, w+4, ...
+  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+  double jn[3] = {0, 0, 0}, en = 0;
+  for (int il = w; il < kb.Nlma; il += 4) {
+    const int a = kb.lma_atom[il];
+    const int j0 = kb.atom_start[a], m = kb.atom_start[a + 1] - j0;
+    double ur = 0, ui = 0, xr[3] = {0, 0, 0}, xi[3] = {0, 0, 0};
+    for (int j = lane; j < m; j += kWave) {
+      const int jp = j0 + j;
+      const double px = kb.pos[3 * jp], py = kb.pos[3 * jp + 1], pz = kb.pos[3 * jp + 2];
+      double s, c;
+      sincos(kx * px + ky * py + kz * pz, &s, &c);
+      const double u = kb.uv[(size_t)il * kb.Nps + j];
+      const cplx<T> e = E[kb.pidx[jp]];
+      const double tr = u * (c * (double)e.re - s * (double)e.im);
+      const double ti = u * (c * (double)e.im + s * (double)e.re);
+      ur += tr;
+      ui += ti;
+      xr[0] += px * tr; xi[0] += px * ti;
+      xr[1] += py * tr; xi[1] += py * ti;
+      xr[2] += pz * tr; xi[2] += pz * ti;
+    }
+    ur = wave_sum(ur); ui = wave_sum(ui);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) { xr[d] = wave_sum(xr[d]); xi[d] = wave_sum(xi[d]); }
+    const double sg = kb.lma_sign[il];
+    const double Ur = ur * g.Hxyz * sg, Ui = ui * g.Hxyz * sg;  // uVpsi (with iuV)
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const double Xr = xr[d] * g.Hxyz, Xi = xi[d] * g.Hxyz;
+      jn[d] += occ / g.aLxyz * 2.0 * (Xr * Ui - Xi * Ur);  // Im(conj(X) U)
+    }
+    const double hr = ur * g.Hxyz, hi = ui * g.Hxyz;
+    en += occ * (hr * hr + hi * hi) * sg;
+  }
+  if (lane == 0) { nl[w][0] = jn[0]; nl[w][1] = jn[1]; nl[w][2] = jn[2]; nl[w][3] = en; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double J[3], Enl = 0;
+    for (int d = 0; d < 3; ++d) J[d] = (nl[0][d] + nl[1][d]) + (nl[2][d] + nl[3][d]);
+    Enl = (nl[0][3] + nl[1][3]) + (nl[2][3] + nl[3][3]);
+    double* r = out + (size_t)ikb * 8;
+    const double s = g.Hxyz / g.aLxyz;
+    for (int d = 0; d < 3; ++d) r[d] = (occ * kk[d] * v[0] + 2.0 * v[3 + d] * occ) * s + J[d];
+    r[3] = occ * v[2] * g.Hxyz + occ * (kx * kx + ky * ky + kz * kz) / 2.0;
+    r[4] = Enl;
+    r[5] = occ * v[1] * g.Hxyz;
+    r[6] = occ * v[0] * g.Hxyz;
+    r[7] = 0.0;
+  }
+}
+
+// Deterministic sum of per-orbital rows (NKB, 8) -> dst[0..7]; one block.
+__global__ void __launch_bounds__(256) k_sum_rows(const double* __restrict__ rows, int nrows,
+                                                  double* __restrict__ dst) {
+  __shared__ double smem[4 * 8];
+  double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (int r = threadIdx.x; r < nrows; r += 256) {
+#pragma unroll
+    for (int i = 0; i < 8; ++i) v[i] += rows[(size_t)r * 8 + i];
+  }
+  block_sum256<8>(v, smem);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int i = 0; i < 8; ++i) dst[i] = v[i];
+  }
+}
+
+// Density partials: part(c, p) = sum over orbitals of chunk c of occ |psi(p)|^2
+template <typename T>
+__global__ void __launch_bounds__(256) k_density_part(const cplx<T>* __restrict__ zu, int NKB, int NL, int per_chunk,
+                                                      const double* __restrict__ occv, double* __restrict__ part) {
+  const int p = blockIdx.x * 256 + threadIdx.x;
+  const int c = blockIdx.y;
+  if (p >= NL) return;
+  const int o0 = c * per_chunk, o1 = min(NKB, o0 + per_chunk);
+  double r = 0;
+  for (int o = o0; o < o1; ++o) {
+    const cplx<T> e = zu[(size_t)o * NL + p];
+    const double er = e.re, ei = e.im;
+    r += occv[o] * (er * er + ei * ei);
+  }
+  part[(size_t)c * NL + p] = r;
+}
+
+__global__ void __launch_bounds__(256) k_density_final(const double* __restrict__ part, int nchunk, int NL,
+                                                       double* __restrict__ rho) {
+  const int p = blockIdx.x * 256 + threadIdx.x;
+  if (p >= NL) return;
+  double r = 0;
+  for (int c = 0; c < nchunk; ++c) r += part[(size_t)c * NL + p];
+  rho[p] = r;
+}
+
+// FP64 Gram-Schmidt of NB bands of each k (one block per k), modified form of
+// Gram_Schmidt_ompk (ARTED/GS/Gram_Schmidt.f90:41-51).
+__global__ void __launch_bounds__(256) k_gram_schmidt(cplx<double>* __restrict__ zu, int NB, int NL, double Hxyz) {
+  __shared__ double smem[4 * 2];
+  __shared__ double bc[2];
+  cplx<double>* b0 = zu + (size_t)blockIdx.x * NB * NL;
+  for (int ib = 0; ib < NB; ++ib) {
+    cplx<double>* x = b0 + (size_t)ib * NL;
+    for (int ibt = 0; ibt < ib; ++ibt) {
+      const cplx<double>* y = b0 + (size_t)ibt * NL;
+      double v[2] = {0, 0};
+      for (int p = threadIdx.x; p < NL; p += 256) {
+        v[0] += y[p].re * x[p].re + y[p].im * x[p].im;
+        v[1] += y[p].re * x[p].im - y[p].im * x[p].re;
+      }
+      block_sum256<2>(v, smem);
+      if (threadIdx.x == 0) { bc[0] = v[0] * Hxyz; bc[1] = v[1] * Hxyz; }
+      __syncthreads();
+      const double zr = bc[0], zi = bc[1];
+      for (int p = threadIdx.x; p < NL; p += 256) {
+        const cplx<double> yy = y[p];
+        x[p].re -= yy.re * zr - yy.im * zi;
+        x[p].im -= yy.re * zi + yy.im * zr;
+      }
+      __syncthreads();
+    }
+    double v[1] = {0};
+    for (int p = threadIdx.x; p < NL; p += 256) v[0] += x[p].re * x[p].re + x[p].im * x[p].im;
+    block_sum256<1>(v, smem);
+    if (threadIdx.x == 0) bc[0] = sqrt(v[0] * Hxyz);
+    __syncthreads();
+    const double r = bc[0];
+    for (int p = threadIdx.x; p < NL; p += 256) { x[p].re /= r; x[p].im /= r; }
+    __syncthreads();
+  }
+}
+
+// <x|y> Hxyz over rows of (n, NL) pairs: dst(o) = sum conj(x) y * Hxyz (complex)
+template <typename T>
+__global__ void __launch_bounds__(256) k_dot_rows(const cplx<T>* __restrict__ x, const cplx<T>* __restrict__ y,
+                                                  int NL, double Hxyz, double* __restrict__ dst) {
+  __shared__ double smem[4 * 2];
+  const cplx<T>* a = x + (size_t)blockIdx.x * NL;
+  const cplx<T>* b = y + (size_t)blockIdx.x * NL;
+  double v[2] = {0, 0};
+  for (int p = threadIdx.x; p < NL; p += 256) {
+    const double ar = a[p].re, ai = a[p].im, br = b[p].re, bi = b[p].im;
+    v[0] += ar * br + ai * bi;
+    v[1] += ar * bi - ai * br;
+  }
+  block_sum256<2>(v, smem);
+  if (threadIdx.x == 0) { dst[2 * blockIdx.x] = v[0] * Hxyz; dst[2 * blockIdx.x + 1] = v[1] * Hxyz; }
+}
+
+template <typename S, typename D>
+__global__ void __launch_bounds__(256) k_convert(const cplx<S>* __restrict__ src, cplx<D>* __restrict__ dst, size_t n) {
+  const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) dst[i] = cplx<D>{(D)src[i].re, (D)src[i].im};
+}
+
+}  // namespace salmon

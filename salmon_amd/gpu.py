@@ -1,0 +1,223 @@
+"""Python host binding of the gfx950 ARTED hot path (ctypes over the C-ABI
+declared in include/salmon_gpu.h).
+
+The class mirrors the reference's call interface for this path: the module
+state of ``Global_Variables`` becomes a ``Context`` and the routines keep their
+names and argument meaning:
+
+=====================  ==============================================  ===========================
+method                 reference routine                               file:line
+=====================  ==============================================  ===========================
+``hpsi``               hpsi_omp_KB_GS / hpsi_omp_KB_RT                 ARTED/common/hpsi.f90:33,89
+``hamiltonian``        hamiltonian(zu, flag_current)                   ARTED/RT/hamiltonian.f90:27
+``current``            current_RT -> current('RT')                     ARTED/RT/current.f90:65,78
+``psi_rho_RT``         psi_rho_RT(zu)                                  ARTED/common/psi_rho.f90:40
+``total_energy``       Total_Energy_omp (Ekin, Enl) + frozen-V Eloc    ARTED/common/total_energy.f90:17
+``gram_schmidt``       Gram_Schmidt                                    ARTED/GS/Gram_Schmidt.f90:20
+``sp_energy``          sp_energy_omp                                   ARTED/GS/sp_energy.f90:20
+``rt_run``             the tddft_sc time loop (middlepoint, frozen V)  ARTED/control/control_sc.f90:150
+=====================  ==============================================  ===========================
+
+The reference aborts through ``err_finalize`` on any error; here every failed
+call raises ``SalmonGPUError`` with the library's message.  There is no CPU
+fallback: without the built HIP library the import of ``lib()`` fails.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libsalmon_gpu.so")
+
+FP64 = 0
+MIXED = 1
+
+
+class SalmonGPUError(RuntimeError):
+    pass
+
+
+class GridT(ctypes.Structure):
+    _fields_ = [("NLx", ctypes.c_int), ("NLy", ctypes.c_int), ("NLz", ctypes.c_int),
+                ("aLx", ctypes.c_double), ("aLy", ctypes.c_double), ("aLz", ctypes.c_double),
+                ("lapx", ctypes.c_double * 5), ("lapy", ctypes.c_double * 5), ("lapz", ctypes.c_double * 5),
+                ("nabx", ctypes.c_double * 5), ("naby", ctypes.c_double * 5), ("nabz", ctypes.c_double * 5)]
+
+
+# entry points declared in include/salmon_gpu.h (name -> argtypes)
+_vp, _ci, _cd = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
+SIGNATURES = {
+    "salmon_gpu_init": [ctypes.POINTER(GridT), _ci, ctypes.POINTER(_vp)],
+    "salmon_gpu_set_potential": [_vp, _vp],
+    "salmon_gpu_set_projectors": [_vp, _ci, _ci, _vp, _vp, _vp, _vp, _vp, _ci, _vp, _vp, _vp],
+    "salmon_gpu_set_kpoints": [_vp, _ci, _ci, _vp, _vp],
+    "salmon_gpu_put_orbitals": [_vp, _vp],
+    "salmon_gpu_get_orbitals": [_vp, _vp],
+    "salmon_gpu_hpsi": [_vp, _vp, _vp, _vp, _vp],
+    "salmon_gpu_hamiltonian": [_vp, _vp, _cd],
+    "salmon_gpu_current": [_vp, _vp, _vp],
+    "salmon_gpu_density": [_vp, _vp],
+    "salmon_gpu_energy": [_vp, _vp, _vp, _vp, _vp],
+    "salmon_gpu_norm": [_vp, _vp],
+    "salmon_gpu_gram_schmidt": [_vp],
+    "salmon_gpu_sp_energy": [_vp, _vp, _vp],
+    "salmon_gpu_rt_run": [_vp, _vp, _ci, _cd, _ci, _vp, _vp, _vp],
+    "salmon_gpu_synchronize": [_vp],
+    "salmon_gpu_errstr": [_ci],
+    "salmon_gpu_last_error": [_vp],
+    "salmon_gpu_finalize": [_vp],
+}
+
+_lib = None
+
+
+def lib():
+    """Load the HIP library; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise SalmonGPUError(f"HIP library not built: {LIB_PATH} (run __graft_entry__.build())")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, argt in SIGNATURES.items():
+            f = getattr(L, name)
+            f.argtypes = argt
+            f.restype = ctypes.c_char_p if name in ("salmon_gpu_errstr", "salmon_gpu_last_error") else ctypes.c_int
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+class Context:
+    """Device-resident state of one periodic cell (the reference's module globals)."""
+
+    def __init__(self, sys, precision=FP64, orbitals=True):
+        self.sys = sys
+        self.precision = precision
+        self._h = ctypes.c_void_p()
+        g = GridT()
+        g.NLx, g.NLy, g.NLz = sys.NLx, sys.NLy, sys.NLz
+        g.aLx, g.aLy, g.aLz = sys.aLx, sys.aLy, sys.aLz
+        for name in ("lapx", "lapy", "lapz", "nabx", "naby", "nabz"):
+            setattr(g, name, (ctypes.c_double * 5)(*[float(v) for v in getattr(sys, name)]))
+        rc = lib().salmon_gpu_init(ctypes.byref(g), int(precision), ctypes.byref(self._h))
+        if rc != 0:
+            raise SalmonGPUError(f"salmon_gpu_init: {lib().salmon_gpu_errstr(rc).decode()}")
+        self._check(lib().salmon_gpu_set_potential(self._h, _p(_f64(sys.vloc))), "set_potential")
+        keep = [_i32(sys.Mps), _i32(sys.Jxyz), _i32(sys.Jxx), _i32(sys.Jyy), _i32(sys.Jzz), _i32(sys.a_tbl),
+                _f64(sys.uV), _i32(sys.iuV)]
+        self._check(lib().salmon_gpu_set_projectors(self._h, sys.NI, sys.Nps, _p(keep[0]), _p(keep[1]), _p(keep[2]),
+                                                    _p(keep[3]), _p(keep[4]), sys.Nlma, _p(keep[5]), _p(keep[6]),
+                                                    _p(keep[7])), "set_projectors")
+        self._check(lib().salmon_gpu_set_kpoints(self._h, sys.NK, sys.NB, _p(_f64(sys.kAc0)), _p(_f64(sys.occ))),
+                    "set_kpoints")
+        if orbitals:
+            self.put_orbitals(sys.zu)
+
+    def _check(self, rc, what):
+        if rc != 0:
+            msg = lib().salmon_gpu_last_error(self._h).decode() if self._h else ""
+            raise SalmonGPUError(f"{what}: {lib().salmon_gpu_errstr(rc).decode()}: {msg}")
+
+    def close(self):
+        if self._h:
+            lib().salmon_gpu_finalize(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # --- data movement --------------------------------------------------
+    def put_orbitals(self, zu):
+        s = self.sys
+        zu = np.ascontiguousarray(zu, dtype=np.complex128)
+        if zu.shape != (s.NK, s.NB, s.NL):
+            raise SalmonGPUError(f"zu must have shape (NK, NB, NL) = {(s.NK, s.NB, s.NL)}")
+        self._check(lib().salmon_gpu_put_orbitals(self._h, _p(zu)), "put_orbitals")
+
+    def get_orbitals(self):
+        s = self.sys
+        zu = np.zeros((s.NK, s.NB, s.NL), np.complex128)
+        self._check(lib().salmon_gpu_get_orbitals(self._h, _p(zu)), "get_orbitals")
+        return zu
+
+    # --- operators ------------------------------------------------------
+    def hpsi(self, kvec, psi, with_tt=False):
+        psi = np.ascontiguousarray(psi, dtype=np.complex128)
+        out = np.zeros_like(psi)
+        tt = np.zeros_like(psi) if with_tt else None
+        self._check(lib().salmon_gpu_hpsi(self._h, _p(_f64(kvec)), _p(psi), _p(out), _p(tt) if with_tt else None),
+                    "hpsi")
+        return (out, tt) if with_tt else out
+
+    hpsi_omp_KB_RT = hpsi
+
+    def hamiltonian(self, kAc, dt):
+        self._check(lib().salmon_gpu_hamiltonian(self._h, _p(_f64(kAc)), float(dt)), "hamiltonian")
+
+    def current(self, kAc):
+        j = np.zeros(3)
+        self._check(lib().salmon_gpu_current(self._h, _p(_f64(kAc)), _p(j)), "current")
+        return j
+
+    current_RT = current
+
+    def psi_rho_RT(self):
+        rho = np.zeros(self.sys.NL)
+        self._check(lib().salmon_gpu_density(self._h, _p(rho)), "psi_rho_RT")
+        return rho
+
+    def total_energy(self, kAc):
+        ek, en, el = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        self._check(lib().salmon_gpu_energy(self._h, _p(_f64(kAc)), ctypes.byref(ek), ctypes.byref(en),
+                                            ctypes.byref(el)), "total_energy")
+        return ek.value, en.value, el.value
+
+    def norm(self):
+        n = ctypes.c_double()
+        self._check(lib().salmon_gpu_norm(self._h, ctypes.byref(n)), "norm")
+        return n.value
+
+    def gram_schmidt(self):
+        self._check(lib().salmon_gpu_gram_schmidt(self._h), "gram_schmidt")
+
+    def sp_energy(self, kAc):
+        esp = np.zeros((self.sys.NK, self.sys.NB))
+        self._check(lib().salmon_gpu_sp_energy(self._h, _p(_f64(kAc)), _p(esp)), "sp_energy")
+        return esp
+
+    def rt_run(self, ac, nsteps, dt=None, reortho_every=0):
+        ac = _f64(np.asarray(ac)[: nsteps + 1])
+        if ac.shape != (nsteps + 1, 3):
+            raise SalmonGPUError("ac must hold nsteps+1 rows of 3")
+        jt = np.zeros((nsteps, 3))
+        et = np.zeros(nsteps)
+        nt = np.zeros(nsteps)
+        self._check(lib().salmon_gpu_rt_run(self._h, _p(ac), int(nsteps), float(self.sys.dt if dt is None else dt),
+                                            int(reortho_every), _p(jt), _p(et), _p(nt)), "rt_run")
+        return dict(J=jt, E=et, norm=nt)
+
+    def synchronize(self):
+        self._check(lib().salmon_gpu_synchronize(self._h), "synchronize")
