@@ -1,0 +1,194 @@
+"""Throughput of ARTED's real-time step on one MI355X.
+
+Workload: the 16^3 cell (L = 10.26 bohr) with 32 bands at each of 512 k points
+(Monkhorst-Pack 8^3), frozen synthetic potential with 8 atoms x 4 KB
+projectors, Acos2 pulse (A0 = 0.05, w = 0.057 Ha, 20 fs), dt = 0.08, Taylor
+order 4.  One step = Taylor-4 propagation of every (k, band) orbital at the
+midpoint k+A, the density, and the current and energy at k+A(t+dt).
+
+Metric: Taylor steps * k * band per second (whole job), for the mixed path
+(complex64 stencil/KB/Taylor, FP64 sums) with the FP64 path measured on the
+same orbitals; the J(t) relative error max|J32 - J64| / max|J64| over the run
+is reported beside it.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import os
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+
+
+class HipEvents:
+    """hipEvent timing on a given stream (libamdhip64 through ctypes)."""
+
+    def __init__(self):
+        self.L = ctypes.CDLL("libamdhip64.so")
+        self.L.hipEventCreate.argtypes = [ctypes.POINTER(ctypes.c_void_p)]
+        self.L.hipEventRecord.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        self.L.hipEventSynchronize.argtypes = [ctypes.c_void_p]
+        self.L.hipEventElapsedTime.argtypes = [ctypes.POINTER(ctypes.c_float), ctypes.c_void_p, ctypes.c_void_p]
+        self.L.hipEventDestroy.argtypes = [ctypes.c_void_p]
+
+    def event(self):
+        e = ctypes.c_void_p()
+        assert self.L.hipEventCreate(ctypes.byref(e)) == 0
+        return e
+
+    def record(self, e, stream):
+        assert self.L.hipEventRecord(e, ctypes.c_void_p(stream)) == 0
+
+    def elapsed_ms(self, a, b):
+        assert self.L.hipEventSynchronize(b) == 0
+        ms = ctypes.c_float()
+        assert self.L.hipEventElapsedTime(ctypes.byref(ms), a, b) == 0
+        return float(ms.value)
+
+
+def workload(seed=0, nsteps=2000):
+    from salmon_amd import synth
+    s, ac = synth.config2(seed=seed, nsteps=nsteps, orbitals=False)
+    rng = np.random.default_rng(seed + 1)
+    s.zu = rng.standard_normal((s.NK, s.NB, s.NL)) + 1j * rng.standard_normal((s.NK, s.NB, s.NL))
+    return s, ac
+
+
+def run_gpu(s, ac, prec, warmup, steps, ev):
+    from salmon_amd import gpu
+    with gpu.Context(s, prec) as c:
+        c.rt_run(ac[: warmup + 1], warmup)
+        c.synchronize()
+        st = c.stream()
+        e0, e1 = ev.event(), ev.event()
+        ev.record(e0, st)
+        t0 = time.perf_counter()
+        r = c.rt_run(ac[warmup: warmup + steps + 1], steps)
+        ev.record(e1, st)
+        c.synchronize()
+        t1 = time.perf_counter()
+        ms = ev.elapsed_ms(e0, e1)
+    return dict(ms=ms, wall=t1 - t0, J=r["J"], E=r["E"], norm=r["norm"])
+
+
+CPU_SAMPLE_SCRIPT = r"""
+import sys, time, json, os
+sys.path.insert(0, sys.argv[1])
+import numpy as np
+from salmon_amd import synth
+nk, nsteps, kind = int(sys.argv[2]), int(sys.argv[3]), sys.argv[4]
+s, ac = synth.config2(nsteps=nsteps + 1, orbitals=False)
+s.kAc0 = np.ascontiguousarray(s.kAc0[:, :nk]); s.occ = np.ascontiguousarray(s.occ[:nk])
+s.zu = synth.random_orbitals(np.random.default_rng(7), nk, s.NB, s.NL, s.Hxyz)
+if kind == "reference":
+    from oracle.ref import Ref
+    R = Ref(s)
+    threads = R.num_threads()
+    R.rt_run(ac[:2], 1)
+    t0 = time.perf_counter(); R.rt_run(ac[1:nsteps + 2], nsteps); t = time.perf_counter() - t0
+else:
+    from oracle.cpu import Oracle, lib
+    O = Oracle(s)
+    threads = lib().or_num_threads()
+    O.rt_run(ac[:2], 1)
+    t0 = time.perf_counter(); O.rt_run(ac[1:nsteps + 2], nsteps); t = time.perf_counter() - t0
+print(json.dumps(dict(seconds=t, threads=threads, nkb=nk * s.NB, steps=nsteps)))
+"""
+
+
+def cpu_baseline(nk=64, nsteps=4):
+    """The reference's Fortran (oracle/_ref) on a bounded sample of the same
+    workload, on the host cores, in a subprocess (its module globals hold one
+    system per process)."""
+    from oracle import ref
+    kind = "reference" if ref.available() else "port"
+    env = dict(os.environ)
+    env.setdefault("OMP_NUM_THREADS", str(min(os.cpu_count() or 1, 16)))
+    env.setdefault("OMP_PROC_BIND", "close")
+    env.setdefault("OMP_PLACES", "cores")
+    out = subprocess.run([sys.executable, "-c", CPU_SAMPLE_SCRIPT, ROOT, str(nk), str(nsteps), kind],
+                         capture_output=True, text=True, env=env, timeout=600)
+    if out.returncode != 0:
+        raise RuntimeError("cpu baseline failed: " + out.stderr[-2000:])
+    d = json.loads(out.stdout.strip().splitlines()[-1])
+    rate = d["nkb"] * d["steps"] / d["seconds"]
+    return dict(value=rate, unit="k*band*steps/s", cores=int(d["threads"]), kind=kind,
+                sample=f"{nk} of 512 k points x 32 bands on the 16^3 cell, {nsteps} steps after 1 warm-up "
+                       f"(Taylor-4 + density + current + energy per step), {d['seconds']:.2f} s")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-steps", type=int, default=20)
+    ap.add_argument("--cpu-nk", type=int, default=128)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: the k-point sharding over several GPUs is not built "
+              "(every GPU box here has one GPU); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    from salmon_amd import gpu
+    ev = HipEvents()
+    s, ac = workload(nsteps=args.warmup + args.steps + 1)
+    # orthonormalise once in FP64 on the GPU; both precisions start from it
+    with gpu.Context(s, gpu.FP64) as c:
+        c.gram_schmidt()
+        s.zu = c.get_orbitals()
+    nkb, steps = s.NK * s.NB, args.steps
+    r64 = run_gpu(s, ac, gpu.FP64, args.warmup, steps, ev)
+    r32 = run_gpu(s, ac, gpu.MIXED, args.warmup, steps, ev)
+    jerr = float(np.max(np.abs(r32["J"] - r64["J"])) / np.max(np.abs(r64["J"])))
+    eerr = float(np.max(np.abs(r32["E"] - r64["E"])) / np.max(np.abs(r64["E"])))
+    rate = lambda r: nkb * steps / (r["ms"] * 1e-3)
+    v32, v64 = rate(r32), rate(r64)
+    ms32, ms64 = r32["ms"] / steps, r64["ms"] / steps
+    # algorithmic bytes per orbital-step: 12*S*NL (11 S NL for four streamed
+    # H applications + 1 S NL for one observables pass); one step covers all
+    # NKB orbitals
+    B32 = 12 * 8 * s.NL * nkb
+    B64 = 12 * 16 * s.NL * nkb
+    ach = B32 / (ms32 * 1e-3) / 1e9
+    roof = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
+            "traffic": None,
+            "kernel": "whole RT step, mixed path (per-step average over the timed region)",
+            "fp64": {"achieved": B64 / (ms64 * 1e-3) / 1e9, "frac": B64 / (ms64 * 1e-3) / 1e9 / HBM_PEAK_GBS}}
+    cpu = None
+    if not args.no_cpu:
+        try:
+            cpu = cpu_baseline(args.cpu_nk, args.cpu_steps)
+        except Exception as e:  # reported, not fatal
+            cpu = {"error": str(e)[:300]}
+    line = {
+        "metric": "Taylor steps*k*band/s (16^3, 512k x 32) FP64 vs mixed @ J(t) rel err<=1e-5",
+        "value": v32, "unit": "k*band*steps/s", "n_gpus": 1, "steps": steps, "warmup": args.warmup,
+        "ms_per_step": ms32, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "c64 (FP64 sums)", "data": "synthetic",
+        "config": {"workload": "16^3 cell, 512 k x 32 bands (NKB=16384), 8 atoms x 4 KB projectors, Acos2 pulse, "
+                               "dt=0.08, Taylor-4, J+E+rho every step",
+                   "NL": s.NL, "NK": s.NK, "NB": s.NB, "precision": "mixed"},
+        "fp64": {"value": v64, "ms_per_step": ms64},
+        "mixed_vs_fp64_speedup": v32 / v64,
+        "J_rel_err": jerr, "E_rel_err": eerr,
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -129,6 +129,11 @@ int salmon_gpu_rt_run(salmon_ctx *ctx, const double *ac, int nsteps, double dt, 
 /* Waits for all work of the context. */
 int salmon_gpu_synchronize(salmon_ctx *ctx);
 
+/* The context's HIP stream (a hipStream_t), for callers that order their own
+ * work or timing events with the library's.  No reference counterpart (the
+ * reference's OpenACC build uses the default queue). */
+void *salmon_gpu_stream(salmon_ctx *ctx);
+
 const char *salmon_gpu_errstr(int code);
 const char *salmon_gpu_last_error(const salmon_ctx *ctx);
 int salmon_gpu_finalize(salmon_ctx *ctx);

@@ -687,6 +687,8 @@ int salmon_gpu_synchronize(salmon_ctx* c) {
   return SALMON_OK;
 }
 
+void* salmon_gpu_stream(salmon_ctx* c) { return c ? (void*)c->stream : nullptr; }
+
 int salmon_gpu_finalize(salmon_ctx* c) {
   if (!c) return SALMON_ERR_ARG;
   if (c->stream) (void)hipStreamSynchronize(c->stream);

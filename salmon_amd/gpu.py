@@ -66,6 +66,7 @@ SIGNATURES = {
     "salmon_gpu_sp_energy": [_vp, _vp, _vp],
     "salmon_gpu_rt_run": [_vp, _vp, _ci, _cd, _ci, _vp, _vp, _vp],
     "salmon_gpu_synchronize": [_vp],
+    "salmon_gpu_stream": [_vp],
     "salmon_gpu_errstr": [_ci],
     "salmon_gpu_last_error": [_vp],
     "salmon_gpu_finalize": [_vp],
@@ -84,7 +85,12 @@ def lib():
         for name, argt in SIGNATURES.items():
             f = getattr(L, name)
             f.argtypes = argt
-            f.restype = ctypes.c_char_p if name in ("salmon_gpu_errstr", "salmon_gpu_last_error") else ctypes.c_int
+            if name in ("salmon_gpu_errstr", "salmon_gpu_last_error"):
+                f.restype = ctypes.c_char_p
+            elif name == "salmon_gpu_stream":
+                f.restype = ctypes.c_void_p
+            else:
+                f.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -218,6 +224,10 @@ class Context:
         self._check(lib().salmon_gpu_rt_run(self._h, _p(ac), int(nsteps), float(self.sys.dt if dt is None else dt),
                                             int(reortho_every), _p(jt), _p(et), _p(nt)), "rt_run")
         return dict(J=jt, E=et, norm=nt)
+
+    def stream(self):
+        """The context's hipStream_t (as an integer handle)."""
+        return lib().salmon_gpu_stream(self._h)
 
     def synchronize(self):
         self._check(lib().salmon_gpu_synchronize(self._h), "synchronize")
