@@ -126,6 +126,17 @@ int salmon_gpu_sp_energy(salmon_ctx *ctx, const double *kAc, double *esp);
 int salmon_gpu_rt_run(salmon_ctx *ctx, const double *ac, int nsteps, double dt, int reortho_every,
                       double *jt, double *et, double *norm);
 
+/* Kernel path of the RT step.  SALMON_PATH_AUTO uses the LDS-resident fused
+ * step where the grid has a compiled shape (16^3) and the streamed kernels
+ * otherwise; SALMON_PATH_STREAMED forces the streamed kernels.  Like the
+ * reference's build-time stencil selection (ARTED/stencil/CMakeLists.txt:1-52)
+ * the choice changes the code path, not the results beyond rounding.
+ * salmon_gpu_path returns the path rt_run takes (SALMON_PATH_FUSED or
+ * SALMON_PATH_STREAMED). */
+enum { SALMON_PATH_AUTO = 0, SALMON_PATH_STREAMED = 1, SALMON_PATH_FUSED = 2 };
+int salmon_gpu_set_path(salmon_ctx *ctx, int path);
+int salmon_gpu_path(salmon_ctx *ctx);
+
 /* Waits for all work of the context. */
 int salmon_gpu_synchronize(salmon_ctx *ctx);
 

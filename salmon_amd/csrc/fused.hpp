@@ -1,10 +1,634 @@
-// LDS-resident Taylor step (placeholder: the streamed path serves every grid).
+// LDS-resident RT step for grids whose orbital fits in LDS (16^3).
+//
+// One workgroup owns the bands [b0, b1) of one k point.  For each orbital it
+// reads psi once from HBM into LDS, applies H four times out of LDS (stencil
+// + local + KB), accumulates the Taylor-4 sum in registers
+// (ARTED/RT/hamiltonian.f90:46-66,99-122), writes psi(t+dt) once, and then --
+// with psi(t+dt) still in LDS -- evaluates the observables at k+A(t+dt):
+// density (psi_rho.f90:130-144), current (RT/current.f90:172-273), kinetic
+// and nonlocal energy (total_energy.f90:172-203) and the norm, all summed in
+// FP64.  Per-workgroup sums and density partials are reduced afterwards in a
+// fixed order, so results do not depend on scheduling.
+//
+// Thread layout: a thread owns PX = NX/XS consecutive x points of one (y,z)
+// line; z is the fastest thread index, so every LDS and HBM access of a wave is
+// to consecutive complex values.  y and z neighbours come from LDS, x
+// neighbours from a register window (halo of 4 on each side, periodic).
+// Two LDS buffers alternate between input and output of each H application.
+// KB phases exp(i k.(r - T)) are built from per-axis tables
+// exp(i k_a (l H_a - j L_a)), l = 0..N_a-1, j = -2..2, computed once per
+// workgroup for the midpoint k (propagation) and for k+A(t+dt) (observables).
 #pragma once
 
-static int fused_config(salmon_ctx*) { return 0; }
-static int fused_set_tables(salmon_ctx*) { return SALMON_OK; }
-static void fused_free(salmon_ctx*) {}
+namespace salmon {
+
+constexpr int kFusedMaxNPI = 1152;      // sphere points per cell (LDS correction array)
+constexpr int kFusedMaxLmaPerAtom = 4;  // projectors per atom
+
 template <typename T>
-static int fused_step(salmon_ctx* c, const double*, const double*, double, double*) {
-  return fail(c, SALMON_ERR_UNSUPPORTED, "fused step not available");
+struct FusedArgs {
+  cplx<T>* zu;
+  const double* kac0;
+  const double* occ;
+  const T* vloc;
+  const double* vloc64;
+  int NK, NB, bpw, nbg;
+  double am[3], an[3];
+  T z1i, z2r, z3i, z4r;
+  T lapc[3][5];          // -lap(n)/2 per axis (propagation) -- uniform, kept in SGPRs
+  T lap[3][5];           // lap(n) per axis (observables)
+  T nab[3][5];           // nab(n) per axis
+  // KB tables
+  const int* pt_prim;    // (NL) first sphere point covering a grid point, or -1
+  const int* ov_start;   // (NOV+1) grid points covered by several sphere points:
+  const int* ov_jp;      //   entries [ov_start[o]+1, ov_start[o+1]) add into ov_jp[ov_start[o]]
+  int NOV;
+  const int* jp_atom;    // (NPI)
+  const int* jp_j;       // (NPI) index within its atom's sphere
+  const int* jp_grid;    // (NPI) grid index
+  const int* jp_phase;   // (NPI) px | py << 10 | pz << 20 into the phase tables
+  const double* jp_pos;  // (NPI*3) unwrapped coordinates
+  const int* atom_start; // (NI+1)
+  const int* atom_lma;   // (NI+1) projector range of each atom
+  const double* lma_sign;
+  const T* uv;           // (Nlma*Nps)
+  const double* uv64;    // (Nlma*Nps)
+  int NI, Nps, Nlma, NPI;
+  Grid g;
+  double* wg_rows;       // (nwg*8)
+  double* rho_part;      // (nwg*NL)
+};
+
+template <typename T>
+__device__ __forceinline__ cplx<T> cmul(cplx<T> a, cplx<T> b) {
+  return cplx<T>{a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+
+template <typename T, int NX, int NY, int NZ>
+__device__ __forceinline__ cplx<T> phase_at(const cplx<T>* ph, int code) {
+  const int px = code & 1023, py = (code >> 10) & 1023, pz = (code >> 20) & 1023;
+  return cmul(cmul(ph[px], ph[5 * NX + py]), ph[5 * NX + 5 * NY + pz]);
+}
+
+// x neighbour o (own-point offset, -4..PX+3) from the register window; o is a
+// compile-time constant after unrolling, so the window stays in registers
+template <typename T, int NX, int H, int XS, int W>
+__device__ __forceinline__ cplx<T> win_at(const cplx<T> (&win)[W], int o) {
+  if constexpr (XS > 1) {
+    return win[H + o];
+  } else {
+    return win[(o + NX) % NX];
+  }
+}
+
+template <typename T, int NX, int NY, int NZ, int XS, int WPE>
+__global__ void __launch_bounds__(NY * NZ * XS, WPE) k_fused_step(FusedArgs<T> a) {
+  constexpr int NYZ = NY * NZ, NT = NYZ * XS, NL = NX * NYZ;
+  constexpr int PX = NX / XS;                 // own x points per thread
+  constexpr int H = (XS > 1) ? 4 : 0;         // register halo on each side
+  constexpr int W = (XS > 1) ? PX + 8 : NX;   // register window
+  constexpr int NPH = 5 * (NX + NY + NZ);
+  constexpr int NW = (NT + kWave - 1) / kWave;
+  static_assert(NX % XS == 0, "x split");
+  static_assert(XS == 1 || PX >= 4, "halo of 4 needs PX >= 4");
+
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cplx<T>* bufA = reinterpret_cast<cplx<T>*>(smem_raw);
+  cplx<T>* bufB = bufA + NL;
+  cplx<T>* phm = bufB + NL;      // phase tables at the midpoint k
+  cplx<T>* phn = phm + NPH;      // phase tables at k + A(t+dt)
+  cplx<T>* dps = phn + NPH;      // KB term at each sphere point of the current H
+  double* red = reinterpret_cast<double*>(dps + kFusedMaxNPI);
+
+  const int t = threadIdx.x;
+  const int lane = t & (kWave - 1), wv = t / kWave;
+  const int xs = t / NYZ, yz = t - xs * NYZ;
+  const int iy = yz / NZ, iz = yz - iy * NZ;
+  const int x0 = xs * PX;
+  const int ik = blockIdx.x / a.nbg;
+  const int b0 = (blockIdx.x - ik * a.nbg) * a.bpw;
+  const int b1 = min(a.NB, b0 + a.bpw);
+  const Grid& g = a.g;
+
+  double km[3], kn[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    km[d] = a.kac0[d * a.NK + ik] + a.am[d];
+    kn[d] = a.kac0[d * a.NK + ik] + a.an[d];
+  }
+  // phase tables
+  for (int q = t; q < NPH; q += NT) {
+    int ax, n, r = q;
+    if (r < 5 * NX) { ax = 0; n = NX; } else if (r < 5 * (NX + NY)) { ax = 1; n = NY; r -= 5 * NX; }
+    else { ax = 2; n = NZ; r -= 5 * (NX + NY); }
+    const int img = r / n, l = r - img * n;
+    const double h = ax == 0 ? g.Hx : (ax == 1 ? g.Hy : g.Hz);
+    const double L = ax == 0 ? g.aLx : (ax == 1 ? g.aLy : g.aLz);
+    const double pos = l * h - (img - 2) * L;
+    double s, c;
+    sincos(km[ax] * pos, &s, &c);
+    phm[q] = cplx<T>{(T)c, (T)s};
+    sincos(kn[ax] * pos, &s, &c);
+    phn[q] = cplx<T>{(T)c, (T)s};
+  }
+  // the k dependence of the gradient term is applied after the per-axis sums
+  const T kmx = (T)km[0], kmy = (T)km[1], kmz = (T)km[2];
+  const T A = (T)((km[0] * km[0] + km[1] * km[1] + km[2] * km[2] - (g.lap[0][0] + g.lap[1][0] + g.lap[2][0])) * 0.5);
+  // the (first) sphere point covering each own grid point, or the zero entry NPI
+  int prim[PX];
+#pragma unroll
+  for (int i = 0; i < PX; ++i) {
+    const int q = a.pt_prim[(x0 + i) * NYZ + yz];
+    prim[i] = q >= 0 ? q : a.NPI;
+  }
+  if (t == 0) dps[a.NPI] = cplx<T>{0, 0};
+  const double A0 = -(g.lap[0][0] + g.lap[1][0] + g.lap[2][0]) * 0.5;
+  double n2w = 0, vsw = 0, tew = 0, jsw[3] = {0, 0, 0}, jnl[3] = {0, 0, 0}, enl = 0, occsum = 0;
+  // density partial of this workgroup: each thread owns its points, so the
+  // slice is updated without atomics
+  double* rho = a.rho_part + (size_t)blockIdx.x * NL;
+#pragma unroll
+  for (int i = 0; i < PX; ++i) rho[(x0 + i) * NYZ + yz] = 0.0;
+  __syncthreads();
+
+  for (int ib = b0; ib < b1; ++ib) {
+    const size_t ikb = (size_t)ik * a.NB + ib;
+    const double occ = a.occ[ikb];
+    occsum += occ;
+    cplx<T>* zo = a.zu + ikb * NL;
+    cplx<T> win[W], acc[PX];
+#pragma unroll
+    for (int i = 0; i < PX; ++i) {
+      const int p = (x0 + i) * NYZ + yz;
+      const cplx<T> v = zo[p];
+      acc[i] = v;
+      win[H + i] = v;
+      bufA[p] = v;
+    }
+    __syncthreads();
+    cplx<T>* in = bufA;
+    cplx<T>* out = bufB;
+#pragma unroll 1
+    for (int n = 0; n < 4; ++n) {
+      if constexpr (XS > 1) {
+#pragma unroll
+        for (int i = 0; i < H; ++i) {
+          win[i] = in[((x0 - H + i + NX) % NX) * NYZ + yz];
+          win[H + PX + i] = in[((x0 + PX + i) % NX) * NYZ + yz];
+        }
+      }
+      // KB term of pseudo_pt (hpsi.f90:176-206) per atom in one wave:
+      //   uVpsi(l) = Hxyz iuV(l) sum_j uV(j,l) e^{ik.r_j} psi(J_j)       (projection)
+      //   dps(j)   = e^{-ik.r_j} sum_l uV(j,l) uVpsi(l)                  (back-projection)
+      // the wave sum leaves uVpsi in every lane, so no barrier separates the two
+      for (int at = wv; at < a.NI; at += NW) {
+        const int j0 = a.atom_start[at], m = a.atom_start[at + 1] - j0;
+        const int l0 = a.atom_lma[at], nl = a.atom_lma[at + 1] - l0;
+        T sr[kFusedMaxLmaPerAtom], si[kFusedMaxLmaPerAtom];
+#pragma unroll
+        for (int l = 0; l < kFusedMaxLmaPerAtom; ++l) { sr[l] = 0; si[l] = 0; }
+        for (int j = lane; j < m; j += kWave) {
+          const int jp = j0 + j;
+          const cplx<T> e = phase_at<T, NX, NY, NZ>(phm, a.jp_phase[jp]);
+          const cplx<T> w = cmul(e, in[a.jp_grid[jp]]);
+#pragma unroll
+          for (int l = 0; l < kFusedMaxLmaPerAtom; ++l) {
+            if (l < nl) {
+              const T u = a.uv[(size_t)(l0 + l) * a.Nps + j];
+              sr[l] += u * w.re;
+              si[l] += u * w.im;
+            }
+          }
+        }
+#pragma unroll
+        for (int l = 0; l < kFusedMaxLmaPerAtom; ++l) {
+          const T f = l < nl ? (T)(g.Hxyz * a.lma_sign[l0 + l]) : (T)0;
+          sr[l] = wave_sum(sr[l]) * f;
+          si[l] = wave_sum(si[l]) * f;
+        }
+        for (int j = lane; j < m; j += kWave) {
+          const int jp = j0 + j;
+          T tr = 0, ti = 0;
+#pragma unroll
+          for (int l = 0; l < kFusedMaxLmaPerAtom; ++l) {
+            if (l < nl) {
+              const T u = a.uv[(size_t)(l0 + l) * a.Nps + j];
+              tr += u * sr[l];
+              ti += u * si[l];
+            }
+          }
+          const cplx<T> e = phase_at<T, NX, NY, NZ>(phm, a.jp_phase[jp]);
+          dps[jp] = cplx<T>{e.re * tr + e.im * ti, e.re * ti - e.im * tr};  // conj(e) * t
+        }
+      }
+      __syncthreads();  // KB terms complete
+      if (a.NOV > 0) {
+        // points inside several spheres (overlapping atoms): fold the other
+        // sphere points' terms into the first, in table order
+        for (int o = t; o < a.NOV; o += NT) {
+          const int q0 = a.ov_start[o], q1 = a.ov_start[o + 1];
+          cplx<T> s0 = dps[a.ov_jp[q0]];
+          for (int q = q0 + 1; q < q1; ++q) {
+            const cplx<T> d = dps[a.ov_jp[q]];
+            s0.re += d.re;
+            s0.im += d.im;
+          }
+          dps[a.ov_jp[q0]] = s0;
+        }
+        __syncthreads();
+      }
+      // stencil: h = (V+A) psi - 1/2 sum_a sum_n lap_n (psi+ + psi-) - i sum_a k_a sum_n nab_n (psi+ - psi-)
+      cplx<T> h[PX];
+#pragma unroll
+      for (int i = 0; i < PX; ++i) {
+        const int x = x0 + i;
+        const cplx<T> e0 = win[H + i];
+        const T vAi = a.vloc[x * NYZ + yz] + A;
+        T lr = vAi * e0.re, li = vAi * e0.im;
+        T gzr = 0, gzi = 0, gyr = 0, gyi = 0, gxr = 0, gxi = 0;
+        const cplx<T>* rowp = in + x * NYZ;
+#pragma unroll
+        for (int nn = 1; nn <= 4; ++nn) {
+          const cplx<T> zp = rowp[iy * NZ + (iz + nn) % NZ];
+          const cplx<T> zm = rowp[iy * NZ + (iz - nn + NZ) % NZ];
+          lr += a.lapc[2][nn] * (zp.re + zm.re);
+          li += a.lapc[2][nn] * (zp.im + zm.im);
+          gzr += a.nab[2][nn] * (zp.re - zm.re);
+          gzi += a.nab[2][nn] * (zp.im - zm.im);
+          const cplx<T> yp = rowp[((iy + nn) % NY) * NZ + iz];
+          const cplx<T> ym = rowp[((iy - nn + NY) % NY) * NZ + iz];
+          lr += a.lapc[1][nn] * (yp.re + ym.re);
+          li += a.lapc[1][nn] * (yp.im + ym.im);
+          gyr += a.nab[1][nn] * (yp.re - ym.re);
+          gyi += a.nab[1][nn] * (yp.im - ym.im);
+          const cplx<T> xp = win_at<T, NX, H, XS, W>(win, i + nn);
+          const cplx<T> xm = win_at<T, NX, H, XS, W>(win, i - nn);
+          lr += a.lapc[0][nn] * (xp.re + xm.re);
+          li += a.lapc[0][nn] * (xp.im + xm.im);
+          gxr += a.nab[0][nn] * (xp.re - xm.re);
+          gxi += a.nab[0][nn] * (xp.im - xm.im);
+        }
+        // -i * (k.G)
+        h[i].re = lr + (kmx * gxi + kmy * gyi + kmz * gzi);
+        h[i].im = li - (kmx * gxr + kmy * gyr + kmz * gzr);
+      }
+      // add the KB term of the sphere point covering each own point (branch-free:
+      // uncovered points read the zero entry dps[NPI])
+#pragma unroll
+      for (int i = 0; i < PX; ++i) {
+        const cplx<T> d = dps[prim[i]];
+        h[i].re += d.re;
+        h[i].im += d.im;
+      }
+      const T zr = (n == 1) ? a.z2r : ((n == 3) ? a.z4r : (T)0);
+      const T zi = (n == 0) ? a.z1i : ((n == 2) ? a.z3i : (T)0);
+#pragma unroll
+      for (int i = 0; i < PX; ++i) {
+        out[(x0 + i) * NYZ + yz] = h[i];
+        win[H + i] = h[i];
+        acc[i].re += zr * h[i].re - zi * h[i].im;
+        acc[i].im += zr * h[i].im + zi * h[i].re;
+      }
+      __syncthreads();
+      cplx<T>* tmp = in;
+      in = out;
+      out = tmp;
+    }
+    // psi(t+dt): to HBM and to the free LDS buffer
+#pragma unroll
+    for (int i = 0; i < PX; ++i) {
+      const int p = (x0 + i) * NYZ + yz;
+      zo[p] = acc[i];
+      out[p] = acc[i];
+    }
+    __syncthreads();
+    // observables at k + A(t+dt), FP64 sums; all neighbours from LDS (a rolled
+    // loop keeps this sweep's registers low)
+#pragma unroll 1
+    for (int i = 0; i < PX; ++i) {
+      const int x = x0 + i;
+      const int p = x * NYZ + yz;
+      const cplx<T> e0 = out[p];
+      const double er = (double)e0.re, ei = (double)e0.im;
+      const double a2 = er * er + ei * ei;
+      n2w += occ * a2;
+      vsw += occ * a.vloc64[p] * a2;
+      rho[p] += occ * a2;
+      const cplx<T>* rowp = out + x * NYZ;
+      T gr0 = 0, gi0 = 0, gr1 = 0, gi1 = 0, gr2 = 0, gi2 = 0, lr = 0, li = 0;
+#pragma unroll
+      for (int nn = 1; nn <= 4; ++nn) {
+        const cplx<T> zp = rowp[iy * NZ + (iz + nn) % NZ];
+        const cplx<T> zm = rowp[iy * NZ + (iz - nn + NZ) % NZ];
+        const cplx<T> yp = rowp[((iy + nn) % NY) * NZ + iz];
+        const cplx<T> ym = rowp[((iy - nn + NY) % NY) * NZ + iz];
+        const cplx<T> xp = out[((x + nn) % NX) * NYZ + yz];
+        const cplx<T> xm = out[((x - nn + NX) % NX) * NYZ + yz];
+        gr2 += a.nab[2][nn] * (zp.re - zm.re); gi2 += a.nab[2][nn] * (zp.im - zm.im);
+        gr1 += a.nab[1][nn] * (yp.re - ym.re); gi1 += a.nab[1][nn] * (yp.im - ym.im);
+        gr0 += a.nab[0][nn] * (xp.re - xm.re); gi0 += a.nab[0][nn] * (xp.im - xm.im);
+        lr += a.lap[2][nn] * (zp.re + zm.re) + a.lap[1][nn] * (yp.re + ym.re) + a.lap[0][nn] * (xp.re + xm.re);
+        li += a.lap[2][nn] * (zp.im + zm.im) + a.lap[1][nn] * (yp.im + ym.im) + a.lap[0][nn] * (xp.im + xm.im);
+      }
+      // sum_i Im(conj psi_i sum_n nab_n psi_{i+n}) = 1/2 sum_i Im(conj psi_i G_i)
+      const double G0r = gr0, G0i = gi0, G1r = gr1, G1i = gi1, G2r = gr2, G2i = gi2;
+      jsw[0] += occ * 0.5 * (er * G0i - ei * G0r);
+      jsw[1] += occ * 0.5 * (er * G1i - ei * G1r);
+      jsw[2] += occ * 0.5 * (er * G2i - ei * G2r);
+      const double wr = kn[0] * G0r + kn[1] * G1r + kn[2] * G2r;
+      const double wi = kn[0] * G0i + kn[1] * G1i + kn[2] * G2i;
+      const double zr = A0 * er - 0.5 * (double)lr + wi, zi = A0 * ei - 0.5 * (double)li - wr;
+      tew += occ * (er * zr + ei * zi);
+    }
+    // nonlocal current and energy (current.f90:247-269, total_energy.f90:191-199):
+    // waves over (atom, projector)
+    for (int at = wv; at < a.NI; at += NW) {
+      const int j0 = a.atom_start[at], m = a.atom_start[at + 1] - j0;
+      for (int l = a.atom_lma[at]; l < a.atom_lma[at + 1]; ++l) {
+        double ur = 0, ui = 0, xr0 = 0, xi0 = 0, xr1 = 0, xi1 = 0, xr2 = 0, xi2 = 0;
+        for (int j = lane; j < m; j += kWave) {
+          const int jp = j0 + j;
+          const cplx<T> e = phase_at<T, NX, NY, NZ>(phn, a.jp_phase[jp]);
+          const cplx<T> w = cmul(e, out[a.jp_grid[jp]]);
+          const double u = a.uv64[(size_t)l * a.Nps + j];
+          const double tr = u * (double)w.re, ti = u * (double)w.im;
+          const double px = a.jp_pos[3 * jp], py = a.jp_pos[3 * jp + 1], pz = a.jp_pos[3 * jp + 2];
+          ur += tr; ui += ti;
+          xr0 += px * tr; xi0 += px * ti;
+          xr1 += py * tr; xi1 += py * ti;
+          xr2 += pz * tr; xi2 += pz * ti;
+        }
+        ur = wave_sum(ur); ui = wave_sum(ui);
+        xr0 = wave_sum(xr0); xi0 = wave_sum(xi0);
+        xr1 = wave_sum(xr1); xi1 = wave_sum(xi1);
+        xr2 = wave_sum(xr2); xi2 = wave_sum(xi2);
+        if (lane == 0) {
+          const double sg = a.lma_sign[l], hx = g.Hxyz;
+          const double Ur = ur * hx * sg, Ui = ui * hx * sg;
+          const double f = occ / g.aLxyz * 2.0 * hx;
+          jnl[0] += f * (xr0 * Ui - xi0 * Ur);
+          jnl[1] += f * (xr1 * Ui - xi1 * Ur);
+          jnl[2] += f * (xr2 * Ui - xi2 * Ur);
+          enl += occ * (ur * ur + ui * ui) * hx * hx * sg;
+        }
+      }
+    }
+    // the next orbital overwrites bufA only; bufB (psi(t+dt)) is read above
+    // and is next written after the barrier that follows the next load.
+  }
+  // per-workgroup sums
+  double v[10] = {n2w, vsw, tew, jsw[0], jsw[1], jsw[2], jnl[0], jnl[1], jnl[2], enl};
+#pragma unroll
+  for (int i = 0; i < 10; ++i) v[i] = wave_sum(v[i]);
+  if (lane == 0) {
+#pragma unroll
+    for (int i = 0; i < 10; ++i) red[wv * 10 + i] = v[i];
+  }
+  __syncthreads();
+  if (t == 0) {
+    double s[10];
+#pragma unroll
+    for (int i = 0; i < 10; ++i) {
+      s[i] = 0;
+      for (int w = 0; w < NW; ++w) s[i] += red[w * 10 + i];
+    }
+    double* r = a.wg_rows + (size_t)blockIdx.x * 8;
+    const double f = g.Hxyz / g.aLxyz;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) r[d] = (kn[d] * s[0] + 2.0 * s[3 + d]) * f + s[6 + d];
+    r[3] = s[2] * g.Hxyz + occsum * (kn[0] * kn[0] + kn[1] * kn[1] + kn[2] * kn[2]) / 2.0;
+    r[4] = s[9];
+    r[5] = s[1] * g.Hxyz;
+    r[6] = s[0] * g.Hxyz;
+    r[7] = 0.0;
+  }
+}
+
+}  // namespace salmon
+
+// ---------------------------------------------------------------------------
+// host side (included into salmon_gpu.hip after the context definition)
+// ---------------------------------------------------------------------------
+// x split of the 16^3 kernels: threads per workgroup = 256 * XS
+constexpr int kFusedXS64 = 1;
+constexpr int kFusedXS32 = 1;
+// waves per SIMD the register allocation targets: the FP64 kernel's 152 KiB of
+// LDS allows one workgroup per CU, the complex64 kernel's 77 KiB two
+constexpr int kFusedWPE64 = 1;
+constexpr int kFusedWPE32 = 1;
+
+struct FusedState {
+  int shape = 0;  // 1: 16^3
+  std::vector<void*> bufs;
+  const int *pt_prim = nullptr, *ov_start = nullptr, *ov_jp = nullptr, *jp_atom = nullptr, *jp_j = nullptr,
+            *jp_grid = nullptr, *jp_phase = nullptr, *atom_lma = nullptr;
+  int NOV = 0;
+  const double* jp_pos = nullptr;
+  float* uv32 = nullptr;
+  double* wg_rows = nullptr;
+  double* rho_part = nullptr;
+  int nwg_alloc = 0;
+  bool tables = false;
+};
+
+static int fused_shape(const salmon::Grid& g) {
+  if (g.NLx == 16 && g.NLy == 16 && g.NLz == 16) return 1;
+  return 0;
+}
+
+static int fused_config(salmon_ctx* c) {
+  if (!c->fz) c->fz = new FusedState();
+  FusedState* f = (FusedState*)c->fz;
+  f->shape = fused_shape(c->g);
+  return f->shape;
+}
+
+static void fused_free_tables(FusedState* f) {
+  for (void* p : f->bufs) (void)hipFree(p);
+  f->bufs.clear();
+  f->tables = false;
+  f->uv32 = nullptr;
+}
+
+static void fused_free(salmon_ctx* c) {
+  FusedState* f = (FusedState*)c->fz;
+  if (!f) return;
+  fused_free_tables(f);
+  if (f->wg_rows) (void)hipFree(f->wg_rows);
+  if (f->rho_part) (void)hipFree(f->rho_part);
+  delete f;
+  c->fz = nullptr;
+}
+
+// Builds the per-point and per-sphere-point tables of the fused kernel from
+// the streamed path's tables (host copies are rebuilt from the device ones).
+static bool fused_ready(const salmon_ctx* c) {
+  const FusedState* f = (const FusedState*)c->fz;
+  return f && f->shape > 0 && f->tables && c->path == SALMON_PATH_AUTO;
+}
+
+static int fused_set_tables(salmon_ctx* c) {
+  FusedState* f = (FusedState*)c->fz;
+  if (!f || f->shape == 0) return SALMON_OK;
+  fused_free_tables(f);
+  const salmon::KBTables& kb = c->kb;
+  const salmon::Grid& g = c->g;
+  if (kb.NPI + 1 > salmon::kFusedMaxNPI) return SALMON_OK;  // the streamed path serves this case
+  std::vector<int> start(kb.NI + 1), pidx(kb.NPI), lma_atom(kb.Nlma);
+  std::vector<double> pos(3 * (size_t)kb.NPI), uv((size_t)kb.Nlma * kb.Nps);
+  if (hipMemcpy(start.data(), kb.atom_start, sizeof(int) * start.size(), hipMemcpyDeviceToHost) != hipSuccess ||
+      (kb.NPI && hipMemcpy(pidx.data(), kb.pidx, sizeof(int) * pidx.size(), hipMemcpyDeviceToHost) != hipSuccess) ||
+      (kb.NPI && hipMemcpy(pos.data(), kb.pos, sizeof(double) * pos.size(), hipMemcpyDeviceToHost) != hipSuccess) ||
+      (kb.Nlma && hipMemcpy(lma_atom.data(), kb.lma_atom, sizeof(int) * lma_atom.size(), hipMemcpyDeviceToHost) != hipSuccess) ||
+      (kb.Nlma && hipMemcpy(uv.data(), kb.uv, sizeof(double) * uv.size(), hipMemcpyDeviceToHost) != hipSuccess))
+    return fail(c, SALMON_ERR_HIP, "fused tables: device read-back failed");
+  std::vector<int> atom_lma(kb.NI + 1, 0);
+  for (int l = 0; l < kb.Nlma; ++l) atom_lma[lma_atom[l] + 1]++;
+  for (int at = 0; at < kb.NI; ++at) {
+    if (atom_lma[at + 1] > salmon::kFusedMaxLmaPerAtom) return SALMON_OK;  // streamed path
+    atom_lma[at + 1] += atom_lma[at];
+  }
+  std::vector<int> jp_atom(kb.NPI), jp_j(kb.NPI), jp_phase(kb.NPI);
+  const int N[3] = {g.NLx, g.NLy, g.NLz};
+  const double Hs[3] = {g.Hx, g.Hy, g.Hz}, Ls[3] = {g.aLx, g.aLy, g.aLz};
+  for (int at = 0; at < kb.NI; ++at)
+    for (int jp = start[at]; jp < start[at + 1]; ++jp) {
+      jp_atom[jp] = at;
+      jp_j[jp] = jp - start[at];
+      const int p = pidx[jp];
+      const int l[3] = {p / (g.NLy * g.NLz), (p / g.NLz) % g.NLy, p % g.NLz};
+      int code = 0;
+      for (int d = 0; d < 3; ++d) {
+        // pos = l*H - img*L with img in -2..2
+        const double img = (l[d] * Hs[d] - pos[3 * (size_t)jp + d]) / Ls[d];
+        const int ii = (int)std::lround(img);
+        if (ii < -2 || ii > 2) return fail(c, SALMON_ERR_ARG, "image offset outside -2..2");
+        code |= ((ii + 2) * N[d] + l[d]) << (10 * d);
+      }
+      jp_phase[jp] = code;
+    }
+  std::vector<std::vector<int>> per(g.NL);
+  for (int jp = 0; jp < kb.NPI; ++jp) per[pidx[jp]].push_back(jp);
+  std::vector<int> pt_prim(g.NL, -1), ov_start(1, 0), ov_jp;
+  for (int p = 0; p < g.NL; ++p) {
+    if (per[p].empty()) continue;
+    pt_prim[p] = per[p][0];
+    if (per[p].size() > 1) {
+      for (int jp : per[p]) ov_jp.push_back(jp);
+      ov_start.push_back((int)ov_jp.size());
+    }
+  }
+  f->NOV = (int)ov_start.size() - 1;
+  std::vector<float> uv32(uv.size());
+  for (size_t i = 0; i < uv.size(); ++i) uv32[i] = (float)uv[i];
+  auto up = [&](const void* src, size_t bytes, void** dst) -> int {
+    void* d = nullptr;
+    if (hipMalloc(&d, bytes ? bytes : 4) != hipSuccess) return fail(c, SALMON_ERR_NOMEM, "fused tables");
+    f->bufs.push_back(d);
+    if (bytes && hipMemcpy(d, src, bytes, hipMemcpyHostToDevice) != hipSuccess)
+      return fail(c, SALMON_ERR_HIP, "fused table upload");
+    *dst = d;
+    return SALMON_OK;
+  };
+  int r;
+  if ((r = up(pt_prim.data(), sizeof(int) * pt_prim.size(), (void**)&f->pt_prim)) ||
+      (r = up(ov_start.data(), sizeof(int) * ov_start.size(), (void**)&f->ov_start)) ||
+      (r = up(ov_jp.data(), sizeof(int) * ov_jp.size(), (void**)&f->ov_jp)) ||
+      (r = up(jp_atom.data(), sizeof(int) * jp_atom.size(), (void**)&f->jp_atom)) ||
+      (r = up(jp_j.data(), sizeof(int) * jp_j.size(), (void**)&f->jp_j)) ||
+      (r = up(pidx.data(), sizeof(int) * pidx.size(), (void**)&f->jp_grid)) ||
+      (r = up(jp_phase.data(), sizeof(int) * jp_phase.size(), (void**)&f->jp_phase)) ||
+      (r = up(atom_lma.data(), sizeof(int) * atom_lma.size(), (void**)&f->atom_lma)) ||
+      (r = up(uv32.data(), sizeof(float) * uv32.size(), (void**)&f->uv32)))
+    return r;
+  f->jp_pos = kb.pos;
+  f->tables = true;
+  return SALMON_OK;
+}
+
+template <typename T, int NX, int NY, int NZ, int XS>
+static size_t fused_lds_bytes() {
+  return sizeof(salmon::cplx<T>) * (2 * NX * NY * NZ + 2 * 5 * (NX + NY + NZ) + salmon::kFusedMaxNPI) +
+         sizeof(double) * 10 * ((NY * NZ * XS + 63) / 64);
+}
+
+template <typename T, int NX, int NY, int NZ, int XS, int WPE>
+static int fused_launch(salmon_ctx* c, const salmon::FusedArgs<T>& args, int nwg) {
+  auto kern = salmon::k_fused_step<T, NX, NY, NZ, XS, WPE>;
+  const size_t lds = fused_lds_bytes<T, NX, NY, NZ, XS>();
+  static bool attr_set = false;
+  if (!attr_set) {
+    HIPCHK(c, hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    attr_set = true;
+  }
+  hipLaunchKernelGGL(kern, dim3(nwg), dim3(NY * NZ * XS), lds, c->stream, args);
+  LAUNCHCHK(c);
+  return SALMON_OK;
+}
+
+// workgroups resident per CU for a launch (1 for the FP64 16^3 kernel, whose
+// LDS is 141 KiB; 2 for the complex64 kernels)
+template <typename T>
+static int fused_step(salmon_ctx* c, const double amid[3], const double anew[3], double dt, double* res_dev) {
+  FusedState* f = (FusedState*)c->fz;
+  if (!f || !f->tables) return fail(c, SALMON_ERR_UNSUPPORTED, "fused tables not built");
+  int ncu = 256;
+  {
+    int dev = 0;
+    if (hipGetDevice(&dev) == hipSuccess) {
+      int v = 0;
+      if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && v > 0) ncu = v;
+    }
+  }
+  const int per_cu = sizeof(T) == 8 ? 1 : 2;
+  const int target = 2 * ncu * per_cu;
+  int bpw = (c->NKB + target - 1) / target;
+  bpw = std::max(1, std::min(bpw, c->NB));
+  const int nbg = (c->NB + bpw - 1) / bpw;
+  const int nwg = c->NK * nbg;
+  if (nwg > f->nwg_alloc) {
+    if (f->wg_rows) (void)hipFree(f->wg_rows);
+    if (f->rho_part) (void)hipFree(f->rho_part);
+    f->wg_rows = nullptr;
+    f->rho_part = nullptr;
+    int r;
+    if ((r = dalloc(c, &f->wg_rows, 8 * (size_t)nwg)) || (r = dalloc(c, &f->rho_part, (size_t)nwg * c->g.NL))) return r;
+    f->nwg_alloc = nwg;
+  }
+  salmon::FusedArgs<T> a{};
+  a.zu = (salmon::cplx<T>*)c->d_zu;
+  a.kac0 = c->d_kac0;
+  a.occ = c->d_occ;
+  a.vloc = vloc_ptr<T>(c);
+  a.vloc64 = c->d_vloc;
+  a.NK = c->NK; a.NB = c->NB; a.bpw = bpw; a.nbg = nbg;
+  for (int d = 0; d < 3; ++d) { a.am[d] = amid[d]; a.an[d] = anew[d]; }
+  a.z1i = (T)(-dt); a.z2r = (T)(-dt * dt / 2.0); a.z3i = (T)(dt * dt * dt / 6.0); a.z4r = (T)(dt * dt * dt * dt / 24.0);
+  for (int d = 0; d < 3; ++d)
+    for (int n = 0; n < 5; ++n) {
+      a.lapc[d][n] = (T)(-0.5 * c->g.lap[d][n]);
+      a.lap[d][n] = (T)c->g.lap[d][n];
+      a.nab[d][n] = (T)c->g.nab[d][n];
+    }
+  a.pt_prim = f->pt_prim; a.ov_start = f->ov_start; a.ov_jp = f->ov_jp; a.NOV = f->NOV; a.jp_atom = f->jp_atom; a.jp_j = f->jp_j; a.jp_grid = f->jp_grid;
+  a.jp_phase = f->jp_phase; a.jp_pos = f->jp_pos; a.atom_start = c->kb.atom_start; a.atom_lma = f->atom_lma;
+  a.lma_sign = c->kb.lma_sign;
+  a.uv = sizeof(T) == 8 ? (const T*)c->kb.uv : (const T*)f->uv32;
+  a.uv64 = c->kb.uv;
+  a.NI = c->kb.NI; a.Nps = c->kb.Nps; a.Nlma = c->kb.Nlma; a.NPI = c->kb.NPI;
+  a.g = c->g;
+  a.wg_rows = f->wg_rows;
+  a.rho_part = f->rho_part;
+  int r;
+  if constexpr (sizeof(T) == 8)
+    r = fused_launch<T, 16, 16, 16, kFusedXS64, kFusedWPE64>(c, a, nwg);
+  else
+    r = fused_launch<T, 16, 16, 16, kFusedXS32, kFusedWPE32>(c, a, nwg);
+  if (r) return r;
+  hipLaunchKernelGGL(salmon::k_sum_rows, dim3(1), dim3(256), 0, c->stream, f->wg_rows, nwg, res_dev);
+  LAUNCHCHK(c);
+  hipLaunchKernelGGL(salmon::k_density_final, dim3((c->g.NL + 255) / 256), dim3(256), 0, c->stream, f->rho_part, nwg,
+                     c->g.NL, c->d_rho);
+  LAUNCHCHK(c);
+  return SALMON_OK;
 }

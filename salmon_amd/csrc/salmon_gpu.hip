@@ -54,6 +54,7 @@ struct salmon_ctx {
   cplx<double>* d_f64 = nullptr;  // FP64 scratch copy of the orbitals (mixed Gram-Schmidt)
   // fused step kernel configuration (0 = streamed path)
   int fused = -1;
+  int path = SALMON_PATH_AUTO;
   void* fz = nullptr;  // state owned by the fused path
 };
 
@@ -251,10 +252,7 @@ namespace {
 template <typename T>
 int rt_step(salmon_ctx* c, const double amid[3], const double anew[3], double dt, double* res_dev) {
   int r;
-  if (c->fused > 0) {
-    r = fused_step<T>(c, amid, anew, dt, res_dev);
-    return r;
-  }
+  if (fused_ready(c)) return fused_step<T>(c, amid, anew, dt, res_dev);
   if ((r = taylor_streamed<T>(c, c->d_kac0, amid[0], amid[1], amid[2], dt))) return r;
   if ((r = density_dev<T>(c))) return r;
   if ((r = observe_rows<T>(c, c->d_kac0, anew[0], anew[1], anew[2]))) return r;
@@ -685,6 +683,18 @@ int salmon_gpu_synchronize(salmon_ctx* c) {
   if (!c) return SALMON_ERR_ARG;
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return SALMON_OK;
+}
+
+int salmon_gpu_set_path(salmon_ctx* c, int path) {
+  if (!c) return SALMON_ERR_ARG;
+  if (path != SALMON_PATH_AUTO && path != SALMON_PATH_STREAMED) return fail(c, SALMON_ERR_ARG, "unknown path %d", path);
+  c->path = path;
+  return SALMON_OK;
+}
+
+int salmon_gpu_path(salmon_ctx* c) {
+  if (!c) return -1;
+  return fused_ready(c) ? SALMON_PATH_FUSED : SALMON_PATH_STREAMED;
 }
 
 void* salmon_gpu_stream(salmon_ctx* c) { return c ? (void*)c->stream : nullptr; }

@@ -34,6 +34,7 @@ LIB_PATH = os.path.join(HERE, "lib", "libsalmon_gpu.so")
 
 FP64 = 0
 MIXED = 1
+PATH_AUTO, PATH_STREAMED, PATH_FUSED = 0, 1, 2
 
 
 class SalmonGPUError(RuntimeError):
@@ -65,6 +66,8 @@ SIGNATURES = {
     "salmon_gpu_gram_schmidt": [_vp],
     "salmon_gpu_sp_energy": [_vp, _vp, _vp],
     "salmon_gpu_rt_run": [_vp, _vp, _ci, _cd, _ci, _vp, _vp, _vp],
+    "salmon_gpu_set_path": [_vp, _ci],
+    "salmon_gpu_path": [_vp],
     "salmon_gpu_synchronize": [_vp],
     "salmon_gpu_stream": [_vp],
     "salmon_gpu_errstr": [_ci],
@@ -228,6 +231,12 @@ class Context:
     def stream(self):
         """The context's hipStream_t (as an integer handle)."""
         return lib().salmon_gpu_stream(self._h)
+
+    def set_path(self, path):
+        self._check(lib().salmon_gpu_set_path(self._h, int(path)), "set_path")
+
+    def path(self):
+        return lib().salmon_gpu_path(self._h)
 
     def synchronize(self):
         self._check(lib().salmon_gpu_synchronize(self._h), "synchronize")

@@ -151,3 +151,43 @@ def test_error_paths_raise():
     bad.Jxyz[0, 0] = s.NL + 5
     with pytest.raises(gpu.SalmonGPUError):
         gpu.Context(bad)
+
+
+def test_fused_path_selected_and_matches_streamed():
+    """16^3 grids take the LDS-resident fused step; it agrees with the
+    streamed kernels and with the reference trajectory."""
+    s, d = load_case("cfg1")
+    with _ctx(s) as c:
+        assert c.path() == gpu.PATH_FUSED
+        rf = c.rt_run(d["ac"], 20)
+        zf = c.get_orbitals()
+    with _ctx(s) as c:
+        c.set_path(gpu.PATH_STREAMED)
+        assert c.path() == gpu.PATH_STREAMED
+        rs = c.rt_run(d["ac"], 20)
+        zs = c.get_orbitals()
+    assert rel(zf, zs) < TOL64
+    assert rel(rf["J"], rs["J"]) < 1e-11
+    assert rel(rf["E"], rs["E"]) < TOL64
+    assert rel(rf["J"], d["rt_J"][:20]) < 1e-11
+
+
+def test_fused_overlapping_spheres_against_oracle():
+    """Random atoms on the 16^3 grid: sphere points shared by two atoms go
+    through the overlap fold of the fused kernel."""
+    s = synth.make_system(N=(16, 16, 16), L=10.26, nk3=(2, 1, 1), NB=3, seed=21, atoms="random", natoms=14)
+    cover = np.zeros(s.NL, int)
+    for a in range(s.NI):
+        np.add.at(cover, s.Jxyz[a, : s.Mps[a]] - 1, 1)
+    assert cover.max() >= 2
+    ac = synth.acos2_pulse(3, s.dt, 0.05, 0.3, 2.0)
+    with _ctx(s) as c:
+        assert c.path() == gpu.PATH_FUSED
+        r = c.rt_run(ac, 3)
+        z = c.get_orbitals()
+        rho = c.psi_rho_RT()
+    o = Oracle(s).rt_run(ac, 3, want_rho=True)
+    assert rel(z, o["zu"]) < TOL64
+    assert rel(r["J"], o["J"]) < 1e-11
+    assert rel(r["E"], o["E"]) < TOL64
+    assert rel(rho, o["rho"]) < TOL64
