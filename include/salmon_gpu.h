@@ -121,10 +121,11 @@ int salmon_gpu_sp_energy(salmon_ctx *ctx, const double *kAc, double *esp);
  * A(t_n) for n = 0..nsteps as (nsteps+1) rows of 3.  Step n propagates at
  * kAc0 + (A_n+A_{n+1})/2, computes the density, then current and energy at
  * kAc0 + A_{n+1}.  Outputs (nullable): jt (nsteps rows of 3), et(nsteps) =
- * Ekin+Enl+Eloc, norm(nsteps).  reortho_every > 0 applies an FP64
- * Gram-Schmidt after every reortho_every steps. */
+ * Ekin+Enl+Eloc, norm(nsteps), rho(NL) = the density of the last step.
+ * reortho_every > 0 applies an FP64 Gram-Schmidt after every reortho_every
+ * steps. */
 int salmon_gpu_rt_run(salmon_ctx *ctx, const double *ac, int nsteps, double dt, int reortho_every,
-                      double *jt, double *et, double *norm);
+                      double *jt, double *et, double *norm, double *rho);
 
 /* Kernel path of the RT step.  SALMON_PATH_AUTO uses the LDS-resident fused
  * step where the grid has a compiled shape (16^3) and the streamed kernels

@@ -642,7 +642,7 @@ int salmon_gpu_sp_energy(salmon_ctx* c, const double* kAc, double* esp) {
 }
 
 int salmon_gpu_rt_run(salmon_ctx* c, const double* ac, int nsteps, double dt, int reortho_every, double* jt, double* et,
-                      double* norm) {
+                      double* norm, double* rho) {
   int r = check_ready(c, true);
   if (r) return r;
   if (!ac || nsteps < 0) return fail(c, SALMON_ERR_ARG, "ac is NULL or nsteps < 0");
@@ -671,6 +671,10 @@ int salmon_gpu_rt_run(salmon_ctx* c, const double* ac, int nsteps, double dt, in
         if (jt) { jt[3 * n] = q[0]; jt[3 * n + 1] = q[1]; jt[3 * n + 2] = q[2]; }
         if (et) et[n] = q[3] + q[4] + q[5];
         if (norm) norm[n] = q[6];
+      }
+      if (rho) {
+        e = hipMemcpy(rho, c->d_rho, sizeof(double) * c->g.NL, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) r = fail(c, SALMON_ERR_HIP, "rt_run density: %s", hipGetErrorString(e));
       }
     }
   }

@@ -65,7 +65,7 @@ SIGNATURES = {
     "salmon_gpu_norm": [_vp, _vp],
     "salmon_gpu_gram_schmidt": [_vp],
     "salmon_gpu_sp_energy": [_vp, _vp, _vp],
-    "salmon_gpu_rt_run": [_vp, _vp, _ci, _cd, _ci, _vp, _vp, _vp],
+    "salmon_gpu_rt_run": [_vp, _vp, _ci, _cd, _ci, _vp, _vp, _vp, _vp],
     "salmon_gpu_set_path": [_vp, _ci],
     "salmon_gpu_path": [_vp],
     "salmon_gpu_synchronize": [_vp],
@@ -224,9 +224,10 @@ class Context:
         jt = np.zeros((nsteps, 3))
         et = np.zeros(nsteps)
         nt = np.zeros(nsteps)
+        rho = np.zeros(self.sys.NL)
         self._check(lib().salmon_gpu_rt_run(self._h, _p(ac), int(nsteps), float(self.sys.dt if dt is None else dt),
-                                            int(reortho_every), _p(jt), _p(et), _p(nt)), "rt_run")
-        return dict(J=jt, E=et, norm=nt)
+                                            int(reortho_every), _p(jt), _p(et), _p(nt), _p(rho)), "rt_run")
+        return dict(J=jt, E=et, norm=nt, rho=rho)
 
     def stream(self):
         """The context's hipStream_t (as an integer handle)."""

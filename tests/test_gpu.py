@@ -170,6 +170,7 @@ def test_fused_path_selected_and_matches_streamed():
     assert rel(rf["J"], rs["J"]) < 1e-11
     assert rel(rf["E"], rs["E"]) < TOL64
     assert rel(rf["J"], d["rt_J"][:20]) < 1e-11
+    assert rel(rf["rho"], rs["rho"]) < TOL64
 
 
 def test_fused_overlapping_spheres_against_oracle():
@@ -185,9 +186,8 @@ def test_fused_overlapping_spheres_against_oracle():
         assert c.path() == gpu.PATH_FUSED
         r = c.rt_run(ac, 3)
         z = c.get_orbitals()
-        rho = c.psi_rho_RT()
     o = Oracle(s).rt_run(ac, 3, want_rho=True)
     assert rel(z, o["zu"]) < TOL64
     assert rel(r["J"], o["J"]) < 1e-11
     assert rel(r["E"], o["E"]) < TOL64
-    assert rel(rho, o["rho"]) < TOL64
+    assert rel(r["rho"], o["rho"]) < TOL64  # density partials of the fused kernel
