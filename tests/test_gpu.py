@@ -176,7 +176,7 @@ def test_fused_path_selected_and_matches_streamed():
 def test_fused_overlapping_spheres_against_oracle():
     """Random atoms on the 16^3 grid: sphere points shared by two atoms go
     through the overlap fold of the fused kernel."""
-    s = synth.make_system(N=(16, 16, 16), L=10.26, nk3=(2, 1, 1), NB=3, seed=21, atoms="random", natoms=14)
+    s = synth.make_system(N=(16, 16, 16), L=10.26, nk3=(2, 1, 1), NB=3, seed=22, atoms="random", natoms=7)
     cover = np.zeros(s.NL, int)
     for a in range(s.NI):
         np.add.at(cover, s.Jxyz[a, : s.Mps[a]] - 1, 1)
