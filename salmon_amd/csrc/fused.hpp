@@ -70,6 +70,15 @@ __device__ __forceinline__ cplx<T> phase_at(const cplx<T>* ph, int code) {
   return cmul(cmul(ph[px], ph[5 * NX + py]), ph[5 * NX + 5 * NY + pz]);
 }
 
+// Returns p unchanged but opaque to the optimiser, so loads through it are
+// not hoisted out of the orbital and H loops (hoisted table values would stay
+// live in registers for the whole kernel).
+template <typename P>
+__device__ __forceinline__ P* opaque(P* p) {
+  asm volatile("" : "+s"(p));
+  return p;
+}
+
 // x neighbour o (own-point offset, -4..PX+3) from the register window; o is a
 // compile-time constant after unrolling, so the window stays in registers
 template <typename T, int NX, int H, int XS, int W>
@@ -177,6 +186,9 @@ __global__ void __launch_bounds__(NY * NZ * XS, WPE) k_fused_step(FusedArgs<T> a
           win[H + PX + i] = in[((x0 + PX + i) % NX) * NYZ + yz];
         }
       }
+      const int* jp_phase = opaque(a.jp_phase);
+      const int* jp_grid = opaque(a.jp_grid);
+      const T* uvt = opaque(a.uv);
       // KB term of pseudo_pt (hpsi.f90:176-206) per atom in one wave:
       //   uVpsi(l) = Hxyz iuV(l) sum_j uV(j,l) e^{ik.r_j} psi(J_j)       (projection)
       //   dps(j)   = e^{-ik.r_j} sum_l uV(j,l) uVpsi(l)                  (back-projection)
@@ -189,12 +201,12 @@ __global__ void __launch_bounds__(NY * NZ * XS, WPE) k_fused_step(FusedArgs<T> a
         for (int l = 0; l < kFusedMaxLmaPerAtom; ++l) { sr[l] = 0; si[l] = 0; }
         for (int j = lane; j < m; j += kWave) {
           const int jp = j0 + j;
-          const cplx<T> e = phase_at<T, NX, NY, NZ>(phm, a.jp_phase[jp]);
-          const cplx<T> w = cmul(e, in[a.jp_grid[jp]]);
+          const cplx<T> e = phase_at<T, NX, NY, NZ>(phm, jp_phase[jp]);
+          const cplx<T> w = cmul(e, in[jp_grid[jp]]);
 #pragma unroll
           for (int l = 0; l < kFusedMaxLmaPerAtom; ++l) {
             if (l < nl) {
-              const T u = a.uv[(size_t)(l0 + l) * a.Nps + j];
+              const T u = uvt[(size_t)(l0 + l) * a.Nps + j];
               sr[l] += u * w.re;
               si[l] += u * w.im;
             }
@@ -212,12 +224,12 @@ __global__ void __launch_bounds__(NY * NZ * XS, WPE) k_fused_step(FusedArgs<T> a
 #pragma unroll
           for (int l = 0; l < kFusedMaxLmaPerAtom; ++l) {
             if (l < nl) {
-              const T u = a.uv[(size_t)(l0 + l) * a.Nps + j];
+              const T u = uvt[(size_t)(l0 + l) * a.Nps + j];
               tr += u * sr[l];
               ti += u * si[l];
             }
           }
-          const cplx<T> e = phase_at<T, NX, NY, NZ>(phm, a.jp_phase[jp]);
+          const cplx<T> e = phase_at<T, NX, NY, NZ>(phm, jp_phase[jp]);
           dps[jp] = cplx<T>{e.re * tr + e.im * ti, e.re * ti - e.im * tr};  // conj(e) * t
         }
       }
@@ -238,25 +250,36 @@ __global__ void __launch_bounds__(NY * NZ * XS, WPE) k_fused_step(FusedArgs<T> a
         __syncthreads();
       }
       // stencil: h = (V+A) psi - 1/2 sum_a sum_n lap_n (psi+ + psi-) - i sum_a k_a sum_n nab_n (psi+ - psi-)
+      // One LDS base address per neighbour direction; the x offset of each own
+      // point is a compile-time constant that folds into the ds_read offset.
+      const cplx<T>* zb[8];
+      const cplx<T>* yb[8];
+#pragma unroll
+      for (int nn = 1; nn <= 4; ++nn) {
+        const cplx<T>* base = in + x0 * NYZ;
+        zb[nn - 1] = base + iy * NZ + (iz + nn) % NZ;
+        zb[nn + 3] = base + iy * NZ + (iz - nn + NZ) % NZ;
+        yb[nn - 1] = base + ((iy + nn) % NY) * NZ + iz;
+        yb[nn + 3] = base + ((iy - nn + NY) % NY) * NZ + iz;
+      }
+      const T* vrow = opaque(a.vloc) + x0 * NYZ + yz;
       cplx<T> h[PX];
 #pragma unroll
       for (int i = 0; i < PX; ++i) {
-        const int x = x0 + i;
         const cplx<T> e0 = win[H + i];
-        const T vAi = a.vloc[x * NYZ + yz] + A;
+        const T vAi = vrow[i * NYZ] + A;
         T lr = vAi * e0.re, li = vAi * e0.im;
         T gzr = 0, gzi = 0, gyr = 0, gyi = 0, gxr = 0, gxi = 0;
-        const cplx<T>* rowp = in + x * NYZ;
 #pragma unroll
         for (int nn = 1; nn <= 4; ++nn) {
-          const cplx<T> zp = rowp[iy * NZ + (iz + nn) % NZ];
-          const cplx<T> zm = rowp[iy * NZ + (iz - nn + NZ) % NZ];
+          const cplx<T> zp = zb[nn - 1][i * NYZ];
+          const cplx<T> zm = zb[nn + 3][i * NYZ];
           lr += a.lapc[2][nn] * (zp.re + zm.re);
           li += a.lapc[2][nn] * (zp.im + zm.im);
           gzr += a.nab[2][nn] * (zp.re - zm.re);
           gzi += a.nab[2][nn] * (zp.im - zm.im);
-          const cplx<T> yp = rowp[((iy + nn) % NY) * NZ + iz];
-          const cplx<T> ym = rowp[((iy - nn + NY) % NY) * NZ + iz];
+          const cplx<T> yp = yb[nn - 1][i * NYZ];
+          const cplx<T> ym = yb[nn + 3][i * NYZ];
           lr += a.lapc[1][nn] * (yp.re + ym.re);
           li += a.lapc[1][nn] * (yp.im + ym.im);
           gyr += a.nab[1][nn] * (yp.re - ym.re);
@@ -340,6 +363,10 @@ __global__ void __launch_bounds__(NY * NZ * XS, WPE) k_fused_step(FusedArgs<T> a
       const double zr = A0 * er - 0.5 * (double)lr + wi, zi = A0 * ei - 0.5 * (double)li - wr;
       tew += occ * (er * zr + ei * zi);
     }
+    const int* jp_phase_n = opaque(a.jp_phase);
+    const int* jp_grid_n = opaque(a.jp_grid);
+    const double* uv64 = opaque(a.uv64);
+    const double* jp_pos = opaque(a.jp_pos);
     // nonlocal current and energy (current.f90:247-269, total_energy.f90:191-199):
     // waves over (atom, projector)
     for (int at = wv; at < a.NI; at += NW) {
@@ -348,11 +375,11 @@ __global__ void __launch_bounds__(NY * NZ * XS, WPE) k_fused_step(FusedArgs<T> a
         double ur = 0, ui = 0, xr0 = 0, xi0 = 0, xr1 = 0, xi1 = 0, xr2 = 0, xi2 = 0;
         for (int j = lane; j < m; j += kWave) {
           const int jp = j0 + j;
-          const cplx<T> e = phase_at<T, NX, NY, NZ>(phn, a.jp_phase[jp]);
-          const cplx<T> w = cmul(e, out[a.jp_grid[jp]]);
-          const double u = a.uv64[(size_t)l * a.Nps + j];
+          const cplx<T> e = phase_at<T, NX, NY, NZ>(phn, jp_phase_n[jp]);
+          const cplx<T> w = cmul(e, out[jp_grid_n[jp]]);
+          const double u = uv64[(size_t)l * a.Nps + j];
           const double tr = u * (double)w.re, ti = u * (double)w.im;
-          const double px = a.jp_pos[3 * jp], py = a.jp_pos[3 * jp + 1], pz = a.jp_pos[3 * jp + 2];
+          const double px = jp_pos[3 * jp], py = jp_pos[3 * jp + 1], pz = jp_pos[3 * jp + 2];
           ur += tr; ui += ti;
           xr0 += px * tr; xi0 += px * ti;
           xr1 += py * tr; xi1 += py * ti;
@@ -410,12 +437,25 @@ __global__ void __launch_bounds__(NY * NZ * XS, WPE) k_fused_step(FusedArgs<T> a
 // host side (included into salmon_gpu.hip after the context definition)
 // ---------------------------------------------------------------------------
 // x split of the 16^3 kernels: threads per workgroup = 256 * XS
-constexpr int kFusedXS64 = 1;
-constexpr int kFusedXS32 = 1;
-// waves per SIMD the register allocation targets: the FP64 kernel's 152 KiB of
+// (build-time tunables, like the reference's stencil build options)
+#ifndef SALMON_FUSED_XS64
+#define SALMON_FUSED_XS64 1
+#endif
+#ifndef SALMON_FUSED_XS32
+#define SALMON_FUSED_XS32 1
+#endif
+// waves per SIMD the register allocation targets: the FP64 kernel's 154 KiB of
 // LDS allows one workgroup per CU, the complex64 kernel's 77 KiB two
-constexpr int kFusedWPE64 = 1;
-constexpr int kFusedWPE32 = 1;
+#ifndef SALMON_FUSED_WPE64
+#define SALMON_FUSED_WPE64 1
+#endif
+#ifndef SALMON_FUSED_WPE32
+#define SALMON_FUSED_WPE32 1
+#endif
+constexpr int kFusedXS64 = SALMON_FUSED_XS64;
+constexpr int kFusedXS32 = SALMON_FUSED_XS32;
+constexpr int kFusedWPE64 = SALMON_FUSED_WPE64;
+constexpr int kFusedWPE32 = SALMON_FUSED_WPE32;
 
 struct FusedState {
   int shape = 0;  // 1: 16^3
@@ -580,7 +620,11 @@ static int fused_step(salmon_ctx* c, const double amid[3], const double anew[3],
       if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && v > 0) ncu = v;
     }
   }
-  const int per_cu = sizeof(T) == 8 ? 1 : 2;
+  // workgroups resident per CU: LDS allows 1 (FP64) or 2 (complex64), the
+  // register target allows WPE*4/(threads/64)
+  const int threads = 256 * (sizeof(T) == 8 ? kFusedXS64 : kFusedXS32);
+  const int by_regs = std::max(1, (sizeof(T) == 8 ? kFusedWPE64 : kFusedWPE32) * 4 * 64 / threads);
+  const int per_cu = std::min(sizeof(T) == 8 ? 1 : 2, by_regs);
   const int target = 2 * ncu * per_cu;
   int bpw = (c->NKB + target - 1) / target;
   bpw = std::max(1, std::min(bpw, c->NB));
@@ -627,8 +671,6 @@ static int fused_step(salmon_ctx* c, const double amid[3], const double anew[3],
   if (r) return r;
   hipLaunchKernelGGL(salmon::k_sum_rows, dim3(1), dim3(256), 0, c->stream, f->wg_rows, nwg, res_dev);
   LAUNCHCHK(c);
-  hipLaunchKernelGGL(salmon::k_density_final, dim3((c->g.NL + 255) / 256), dim3(256), 0, c->stream, f->rho_part, nwg,
-                     c->g.NL, c->d_rho);
-  LAUNCHCHK(c);
+  return density_reduce(c, f->rho_part, nwg);
   return SALMON_OK;
 }

@@ -46,6 +46,7 @@ struct salmon_ctx {
   double* d_sum = nullptr;    // (8)
   double* d_rho = nullptr;    // (NL)
   double* d_rho_part = nullptr;
+  double* d_rho_part2 = nullptr;  // second stage of the density reduction
   int rho_chunks = 0;
   // single-orbital staging
   double* d_kone = nullptr;   // (3)
@@ -184,6 +185,26 @@ int observe_rows(salmon_ctx* c, const double* kbase, double ax, double ay, doubl
   return SALMON_OK;
 }
 
+// rho(p) = sum over nchunk partials (nchunk, NL), in a fixed order and with
+// enough threads: slices of <= 16 partials, then the sum of the slices
+constexpr int kRhoMaxSlices = 64;
+int density_reduce(salmon_ctx* c, const double* part, int nchunk) {
+  const int NL = c->g.NL;
+  if (!c->d_rho_part2) {
+    int r = dalloc(c, &c->d_rho_part2, (size_t)kRhoMaxSlices * NL);
+    if (r) return r;
+  }
+  const int per = std::max(16, (nchunk + kRhoMaxSlices - 1) / kRhoMaxSlices);
+  const int ns = (nchunk + per - 1) / per;
+  hipLaunchKernelGGL(k_density_slices, dim3((NL + 255) / 256, ns), dim3(256), 0, c->stream, part, nchunk, per, NL,
+                     c->d_rho_part2);
+  LAUNCHCHK(c);
+  hipLaunchKernelGGL(k_density_final, dim3((NL + 255) / 256), dim3(256), 0, c->stream, c->d_rho_part2, ns, NL,
+                     c->d_rho);
+  LAUNCHCHK(c);
+  return SALMON_OK;
+}
+
 template <typename T>
 int density_dev(salmon_ctx* c) {
   const int NL = c->g.NL;
@@ -197,10 +218,7 @@ int density_dev(salmon_ctx* c) {
   hipLaunchKernelGGL(k_density_part<T>, dim3((NL + 255) / 256, nch), dim3(256), 0, c->stream,
                      (const cplx<T>*)c->d_zu, c->NKB, NL, per, c->d_occ, c->d_rho_part);
   LAUNCHCHK(c);
-  hipLaunchKernelGGL(k_density_final, dim3((NL + 255) / 256), dim3(256), 0, c->stream, c->d_rho_part, nch, NL,
-                     c->d_rho);
-  LAUNCHCHK(c);
-  return SALMON_OK;
+  return density_reduce(c, c->d_rho_part, nch);
 }
 
 int gram_schmidt_dev(salmon_ctx* c) {
@@ -708,6 +726,7 @@ int salmon_gpu_finalize(salmon_ctx* c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   fused_free(c);
   dfree((void**)&c->d_vloc); dfree((void**)&c->d_vloc_f); dfree((void**)&c->d_rho); dfree((void**)&c->d_rho_part);
+  dfree((void**)&c->d_rho_part2);
   dfree((void**)&c->d_kac0); dfree((void**)&c->d_kac); dfree((void**)&c->d_occ); dfree((void**)&c->d_rows);
   dfree((void**)&c->d_sum); dfree((void**)&c->d_kone); dfree((void**)&c->d_oneocc); dfree(&c->d_zu);
   dfree((void**)&c->d_f64); dfree(&c->d_uvpsi);

@@ -313,6 +313,20 @@ __global__ void __launch_bounds__(256) k_density_final(const double* __restrict_
   rho[p] = r;
 }
 
+// First stage of the density reduction: slice s of the nchunk partials,
+// summed in chunk order -> part2(s, p).  The second stage (k_density_final over
+// the slices) completes a fixed-order sum.
+__global__ void __launch_bounds__(256) k_density_slices(const double* __restrict__ part, int nchunk, int per_slice,
+                                                        int NL, double* __restrict__ part2) {
+  const int p = blockIdx.x * 256 + threadIdx.x;
+  const int sl = blockIdx.y;
+  if (p >= NL) return;
+  const int c0 = sl * per_slice, c1 = min(nchunk, c0 + per_slice);
+  double r = 0;
+  for (int c = c0; c < c1; ++c) r += part[(size_t)c * NL + p];
+  part2[(size_t)sl * NL + p] = r;
+}
+
 // FP64 Gram-Schmidt of NB bands of each k (one block per k), modified form of
 // Gram_Schmidt_ompk (ARTED/GS/Gram_Schmidt.f90:41-51).
 __global__ void __launch_bounds__(256) k_gram_schmidt(cplx<double>* __restrict__ zu, int NB, int NL, double Hxyz) {
