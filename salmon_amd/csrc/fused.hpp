@@ -439,7 +439,7 @@ __global__ void __launch_bounds__(NY * NZ * XS, WPE) k_fused_step(FusedArgs<T> a
 // x split of the 16^3 kernels: threads per workgroup = 256 * XS
 // (build-time tunables, like the reference's stencil build options)
 #ifndef SALMON_FUSED_XS64
-#define SALMON_FUSED_XS64 1
+#define SALMON_FUSED_XS64 2
 #endif
 #ifndef SALMON_FUSED_XS32
 #define SALMON_FUSED_XS32 1
@@ -447,10 +447,10 @@ __global__ void __launch_bounds__(NY * NZ * XS, WPE) k_fused_step(FusedArgs<T> a
 // waves per SIMD the register allocation targets: the FP64 kernel's 154 KiB of
 // LDS allows one workgroup per CU, the complex64 kernel's 77 KiB two
 #ifndef SALMON_FUSED_WPE64
-#define SALMON_FUSED_WPE64 1
+#define SALMON_FUSED_WPE64 2
 #endif
 #ifndef SALMON_FUSED_WPE32
-#define SALMON_FUSED_WPE32 1
+#define SALMON_FUSED_WPE32 2
 #endif
 constexpr int kFusedXS64 = SALMON_FUSED_XS64;
 constexpr int kFusedXS32 = SALMON_FUSED_XS32;
