@@ -41,11 +41,49 @@ struct KBTables {
 
 constexpr int kWave = 64;
 
+// Wave-wide sum with DPP lane moves (VALU only; no LDS round trips):
+// rotations by 8, 4, 2, 1 inside each 16-lane row leave every lane with its
+// row sum; row_bcast:15 and row_bcast:31 then fold rows 0..3 into lane 63,
+// whose value is broadcast with readlane.  The summation order is fixed.
+namespace dpp {
+constexpr int kRowRor1 = 0x121, kRowRor2 = 0x122, kRowRor4 = 0x124, kRowRor8 = 0x128;
+constexpr int kRowBcast15 = 0x142, kRowBcast31 = 0x143;
+}  // namespace dpp
+
+template <int CTRL, int ROWS>
+__device__ __forceinline__ float dpp_mov(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROWS, 0xf, false));
+}
+
+template <int CTRL, int ROWS>
+__device__ __forceinline__ double dpp_mov(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffll), CTRL, ROWS, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, ROWS, 0xf, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+__device__ __forceinline__ float lane_read(float v, int lane) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+
+__device__ __forceinline__ double lane_read(double v, int lane) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), lane);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// requires all 64 lanes active
 template <typename T>
 __device__ __forceinline__ T wave_sum(T v) {
-#pragma unroll
-  for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
-  return v;
+  v += dpp_mov<dpp::kRowRor8, 0xf>(v);
+  v += dpp_mov<dpp::kRowRor4, 0xf>(v);
+  v += dpp_mov<dpp::kRowRor2, 0xf>(v);
+  v += dpp_mov<dpp::kRowRor1, 0xf>(v);
+  v += dpp_mov<dpp::kRowBcast15, 0xa>(v);
+  v += dpp_mov<dpp::kRowBcast31, 0xc>(v);
+  return lane_read(v, 63);
 }
 
 // Deterministic sum of NV values over a 256-thread block; the result is

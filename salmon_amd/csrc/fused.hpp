@@ -90,7 +90,7 @@ __device__ __forceinline__ cplx<T> win_at(const cplx<T> (&win)[W], int o) {
   }
 }
 
-template <typename T, int NX, int NY, int NZ, int XS, int WPE>
+template <typename T, int NX, int NY, int NZ, int XS, int WPE, bool OV>
 __global__ void __launch_bounds__(NY * NZ * XS, WPE) k_fused_step(FusedArgs<T> a) {
   constexpr int NYZ = NY * NZ, NT = NYZ * XS, NL = NX * NYZ;
   constexpr int PX = NX / XS;                 // own x points per thread
@@ -234,7 +234,7 @@ __global__ void __launch_bounds__(NY * NZ * XS, WPE) k_fused_step(FusedArgs<T> a
         }
       }
       __syncthreads();  // KB terms complete
-      if (a.NOV > 0) {
+      if constexpr (OV) {
         // points inside several spheres (overlapping atoms): fold the other
         // sphere points' terms into the first, in table order
         for (int o = t; o < a.NOV; o += NT) {
@@ -592,9 +592,9 @@ static size_t fused_lds_bytes() {
          sizeof(double) * 10 * ((NY * NZ * XS + 63) / 64);
 }
 
-template <typename T, int NX, int NY, int NZ, int XS, int WPE>
+template <typename T, int NX, int NY, int NZ, int XS, int WPE, bool OV>
 static int fused_launch(salmon_ctx* c, const salmon::FusedArgs<T>& args, int nwg) {
-  auto kern = salmon::k_fused_step<T, NX, NY, NZ, XS, WPE>;
+  auto kern = salmon::k_fused_step<T, NX, NY, NZ, XS, WPE, OV>;
   const size_t lds = fused_lds_bytes<T, NX, NY, NZ, XS>();
   static bool attr_set = false;
   if (!attr_set) {
@@ -664,10 +664,15 @@ static int fused_step(salmon_ctx* c, const double amid[3], const double anew[3],
   a.wg_rows = f->wg_rows;
   a.rho_part = f->rho_part;
   int r;
-  if constexpr (sizeof(T) == 8)
-    r = fused_launch<T, 16, 16, 16, kFusedXS64, kFusedWPE64>(c, a, nwg);
-  else
-    r = fused_launch<T, 16, 16, 16, kFusedXS32, kFusedWPE32>(c, a, nwg);
+  // the overlap fold is compiled only into the instantiation for cells whose
+  // atom spheres share grid points
+  if constexpr (sizeof(T) == 8) {
+    r = a.NOV > 0 ? fused_launch<T, 16, 16, 16, kFusedXS64, kFusedWPE64, true>(c, a, nwg)
+                  : fused_launch<T, 16, 16, 16, kFusedXS64, kFusedWPE64, false>(c, a, nwg);
+  } else {
+    r = a.NOV > 0 ? fused_launch<T, 16, 16, 16, kFusedXS32, kFusedWPE32, true>(c, a, nwg)
+                  : fused_launch<T, 16, 16, 16, kFusedXS32, kFusedWPE32, false>(c, a, nwg);
+  }
   if (r) return r;
   hipLaunchKernelGGL(salmon::k_sum_rows, dim3(1), dim3(256), 0, c->stream, f->wg_rows, nwg, res_dev);
   LAUNCHCHK(c);
