@@ -17,6 +17,8 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "_ref", "libsalmon_ref.so")
+# the same reference code with RT/hamiltonian.f90 replaced by the GPU drop-in
+LIB_GPU = os.path.join(HERE, "_ref", "libsalmon_ref_gpu.so")
 REF_SRC = "/root/reference/src"
 
 
@@ -24,11 +26,15 @@ def available():
     return os.path.exists(LIB)
 
 
-def build():
-    """Compile the reference hot path where the reference tree is present."""
+def build(gpu_dropin=True):
+    """Compile the reference hot path where the reference tree is present
+    (and, when the HIP library is built, its variant with the GPU drop-in)."""
     if not os.path.isdir(REF_SRC):
         return available()
     subprocess.check_call(["make", "-s", "-C", os.path.join(HERE, "ref")])
+    hip = os.path.join(os.path.dirname(HERE), "salmon_amd", "lib", "libsalmon_gpu.so")
+    if gpu_dropin and os.path.exists(hip):
+        subprocess.check_call(["make", "-s", "-C", os.path.join(HERE, "ref"), "gpu"])
     return available()
 
 
@@ -40,10 +46,10 @@ def _p(a):
     return a.ctypes.data_as(ctypes.c_void_p)
 
 
-def lib():
+def lib(path=None):
     global _lib
     if _lib is None:
-        L = ctypes.CDLL(LIB)
+        L = ctypes.CDLL(path or LIB)
         vp, ci, cd = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
         L.ref_setup.argtypes = [ci, ci, ci, cd, cd, cd, ci, ci, vp, vp, vp, ci, ci, vp, vp, vp, vp, vp,
                                 ci, vp, vp, vp, vp, cd, cd, vp]
@@ -65,11 +71,12 @@ def lib():
 
 
 class Ref:
-    def __init__(self, sys):
+    def __init__(self, sys, lib_path=None):
         global _owner
         if _owner is not None:
             raise RuntimeError("the reference keeps module globals: one system per process")
         _owner = self
+        lib(lib_path)
         self.sys = sys
         s = sys
         self._keep = [np.ascontiguousarray(x) for x in (s.kAc0, s.occ, s.vloc)]
