@@ -51,11 +51,39 @@ class HipEvents:
     def record(self, e, stream):
         assert self.L.hipEventRecord(e, ctypes.c_void_p(stream)) == 0
 
+    def copy_calibration(self, nbytes):
+        """One device-to-device copy of nbytes (the runtime's copy kernel reads
+        and writes nbytes): the known-size reference for the FETCH_SIZE and
+        WRITE_SIZE counters."""
+        L = self.L
+        L.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+        L.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+        L.hipFree.argtypes = [ctypes.c_void_p]
+        L.hipDeviceSynchronize.argtypes = []
+        a, b = ctypes.c_void_p(), ctypes.c_void_p()
+        assert L.hipMalloc(ctypes.byref(a), nbytes) == 0 and L.hipMalloc(ctypes.byref(b), nbytes) == 0
+        assert L.hipMemcpy(b, a, nbytes, 3) == 0  # hipMemcpyDeviceToDevice
+        assert L.hipDeviceSynchronize() == 0
+        L.hipFree(a)
+        L.hipFree(b)
+
     def elapsed_ms(self, a, b):
         assert self.L.hipEventSynchronize(b) == 0
         ms = ctypes.c_float()
         assert self.L.hipEventElapsedTime(ctypes.byref(ms), a, b) == 0
         return float(ms.value)
+
+
+def measured_traffic():
+    """HBM bytes per launch of the fused step kernels, from the committed
+    rocprofv3 FETCH_SIZE/WRITE_SIZE summary (profiles/), scaled by the same
+    counters read on a 1 GiB copy; {} when the summary is absent."""
+    path = os.path.join(ROOT, "profiles", "fused_step_16c_512k32b_traffic.json")
+    if not os.path.exists(path):
+        return {}
+    with open(path) as f:
+        d = json.load(f)
+    return {k: d[k]["hbm_bytes_per_launch"] for k in ("mixed", "fp64") if k in d}
 
 
 def workload(seed=0, nsteps=2000):
@@ -137,6 +165,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=20)
     ap.add_argument("--cpu-nk", type=int, default=128)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run one 1 GiB device copy (reference size for the HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the k-point sharding over several GPUs is not built "
@@ -144,6 +174,8 @@ def main():
         sys.exit(2)
     from salmon_amd import gpu
     ev = HipEvents()
+    if args.copy_calibration:
+        ev.copy_calibration(1 << 30)
     s, ac = workload(nsteps=args.warmup + args.steps + 1)
     # orthonormalise once in FP64 on the GPU; both precisions start from it
     with gpu.Context(s, gpu.FP64) as c:
@@ -163,10 +195,15 @@ def main():
     B32 = 12 * 8 * s.NL * nkb
     B64 = 12 * 16 * s.NL * nkb
     ach = B32 / (ms32 * 1e-3) / 1e9
+    traffic = measured_traffic()
     roof = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
-            "traffic": None,
-            "kernel": "whole RT step, mixed path (per-step average over the timed region)",
-            "fp64": {"achieved": B64 / (ms64 * 1e-3) / 1e9, "frac": B64 / (ms64 * 1e-3) / 1e9 / HBM_PEAK_GBS}}
+            "traffic": traffic.get("mixed"),
+            "kernel": "k_fused_step<float> (one launch per RT step; duration = timed region / steps, which "
+                      "also holds two small reduction kernels per step)",
+            "algorithmic_bytes_per_launch": B32,
+            "fp64": {"kernel": "k_fused_step<double>", "achieved": B64 / (ms64 * 1e-3) / 1e9,
+                     "frac": B64 / (ms64 * 1e-3) / 1e9 / HBM_PEAK_GBS, "algorithmic_bytes_per_launch": B64,
+                     "traffic": traffic.get("fp64")}}
     cpu = None
     if not args.no_cpu:
         try:
