@@ -143,16 +143,21 @@ __global__ void __launch_bounds__(NY * NZ * XS, WPE) k_fused_step(FusedArgs<T> a
   // the k dependence of the gradient term is applied after the per-axis sums
   const T kmx = (T)km[0], kmy = (T)km[1], kmz = (T)km[2];
   const T A = (T)((km[0] * km[0] + km[1] * km[1] + km[2] * km[2] - (g.lap[0][0] + g.lap[1][0] + g.lap[2][0])) * 0.5);
-  // the (first) sphere point covering each own grid point, or the zero entry NPI
-  int prim[PX];
+  // the (first) sphere point covering each own grid point, or the zero entry
+  // NPI; two 16-bit indices per register
+  int prim[(PX + 1) / 2];
+#pragma unroll
+  for (int i = 0; i < (PX + 1) / 2; ++i) prim[i] = 0;
 #pragma unroll
   for (int i = 0; i < PX; ++i) {
     const int q = a.pt_prim[(x0 + i) * NYZ + yz];
-    prim[i] = q >= 0 ? q : a.NPI;
+    prim[i >> 1] |= (q >= 0 ? q : a.NPI) << (16 * (i & 1));
   }
   if (t == 0) dps[a.NPI] = cplx<T>{0, 0};
   const double A0 = -(g.lap[0][0] + g.lap[1][0] + g.lap[2][0]) * 0.5;
-  double n2w = 0, vsw = 0, tew = 0, jsw[3] = {0, 0, 0}, jnl[3] = {0, 0, 0}, enl = 0, occsum = 0;
+  // per-wave FP64 accumulators of the workgroup's observables, in LDS:
+  // |psi|^2, V|psi|^2, kinetic, current stencil x/y/z, nonlocal current x/y/z, Enl
+  if (lane < 10) red[wv * 10 + lane] = 0.0;
   // density partial of this workgroup: each thread owns its points, so the
   // slice is updated without atomics
   double* rho = a.rho_part + (size_t)blockIdx.x * NL;
@@ -163,7 +168,6 @@ __global__ void __launch_bounds__(NY * NZ * XS, WPE) k_fused_step(FusedArgs<T> a
   for (int ib = b0; ib < b1; ++ib) {
     const size_t ikb = (size_t)ik * a.NB + ib;
     const double occ = a.occ[ikb];
-    occsum += occ;
     cplx<T>* zo = a.zu + ikb * NL;
     cplx<T> win[W], acc[PX];
 #pragma unroll
@@ -263,6 +267,12 @@ __global__ void __launch_bounds__(NY * NZ * XS, WPE) k_fused_step(FusedArgs<T> a
         yb[nn + 3] = base + ((iy - nn + NY) % NY) * NZ + iz;
       }
       const T* vrow = opaque(a.vloc) + x0 * NYZ + yz;
+      const T zr = (n == 1) ? a.z2r : ((n == 3) ? a.z4r : (T)0);
+      const T zi = (n == 0) ? a.z1i : ((n == 2) ? a.z3i : (T)0);
+      // Each result goes to LDS and into the Taylor sum at once; it replaces
+      // its input in the x window only when no later own point needs that
+      // input (a delay of 4 points; the first 4 of a periodic column wait to
+      // the end), which keeps few results live in registers.
       cplx<T> h[PX];
 #pragma unroll
       for (int i = 0; i < PX; ++i) {
@@ -291,26 +301,28 @@ __global__ void __launch_bounds__(NY * NZ * XS, WPE) k_fused_step(FusedArgs<T> a
           gxr += a.nab[0][nn] * (xp.re - xm.re);
           gxi += a.nab[0][nn] * (xp.im - xm.im);
         }
+        // KB term of the sphere point covering this point (branch-free:
+        // uncovered points read the zero entry dps[NPI])
+        const cplx<T> d = dps[(prim[i >> 1] >> (16 * (i & 1))) & 0xffff];
         // -i * (k.G)
-        h[i].re = lr + (kmx * gxi + kmy * gyi + kmz * gzi);
-        h[i].im = li - (kmx * gxr + kmy * gyr + kmz * gzr);
-      }
-      // add the KB term of the sphere point covering each own point (branch-free:
-      // uncovered points read the zero entry dps[NPI])
-#pragma unroll
-      for (int i = 0; i < PX; ++i) {
-        const cplx<T> d = dps[prim[i]];
-        h[i].re += d.re;
-        h[i].im += d.im;
-      }
-      const T zr = (n == 1) ? a.z2r : ((n == 3) ? a.z4r : (T)0);
-      const T zi = (n == 0) ? a.z1i : ((n == 2) ? a.z3i : (T)0);
-#pragma unroll
-      for (int i = 0; i < PX; ++i) {
+        h[i].re = lr + (kmx * gxi + kmy * gyi + kmz * gzi) + d.re;
+        h[i].im = li - (kmx * gxr + kmy * gyr + kmz * gzr) + d.im;
         out[(x0 + i) * NYZ + yz] = h[i];
-        win[H + i] = h[i];
         acc[i].re += zr * h[i].re - zi * h[i].im;
         acc[i].im += zr * h[i].im + zi * h[i].re;
+        if constexpr (XS > 1) {
+          if (i >= 4) win[H + i - 4] = h[i - 4];
+        } else {
+          if (i >= 8 && i - 4 < NX - 4) win[i - 4] = h[i - 4];
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < PX; ++i) {
+        if constexpr (XS > 1) {
+          if (i >= PX - 4) win[H + i] = h[i];
+        } else {
+          if (i < 4 || i >= NX - 4) win[i] = h[i];
+        }
       }
       __syncthreads();
       cplx<T>* tmp = in;
@@ -327,6 +339,7 @@ __global__ void __launch_bounds__(NY * NZ * XS, WPE) k_fused_step(FusedArgs<T> a
     __syncthreads();
     // observables at k + A(t+dt), FP64 sums; all neighbours from LDS (a rolled
     // loop keeps this sweep's registers low)
+    double n2w = 0, vsw = 0, tew = 0, jsw[3] = {0, 0, 0};
 #pragma unroll 1
     for (int i = 0; i < PX; ++i) {
       const int x = x0 + i;
@@ -363,6 +376,16 @@ __global__ void __launch_bounds__(NY * NZ * XS, WPE) k_fused_step(FusedArgs<T> a
       const double zr = A0 * er - 0.5 * (double)lr + wi, zi = A0 * ei - 0.5 * (double)li - wr;
       tew += occ * (er * zr + ei * zi);
     }
+    {
+      // this orbital's grid sums into the wave's accumulators
+      double v[6] = {n2w, vsw, tew, jsw[0], jsw[1], jsw[2]};
+#pragma unroll
+      for (int i = 0; i < 6; ++i) v[i] = wave_sum(v[i]);
+      if (lane == 0) {
+#pragma unroll
+        for (int i = 0; i < 6; ++i) red[wv * 10 + i] += v[i];
+      }
+    }
     const int* jp_phase_n = opaque(a.jp_phase);
     const int* jp_grid_n = opaque(a.jp_grid);
     const double* uv64 = opaque(a.uv64);
@@ -393,24 +416,17 @@ __global__ void __launch_bounds__(NY * NZ * XS, WPE) k_fused_step(FusedArgs<T> a
           const double sg = a.lma_sign[l], hx = g.Hxyz;
           const double Ur = ur * hx * sg, Ui = ui * hx * sg;
           const double f = occ / g.aLxyz * 2.0 * hx;
-          jnl[0] += f * (xr0 * Ui - xi0 * Ur);
-          jnl[1] += f * (xr1 * Ui - xi1 * Ur);
-          jnl[2] += f * (xr2 * Ui - xi2 * Ur);
-          enl += occ * (ur * ur + ui * ui) * hx * hx * sg;
+          red[wv * 10 + 6] += f * (xr0 * Ui - xi0 * Ur);
+          red[wv * 10 + 7] += f * (xr1 * Ui - xi1 * Ur);
+          red[wv * 10 + 8] += f * (xr2 * Ui - xi2 * Ur);
+          red[wv * 10 + 9] += occ * (ur * ur + ui * ui) * hx * hx * sg;
         }
       }
     }
     // the next orbital overwrites bufA only; bufB (psi(t+dt)) is read above
     // and is next written after the barrier that follows the next load.
   }
-  // per-workgroup sums
-  double v[10] = {n2w, vsw, tew, jsw[0], jsw[1], jsw[2], jnl[0], jnl[1], jnl[2], enl};
-#pragma unroll
-  for (int i = 0; i < 10; ++i) v[i] = wave_sum(v[i]);
-  if (lane == 0) {
-#pragma unroll
-    for (int i = 0; i < 10; ++i) red[wv * 10 + i] = v[i];
-  }
+  // per-workgroup sums of the per-wave accumulators, in wave order
   __syncthreads();
   if (t == 0) {
     double s[10];
@@ -419,6 +435,8 @@ __global__ void __launch_bounds__(NY * NZ * XS, WPE) k_fused_step(FusedArgs<T> a
       s[i] = 0;
       for (int w = 0; w < NW; ++w) s[i] += red[w * 10 + i];
     }
+    double occsum = 0;
+    for (int ib = b0; ib < b1; ++ib) occsum += a.occ[(size_t)ik * a.NB + ib];
     double* r = a.wg_rows + (size_t)blockIdx.x * 8;
     const double f = g.Hxyz / g.aLxyz;
 #pragma unroll
