@@ -36,7 +36,6 @@ struct FusedArgs {
   double am[3], an[3];
   T z1i, z2r, z3i, z4r;
   T lapc[3][5];          // -lap(n)/2 per axis (propagation) -- uniform, kept in SGPRs
-  T lap[3][5];           // lap(n) per axis (observables)
   T nab[3][5];           // nab(n) per axis
   // KB tables
   const int* pt_prim;    // (NL) first sphere point covering a grid point, or -1
@@ -339,7 +338,18 @@ __global__ void __launch_bounds__(NY * NZ * XS, WPE) k_fused_step(FusedArgs<T> a
     __syncthreads();
     // observables at k + A(t+dt), FP64 sums; all neighbours from LDS (a rolled
     // loop keeps this sweep's registers low)
-    double n2w = 0, vsw = 0, tew = 0, jsw[3] = {0, 0, 0};
+    // Forward correlations C(a,n) = sum_i conj(psi_i) psi_{i + n e_a} (FP64):
+    // the kinetic energy (total_energy_stencil), the -i k.grad energy term and
+    // the stencil current (current_stencil, forward neighbours) are linear in
+    // them:  sum_i Re conj(psi)(-1/2 L psi) = -sum lap_n Re C,
+    //        sum_i Re conj(psi)(-i w)       = 2 sum_a k_a sum_n nab_n Im C,
+    //        current stencil term (axis a)  = sum_n nab_n Im C(a,n).
+    double n2 = 0, vs = 0;
+    double cr[3][4], ci[3][4];
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+#pragma unroll
+      for (int nn = 0; nn < 4; ++nn) cr[d][nn] = ci[d][nn] = 0;
 #pragma unroll 1
     for (int i = 0; i < PX; ++i) {
       const int x = x0 + i;
@@ -347,43 +357,45 @@ __global__ void __launch_bounds__(NY * NZ * XS, WPE) k_fused_step(FusedArgs<T> a
       const cplx<T> e0 = out[p];
       const double er = (double)e0.re, ei = (double)e0.im;
       const double a2 = er * er + ei * ei;
-      n2w += occ * a2;
-      vsw += occ * a.vloc64[p] * a2;
+      n2 += a2;
+      vs += a.vloc64[p] * a2;
       rho[p] += occ * a2;
       const cplx<T>* rowp = out + x * NYZ;
-      T gr0 = 0, gi0 = 0, gr1 = 0, gi1 = 0, gr2 = 0, gi2 = 0, lr = 0, li = 0;
 #pragma unroll
       for (int nn = 1; nn <= 4; ++nn) {
         const cplx<T> zp = rowp[iy * NZ + (iz + nn) % NZ];
-        const cplx<T> zm = rowp[iy * NZ + (iz - nn + NZ) % NZ];
         const cplx<T> yp = rowp[((iy + nn) % NY) * NZ + iz];
-        const cplx<T> ym = rowp[((iy - nn + NY) % NY) * NZ + iz];
         const cplx<T> xp = out[((x + nn) % NX) * NYZ + yz];
-        const cplx<T> xm = out[((x - nn + NX) % NX) * NYZ + yz];
-        gr2 += a.nab[2][nn] * (zp.re - zm.re); gi2 += a.nab[2][nn] * (zp.im - zm.im);
-        gr1 += a.nab[1][nn] * (yp.re - ym.re); gi1 += a.nab[1][nn] * (yp.im - ym.im);
-        gr0 += a.nab[0][nn] * (xp.re - xm.re); gi0 += a.nab[0][nn] * (xp.im - xm.im);
-        lr += a.lap[2][nn] * (zp.re + zm.re) + a.lap[1][nn] * (yp.re + ym.re) + a.lap[0][nn] * (xp.re + xm.re);
-        li += a.lap[2][nn] * (zp.im + zm.im) + a.lap[1][nn] * (yp.im + ym.im) + a.lap[0][nn] * (xp.im + xm.im);
+        const double q[3][2] = {{xp.re, xp.im}, {yp.re, yp.im}, {zp.re, zp.im}};
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          cr[d][nn - 1] += er * q[d][0] + ei * q[d][1];
+          ci[d][nn - 1] += er * q[d][1] - ei * q[d][0];
+        }
       }
-      // sum_i Im(conj psi_i sum_n nab_n psi_{i+n}) = 1/2 sum_i Im(conj psi_i G_i)
-      const double G0r = gr0, G0i = gi0, G1r = gr1, G1i = gi1, G2r = gr2, G2i = gi2;
-      jsw[0] += occ * 0.5 * (er * G0i - ei * G0r);
-      jsw[1] += occ * 0.5 * (er * G1i - ei * G1r);
-      jsw[2] += occ * 0.5 * (er * G2i - ei * G2r);
-      const double wr = kn[0] * G0r + kn[1] * G1r + kn[2] * G2r;
-      const double wi = kn[0] * G0i + kn[1] * G1i + kn[2] * G2i;
-      const double zr = A0 * er - 0.5 * (double)lr + wi, zi = A0 * ei - 0.5 * (double)li - wr;
-      tew += occ * (er * zr + ei * zi);
     }
     {
-      // this orbital's grid sums into the wave's accumulators
-      double v[6] = {n2w, vsw, tew, jsw[0], jsw[1], jsw[2]};
+      // this orbital's sums into the wave's accumulators
+      double te = A0 * wave_sum(n2), js[3];
 #pragma unroll
-      for (int i = 0; i < 6; ++i) v[i] = wave_sum(v[i]);
+      for (int d = 0; d < 3; ++d) {
+        js[d] = 0;
+#pragma unroll
+        for (int nn = 0; nn < 4; ++nn) {
+          const double Cr = wave_sum(cr[d][nn]), Ci = wave_sum(ci[d][nn]);
+          te += -g.lap[d][nn + 1] * Cr + 2.0 * kn[d] * g.nab[d][nn + 1] * Ci;
+          js[d] += g.nab[d][nn + 1] * Ci;
+        }
+      }
+      const double n2s = wave_sum(n2), vss = wave_sum(vs);
       if (lane == 0) {
-#pragma unroll
-        for (int i = 0; i < 6; ++i) red[wv * 10 + i] += v[i];
+        red[wv * 10 + 0] += occ * n2s;
+        red[wv * 10 + 1] += occ * vss;
+        red[wv * 10 + 2] += occ * te;
+        // reference: jx = jx + jxt*occ with jxt = 2*sum Im(...) (current.f90:206-210)
+        red[wv * 10 + 3] += occ * js[0];
+        red[wv * 10 + 4] += occ * js[1];
+        red[wv * 10 + 5] += occ * js[2];
       }
     }
     const int* jp_phase_n = opaque(a.jp_phase);
@@ -669,7 +681,6 @@ static int fused_step(salmon_ctx* c, const double amid[3], const double anew[3],
   for (int d = 0; d < 3; ++d)
     for (int n = 0; n < 5; ++n) {
       a.lapc[d][n] = (T)(-0.5 * c->g.lap[d][n]);
-      a.lap[d][n] = (T)c->g.lap[d][n];
       a.nab[d][n] = (T)c->g.nab[d][n];
     }
   a.pt_prim = f->pt_prim; a.ov_start = f->ov_start; a.ov_jp = f->ov_jp; a.NOV = f->NOV; a.jp_atom = f->jp_atom; a.jp_j = f->jp_j; a.jp_grid = f->jp_grid;
