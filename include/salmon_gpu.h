@@ -115,6 +115,48 @@ int salmon_gpu_gram_schmidt(salmon_ctx *ctx);
 /* esp(NB,NK) = <psi|H|psi> Hxyz at kAc(NK,3): sp_energy_omp, ARTED/GS/sp_energy.f90:20-60. */
 int salmon_gpu_sp_energy(salmon_ctx *ctx, const double *kAc, double *esp);
 
+/* Preconditioner applied to the CG residual.  CG_ompk applies none
+ * (ARTED/GS/CG.f90:85-92); SALMON_PC_KINETIC multiplies the residual's
+ * Fourier components by the Teter-Payne-Allan factor f/(f+16x^4),
+ * f = 27+18x+12x^2+8x^3, x = (finite-difference kinetic symbol)/<x|T|x>, and
+ * conjugates with the preconditioned ratio <Kg|g>/<Kg_old|g_old>; it has no
+ * reference counterpart. */
+enum { SALMON_PC_NONE = 0, SALMON_PC_KINETIC = 1, SALMON_PC_LEARNED = 2 };
+
+/* One call of CG_ompk (ARTED/GS/CG.f90:34-139, skip_gsortho='n') on the
+ * resident orbitals at kAc(NK,3): for each band in turn, the start vector is
+ * orthogonalised to the lower bands and normalised, then up to iter_cg_max
+ * orthogonalised CG iterations with the 2x2 Rayleigh-Ritz update, leaving at
+ * |ev - <x|H|x>| < 1e-15.  esp_var(NB,NK) = occ ||Hx - <x|H|x> x|| sqrt(Hxyz)
+ * (CG.f90:122-126); iters(NB,NK) (nullable) = iterations each band used.
+ * The overlap of the start vector with the lower bands keeps only its real
+ * part, as the reference's real(8) variable does (CG.f90:46,70).  The
+ * coefficients of the reference's sequential Gram-Schmidt loops are formed
+ * from one projection on all lower bands and a triangular solve with their
+ * Gram matrix (equal up to rounding).  FP64 contexts, grid axes <= 64,
+ * NB <= 1024. */
+int salmon_gpu_cg(salmon_ctx *ctx, const double *kAc, int iter_cg_max, int precond, double *esp_var, int *iters);
+
+/* Subspace diagonalisation diag_omp (ARTED/GS/diag.f90:95-130): za(i,j) =
+ * <psi_i|H|psi_j> Hxyz hermitised as the reference does, its eigenvectors U
+ * (ascending eigenvalues), zu <- zu U, esp(NB,NK) = the eigenvalues.
+ * Eigenvector phases are those of a cyclic Jacobi solver (zheev's are
+ * implementation-defined as well). */
+int salmon_gpu_subspace_diag(salmon_ctx *ctx, const double *kAc, double *esp);
+
+/* Sweeps of the ground-state loop with a frozen potential
+ * (ARTED/control/ground_state.f90:136-140: Gram_Schmidt, diag_omp,
+ * Gram_Schmidt, CG_omp(iter_cg_max), Gram_Schmidt), each followed by the band
+ * energies <psi|H|psi> Hxyz (sp_energy_omp, ARTED/GS/sp_energy.f90:20-60),
+ * until no band energy moves by tol or more between two sweeps, or
+ * max_sweeps have run.  (The residual itself stalls near
+ * sqrt(1e-15 * gap): CG_ompk leaves a band once its Rayleigh quotient moves by
+ * less than 1e-15, CG.f90:42,114.)  Outputs (nullable): esp(NB,NK) = the
+ * band energies after the last sweep, esp_var of its CG call, sweeps = sweeps
+ * run. */
+int salmon_gpu_ground_state(salmon_ctx *ctx, const double *kAc, int max_sweeps, int iter_cg_max, int precond,
+                            double tol, double *esp, double *esp_var, int *sweeps);
+
 /* Device-resident RT loop: the time loop of tddft_sc
  * (ARTED/control/control_sc.f90:150-222) with the middlepoint propagator
  * (ARTED/RT/dt_evolve.f90:43-51,106-216) and a frozen potential.  ac holds

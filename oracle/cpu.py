@@ -58,6 +58,7 @@ def lib():
         L.or_gram_schmidt.argtypes = [P, vp]
         L.or_sp_energy.argtypes = [P, vp, vp, vp]
         L.or_cg.argtypes = [P, vp, vp, ctypes.c_int, ctypes.c_int, vp, vp]
+        L.or_kinetic_precondition.argtypes = [P, ctypes.c_double, vp]
         L.or_rt_run.argtypes = [P, vp, vp, ctypes.c_int, ctypes.c_double, vp, vp, vp, vp, vp, ctypes.c_int]
         L.or_num_threads.restype = ctypes.c_int
         _lib = L
@@ -168,6 +169,11 @@ class Oracle:
         it = np.zeros((self.sys.NK, self.sys.NB), np.int32)
         lib().or_cg(self.cp, _p(kAc), _p(zu), int(ncg), int(precond), _p(ev), _p(it))
         return zu, ev, it
+
+    def kinetic_precondition(self, tx, g):
+        g = np.array(g, dtype=np.complex128, order="C", copy=True)
+        lib().or_kinetic_precondition(self.cp, float(tx), _p(g))
+        return g
 
     def rt_run(self, ac, nsteps, zu=None, energy_every=1, want_rho=False):
         s = self.sys

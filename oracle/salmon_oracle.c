@@ -555,57 +555,70 @@ void or_sp_energy(const or_system *s, const double *kAc, const double *zu_, doub
  * wave and Tx the band's kinetic energy <x|T|x> (a Teter-Payne-Allan-type
  * scale).  Applied by a naive DFT per axis (separable), adequate for an oracle.
  */
-static void kinetic_precondition(const or_system *s, double Tx, zc *g)
+/*
+ * Kinetic preconditioner (no reference counterpart; CG_ompk applies none,
+ * GS/CG.f90:85-92): K g = F^-1[k(x) F g] with x = t(q)/Tx, t(q) the
+ * finite-difference kinetic symbol -1/2 sum_axes (lap0 + 2 sum_m lap_m
+ * cos(m 2 pi c/n)) and k(x) = f/(f + 16x^4), f = 27 + 18x + 12x^2 + 8x^3
+ * (the Teter-Payne-Allan form).  F: unnormalised DFT along z, then y, then x;
+ * F^-1: along x, y, z with 1/n per axis.  Twiddles come from the table
+ * (cos, sin)(2 pi j/n) indexed by (c*m) mod n.
+ */
+static void dft_axis(const or_system *s, int ax, int sgn, const zc *in, zc *out)
 {
+  const int N[3] = {s->NLx, s->NLy, s->NLz};
+  const int st[3] = {s->NLy * s->NLz, s->NLz, 1};
+  const int n = N[ax], NL = s->NL;
+  double *tc = (double *)malloc(sizeof(double) * 2 * n);
+  for (int j = 0; j < n; ++j) {
+    tc[2 * j] = cos(2.0 * M_PI * j / n);
+    tc[2 * j + 1] = sin(2.0 * M_PI * j / n);
+  }
+  for (int i = 0; i < NL; ++i) {
+    int c = (i / st[ax]) % n, base = i - c * st[ax];
+    zc t = zmk(0, 0);
+    for (int m = 0; m < n; ++m) {
+      int j = (c * m) % n;
+      t = zadd(t, zmul(zmk(tc[2 * j], sgn * tc[2 * j + 1]), in[base + m * st[ax]]));
+    }
+    out[i] = sgn > 0 ? zscale(1.0 / n, t) : t;
+  }
+  free(tc);
+}
+
+void or_kinetic_precondition(const or_system *s, double Tx, double *g_)
+{
+  zc *g = (zc *)g_;
   const int N[3] = {s->NLx, s->NLy, s->NLz};
   const double *lap[3] = {s->lapx, s->lapy, s->lapz};
   const int NL = s->NL;
+  const int st[3] = {s->NLy * s->NLz, s->NLz, 1};
   zc *a = (zc *)malloc(sizeof(zc) * NL), *b = (zc *)malloc(sizeof(zc) * NL);
-  memcpy(a, g, sizeof(zc) * NL);
-  int st[3] = {s->NLy * s->NLz, s->NLz, 1};
-  /* forward DFT along each axis */
+  double *symb[3];
   for (int ax = 0; ax < 3; ++ax) {
-    int n = N[ax];
-    for (int i = 0; i < NL; ++i) {
-      int c = (i / st[ax]) % n, base = i - c * st[ax];
-      zc t = zmk(0, 0);
-      for (int m = 0; m < n; ++m) {
-        double ph = -2.0 * M_PI * (double)c * m / n;
-        t = zadd(t, zmul(zmk(cos(ph), sin(ph)), a[base + m * st[ax]]));
-      }
-      b[i] = t;
-    }
-    memcpy(a, b, sizeof(zc) * NL);
-  }
-  double sc = Tx > 1e-12 ? Tx : 1e-12;
-  for (int i = 0; i < NL; ++i) {
-    double t = 0.0;
-    for (int ax = 0; ax < 3; ++ax) {
-      int c = (i / st[ax]) % N[ax];
+    symb[ax] = (double *)malloc(sizeof(double) * N[ax]);
+    for (int c = 0; c < N[ax]; ++c) {
       double th = 2.0 * M_PI * c / N[ax];
       double sym = lap[ax][0];
       for (int m = 1; m <= 4; ++m) sym += 2.0 * lap[ax][m] * cos(m * th);
-      t += -0.5 * sym;
+      symb[ax][c] = -0.5 * sym;
     }
+  }
+  dft_axis(s, 2, -1, g, a);
+  dft_axis(s, 1, -1, a, b);
+  dft_axis(s, 0, -1, b, a);
+  double sc = Tx > 1e-12 ? Tx : 1e-12;
+  for (int i = 0; i < NL; ++i) {
+    double t = (symb[0][(i / st[0]) % N[0]] + symb[1][(i / st[1]) % N[1]]) + symb[2][(i / st[2]) % N[2]];
     double x = t / sc;
-    double f = (27.0 + 18.0 * x + 12.0 * x * x + 8.0 * x * x * x);
+    double f = 27.0 + 18.0 * x + 12.0 * x * x + 8.0 * x * x * x;
     double kfac = f / (f + 16.0 * x * x * x * x);
     a[i] = zscale(kfac, a[i]);
   }
-  for (int ax = 0; ax < 3; ++ax) {
-    int n = N[ax];
-    for (int i = 0; i < NL; ++i) {
-      int c = (i / st[ax]) % n, base = i - c * st[ax];
-      zc t = zmk(0, 0);
-      for (int m = 0; m < n; ++m) {
-        double ph = 2.0 * M_PI * (double)c * m / n;
-        t = zadd(t, zmul(zmk(cos(ph), sin(ph)), a[base + m * st[ax]]));
-      }
-      b[i] = zscale(1.0 / n, t);
-    }
-    memcpy(a, b, sizeof(zc) * NL);
-  }
-  memcpy(g, a, sizeof(zc) * NL);
+  dft_axis(s, 0, +1, a, b);
+  dft_axis(s, 1, +1, b, a);
+  dft_axis(s, 2, +1, a, g);
+  for (int ax = 0; ax < 3; ++ax) free(symb[ax]);
   free(a);
   free(b);
 }
@@ -660,7 +673,7 @@ void or_cg(const or_system *s, const double *kAc, double *zu_, int ncg, int prec
           zc *dir = gk;
           if (precond == 1) {
             memcpy(kg, gk, sizeof(zc) * NL);
-            kinetic_precondition(s, xkTxk, kg);
+            or_kinetic_precondition(s, xkTxk, (double *)kg);
             for (int ibt = 0; ibt < ib; ++ibt) {
               zc zs = zscale(Hxyz, zdot(b0 + (size_t)ibt * NL, kg, NL));
               for (int i = 0; i < NL; ++i) kg[i] = zsub(kg[i], zmul(b0[(size_t)ibt * NL + i], zs));

@@ -1,0 +1,372 @@
+// Host side of the ground-state path: CG_ompk (ARTED/GS/CG.f90:34-139) on
+// the resident orbitals with an optional preconditioner, the subspace
+// diagonalisation diag_omp (ARTED/GS/diag.f90:95-130), and the sweep loop of
+// the ground-state driver (ARTED/control/ground_state.f90:136-140) with a
+// frozen potential.  Included by salmon_gpu.hip after its helpers.
+#pragma once
+#include "cg.hpp"
+
+using namespace salmon::cg;
+
+struct GsState {
+  int NK = 0, NB = 0, NL = 0, Nlma = 0, nchunk = 0, ngroup = 0;
+  std::vector<void*> bufs;
+  cplx<double>* w[11] = {};  // X HX TX G P PKO HP TP KG T1 T2, NK*NL each
+  void* uv = nullptr;        // NK*Nlma
+  double* sc = nullptr;
+  int* done = nullptr;
+  int* iters = nullptr;
+  double* esp_var = nullptr;
+  double* part = nullptr;
+  unsigned* cnt = nullptr;
+  cplx<double>* zs = nullptr;
+  cplx<double>* gram = nullptr;
+  double* tw = nullptr;    // DFT twiddles (3*kDftMax complex)
+  double* symb = nullptr;  // kinetic symbol per axis (3*kDftMax)
+  int pstride = 0;
+  // subspace matrices
+  double* dpart = nullptr;
+  size_t dpart_n = 0;
+  double* dM = nullptr;
+  cplx<double>* dU = nullptr;
+  int dk = 0;  // k points per subspace pass
+};
+
+enum { W_X = 0, W_HX, W_TX, W_G, W_P, W_PKO, W_HP, W_TP, W_KG, W_T1, W_T2 };
+
+static void gs_free(salmon_ctx* c) {
+  GsState* s = (GsState*)c->gs;
+  if (!s) return;
+  for (void* p : s->bufs) (void)hipFree(p);
+  delete s;
+  c->gs = nullptr;
+}
+
+template <typename X>
+static int gs_alloc(salmon_ctx* c, GsState* s, X** p, size_t n) {
+  if (n == 0) n = 1;
+  hipError_t e = hipMalloc((void**)p, n * sizeof(X));
+  if (e != hipSuccess) return fail(c, SALMON_ERR_NOMEM, "ground-state buffers (%zu bytes): %s", n * sizeof(X), hipGetErrorString(e));
+  s->bufs.push_back((void*)*p);
+  return SALMON_OK;
+}
+
+// Finite-difference kinetic symbol of one axis, -1/2 (lap0 + 2 sum_m lap_m
+// cos(m 2 pi c/n)), c = 0..n-1, and the DFT twiddles (cos, sin)(2 pi j/n).
+static void kinetic_tables(const Grid& g, std::vector<double>& tw, std::vector<double>& symb) {
+  tw.assign(2 * 3 * kDftMax, 0.0);
+  symb.assign(3 * kDftMax, 0.0);
+  const int N[3] = {g.NLx, g.NLy, g.NLz};
+  for (int ax = 0; ax < 3; ++ax) {
+    const int n = N[ax];
+    for (int j = 0; j < n; ++j) {
+      const double th = 2.0 * M_PI * j / n;
+      tw[2 * (ax * kDftMax + j)] = cos(th);
+      tw[2 * (ax * kDftMax + j) + 1] = sin(th);
+      double sym = g.lap[ax][0];
+      for (int m = 1; m <= 4; ++m) sym += 2.0 * g.lap[ax][m] * cos(m * th);
+      symb[ax * kDftMax + j] = -0.5 * sym;
+    }
+  }
+}
+
+static int gs_ensure(salmon_ctx* c) {
+  GsState* s = (GsState*)c->gs;
+  if (s && s->NK == c->NK && s->NB == c->NB && s->Nlma == c->kb.Nlma) return SALMON_OK;
+  gs_free(c);
+  s = new GsState();
+  c->gs = s;
+  s->NK = c->NK;
+  s->NB = c->NB;
+  s->NL = c->g.NL;
+  s->Nlma = c->kb.Nlma;
+  s->nchunk = (s->NL + kChunk - 1) / kChunk;
+  s->ngroup = (s->NB + kJG - 1) / kJG;
+  s->pstride = s->nchunk * s->ngroup * kPartQ;
+  const size_t nv = (size_t)s->NK * s->NL;
+  int r;
+  for (int i = 0; i < 11; ++i)
+    if ((r = gs_alloc(c, s, &s->w[i], nv))) return r;
+  if ((r = gs_alloc(c, s, (cplx<double>**)&s->uv, (size_t)s->NK * std::max(1, c->kb.Nlma)))) return r;
+  if ((r = gs_alloc(c, s, &s->sc, (size_t)s->NK * kNS))) return r;
+  if ((r = gs_alloc(c, s, &s->done, (size_t)s->NK))) return r;
+  if ((r = gs_alloc(c, s, &s->iters, (size_t)s->NK * s->NB))) return r;
+  if ((r = gs_alloc(c, s, &s->esp_var, (size_t)s->NK * s->NB))) return r;
+  if ((r = gs_alloc(c, s, &s->part, (size_t)s->NK * s->pstride))) return r;
+  if ((r = gs_alloc(c, s, &s->cnt, (size_t)s->NK))) return r;
+  if ((r = gs_alloc(c, s, &s->zs, (size_t)s->NK * s->NB))) return r;
+  if ((r = gs_alloc(c, s, &s->gram, (size_t)s->NK * s->NB * s->NB))) return r;
+  if ((r = gs_alloc(c, s, &s->tw, (size_t)2 * 3 * kDftMax))) return r;
+  if ((r = gs_alloc(c, s, &s->symb, (size_t)3 * kDftMax))) return r;
+  HIPCHK(c, hipMemsetAsync(s->cnt, 0, sizeof(unsigned) * s->NK, c->stream));
+  HIPCHK(c, hipMemsetAsync(s->sc, 0, sizeof(double) * s->NK * kNS, c->stream));
+  HIPCHK(c, hipMemsetAsync(s->done, 0, sizeof(int) * s->NK, c->stream));
+  std::vector<double> tw, symb;
+  kinetic_tables(c->g, tw, symb);
+  HIPCHK(c, hipMemcpyAsync(s->tw, tw.data(), sizeof(double) * tw.size(), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(s->symb, symb.data(), sizeof(double) * symb.size(), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return SALMON_OK;
+}
+
+static State gs_view(salmon_ctx* c, GsState* s) {
+  State S{};
+  S.sc = s->sc;
+  S.done = s->done;
+  S.iters = s->iters;
+  S.esp_var = s->esp_var;
+  S.part = s->part;
+  S.cnt = s->cnt;
+  S.zs = s->zs;
+  S.gram = s->gram;
+  S.occ = c->d_occ;
+  S.NK = s->NK;
+  S.NB = s->NB;
+  S.NL = s->NL;
+  S.nchunk = s->nchunk;
+  S.pstride = s->pstride;
+  S.Hxyz = c->g.Hxyz;
+  return S;
+}
+
+// K g for the kinetic preconditioner: five DFT passes g -> T1 -> T2 -> T1 -> T2 -> out.
+static int kinetic_apply(salmon_ctx* c, GsState* s, const State& S, const cplx<double>* g, cplx<double>* out) {
+  const Grid& G = c->g;
+  cplx<double>* t1 = s->w[W_T1];
+  cplx<double>* t2 = s->w[W_T2];
+  auto pass = [&](const cplx<double>* in, cplx<double>* o, int axis, int dir, const double* symb) -> int {
+    const int n = axis == 2 ? G.NLz : (axis == 1 ? G.NLy : G.NLx);
+    const int lpb = 256 / n;
+    const int nl = G.NL / n;
+    hipLaunchKernelGGL(k_dft_pass, dim3((nl + lpb - 1) / lpb, s->NK), dim3(256), 0, c->stream, S, in, o, G, axis, dir,
+                       s->tw, symb, 1);
+    LAUNCHCHK(c);
+    return SALMON_OK;
+  };
+  int r;
+  if ((r = pass(g, t1, 2, -1, nullptr))) return r;
+  if ((r = pass(t1, t2, 1, -1, nullptr))) return r;
+  if ((r = pass(t2, t1, 0, -1, s->symb))) return r;
+  if ((r = pass(t1, t2, 1, +1, nullptr))) return r;
+  return pass(t2, out, 2, +1, nullptr);
+}
+
+// CG_ompk over all bands of all k (FP64 orbitals).
+static int cg_run(salmon_ctx* c, int ncg, int precond) {
+  GsState* s = (GsState*)c->gs;
+  State S = gs_view(c, s);
+  cplx<double>* zu = (cplx<double>*)c->d_zu;
+  cplx<double>** w = s->w;
+  const int NK = s->NK, NB = s->NB, NL = s->NL;
+  const dim3 gv(s->nchunk, NK), blk(256);
+  const size_t bstride = (size_t)NB * NL;
+  int r;
+  auto H = [&](const cplx<double>* in, cplx<double>* out, cplx<double>* tt) {
+    return apply_h<double>(c, in, out, tt, 0, NK, 1, NK, c->d_kac, 0, 0, 0, s->uv);
+  };
+  auto proj = [&](int ib, Src src, int mode, int check) -> int {
+    if (ib == 0) return SALMON_OK;
+    hipLaunchKernelGGL(k_cg_proj, dim3(s->nchunk, (ib + kJG - 1) / kJG, NK), blk, 0, c->stream, S,
+                       (const cplx<double>*)zu, ib, src, mode, check);
+    LAUNCHCHK(c);
+    return SALMON_OK;
+  };
+  const Src resid{nullptr, 0, w[W_HX], w[W_X]};
+  for (int ib = 0; ib < NB; ++ib) {
+    const Src start{zu + (size_t)ib * NL, bstride, nullptr, nullptr};
+    // start vector orthogonalised to the lower bands and normalised (CG.f90:67-74)
+    if ((r = proj(ib, start, PROJ_START, 0))) return r;
+    hipLaunchKernelGGL(k_cg_sub, gv, blk, 0, c->stream, S, (const cplx<double>*)zu, ib, start, w[W_X],
+                       (const cplx<double>*)nullptr, (int)SUB_START, 0, ncg);
+    LAUNCHCHK(c);
+    hipLaunchKernelGGL(k_cg_scale, gv, blk, 0, c->stream, S, w[W_X], (cplx<double>*)nullptr, (size_t)0, 0);
+    LAUNCHCHK(c);
+    if ((r = H(w[W_X], w[W_HX], w[W_TX]))) return r;
+    hipLaunchKernelGGL(k_cg_xhx, gv, blk, 0, c->stream, S, w[W_X], w[W_HX], w[W_TX]);
+    LAUNCHCHK(c);
+    for (int it = 1; it <= ncg; ++it) {
+      // residual orthogonalised to the lower bands (CG.f90:85-90)
+      if ((r = proj(ib, resid, PROJ_VEC, 1))) return r;
+      const cplx<double>* dir = w[W_G];
+      if (precond == SALMON_PC_NONE) {
+        hipLaunchKernelGGL(k_cg_sub, gv, blk, 0, c->stream, S, (const cplx<double>*)zu, ib, resid, w[W_G],
+                           (const cplx<double>*)nullptr, (int)SUB_GRAD, it, ncg);
+        LAUNCHCHK(c);
+      } else {
+        hipLaunchKernelGGL(k_cg_sub, gv, blk, 0, c->stream, S, (const cplx<double>*)zu, ib, resid, w[W_G],
+                           (const cplx<double>*)nullptr, (int)SUB_NONE, it, ncg);
+        LAUNCHCHK(c);
+        if ((r = kinetic_apply(c, s, S, w[W_G], w[W_KG]))) return r;
+        const Src kg{w[W_KG], (size_t)NL, nullptr, nullptr};
+        if ((r = proj(ib, kg, PROJ_VEC, 1))) return r;
+        hipLaunchKernelGGL(k_cg_sub, gv, blk, 0, c->stream, S, (const cplx<double>*)zu, ib, kg, w[W_KG],
+                           (const cplx<double>*)w[W_G], (int)SUB_PCDIR, it, ncg);
+        LAUNCHCHK(c);
+        dir = w[W_KG];
+      }
+      // search direction, orthogonalised to x and normalised (CG.f90:92-106)
+      hipLaunchKernelGGL(k_cg_pupd, gv, blk, 0, c->stream, S, dir, w[W_P], (const cplx<double>*)w[W_X]);
+      LAUNCHCHK(c);
+      hipLaunchKernelGGL(k_cg_pko, gv, blk, 0, c->stream, S, (const cplx<double>*)w[W_P], (const cplx<double>*)w[W_X],
+                         w[W_PKO]);
+      LAUNCHCHK(c);
+      hipLaunchKernelGGL(k_cg_scale, gv, blk, 0, c->stream, S, w[W_PKO], (cplx<double>*)nullptr, (size_t)0, 1);
+      LAUNCHCHK(c);
+      if ((r = H(w[W_PKO], w[W_HP], w[W_TP]))) return r;
+      // 2x2 Rayleigh-Ritz and the update (CG.f90:107-119)
+      hipLaunchKernelGGL(k_cg_rr, gv, blk, 0, c->stream, S, (const cplx<double>*)w[W_X], (const cplx<double>*)w[W_PKO],
+                         (const cplx<double>*)w[W_HP], ib, it);
+      LAUNCHCHK(c);
+      hipLaunchKernelGGL(k_cg_rotate, gv, blk, 0, c->stream, S, w[W_X], w[W_HX], w[W_TX],
+                         (const cplx<double>*)w[W_PKO], (const cplx<double>*)w[W_HP], (const cplx<double>*)w[W_TP]);
+      LAUNCHCHK(c);
+    }
+    // store the normalised band and its residual (CG.f90:122-126)
+    hipLaunchKernelGGL(k_cg_norm, gv, blk, 0, c->stream, S, (const cplx<double>*)w[W_X]);
+    LAUNCHCHK(c);
+    hipLaunchKernelGGL(k_cg_scale, gv, blk, 0, c->stream, S, w[W_X], zu + (size_t)ib * NL, bstride, 0);
+    LAUNCHCHK(c);
+    // row ib of the lower bands' Gram matrix, for the later bands' projections
+    if (ib + 1 < NB && (r = proj(ib, Src{zu + (size_t)ib * NL, bstride, nullptr, nullptr}, PROJ_GRAM, 0))) return r;
+    if ((r = H(w[W_X], w[W_HP], w[W_TP]))) return r;
+    hipLaunchKernelGGL(k_cg_efin, gv, blk, 0, c->stream, S, (const cplx<double>*)w[W_X], (const cplx<double>*)w[W_HP]);
+    LAUNCHCHK(c);
+    hipLaunchKernelGGL(k_cg_resid, gv, blk, 0, c->stream, S, (const cplx<double>*)w[W_X], (const cplx<double>*)w[W_HP], ib);
+    LAUNCHCHK(c);
+  }
+  return SALMON_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Eigenpairs of a Hermitian matrix (row-major n x n) by cyclic Jacobi
+// rotations; eigenvalues ascending, eigenvectors as the columns of V.
+// ---------------------------------------------------------------------------
+struct zc64 {
+  double re, im;
+};
+
+static void jacobi_herm(int n, std::vector<zc64>& A, std::vector<double>& w, std::vector<zc64>& V) {
+  auto a = [&](int i, int j) -> zc64& { return A[(size_t)i * n + j]; };
+  V.assign((size_t)n * n, zc64{0, 0});
+  for (int i = 0; i < n; ++i) V[(size_t)i * n + i].re = 1.0;
+  double fro = 0.0;
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j) fro += a(i, j).re * a(i, j).re + a(i, j).im * a(i, j).im;
+  for (int sweep = 0; sweep < 100; ++sweep) {
+    double off = 0.0;
+    for (int i = 0; i < n; ++i)
+      for (int j = i + 1; j < n; ++j) off += a(i, j).re * a(i, j).re + a(i, j).im * a(i, j).im;
+    if (off <= 1e-30 * fro || off == 0.0) break;
+    for (int p = 0; p < n; ++p) {
+      for (int q = p + 1; q < n; ++q) {
+        const zc64 apq = a(p, q);
+        const double gabs = std::hypot(apq.re, apq.im);
+        if (gabs == 0.0) continue;
+        const zc64 e{apq.re / gabs, apq.im / gabs};  // a_pq = g e
+        const double tau = (a(q, q).re - a(p, p).re) / (2.0 * gabs);
+        const double t = (tau >= 0 ? 1.0 : -1.0) / (std::fabs(tau) + std::sqrt(1.0 + tau * tau));
+        const double cs = 1.0 / std::sqrt(1.0 + t * t), sn = t * cs;
+        // U: U_pp = c, U_pq = s e, U_qp = -s conj(e), U_qq = c;  A <- U^H A U, V <- V U
+        for (int k = 0; k < n; ++k) {  // columns: A U
+          const zc64 akp = a(k, p), akq = a(k, q);
+          // akp' = c akp - s conj(e) akq ; akq' = s e akp + c akq
+          a(k, p) = zc64{cs * akp.re - sn * (e.re * akq.re + e.im * akq.im), cs * akp.im - sn * (e.re * akq.im - e.im * akq.re)};
+          a(k, q) = zc64{sn * (e.re * akp.re - e.im * akp.im) + cs * akq.re, sn * (e.re * akp.im + e.im * akp.re) + cs * akq.im};
+          const zc64 vkp = V[(size_t)k * n + p], vkq = V[(size_t)k * n + q];
+          V[(size_t)k * n + p] = zc64{cs * vkp.re - sn * (e.re * vkq.re + e.im * vkq.im), cs * vkp.im - sn * (e.re * vkq.im - e.im * vkq.re)};
+          V[(size_t)k * n + q] = zc64{sn * (e.re * vkp.re - e.im * vkp.im) + cs * vkq.re, sn * (e.re * vkp.im + e.im * vkp.re) + cs * vkq.im};
+        }
+        for (int k = 0; k < n; ++k) {  // rows: U^H B
+          const zc64 bpk = a(p, k), bqk = a(q, k);
+          // bpk' = c bpk - s e bqk ; bqk' = s conj(e) bpk + c bqk
+          a(p, k) = zc64{cs * bpk.re - sn * (e.re * bqk.re - e.im * bqk.im), cs * bpk.im - sn * (e.re * bqk.im + e.im * bqk.re)};
+          a(q, k) = zc64{sn * (e.re * bpk.re + e.im * bpk.im) + cs * bqk.re, sn * (e.re * bpk.im - e.im * bpk.re) + cs * bqk.im};
+        }
+        a(p, q) = zc64{0, 0};
+        a(q, p) = zc64{0, 0};
+        a(p, p).im = 0.0;
+        a(q, q).im = 0.0;
+      }
+    }
+  }
+  std::vector<int> ord(n);
+  for (int i = 0; i < n; ++i) ord[i] = i;
+  std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) { return a(x, x).re < a(y, y).re; });
+  std::vector<zc64> W((size_t)n * n);
+  w.resize(n);
+  for (int i = 0; i < n; ++i) {
+    w[i] = a(ord[i], ord[i]).re;
+    for (int k = 0; k < n; ++k) W[(size_t)k * n + i] = V[(size_t)k * n + ord[i]];
+  }
+  V.swap(W);
+}
+
+// diag_omp: za = <psi_i|H psi_j> Hxyz, eigenvectors U, zu <- zu U, esp = eigenvalues.
+static int subspace_diag(salmon_ctx* c, double* esp) {
+  GsState* s = (GsState*)c->gs;
+  const int NK = c->NK, NB = c->NB, NL = c->g.NL;
+  int r;
+  if ((r = ensure_work(c))) return r;
+  if (c->chunk < NB) return fail(c, SALMON_ERR_UNSUPPORTED, "subspace diagonalisation needs %d orbitals of work space", NB);
+  const int nslice = std::max(1, std::min(32, NL / 1024));
+  const int pts = ((NL + nslice - 1) / nslice + kSlicePts - 1) / kSlicePts * kSlicePts;
+  const size_t per_k = (size_t)nslice * NB * NB * 2;
+  int dk = std::max(1, std::min({NK, c->chunk / NB, (int)std::max<size_t>(1, ((size_t)32 << 20) / (per_k * 8))}));
+  if (!s->dpart || s->dk != dk) {
+    if ((r = gs_alloc(c, s, &s->dpart, per_k * dk))) return r;
+    if ((r = gs_alloc(c, s, &s->dM, (size_t)NK * NB * NB * 2))) return r;
+    if ((r = gs_alloc(c, s, &s->dU, (size_t)NK * NB * NB))) return r;
+    s->dk = dk;
+  }
+  cplx<double>* zu = (cplx<double>*)c->d_zu;
+  cplx<double>* hb = (cplx<double>*)c->d_work[0];
+  const int ntile = (NB + kTile - 1) / kTile;
+  for (int k0 = 0; k0 < NK; k0 += dk) {
+    const int kn = std::min(dk, NK - k0);
+    cplx<double>* zk = zu + (size_t)k0 * NB * NL;
+    if ((r = apply_h<double>(c, zk, hb, nullptr, k0 * NB, kn * NB, NB, NK, c->d_kac, 0, 0, 0))) return r;
+    hipLaunchKernelGGL(k_sub_matrix, dim3(ntile * ntile, nslice, kn), dim3(256), 0, c->stream,
+                       (const cplx<double>*)zk, (const cplx<double>*)hb, NB, NL, pts, s->dpart);
+    LAUNCHCHK(c);
+    hipLaunchKernelGGL(k_sub_matrix_sum, dim3((NB * NB * 2 + 255) / 256, kn), dim3(256), 0, c->stream,
+                       (const double*)s->dpart, NB, nslice, c->g.Hxyz, s->dM + (size_t)k0 * NB * NB * 2);
+    LAUNCHCHK(c);
+  }
+  std::vector<double> M((size_t)NK * NB * NB * 2);
+  HIPCHK(c, hipMemcpyAsync(M.data(), s->dM, sizeof(double) * M.size(), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  std::vector<zc64> Uall((size_t)NK * NB * NB);
+  for (int k = 0; k < NK; ++k) {
+    const double* m = &M[(size_t)k * NB * NB * 2];
+    // za(ib2,ib1) = <ib2|H|ib1> for ib2 > ib1, za(ib1,ib2) its conjugate,
+    // za(ib1,ib1) = Re<ib1|H|ib1> (diag.f90:111-117)
+    std::vector<zc64> A((size_t)NB * NB);
+    for (int i = 0; i < NB; ++i) {
+      A[(size_t)i * NB + i] = zc64{m[2 * ((size_t)i * NB + i)], 0.0};
+      for (int j = 0; j < i; ++j) {
+        const zc64 v{m[2 * ((size_t)i * NB + j)], m[2 * ((size_t)i * NB + j) + 1]};
+        A[(size_t)i * NB + j] = v;
+        A[(size_t)j * NB + i] = zc64{v.re, -v.im};
+      }
+    }
+    std::vector<double> w;
+    std::vector<zc64> V;
+    jacobi_herm(NB, A, w, V);
+    for (int i = 0; i < NB; ++i) {
+      if (esp) esp[(size_t)k * NB + i] = w[i];
+      for (int j = 0; j < NB; ++j) Uall[((size_t)k * NB + i) * NB + j] = V[(size_t)j * NB + i];  // column i
+    }
+  }
+  HIPCHK(c, hipMemcpyAsync(s->dU, Uall.data(), sizeof(zc64) * Uall.size(), hipMemcpyHostToDevice, c->stream));
+  // zu(:, i) = sum_j zu(:, j) U(j, i)  (diag.f90:120-127)
+  for (int k0 = 0; k0 < NK; k0 += dk) {
+    const int kn = std::min(dk, NK - k0);
+    cplx<double>* zk = zu + (size_t)k0 * NB * NL;
+    hipLaunchKernelGGL(k_rotate_bands, dim3((NL + 255) / 256, ntile, kn), dim3(256), sizeof(cplx<double>) * NB * kTile,
+                       c->stream, (const cplx<double>*)zk, (const cplx<double>*)(s->dU + (size_t)k0 * NB * NB), hb, NB, NL);
+    LAUNCHCHK(c);
+    HIPCHK(c, hipMemcpyAsync(zk, hb, sizeof(cplx<double>) * (size_t)kn * NB * NL, hipMemcpyDeviceToDevice, c->stream));
+  }
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return SALMON_OK;
+}

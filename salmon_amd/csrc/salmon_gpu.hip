@@ -57,6 +57,7 @@ struct salmon_ctx {
   int fused = -1;
   int path = SALMON_PATH_AUTO;
   void* fz = nullptr;  // state owned by the fused path
+  void* gs = nullptr;  // state owned by the ground-state path
 };
 
 namespace {
@@ -138,8 +139,8 @@ const float* vloc_ptr<float>(const salmon_ctx* c) { return c->d_vloc_f; }
 // H applied to nOrb orbitals (rows of NL) from `in` to `out` (optionally tt).
 template <typename T>
 int apply_h(salmon_ctx* c, const cplx<T>* in, cplx<T>* out, cplx<T>* tt, int orb0, int nOrb, int NB, int NK,
-            const double* kbase, double ax, double ay, double az) {
-  cplx<T>* uv = (cplx<T>*)c->d_uvpsi;
+            const double* kbase, double ax, double ay, double az, void* uvbuf = nullptr) {
+  cplx<T>* uv = (cplx<T>*)(uvbuf ? uvbuf : c->d_uvpsi);
   dim3 gp((c->kb.Nlma + 3) / 4, nOrb);
   if (c->kb.Nlma > 0) {
     hipLaunchKernelGGL(k_kb_project<T>, gp, dim3(256), 0, c->stream, in, orb0, NB, NK, kbase, ax, ay, az, c->g,
@@ -262,6 +263,7 @@ int read_sums(salmon_ctx* c, double out[8]) {
 }  // namespace
 
 #include "fused.hpp"
+#include "gs.hpp"
 
 namespace {
 
@@ -659,6 +661,72 @@ int salmon_gpu_sp_energy(salmon_ctx* c, const double* kAc, double* esp) {
   return SALMON_OK;
 }
 
+static int gs_check(salmon_ctx* c, const double* kAc) {
+  int r = check_ready(c, true);
+  if (r) return r;
+  if (c->precision != SALMON_FP64) return fail(c, SALMON_ERR_UNSUPPORTED, "the ground-state path runs on FP64 contexts");
+  if (c->g.NLx > kDftMax || c->g.NLy > kDftMax || c->g.NLz > kDftMax)
+    return fail(c, SALMON_ERR_UNSUPPORTED, "grid axes longer than %d", kDftMax);
+  if (c->NB > kCgMaxNB) return fail(c, SALMON_ERR_UNSUPPORTED, "more than %d bands", kCgMaxNB);
+  if ((r = upload_kac(c, kAc))) return r;
+  if ((r = ensure_work(c))) return r;
+  return gs_ensure(c);
+}
+
+static int cg_results(salmon_ctx* c, double* esp_var, int* iters) {
+  GsState* s = (GsState*)c->gs;
+  if (esp_var)
+    HIPCHK(c, hipMemcpyAsync(esp_var, s->esp_var, sizeof(double) * c->NKB, hipMemcpyDeviceToHost, c->stream));
+  if (iters) HIPCHK(c, hipMemcpyAsync(iters, s->iters, sizeof(int) * c->NKB, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return SALMON_OK;
+}
+
+int salmon_gpu_cg(salmon_ctx* c, const double* kAc, int iter_cg_max, int precond, double* esp_var, int* iters) {
+  int r = gs_check(c, kAc);
+  if (r) return r;
+  if (iter_cg_max < 0) return fail(c, SALMON_ERR_ARG, "iter_cg_max < 0");
+  if (precond != SALMON_PC_NONE && precond != SALMON_PC_KINETIC)
+    return fail(c, SALMON_ERR_ARG, "unknown preconditioner %d", precond);
+  if ((r = cg_run(c, iter_cg_max, precond))) return r;
+  return cg_results(c, esp_var, iters);
+}
+
+int salmon_gpu_subspace_diag(salmon_ctx* c, const double* kAc, double* esp) {
+  int r = gs_check(c, kAc);
+  if (r) return r;
+  return subspace_diag(c, esp);
+}
+
+int salmon_gpu_ground_state(salmon_ctx* c, const double* kAc, int max_sweeps, int iter_cg_max, int precond,
+                            double tol, double* esp, double* esp_var, int* sweeps) {
+  int r = gs_check(c, kAc);
+  if (r) return r;
+  if (max_sweeps < 1 || iter_cg_max < 1) return fail(c, SALMON_ERR_ARG, "max_sweeps and iter_cg_max must be >= 1");
+  if (precond != SALMON_PC_NONE && precond != SALMON_PC_KINETIC)
+    return fail(c, SALMON_ERR_ARG, "unknown preconditioner %d", precond);
+  std::vector<double> e0(c->NKB, 0.0), e1(c->NKB), var(c->NKB);
+  int n = 0;
+  for (n = 1; n <= max_sweeps; ++n) {
+    // ground_state.f90:136-140
+    if ((r = gram_schmidt_dev(c))) return r;
+    if ((r = subspace_diag(c, nullptr))) return r;
+    if ((r = gram_schmidt_dev(c))) return r;
+    if ((r = cg_run(c, iter_cg_max, precond))) return r;
+    if ((r = gram_schmidt_dev(c))) return r;
+    if ((r = cg_results(c, var.data(), nullptr))) return r;
+    if ((r = salmon_gpu_sp_energy(c, kAc, e1.data()))) return r;
+    double change = 0.0;
+    for (int o = 0; o < c->NKB; ++o) change = std::max(change, std::fabs(e1[o] - e0[o]));
+    e0.swap(e1);
+    if (n > 1 && change < tol) break;
+  }
+  if (sweeps) *sweeps = std::min(n, max_sweeps);
+  if (esp_var) std::copy(var.begin(), var.end(), esp_var);
+  if (esp) std::copy(e0.begin(), e0.end(), esp);
+  return SALMON_OK;
+}
+
 int salmon_gpu_rt_run(salmon_ctx* c, const double* ac, int nsteps, double dt, int reortho_every, double* jt, double* et,
                       double* norm, double* rho) {
   int r = check_ready(c, true);
@@ -725,6 +793,7 @@ int salmon_gpu_finalize(salmon_ctx* c) {
   if (!c) return SALMON_ERR_ARG;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   fused_free(c);
+  gs_free(c);
   dfree((void**)&c->d_vloc); dfree((void**)&c->d_vloc_f); dfree((void**)&c->d_rho); dfree((void**)&c->d_rho_part);
   dfree((void**)&c->d_rho_part2);
   dfree((void**)&c->d_kac0); dfree((void**)&c->d_kac); dfree((void**)&c->d_occ); dfree((void**)&c->d_rows);

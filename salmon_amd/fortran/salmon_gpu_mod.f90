@@ -17,6 +17,9 @@ module salmon_gpu
   integer(c_int), parameter :: SALMON_PATH_AUTO = 0
   integer(c_int), parameter :: SALMON_PATH_STREAMED = 1
   integer(c_int), parameter :: SALMON_PATH_FUSED = 2
+  integer(c_int), parameter :: SALMON_PC_NONE = 0
+  integer(c_int), parameter :: SALMON_PC_KINETIC = 1
+  integer(c_int), parameter :: SALMON_PC_LEARNED = 2
 
   type, bind(C) :: salmon_grid_t
     integer(c_int) :: NLx, NLy, NLz
@@ -117,6 +120,34 @@ module salmon_gpu
       type(c_ptr), value :: ctx
       real(c_double), intent(in) :: kAc(*)
       real(c_double), intent(out) :: esp(*)
+    end function
+
+    integer(c_int) function salmon_gpu_cg(ctx, kAc, iter_cg_max, precond, esp_var, iters) &
+        bind(C, name="salmon_gpu_cg")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: kAc(*)
+      integer(c_int), value :: iter_cg_max, precond
+      real(c_double), intent(out) :: esp_var(*)
+      type(c_ptr), value :: iters
+    end function
+
+    integer(c_int) function salmon_gpu_subspace_diag(ctx, kAc, esp) bind(C, name="salmon_gpu_subspace_diag")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: kAc(*)
+      real(c_double), intent(out) :: esp(*)
+    end function
+
+    integer(c_int) function salmon_gpu_ground_state(ctx, kAc, max_sweeps, iter_cg_max, precond, tol, esp, esp_var, &
+        sweeps) bind(C, name="salmon_gpu_ground_state")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: kAc(*)
+      integer(c_int), value :: max_sweeps, iter_cg_max, precond
+      real(c_double), value :: tol
+      real(c_double), intent(out) :: esp(*), esp_var(*)
+      integer(c_int), intent(out) :: sweeps
     end function
 
     integer(c_int) function salmon_gpu_rt_run(ctx, ac, nsteps, dt, reortho_every, jt, et, norm, rho) &

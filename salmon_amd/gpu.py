@@ -16,6 +16,9 @@ method                 reference routine                               file:line
 ``gram_schmidt``       Gram_Schmidt                                    ARTED/GS/Gram_Schmidt.f90:20
 ``sp_energy``          sp_energy_omp                                   ARTED/GS/sp_energy.f90:20
 ``rt_run``             the tddft_sc time loop (middlepoint, frozen V)  ARTED/control/control_sc.f90:150
+``cg``                 CG_omp -> CG_ompk                               ARTED/GS/CG.f90:20,34
+``diag``               diag_omp (subspace diagonalisation)             ARTED/GS/diag.f90:20
+``ground_state``       the GS sweep loop (frozen V)                    ARTED/control/ground_state.f90:136
 =====================  ==============================================  ===========================
 
 The reference aborts through ``err_finalize`` on any error; here every failed
@@ -35,6 +38,7 @@ LIB_PATH = os.path.join(HERE, "lib", "libsalmon_gpu.so")
 FP64 = 0
 MIXED = 1
 PATH_AUTO, PATH_STREAMED, PATH_FUSED = 0, 1, 2
+PC_NONE, PC_KINETIC, PC_LEARNED = 0, 1, 2
 
 
 class SalmonGPUError(RuntimeError):
@@ -65,6 +69,9 @@ SIGNATURES = {
     "salmon_gpu_norm": [_vp, _vp],
     "salmon_gpu_gram_schmidt": [_vp],
     "salmon_gpu_sp_energy": [_vp, _vp, _vp],
+    "salmon_gpu_cg": [_vp, _vp, _ci, _ci, _vp, _vp],
+    "salmon_gpu_subspace_diag": [_vp, _vp, _vp],
+    "salmon_gpu_ground_state": [_vp, _vp, _ci, _ci, _ci, _cd, _vp, _vp, _vp],
     "salmon_gpu_rt_run": [_vp, _vp, _ci, _cd, _ci, _vp, _vp, _vp, _vp],
     "salmon_gpu_set_path": [_vp, _ci],
     "salmon_gpu_path": [_vp],
@@ -228,6 +235,36 @@ class Context:
         self._check(lib().salmon_gpu_rt_run(self._h, _p(ac), int(nsteps), float(self.sys.dt if dt is None else dt),
                                             int(reortho_every), _p(jt), _p(et), _p(nt), _p(rho)), "rt_run")
         return dict(J=jt, E=et, norm=nt, rho=rho)
+
+    def cg(self, kAc, iter_cg_max, precond=PC_NONE):
+        """CG_ompk on the resident orbitals; returns (esp_var, iters), each (NK, NB)."""
+        s = self.sys
+        ev = np.zeros((s.NK, s.NB))
+        it = np.zeros((s.NK, s.NB), np.int32)
+        self._check(lib().salmon_gpu_cg(self._h, _p(_f64(kAc)), int(iter_cg_max), int(precond), _p(ev), _p(it)), "cg")
+        return ev, it
+
+    CG_omp = cg
+
+    def diag(self, kAc):
+        """diag_omp: rotates the resident orbitals to the subspace eigenvectors; returns esp (NK, NB)."""
+        esp = np.zeros((self.sys.NK, self.sys.NB))
+        self._check(lib().salmon_gpu_subspace_diag(self._h, _p(_f64(kAc)), _p(esp)), "subspace_diag")
+        return esp
+
+    diag_omp = diag
+
+    def ground_state(self, kAc, max_sweeps, iter_cg_max=4, precond=PC_KINETIC, tol=1e-10):
+        """Sweeps of Gram-Schmidt, diag, Gram-Schmidt, CG, Gram-Schmidt until no band energy moves by tol
+        between two sweeps; returns dict(esp, esp_var, sweeps)."""
+        s = self.sys
+        esp = np.zeros((s.NK, s.NB))
+        ev = np.zeros((s.NK, s.NB))
+        n = ctypes.c_int()
+        self._check(lib().salmon_gpu_ground_state(self._h, _p(_f64(kAc)), int(max_sweeps), int(iter_cg_max),
+                                                  int(precond), float(tol), _p(esp), _p(ev), ctypes.byref(n)),
+                    "ground_state")
+        return dict(esp=esp, esp_var=ev, sweeps=n.value)
 
     def stream(self):
         """The context's hipStream_t (as an integer handle)."""

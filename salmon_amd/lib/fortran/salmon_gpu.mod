@@ -1,4 +1,4 @@
-﻿!mod$ v1 sum:52ae2459403bb64f
+﻿!mod$ v1 sum:59e397f2545f5f00
 !need$ 0bde2ac47243ead2 i iso_c_binding
 !need$ bb381bf46e508468 i __fortran_builtins
 module salmon_gpu
@@ -90,6 +90,9 @@ integer(4),parameter::salmon_mixed=1_4
 integer(4),parameter::salmon_path_auto=0_4
 integer(4),parameter::salmon_path_streamed=1_4
 integer(4),parameter::salmon_path_fused=2_4
+integer(4),parameter::salmon_pc_none=0_4
+integer(4),parameter::salmon_pc_kinetic=1_4
+integer(4),parameter::salmon_pc_learned=2_4
 type,bind(c)::salmon_grid_t
 integer(4)::nlx
 integer(4)::nly
@@ -237,6 +240,42 @@ type(c_ptr),value::ctx
 real(8),intent(in)::kac(1_8:*)
 real(8),intent(out)::esp(1_8:*)
 integer(4)::salmon_gpu_sp_energy
+end
+end interface
+interface
+function salmon_gpu_cg(ctx,kac,iter_cg_max,precond,esp_var,iters) bind(c,name="salmon_gpu_cg")
+import::c_ptr
+type(c_ptr),value::ctx
+real(8),intent(in)::kac(1_8:*)
+integer(4),value::iter_cg_max
+integer(4),value::precond
+real(8),intent(out)::esp_var(1_8:*)
+type(c_ptr),value::iters
+integer(4)::salmon_gpu_cg
+end
+end interface
+interface
+function salmon_gpu_subspace_diag(ctx,kac,esp) bind(c,name="salmon_gpu_subspace_diag")
+import::c_ptr
+type(c_ptr),value::ctx
+real(8),intent(in)::kac(1_8:*)
+real(8),intent(out)::esp(1_8:*)
+integer(4)::salmon_gpu_subspace_diag
+end
+end interface
+interface
+function salmon_gpu_ground_state(ctx,kac,max_sweeps,iter_cg_max,precond,tol,esp,esp_var,sweeps) bind(c,name="salmon_gpu_ground_state")
+import::c_ptr
+type(c_ptr),value::ctx
+real(8),intent(in)::kac(1_8:*)
+integer(4),value::max_sweeps
+integer(4),value::iter_cg_max
+integer(4),value::precond
+real(8),value::tol
+real(8),intent(out)::esp(1_8:*)
+real(8),intent(out)::esp_var(1_8:*)
+integer(4),intent(out)::sweeps
+integer(4)::salmon_gpu_ground_state
 end
 end interface
 interface

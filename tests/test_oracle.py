@@ -271,3 +271,32 @@ def test_stability_margin(maker):
     """dt * rho(H) stays inside the Taylor-4 stability region x < 2 sqrt 2."""
     s = maker()
     assert s.dt * synth.spectral_bound(s) < 2 * math.sqrt(2)
+
+
+def test_kinetic_preconditioner_closed_form():
+    """K e^{iG.r} = k(x) e^{iG.r}, x = t(G)/Tx with t the finite-difference
+    kinetic symbol and k(x) = f/(f+16x^4), f = 27+18x+12x^2+8x^3 (Teter-Payne-
+    Allan form) on a non-cubic, non-power-of-two grid; K is linear and
+    Hermitian positive definite."""
+    s = synth.make_system(N=(10, 12, 14), L=(5.0, 6.3, 7.1), nk3=(1, 1, 1), NB=1, seed=2, name="kin")
+    o = Oracle(s)
+    ix, iy, iz = synth.grid_coords(s.NLx, s.NLy, s.NLz)
+    lap = [s.lapx, s.lapy, s.lapz]
+    N = [s.NLx, s.NLy, s.NLz]
+    tx = 1.7
+    for n in [(0, 0, 0), (1, 0, 0), (2, -3, 5), (5, 6, 7), (-4, 1, -6)]:
+        pw = np.exp(2j * np.pi * (n[0] * ix / N[0] + n[1] * iy / N[1] + n[2] * iz / N[2]))
+        t = 0.0
+        for a in range(3):
+            th = 2 * np.pi * n[a] / N[a]
+            t += -0.5 * (lap[a][0] + 2 * sum(lap[a][m] * np.cos(m * th) for m in range(1, 5)))
+        x = t / tx
+        f = 27 + 18 * x + 12 * x ** 2 + 8 * x ** 3
+        assert np.max(np.abs(o.kinetic_precondition(tx, pw) - f / (f + 16 * x ** 4) * pw)) < 1e-13
+    rng = np.random.default_rng(0)
+    a = rng.standard_normal(s.NL) + 1j * rng.standard_normal(s.NL)
+    b = rng.standard_normal(s.NL) + 1j * rng.standard_normal(s.NL)
+    ka, kb = o.kinetic_precondition(tx, a), o.kinetic_precondition(tx, b)
+    assert rel(o.kinetic_precondition(tx, 2 * a - 3j * b), 2 * ka - 3j * kb) < 1e-13
+    assert abs(np.vdot(b, ka) - np.conj(np.vdot(a, kb))) < 1e-12 * abs(np.vdot(b, ka))
+    assert np.vdot(a, ka).real > 0
