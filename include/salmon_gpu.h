@@ -157,6 +157,50 @@ int salmon_gpu_subspace_diag(salmon_ctx *ctx, const double *kAc, double *esp);
 int salmon_gpu_ground_state(salmon_ctx *ctx, const double *kAc, int max_sweeps, int iter_cg_max, int precond,
                             double tol, double *esp, double *esp_var, int *sweeps);
 
+/* ---- Learned CG preconditioner (no reference counterpart) ----------------
+ * A three-layer periodic 3x3x3 convolution network without biases (channels
+ * 2 -> 16 -> 16 -> 2, ReLU after the first two; weights W1(27,2,16),
+ * W2(27,16,16), W3(27,16,2) as [tap][in][out], tap = (dx+1)*9+(dy+1)*3+(dz+1),
+ * SALMON_PCNET_WEIGHTS floats in that order) maps the CG residual g, after
+ * its phase is rotated so that sum g^2 is real and its norm scaled to
+ * sqrt(NL), to a search direction (rotated back).  With SALMON_PC_LEARNED
+ * each CG iteration is a steepest-descent step along that direction with the
+ * 2x2 Rayleigh-Ritz of CG.f90:107-119 (no conjugation: the network is not a
+ * fixed linear operator).  salmon_gpu_ground_state uses it while every sweep
+ * lowers the band-energy sum by at least 0.3 of the previous sweep's drop and
+ * then falls back to SALMON_PC_KINETIC for the rest of the solve.
+ *
+ * Training (one k point per context): salmon_gpu_pc_record makes the next CG
+ * calls store each iteration's orthogonalised residual, iterate, band and
+ * <x|H|x>; salmon_gpu_pc_set_targets takes the resident orbitals as the
+ * converged bands Z and their energies as the targets lam.  The objective of
+ * a sample is the fraction of the band's eigenvalue error removed by one
+ * Rayleigh-Ritz step in span{x, p}, p the direction made orthogonal to x and
+ * the lower bands Z_j (j < band) and normalised:
+ *   loss = -mean(min((theta - ev)/(theta - lam), 1)),
+ *   ev = (theta + c - sqrt((theta - c)^2 + 4|b|^2))/2,
+ *   c = <p|H|p> Hxyz, b = <g|p> Hxyz, theta = <x|H|x>.
+ * Adam (0.9, 0.999, 1e-8) updates FP32 weights. */
+#define SALMON_PCNET_WEIGHTS (27 * 2 * 16 + 27 * 16 * 16 + 27 * 16 * 2)
+typedef struct salmon_pcnet salmon_pcnet;
+int salmon_gpu_pcnet_create(unsigned seed, salmon_pcnet **net);  /* He-normal initial weights */
+int salmon_gpu_pcnet_set_weights(salmon_pcnet *net, const float *w);  /* also resets Adam */
+int salmon_gpu_pcnet_get_weights(salmon_pcnet *net, float *w);
+int salmon_gpu_pcnet_finalize(salmon_pcnet *net);
+/* network used by SALMON_PC_LEARNED on ctx (NULL detaches; not owned) */
+int salmon_gpu_set_pcnet(salmon_ctx *ctx, salmon_pcnet *net);
+/* record up to max_samples residuals of the following CG calls (0 stops) */
+int salmon_gpu_pc_record(salmon_ctx *ctx, int max_samples);
+int salmon_gpu_pc_samples(salmon_ctx *ctx, int *stored, int *valid);
+int salmon_gpu_pc_get_sample(salmon_ctx *ctx, int i, double *g, double *x, int *band, double *theta, int *valid);
+int salmon_gpu_pc_set_targets(salmon_ctx *ctx, const double *kAc);
+/* loss and gradient (SALMON_PCNET_WEIGHTS doubles) of the mean objective over
+ * the samples idx(nsamp), without an update */
+int salmon_gpu_pcnet_gradient(salmon_pcnet *net, salmon_ctx *ctx, int nsamp, const int *idx, double *loss, double *grad);
+/* nsteps Adam steps on batches of `batch` valid samples drawn with seed */
+int salmon_gpu_pcnet_train(salmon_pcnet *net, salmon_ctx *ctx, int nsteps, int batch, double lr, unsigned seed,
+                           double *mean_loss);
+
 /* Device-resident RT loop: the time loop of tddft_sc
  * (ARTED/control/control_sc.f90:150-222) with the middlepoint propagator
  * (ARTED/RT/dt_evolve.f90:43-51,106-216) and a frozen potential.  ac holds

@@ -197,7 +197,10 @@ enum : int {
   SUB_START = 0,  // start vector: SN = 1/sqrt(|out|^2 Hxyz); resets the band's flags
   SUB_GRAD = 1,   // residual, no preconditioner: s = |g|^2 Hxyz (CG.f90:90)
   SUB_PCDIR = 2,  // preconditioned residual: s = Re<Kg|g> Hxyz
-  SUB_NONE = 3    // no reduction
+  SUB_NONE = 3,   // no reduction
+  SUB_LEARN = 4   // learned direction: no conjugation (uk = 0); the network is
+                  // not a fixed linear operator, so each iteration is a
+                  // preconditioned steepest-descent step with the 2x2 Rayleigh-Ritz
 };
 
 // out = src - sum_{j<ib} zs(j) B_j  (CG.f90:69-71, 86-89), then the role's
@@ -235,6 +238,10 @@ __global__ void __launch_bounds__(256) k_cg_sub(State S, const cplx<double>* __r
     }
   }
   if (role == SUB_NONE) return;
+  if (role == SUB_LEARN) {
+    if (c == 0 && threadIdx.x == 0) S.sc[k * kNS + UK] = 0.0;
+    return;
+  }
   double tot[1];
   if (!last_block_sum<1>(v, S, S.nchunk, c, tot)) return;
   if (threadIdx.x != 0) return;
@@ -671,6 +678,25 @@ __global__ void __launch_bounds__(256) k_rotate_bands(const cplx<double>* __rest
 #pragma unroll
   for (int ii = 0; ii < kTile; ++ii)
     if (i0 + ii < NB) C[((size_t)k * NB + i0 + ii) * NL + p] = cplx<double>{re[ii], im[ii]};
+}
+
+// Training sample of the learned preconditioner: the orthogonalised residual
+// and the iterate of k = 0, with the band and <x|H|x>.  valid = 0 when the
+// band has already left its iterations.
+__global__ void __launch_bounds__(256) k_pc_record(State S, const cplx<double>* __restrict__ g,
+                                                   const cplx<double>* __restrict__ x, cplx<double>* __restrict__ sg,
+                                                   cplx<double>* __restrict__ sx, double* __restrict__ theta,
+                                                   int* __restrict__ sib, int* __restrict__ valid, int ib) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < S.NL) {
+    sg[i] = g[i];
+    sx[i] = x[i];
+  }
+  if (i == 0) {
+    theta[0] = S.sc[XHX];
+    sib[0] = ib;
+    valid[0] = S.done[0] ? 0 : 1;
+  }
 }
 
 }  // namespace cg

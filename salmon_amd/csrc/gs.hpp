@@ -8,6 +8,14 @@
 
 using namespace salmon::cg;
 
+// Inference buffers of the learned preconditioner (one residual per k point).
+struct PcInfer {
+  int NK = 0, NL = 0;
+  std::vector<void*> bufs;
+  float *u = nullptr, *a1 = nullptr, *a2 = nullptr, *y = nullptr;
+  double *stats = nullptr, *part = nullptr;
+};
+
 struct GsState {
   int NK = 0, NB = 0, NL = 0, Nlma = 0, nchunk = 0, ngroup = 0;
   std::vector<void*> bufs;
@@ -30,7 +38,24 @@ struct GsState {
   double* dM = nullptr;
   cplx<double>* dU = nullptr;
   int dk = 0;  // k points per subspace pass
+  // learned preconditioner: inference buffers, recorded training samples of
+  // k = 0 (orthogonalised residual, iterate, band, <x|H|x>) and the targets
+  PcInfer inf;
+  int smp_cap = 0, smp_n = 0;
+  bool recording = false;
+  std::vector<void*> smp_bufs;
+  cplx<double>* smp_g = nullptr;
+  cplx<double>* smp_x = nullptr;
+  double* smp_theta = nullptr;
+  int* smp_ib = nullptr;
+  int* smp_valid = nullptr;
+  std::vector<int> h_ib, h_valid;
+  std::vector<double> h_theta;
+  cplx<double>* Zc = nullptr;  // (NB, NL) converged bands
+  std::vector<double> lam;     // their energies
 };
+
+static int pc_infer(salmon_ctx* c, GsState* s, salmon_pcnet* net, const cplx<double>* g, cplx<double>* out);
 
 enum { W_X = 0, W_HX, W_TX, W_G, W_P, W_PKO, W_HP, W_TP, W_KG, W_T1, W_T2 };
 
@@ -38,6 +63,8 @@ static void gs_free(salmon_ctx* c) {
   GsState* s = (GsState*)c->gs;
   if (!s) return;
   for (void* p : s->bufs) (void)hipFree(p);
+  for (void* p : s->smp_bufs) (void)hipFree(p);
+  for (void* p : s->inf.bufs) (void)hipFree(p);
   delete s;
   c->gs = nullptr;
 }
@@ -188,19 +215,27 @@ static int cg_run(salmon_ctx* c, int ncg, int precond) {
       // residual orthogonalised to the lower bands (CG.f90:85-90)
       if ((r = proj(ib, resid, PROJ_VEC, 1))) return r;
       const cplx<double>* dir = w[W_G];
-      if (precond == SALMON_PC_NONE) {
-        hipLaunchKernelGGL(k_cg_sub, gv, blk, 0, c->stream, S, (const cplx<double>*)zu, ib, resid, w[W_G],
-                           (const cplx<double>*)nullptr, (int)SUB_GRAD, it, ncg);
+      hipLaunchKernelGGL(k_cg_sub, gv, blk, 0, c->stream, S, (const cplx<double>*)zu, ib, resid, w[W_G],
+                         (const cplx<double>*)nullptr, (int)(precond == SALMON_PC_NONE ? SUB_GRAD : SUB_NONE), it, ncg);
+      LAUNCHCHK(c);
+      if (s->recording && s->smp_n < s->smp_cap && NK == 1) {
+        const int slot = s->smp_n++;
+        hipLaunchKernelGGL(k_pc_record, dim3((NL + 255) / 256), blk, 0, c->stream, S, (const cplx<double>*)w[W_G],
+                           (const cplx<double>*)w[W_X], s->smp_g + (size_t)slot * NL, s->smp_x + (size_t)slot * NL,
+                           s->smp_theta + slot, s->smp_ib + slot, s->smp_valid + slot, ib);
         LAUNCHCHK(c);
-      } else {
-        hipLaunchKernelGGL(k_cg_sub, gv, blk, 0, c->stream, S, (const cplx<double>*)zu, ib, resid, w[W_G],
-                           (const cplx<double>*)nullptr, (int)SUB_NONE, it, ncg);
-        LAUNCHCHK(c);
-        if ((r = kinetic_apply(c, s, S, w[W_G], w[W_KG]))) return r;
+      }
+      if (precond != SALMON_PC_NONE) {
+        if (precond == SALMON_PC_LEARNED) {
+          if ((r = pc_infer(c, s, (salmon_pcnet*)c->pcnet, w[W_G], w[W_KG]))) return r;
+        } else {
+          if ((r = kinetic_apply(c, s, S, w[W_G], w[W_KG]))) return r;
+        }
         const Src kg{w[W_KG], (size_t)NL, nullptr, nullptr};
         if ((r = proj(ib, kg, PROJ_VEC, 1))) return r;
         hipLaunchKernelGGL(k_cg_sub, gv, blk, 0, c->stream, S, (const cplx<double>*)zu, ib, kg, w[W_KG],
-                           (const cplx<double>*)w[W_G], (int)SUB_PCDIR, it, ncg);
+                           (const cplx<double>*)w[W_G], (int)(precond == SALMON_PC_LEARNED ? SUB_LEARN : SUB_PCDIR),
+                           it, ncg);
         LAUNCHCHK(c);
         dir = w[W_KG];
       }

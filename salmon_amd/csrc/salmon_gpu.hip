@@ -58,6 +58,7 @@ struct salmon_ctx {
   int path = SALMON_PATH_AUTO;
   void* fz = nullptr;  // state owned by the fused path
   void* gs = nullptr;  // state owned by the ground-state path
+  void* pcnet = nullptr;  // network of the learned CG preconditioner (not owned)
 };
 
 namespace {
@@ -264,6 +265,7 @@ int read_sums(salmon_ctx* c, double out[8]) {
 
 #include "fused.hpp"
 #include "gs.hpp"
+#include "pc.hpp"
 
 namespace {
 
@@ -673,6 +675,18 @@ static int gs_check(salmon_ctx* c, const double* kAc) {
   return gs_ensure(c);
 }
 
+// fraction of the previous sweep's energy drop below which the learned
+// preconditioner hands over to the kinetic one
+constexpr double kPcStall = 0.3;
+
+static int check_precond(salmon_ctx* c, int precond) {
+  if (precond != SALMON_PC_NONE && precond != SALMON_PC_KINETIC && precond != SALMON_PC_LEARNED)
+    return fail(c, SALMON_ERR_ARG, "unknown preconditioner %d", precond);
+  if (precond == SALMON_PC_LEARNED && !c->pcnet)
+    return fail(c, SALMON_ERR_STATE, "learned preconditioner selected but no network set (salmon_gpu_set_pcnet)");
+  return SALMON_OK;
+}
+
 static int cg_results(salmon_ctx* c, double* esp_var, int* iters) {
   GsState* s = (GsState*)c->gs;
   if (esp_var)
@@ -686,8 +700,7 @@ int salmon_gpu_cg(salmon_ctx* c, const double* kAc, int iter_cg_max, int precond
   int r = gs_check(c, kAc);
   if (r) return r;
   if (iter_cg_max < 0) return fail(c, SALMON_ERR_ARG, "iter_cg_max < 0");
-  if (precond != SALMON_PC_NONE && precond != SALMON_PC_KINETIC)
-    return fail(c, SALMON_ERR_ARG, "unknown preconditioner %d", precond);
+  if ((r = check_precond(c, precond))) return r;
   if ((r = cg_run(c, iter_cg_max, precond))) return r;
   return cg_results(c, esp_var, iters);
 }
@@ -703,27 +716,213 @@ int salmon_gpu_ground_state(salmon_ctx* c, const double* kAc, int max_sweeps, in
   int r = gs_check(c, kAc);
   if (r) return r;
   if (max_sweeps < 1 || iter_cg_max < 1) return fail(c, SALMON_ERR_ARG, "max_sweeps and iter_cg_max must be >= 1");
-  if (precond != SALMON_PC_NONE && precond != SALMON_PC_KINETIC)
-    return fail(c, SALMON_ERR_ARG, "unknown preconditioner %d", precond);
+  if ((r = check_precond(c, precond))) return r;
   std::vector<double> e0(c->NKB, 0.0), e1(c->NKB), var(c->NKB);
-  int n = 0;
+  int n = 0, pc = precond;
+  double prev_drop = 0.0, prev_sum = 0.0;
   for (n = 1; n <= max_sweeps; ++n) {
     // ground_state.f90:136-140
     if ((r = gram_schmidt_dev(c))) return r;
     if ((r = subspace_diag(c, nullptr))) return r;
     if ((r = gram_schmidt_dev(c))) return r;
-    if ((r = cg_run(c, iter_cg_max, precond))) return r;
+    if ((r = cg_run(c, iter_cg_max, pc))) return r;
     if ((r = gram_schmidt_dev(c))) return r;
     if ((r = cg_results(c, var.data(), nullptr))) return r;
     if ((r = salmon_gpu_sp_energy(c, kAc, e1.data()))) return r;
-    double change = 0.0;
-    for (int o = 0; o < c->NKB; ++o) change = std::max(change, std::fabs(e1[o] - e0[o]));
+    double change = 0.0, sum = 0.0;
+    for (int o = 0; o < c->NKB; ++o) {
+      change = std::max(change, std::fabs(e1[o] - e0[o]));
+      sum += e1[o];
+    }
+    // the learned direction leads while each sweep lowers the band-energy sum
+    // by at least kPcStall of the previous sweep's drop; then the kinetic
+    // preconditioner takes over for the rest of the solve
+    if (pc == SALMON_PC_LEARNED && n > 1) {
+      const double drop = prev_sum - sum;
+      if (n > 2 && !(drop >= kPcStall * prev_drop)) pc = SALMON_PC_KINETIC;
+      prev_drop = drop;
+    }
+    prev_sum = sum;
     e0.swap(e1);
     if (n > 1 && change < tol) break;
   }
   if (sweeps) *sweeps = std::min(n, max_sweeps);
   if (esp_var) std::copy(var.begin(), var.end(), esp_var);
   if (esp) std::copy(e0.begin(), e0.end(), esp);
+  return SALMON_OK;
+}
+
+int salmon_gpu_pcnet_create(unsigned seed, salmon_pcnet** out) {
+  if (!out) return SALMON_ERR_ARG;
+  *out = nullptr;
+  salmon_pcnet* n = new salmon_pcnet();
+  int r = 0;
+  std::vector<void*> own;
+  r |= pc_alloc(own, &n->w, NW);
+  r |= pc_alloc(own, &n->m, NW);
+  r |= pc_alloc(own, &n->v, NW);
+  r |= pc_alloc(own, &n->grad, NW);
+  n->owned = own;
+  if (r) {
+    salmon_gpu_pcnet_finalize(n);
+    return SALMON_ERR_NOMEM;
+  }
+  // He-normal initial weights (fan-in 27*cin; the output layer at half the variance)
+  std::mt19937_64 rng(seed);
+  std::normal_distribution<double> nd(0.0, 1.0);
+  std::vector<float> w(NW);
+  for (int i = 0; i < NW; ++i) {
+    const double sd = i < NW1 ? std::sqrt(2.0 / (27 * C0)) : (i < NW1 + NW2 ? std::sqrt(2.0 / (27 * C1)) : std::sqrt(1.0 / (27 * C2)));
+    w[i] = (float)(nd(rng) * sd);
+  }
+  if (hipMemcpy(n->w, w.data(), sizeof(float) * NW, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemset(n->m, 0, sizeof(float) * NW) != hipSuccess || hipMemset(n->v, 0, sizeof(float) * NW) != hipSuccess) {
+    salmon_gpu_pcnet_finalize(n);
+    return SALMON_ERR_HIP;
+  }
+  *out = n;
+  return SALMON_OK;
+}
+
+int salmon_gpu_pcnet_set_weights(salmon_pcnet* n, const float* w) {
+  if (!n || !w) return SALMON_ERR_ARG;
+  if (hipMemcpy(n->w, w, sizeof(float) * NW, hipMemcpyHostToDevice) != hipSuccess) return SALMON_ERR_HIP;
+  if (hipMemset(n->m, 0, sizeof(float) * NW) != hipSuccess || hipMemset(n->v, 0, sizeof(float) * NW) != hipSuccess)
+    return SALMON_ERR_HIP;
+  n->step = 0;
+  return SALMON_OK;
+}
+
+int salmon_gpu_pcnet_get_weights(salmon_pcnet* n, float* w) {
+  if (!n || !w) return SALMON_ERR_ARG;
+  return hipMemcpy(w, n->w, sizeof(float) * NW, hipMemcpyDeviceToHost) == hipSuccess ? SALMON_OK : SALMON_ERR_HIP;
+}
+
+int salmon_gpu_pcnet_finalize(salmon_pcnet* n) {
+  if (!n) return SALMON_ERR_ARG;
+  (void)hipDeviceSynchronize();
+  pcnet_free_buffers(n);
+  for (void* b : n->owned) (void)hipFree(b);
+  delete n;
+  return SALMON_OK;
+}
+
+int salmon_gpu_set_pcnet(salmon_ctx* c, salmon_pcnet* n) {
+  if (!c) return SALMON_ERR_ARG;
+  c->pcnet = n;
+  return SALMON_OK;
+}
+
+int salmon_gpu_pc_record(salmon_ctx* c, int max_samples) {
+  int r = check_ready(c, true);
+  if (r) return r;
+  if (c->NK != 1) return fail(c, SALMON_ERR_UNSUPPORTED, "training samples are recorded for one k point");
+  if (max_samples < 0) return fail(c, SALMON_ERR_ARG, "max_samples < 0");
+  if ((r = ensure_work(c)) || (r = gs_ensure(c))) return r;
+  GsState* s = (GsState*)c->gs;
+  for (void* p : s->smp_bufs) (void)hipFree(p);
+  s->smp_bufs.clear();
+  s->smp_cap = s->smp_n = 0;
+  s->recording = max_samples > 0;
+  if (!s->recording) return SALMON_OK;
+  const size_t NL = c->g.NL;
+  int e = 0;
+  e |= pc_alloc(s->smp_bufs, &s->smp_g, (size_t)max_samples * NL);
+  e |= pc_alloc(s->smp_bufs, &s->smp_x, (size_t)max_samples * NL);
+  e |= pc_alloc(s->smp_bufs, &s->smp_theta, (size_t)max_samples);
+  e |= pc_alloc(s->smp_bufs, &s->smp_ib, (size_t)max_samples);
+  e |= pc_alloc(s->smp_bufs, &s->smp_valid, (size_t)max_samples);
+  if (e) {
+    s->recording = false;
+    return fail(c, SALMON_ERR_NOMEM, "sample store of %d residuals", max_samples);
+  }
+  s->smp_cap = max_samples;
+  return SALMON_OK;
+}
+
+int salmon_gpu_pc_samples(salmon_ctx* c, int* stored, int* valid) {
+  if (!c) return SALMON_ERR_ARG;
+  GsState* s = (GsState*)c->gs;
+  int r;
+  if (!s) {
+    if (stored) *stored = 0;
+    if (valid) *valid = 0;
+    return SALMON_OK;
+  }
+  if ((r = pc_sync_samples(c, s))) return r;
+  if (stored) *stored = s->smp_n;
+  if (valid) {
+    int v = 0;
+    for (int x : s->h_valid) v += x;
+    *valid = v;
+  }
+  return SALMON_OK;
+}
+
+int salmon_gpu_pc_get_sample(salmon_ctx* c, int i, double* g, double* x, int* ib, double* theta, int* valid) {
+  if (!c || !c->gs) return SALMON_ERR_ARG;
+  GsState* s = (GsState*)c->gs;
+  if (i < 0 || i >= s->smp_n) return fail(c, SALMON_ERR_ARG, "sample %d of %d", i, s->smp_n);
+  int r = pc_sync_samples(c, s);
+  if (r) return r;
+  const size_t NL = c->g.NL;
+  if (g) HIPCHK(c, hipMemcpy(g, s->smp_g + (size_t)i * NL, sizeof(cplx<double>) * NL, hipMemcpyDeviceToHost));
+  if (x) HIPCHK(c, hipMemcpy(x, s->smp_x + (size_t)i * NL, sizeof(cplx<double>) * NL, hipMemcpyDeviceToHost));
+  if (ib) *ib = s->h_ib[i];
+  if (theta) *theta = s->h_theta[i];
+  if (valid) *valid = s->h_valid[i];
+  return SALMON_OK;
+}
+
+int salmon_gpu_pc_set_targets(salmon_ctx* c, const double* kAc) {
+  int r = gs_check(c, kAc);
+  if (r) return r;
+  if (c->NK != 1) return fail(c, SALMON_ERR_UNSUPPORTED, "training targets are set for one k point");
+  GsState* s = (GsState*)c->gs;
+  const size_t n = (size_t)c->NB * c->g.NL;
+  if (!s->Zc && (r = gs_alloc(c, s, &s->Zc, n))) return r;
+  HIPCHK(c, hipMemcpyAsync(s->Zc, c->d_zu, sizeof(cplx<double>) * n, hipMemcpyDeviceToDevice, c->stream));
+  s->lam.resize(c->NB);
+  return salmon_gpu_sp_energy(c, kAc, s->lam.data());
+}
+
+static int pc_train_check(salmon_pcnet* n, salmon_ctx* c) {
+  if (!n || !c) return SALMON_ERR_ARG;
+  GsState* s = (GsState*)c->gs;
+  if (!s || s->smp_n == 0) return fail(c, SALMON_ERR_STATE, "no recorded samples (salmon_gpu_pc_record, then CG)");
+  if (!s->Zc || (int)s->lam.size() != c->NB) return fail(c, SALMON_ERR_STATE, "salmon_gpu_pc_set_targets not called");
+  return pc_sync_samples(c, s);
+}
+
+int salmon_gpu_pcnet_gradient(salmon_pcnet* n, salmon_ctx* c, int nsamp, const int* idx, double* loss, double* grad) {
+  int r = pc_train_check(n, c);
+  if (r) return r;
+  if (nsamp < 1 || !idx) return fail(c, SALMON_ERR_ARG, "no samples given");
+  std::vector<int> v(idx, idx + nsamp);
+  return pc_step(c, n, v, false, 0.0f, loss, grad);
+}
+
+int salmon_gpu_pcnet_train(salmon_pcnet* n, salmon_ctx* c, int nsteps, int batch, double lr, unsigned seed,
+                           double* mean_loss) {
+  int r = pc_train_check(n, c);
+  if (r) return r;
+  if (nsteps < 0 || batch < 1) return fail(c, SALMON_ERR_ARG, "nsteps < 0 or batch < 1");
+  GsState* s = (GsState*)c->gs;
+  std::vector<int> pool;
+  for (int i = 0; i < s->smp_n; ++i)
+    if (s->h_valid[i]) pool.push_back(i);
+  if (pool.empty()) return fail(c, SALMON_ERR_STATE, "no valid samples");
+  std::mt19937 rng(seed);
+  std::uniform_int_distribution<int> pick(0, (int)pool.size() - 1);
+  double tot = 0.0;
+  for (int k = 0; k < nsteps; ++k) {
+    std::vector<int> idx(batch);
+    for (int& i : idx) i = pool[pick(rng)];
+    double l = 0.0;
+    if ((r = pc_step(c, n, idx, true, (float)lr, &l, nullptr))) return r;
+    tot += l;
+  }
+  if (mean_loss) *mean_loss = nsteps > 0 ? tot / nsteps : 0.0;
   return SALMON_OK;
 }
 

@@ -150,6 +150,80 @@ module salmon_gpu
       integer(c_int), intent(out) :: sweeps
     end function
 
+    integer(c_int) function salmon_gpu_pcnet_create(seed, net) bind(C, name="salmon_gpu_pcnet_create")
+      import :: c_int, c_ptr
+      integer(c_int), value :: seed
+      type(c_ptr), intent(out) :: net
+    end function
+
+    integer(c_int) function salmon_gpu_pcnet_set_weights(net, w) bind(C, name="salmon_gpu_pcnet_set_weights")
+      import :: c_int, c_ptr, c_float
+      type(c_ptr), value :: net
+      real(c_float), intent(in) :: w(*)
+    end function
+
+    integer(c_int) function salmon_gpu_pcnet_get_weights(net, w) bind(C, name="salmon_gpu_pcnet_get_weights")
+      import :: c_int, c_ptr, c_float
+      type(c_ptr), value :: net
+      real(c_float), intent(out) :: w(*)
+    end function
+
+    integer(c_int) function salmon_gpu_pcnet_finalize(net) bind(C, name="salmon_gpu_pcnet_finalize")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: net
+    end function
+
+    integer(c_int) function salmon_gpu_set_pcnet(ctx, net) bind(C, name="salmon_gpu_set_pcnet")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, net
+    end function
+
+    integer(c_int) function salmon_gpu_pc_record(ctx, max_samples) bind(C, name="salmon_gpu_pc_record")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: max_samples
+    end function
+
+    integer(c_int) function salmon_gpu_pc_samples(ctx, stored, valid) bind(C, name="salmon_gpu_pc_samples")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), intent(out) :: stored, valid
+    end function
+
+    integer(c_int) function salmon_gpu_pc_get_sample(ctx, i, g, x, band, theta, valid) &
+        bind(C, name="salmon_gpu_pc_get_sample")
+      import :: c_int, c_ptr, c_double, c_double_complex
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: i
+      complex(c_double_complex), intent(out) :: g(*), x(*)
+      integer(c_int), intent(out) :: band, valid
+      real(c_double), intent(out) :: theta
+    end function
+
+    integer(c_int) function salmon_gpu_pc_set_targets(ctx, kAc) bind(C, name="salmon_gpu_pc_set_targets")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: kAc(*)
+    end function
+
+    integer(c_int) function salmon_gpu_pcnet_gradient(net, ctx, nsamp, idx, loss, grad) &
+        bind(C, name="salmon_gpu_pcnet_gradient")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: net, ctx
+      integer(c_int), value :: nsamp
+      integer(c_int), intent(in) :: idx(*)
+      real(c_double), intent(out) :: loss, grad(*)
+    end function
+
+    integer(c_int) function salmon_gpu_pcnet_train(net, ctx, nsteps, batch, lr, seed, mean_loss) &
+        bind(C, name="salmon_gpu_pcnet_train")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: net, ctx
+      integer(c_int), value :: nsteps, batch, seed
+      real(c_double), value :: lr
+      real(c_double), intent(out) :: mean_loss
+    end function
+
     integer(c_int) function salmon_gpu_rt_run(ctx, ac, nsteps, dt, reortho_every, jt, et, norm, rho) &
         bind(C, name="salmon_gpu_rt_run")
       import :: c_int, c_ptr, c_double

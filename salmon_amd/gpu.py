@@ -39,6 +39,7 @@ FP64 = 0
 MIXED = 1
 PATH_AUTO, PATH_STREAMED, PATH_FUSED = 0, 1, 2
 PC_NONE, PC_KINETIC, PC_LEARNED = 0, 1, 2
+PCNET_WEIGHTS = 27 * 2 * 16 + 27 * 16 * 16 + 27 * 16 * 2
 
 
 class SalmonGPUError(RuntimeError):
@@ -72,6 +73,17 @@ SIGNATURES = {
     "salmon_gpu_cg": [_vp, _vp, _ci, _ci, _vp, _vp],
     "salmon_gpu_subspace_diag": [_vp, _vp, _vp],
     "salmon_gpu_ground_state": [_vp, _vp, _ci, _ci, _ci, _cd, _vp, _vp, _vp],
+    "salmon_gpu_pcnet_create": [ctypes.c_uint, ctypes.POINTER(_vp)],
+    "salmon_gpu_pcnet_set_weights": [_vp, _vp],
+    "salmon_gpu_pcnet_get_weights": [_vp, _vp],
+    "salmon_gpu_pcnet_finalize": [_vp],
+    "salmon_gpu_set_pcnet": [_vp, _vp],
+    "salmon_gpu_pc_record": [_vp, _ci],
+    "salmon_gpu_pc_samples": [_vp, _vp, _vp],
+    "salmon_gpu_pc_get_sample": [_vp, _ci, _vp, _vp, _vp, _vp, _vp],
+    "salmon_gpu_pc_set_targets": [_vp, _vp],
+    "salmon_gpu_pcnet_gradient": [_vp, _vp, _ci, _vp, _vp, _vp],
+    "salmon_gpu_pcnet_train": [_vp, _vp, _ci, _ci, _cd, ctypes.c_uint, _vp],
     "salmon_gpu_rt_run": [_vp, _vp, _ci, _cd, _ci, _vp, _vp, _vp, _vp],
     "salmon_gpu_set_path": [_vp, _ci],
     "salmon_gpu_path": [_vp],
@@ -266,6 +278,31 @@ class Context:
                     "ground_state")
         return dict(esp=esp, esp_var=ev, sweeps=n.value)
 
+    # --- learned preconditioner: network, training samples -----------------
+    def set_pcnet(self, net):
+        self._pcnet = net
+        self._check(lib().salmon_gpu_set_pcnet(self._h, net._h if net is not None else None), "set_pcnet")
+
+    def pc_record(self, max_samples):
+        self._check(lib().salmon_gpu_pc_record(self._h, int(max_samples)), "pc_record")
+
+    def pc_samples(self):
+        st, va = ctypes.c_int(), ctypes.c_int()
+        self._check(lib().salmon_gpu_pc_samples(self._h, ctypes.byref(st), ctypes.byref(va)), "pc_samples")
+        return st.value, va.value
+
+    def pc_get_sample(self, i):
+        g = np.zeros(self.sys.NL, np.complex128)
+        x = np.zeros(self.sys.NL, np.complex128)
+        ib, va = ctypes.c_int(), ctypes.c_int()
+        th = ctypes.c_double()
+        self._check(lib().salmon_gpu_pc_get_sample(self._h, int(i), _p(g), _p(x), ctypes.byref(ib), ctypes.byref(th),
+                                                   ctypes.byref(va)), "pc_get_sample")
+        return dict(g=g, x=x, band=ib.value, theta=th.value, valid=bool(va.value))
+
+    def pc_set_targets(self, kAc):
+        self._check(lib().salmon_gpu_pc_set_targets(self._h, _p(_f64(kAc))), "pc_set_targets")
+
     def stream(self):
         """The context's hipStream_t (as an integer handle)."""
         return lib().salmon_gpu_stream(self._h)
@@ -278,3 +315,54 @@ class Context:
 
     def synchronize(self):
         self._check(lib().salmon_gpu_synchronize(self._h), "synchronize")
+
+
+class PcNet:
+    """The learned CG preconditioner's network (device weights + Adam state)."""
+
+    def __init__(self, seed=0, weights=None):
+        self._h = ctypes.c_void_p()
+        rc = lib().salmon_gpu_pcnet_create(int(seed), ctypes.byref(self._h))
+        if rc != 0:
+            raise SalmonGPUError(f"salmon_gpu_pcnet_create: {lib().salmon_gpu_errstr(rc).decode()}")
+        if weights is not None:
+            self.set_weights(weights)
+
+    def set_weights(self, w):
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        if w.size != PCNET_WEIGHTS:
+            raise SalmonGPUError(f"expected {PCNET_WEIGHTS} weights")
+        rc = lib().salmon_gpu_pcnet_set_weights(self._h, _p(w))
+        if rc != 0:
+            raise SalmonGPUError("pcnet_set_weights failed")
+
+    def weights(self):
+        w = np.zeros(PCNET_WEIGHTS, np.float32)
+        if lib().salmon_gpu_pcnet_get_weights(self._h, _p(w)) != 0:
+            raise SalmonGPUError("pcnet_get_weights failed")
+        return w
+
+    def gradient(self, ctx, idx):
+        idx = _i32(idx)
+        g = np.zeros(PCNET_WEIGHTS)
+        loss = ctypes.c_double()
+        ctx._check(lib().salmon_gpu_pcnet_gradient(self._h, ctx._h, int(idx.size), _p(idx), ctypes.byref(loss), _p(g)),
+                   "pcnet_gradient")
+        return loss.value, g
+
+    def train(self, ctx, nsteps, batch=4, lr=1e-3, seed=0):
+        loss = ctypes.c_double()
+        ctx._check(lib().salmon_gpu_pcnet_train(self._h, ctx._h, int(nsteps), int(batch), float(lr), int(seed),
+                                                ctypes.byref(loss)), "pcnet_train")
+        return loss.value
+
+    def close(self):
+        if self._h:
+            lib().salmon_gpu_pcnet_finalize(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
