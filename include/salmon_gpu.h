@@ -153,9 +153,9 @@ int salmon_gpu_subspace_diag(salmon_ctx *ctx, const double *kAc, double *esp);
  * sqrt(1e-15 * gap): CG_ompk leaves a band once its Rayleigh quotient moves by
  * less than 1e-15, CG.f90:42,114.)  Outputs (nullable): esp(NB,NK) = the
  * band energies after the last sweep, esp_var of its CG call, sweeps = sweeps
- * run. */
+ * run, history(NB*NK, max_sweeps) = the band energies after each sweep. */
 int salmon_gpu_ground_state(salmon_ctx *ctx, const double *kAc, int max_sweeps, int iter_cg_max, int precond,
-                            double tol, double *esp, double *esp_var, int *sweeps);
+                            double tol, double *esp, double *esp_var, int *sweeps, double *history);
 
 /* ---- Learned CG preconditioner (no reference counterpart) ----------------
  * A three-layer periodic 3x3x3 convolution network without biases (channels
@@ -187,6 +187,10 @@ int salmon_gpu_pcnet_create(unsigned seed, salmon_pcnet **net);  /* He-normal in
 int salmon_gpu_pcnet_set_weights(salmon_pcnet *net, const float *w);  /* also resets Adam */
 int salmon_gpu_pcnet_get_weights(salmon_pcnet *net, float *w);
 int salmon_gpu_pcnet_finalize(salmon_pcnet *net);
+/* d(NL) = the learned direction of one residual g(NL) (complex, one k point),
+ * computed as inside CG: BF16 weights and activations on the matrix cores
+ * (v_mfma_f32_16x16x32_bf16, FP32 accumulation), scaled by |g|/sqrt(NL) */
+int salmon_gpu_pcnet_apply(salmon_pcnet *net, salmon_ctx *ctx, const double *g, double *d);
 /* network used by SALMON_PC_LEARNED on ctx (NULL detaches; not owned) */
 int salmon_gpu_set_pcnet(salmon_ctx *ctx, salmon_pcnet *net);
 /* record up to max_samples residuals of the following CG calls (0 stops) */

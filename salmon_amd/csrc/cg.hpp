@@ -104,7 +104,7 @@ struct Src {
 //   PROJ_VEC    complex coefficients (CG.f90:86-89)
 //   PROJ_GRAM   stores row ib of G from the stored band ib (no solve).
 // Grid: (nchunk, ceil(ib/kJG), NK).
-enum : int { PROJ_START = 0, PROJ_VEC = 1, PROJ_GRAM = 2 };
+enum : int { PROJ_START = 0, PROJ_VEC = 1, PROJ_GRAM = 2, PROJ_PLAIN = 3 };
 constexpr int kCgMaxNB = 1024;
 
 __global__ void __launch_bounds__(256) k_cg_proj(State S, const cplx<double>* __restrict__ zu, int ib, Src src,
@@ -167,6 +167,10 @@ __global__ void __launch_bounds__(256) k_cg_proj(State S, const cplx<double>* __
   if (mode == PROJ_GRAM) {
     // raw_j = Hxyz <B_j|B_ib>, so G(ib, j) = conj(raw_j)
     for (int j = threadIdx.x; j < ib; j += 256) gram[(size_t)ib * S.NB + j] = cplx<double>{zsh[j].re, -zsh[j].im};
+    return;
+  }
+  if (mode == PROJ_PLAIN) {  // coefficients against the unmodified vector
+    for (int j = threadIdx.x; j < ib; j += 256) S.zs[(size_t)k * S.NB + j] = zsh[j];
     return;
   }
   if (threadIdx.x < kWave) {
@@ -678,6 +682,50 @@ __global__ void __launch_bounds__(256) k_rotate_bands(const cplx<double>* __rest
 #pragma unroll
   for (int ii = 0; ii < kTile; ++ii)
     if (i0 + ii < NB) C[((size_t)k * NB + i0 + ii) * NL + p] = cplx<double>{re[ii], im[ii]};
+}
+
+// Gram-Schmidt step of band ib in place (Gram_Schmidt_ompk,
+// ARTED/GS/Gram_Schmidt.f90:43-50): v -= sum_{j<ib} zs_j B_j; with_norm ends
+// with SN = 1/sqrt(|v|^2 Hxyz).  Grid (nchunk, NK).
+__global__ void __launch_bounds__(256) k_gs_sub(State S, cplx<double>* __restrict__ zu, int ib, int with_norm) {
+  const int k = blockIdx.y, c = blockIdx.x;
+  cplx<double>* B = zu + (size_t)k * S.NB * S.NL;
+  cplx<double>* v = B + (size_t)ib * S.NL;
+  const cplx<double>* zs = S.zs + (size_t)k * S.NB;
+  double a[1] = {0.0};
+#pragma unroll
+  for (int i = 0; i < kPPT; ++i) {
+    const int p = c * kChunk + i * 256 + threadIdx.x;
+    if (p < S.NL) {
+      cplx<double> e = v[p];
+      for (int j = 0; j < ib; ++j) {
+        const cplx<double> b = B[(size_t)j * S.NL + p], z = zs[j];
+        e.re -= b.re * z.re - b.im * z.im;
+        e.im -= b.re * z.im + b.im * z.re;
+      }
+      v[p] = e;
+      a[0] += e.re * e.re + e.im * e.im;
+    }
+  }
+  if (!with_norm) return;
+  double tot[1];
+  if (!last_block_sum<1>(a, S, S.nchunk, c, tot)) return;
+  if (threadIdx.x == 0) S.sc[k * kNS + SN] = 1.0 / sqrt(tot[0] * S.Hxyz);
+}
+
+// band ib *= SN.  Grid (nchunk, NK).
+__global__ void __launch_bounds__(256) k_gs_scale(State S, cplx<double>* __restrict__ zu, int ib) {
+  const int k = blockIdx.y;
+  const double f = S.sc[k * kNS + SN];
+  cplx<double>* v = zu + ((size_t)k * S.NB + ib) * S.NL;
+#pragma unroll
+  for (int i = 0; i < kPPT; ++i) {
+    const int p = blockIdx.x * kChunk + i * 256 + threadIdx.x;
+    if (p < S.NL) {
+      v[p].re *= f;
+      v[p].im *= f;
+    }
+  }
 }
 
 // Training sample of the learned preconditioner: the orthogonalised residual

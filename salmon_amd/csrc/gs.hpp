@@ -12,7 +12,8 @@ using namespace salmon::cg;
 struct PcInfer {
   int NK = 0, NL = 0;
   std::vector<void*> bufs;
-  float *u = nullptr, *a1 = nullptr, *a2 = nullptr, *y = nullptr;
+  unsigned short *ub = nullptr, *a1b = nullptr, *a2b = nullptr;  // BF16 activations
+  float* y = nullptr;
   double *stats = nullptr, *part = nullptr;
 };
 
@@ -267,6 +268,36 @@ static int cg_run(salmon_ctx* c, int ncg, int precond) {
     hipLaunchKernelGGL(k_cg_efin, gv, blk, 0, c->stream, S, (const cplx<double>*)w[W_X], (const cplx<double>*)w[W_HP]);
     LAUNCHCHK(c);
     hipLaunchKernelGGL(k_cg_resid, gv, blk, 0, c->stream, S, (const cplx<double>*)w[W_X], (const cplx<double>*)w[W_HP], ib);
+    LAUNCHCHK(c);
+  }
+  return SALMON_OK;
+}
+
+// Gram_Schmidt_ompk (ARTED/GS/Gram_Schmidt.f90:41-51) on the resident FP64
+// orbitals, all k at once and every band over the whole chip: the sequential
+// loop's overlaps are taken against the unmodified band and the subtraction
+// is repeated once (classical Gram-Schmidt twice, equal to the reference's
+// modified form up to rounding), then the band is normalised.
+static int gram_schmidt_multi(salmon_ctx* c, cplx<double>* zu) {
+  int r;
+  if ((r = gs_ensure(c))) return r;
+  GsState* s = (GsState*)c->gs;
+  State S = gs_view(c, s);
+  const dim3 gv(s->nchunk, s->NK), blk(256);
+  for (int ib = 0; ib < s->NB; ++ib) {
+    const Src v{zu + (size_t)ib * s->NL, (size_t)s->NB * s->NL, nullptr, nullptr};
+    for (int pass = 0; pass < 2 && ib > 0; ++pass) {
+      hipLaunchKernelGGL(k_cg_proj, dim3(s->nchunk, (ib + kJG - 1) / kJG, s->NK), blk, 0, c->stream, S,
+                         (const cplx<double>*)zu, ib, v, (int)PROJ_PLAIN, 0);
+      LAUNCHCHK(c);
+      hipLaunchKernelGGL(k_gs_sub, gv, blk, 0, c->stream, S, zu, ib, pass == 1 ? 1 : 0);
+      LAUNCHCHK(c);
+    }
+    if (ib == 0) {
+      hipLaunchKernelGGL(k_gs_sub, gv, blk, 0, c->stream, S, zu, 0, 1);
+      LAUNCHCHK(c);
+    }
+    hipLaunchKernelGGL(k_gs_scale, gv, blk, 0, c->stream, S, zu, ib);
     LAUNCHCHK(c);
   }
   return SALMON_OK;

@@ -28,8 +28,12 @@ struct salmon_pcnet {
          *lam = nullptr;
   int* ib = nullptr;
   void* uv = nullptr;
-  std::vector<void*> owned;  // w, m, v, grad
+  std::vector<void*> owned;  // w, m, v, grad, bt
+  unsigned short* bt = nullptr;  // BF16 weights of the three layers, [n][kk] per layer
+  bool packed = false;           // bt holds the current w
 };
+
+constexpr int kBt1 = 16 * kpad<C0>(), kBt2 = 16 * kpad<C1>(), kBt3 = 16 * kpad<C2>();
 
 static int pc_fail(salmon_pcnet* n, int code, const char* msg) {
   if (n) n->err = msg;
@@ -126,17 +130,20 @@ static int pc_wgrad(hipStream_t st, salmon_pcnet* n, const Grid& g, int ns, cons
   return hipGetLastError() == hipSuccess ? SALMON_OK : SALMON_ERR_HIP;
 }
 
+// Learned direction of n residual rows g (stride NL) into out, on the matrix
+// cores with BF16 weights and activations.
 static int pc_infer(salmon_ctx* c, GsState* s, salmon_pcnet* net, const cplx<double>* g, cplx<double>* out) {
   PcInfer& I = s->inf;
   const int NK = s->NK, NL = s->NL;
+  const hipStream_t st = c->stream;
   if (I.NK != NK || I.NL != NL) {
     for (void* b : I.bufs) (void)hipFree(b);
     I.bufs.clear();
     const size_t v = (size_t)NK * NL;
     int r = 0;
-    r |= pc_alloc(I.bufs, &I.u, v * C0);
-    r |= pc_alloc(I.bufs, &I.a1, v * C1);
-    r |= pc_alloc(I.bufs, &I.a2, v * C2);
+    r |= pc_alloc(I.bufs, &I.ub, v * C0);
+    r |= pc_alloc(I.bufs, &I.a1b, v * C1);
+    r |= pc_alloc(I.bufs, &I.a2b, v * C2);
     r |= pc_alloc(I.bufs, &I.y, v * C3);
     r |= pc_alloc(I.bufs, &I.stats, (size_t)4 * NK);
     r |= pc_alloc(I.bufs, &I.part, (size_t)3 * NK * ((NL + kRedChunk - 1) / kRedChunk));
@@ -144,9 +151,28 @@ static int pc_infer(salmon_ctx* c, GsState* s, salmon_pcnet* net, const cplx<dou
     I.NK = NK;
     I.NL = NL;
   }
-  if (pc_canon(c->stream, g, NK, NL, I.part, I.stats, I.u)) return fail(c, SALMON_ERR_HIP, "pc canon launch");
-  if (pc_forward(c->stream, net->w, c->g, NK, I.u, I.a1, I.a2, I.y)) return fail(c, SALMON_ERR_HIP, "pc forward launch");
-  hipLaunchKernelGGL(k_pc_output, dim3((NL + kBlk - 1) / kBlk, NK), dim3(kBlk), 0, c->stream, (const float*)I.y,
+  if (!net->packed) {
+    hipLaunchKernelGGL((k_pc_pack_bf16<C0, C1>), dim3((kBt1 + 255) / 256), dim3(256), 0, st, (const float*)net->w, net->bt);
+    hipLaunchKernelGGL((k_pc_pack_bf16<C1, C2>), dim3((kBt2 + 255) / 256), dim3(256), 0, st, (const float*)(net->w + NW1),
+                       net->bt + kBt1);
+    hipLaunchKernelGGL((k_pc_pack_bf16<C2, C3>), dim3((kBt3 + 255) / 256), dim3(256), 0, st,
+                       (const float*)(net->w + NW1 + NW2), net->bt + kBt1 + kBt2);
+    LAUNCHCHK(c);
+    net->packed = true;
+  }
+  const int nchunk = (NL + kRedChunk - 1) / kRedChunk;
+  hipLaunchKernelGGL(k_pc_stats, dim3(nchunk, NK), dim3(kBlk), 0, st, g, NL, I.part);
+  hipLaunchKernelGGL(k_pc_stats_fin, dim3((NK + 63) / 64), dim3(64), 0, st, (const double*)I.part, nchunk, NK, NL, I.stats);
+  hipLaunchKernelGGL(k_pc_input_bf16, dim3((NL + kBlk - 1) / kBlk, NK), dim3(kBlk), 0, st, g, (const double*)I.stats, NL,
+                     I.ub);
+  const dim3 gm((NL + 63) / 64, NK);
+  hipLaunchKernelGGL((k_conv_mfma<C0, C1, true, true>), gm, dim3(kBlk), 0, st, (const unsigned short*)I.ub,
+                     (const unsigned short*)net->bt, c->g, (void*)I.a1b);
+  hipLaunchKernelGGL((k_conv_mfma<C1, C2, true, true>), gm, dim3(kBlk), 0, st, (const unsigned short*)I.a1b,
+                     (const unsigned short*)(net->bt + kBt1), c->g, (void*)I.a2b);
+  hipLaunchKernelGGL((k_conv_mfma<C2, C3, false, false>), gm, dim3(kBlk), 0, st, (const unsigned short*)I.a2b,
+                     (const unsigned short*)(net->bt + kBt1 + kBt2), c->g, (void*)I.y);
+  hipLaunchKernelGGL(k_pc_output, dim3((NL + kBlk - 1) / kBlk, NK), dim3(kBlk), 0, st, (const float*)I.y,
                      (const double*)I.stats, NL, 1, out);
   LAUNCHCHK(c);
   return SALMON_OK;
@@ -232,6 +258,7 @@ static int pc_step(salmon_ctx* c, salmon_pcnet* n, const std::vector<int>& idx, 
     hipLaunchKernelGGL(k_adam, dim3((NW + kBlk - 1) / kBlk), dim3(kBlk), 0, st, n->w, n->m, n->v,
                        (const double*)n->grad, NW, lr, b1, b2, 1e-8f, c1, c2);
     LAUNCHCHK(c);
+    n->packed = false;
   }
   HIPCHK(c, hipStreamSynchronize(st));
   double l = 0.0;

@@ -401,5 +401,97 @@ __global__ void __launch_bounds__(kBlk) k_adam(float* __restrict__ w, float* __r
   w[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
 }
 
+// ---------------------------------------------------------------------------
+// Inference with BF16 weights and activations on the matrix cores
+// (v_mfma_f32_16x16x32_bf16, FP32 accumulation).  A convolution layer is the
+// GEMM out[p][n] = sum_kk A[p][kk] B[kk][n] with kk = t*CI + ci (27*CI padded
+// to a multiple of 32), A[p][kk] = a[nbr(p,t)][ci], B[kk][n] = W[t][ci][n].
+// One wave computes 16 points x 16 outputs; per 32-wide K step lane l holds
+// A[l%16][8*(l/16) .. +7] and B[8*(l/16) .. +7][l%16] and receives
+// D[4*(l/16) .. +3][l%16].
+// ---------------------------------------------------------------------------
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ unsigned short f2bf(float f) {  // round to nearest even
+  const unsigned int u = __float_as_uint(f);
+  return (unsigned short)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+}
+
+template <int CI>
+constexpr int kpad() { return (27 * CI + 31) / 32 * 32; }
+
+// W (FP32 [t][ci][co]) -> Bt (bf16 [n][kk], n < 16, zero padded).  One thread per entry.
+template <int CI, int CO>
+__global__ void k_pc_pack_bf16(const float* __restrict__ W, unsigned short* __restrict__ Bt) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  constexpr int K = kpad<CI>();
+  if (i >= 16 * K) return;
+  const int n = i / K, kk = i % K;
+  float v = 0.0f;
+  if (n < CO && kk < 27 * CI) v = W[(size_t)kk * CO + n];
+  Bt[i] = f2bf(v);
+}
+
+// u (bf16 [s][p][2]) from the residual rows, as k_pc_input.
+__global__ void __launch_bounds__(kBlk) k_pc_input_bf16(const cplx<double>* __restrict__ g, const double* __restrict__ stats,
+                                                        int NL, unsigned short* __restrict__ u) {
+  const int s = blockIdx.y, p = blockIdx.x * kBlk + threadIdx.x;
+  if (p >= NL) return;
+  const double cr = stats[4 * s], ci = stats[4 * s + 1], sc = stats[4 * s + 2];
+  const cplx<double> e = g[(size_t)s * NL + p];
+  u[((size_t)s * NL + p) * 2] = f2bf((float)((e.re * cr + e.im * ci) * sc));
+  u[((size_t)s * NL + p) * 2 + 1] = f2bf((float)((e.im * cr - e.re * ci) * sc));
+}
+
+// One layer.  in: bf16 [s][p][CI]; out: bf16 [s][p][CO] (OUT_BF16) or FP32.
+// Block = 4 waves = 64 points; grid (ceil(NL/64), n).
+template <int CI, int CO, bool RELU, bool OUT_BF16>
+__global__ void __launch_bounds__(kBlk) k_conv_mfma(const unsigned short* __restrict__ in,
+                                                    const unsigned short* __restrict__ Bt, Grid g, void* __restrict__ out) {
+  constexpr int K = kpad<CI>();
+  __shared__ __attribute__((aligned(16))) unsigned short sb[16 * K];
+  for (int i = threadIdx.x; i < 16 * K / 8; i += kBlk) ((uint4*)sb)[i] = ((const uint4*)Bt)[i];
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, s = blockIdx.y;
+  const int p0 = (blockIdx.x * 4 + wave) * 16;
+  const int m = lane & 15, kq = lane >> 4;
+  const int p = p0 + m;
+  const bool ok = p < g.NL;
+  const int iz = p % g.NLz, iy = (p / g.NLz) % g.NLy, ix = p / (g.NLz * g.NLy);
+  const unsigned short* a = in + (size_t)s * g.NL * CI;
+  f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+  for (int k0 = 0; k0 < K; k0 += 32) {
+    const int kb = k0 + 8 * kq;
+    union { bf16x8 v; unsigned short h[8]; uint4 q; } A, B;
+    if constexpr (CI % 8 == 0) {
+      const int t = kb / CI, c0 = kb % CI;
+      if (ok && t < 27) A.q = *(const uint4*)(a + (size_t)nbr(ix, iy, iz, t, g) * CI + c0);
+      else A.q = make_uint4(0u, 0u, 0u, 0u);
+    } else {
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const int kk = kb + j, t = kk / CI, ci = kk % CI;
+        A.h[j] = (ok && t < 27) ? a[(size_t)nbr(ix, iy, iz, t, g) * CI + ci] : (unsigned short)0;
+      }
+    }
+    B.q = *(const uint4*)(sb + m * K + kb);
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A.v, B.v, acc, 0, 0, 0);
+  }
+  const int n = m;
+  if (n < CO) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int q = p0 + 4 * kq + i;
+      if (q < g.NL) {
+        float v = acc[i];
+        if (RELU) v = fmaxf(v, 0.0f);
+        if constexpr (OUT_BF16) ((unsigned short*)out)[((size_t)s * g.NL + q) * CO + n] = f2bf(v);
+        else ((float*)out)[((size_t)s * g.NL + q) * CO + n] = v;
+      }
+    }
+  }
+}
+
 }  // namespace pc
 }  // namespace salmon

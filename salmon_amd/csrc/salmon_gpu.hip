@@ -223,14 +223,13 @@ int density_dev(salmon_ctx* c) {
   return density_reduce(c, c->d_rho_part, nch);
 }
 
+int gram_schmidt_multi_fwd(salmon_ctx* c, cplx<double>* zu);
+
+// FP64 Gram-Schmidt of the resident orbitals (for complex64 orbitals on an
+// FP64 copy)
 int gram_schmidt_dev(salmon_ctx* c) {
   const size_t n = (size_t)c->NKB * c->g.NL;
-  if (c->precision == SALMON_FP64) {
-    hipLaunchKernelGGL(k_gram_schmidt, dim3(c->NK), dim3(256), 0, c->stream, (cplx<double>*)c->d_zu, c->NB, c->g.NL,
-                       c->g.Hxyz);
-    LAUNCHCHK(c);
-    return SALMON_OK;
-  }
+  if (c->precision == SALMON_FP64) return gram_schmidt_multi_fwd(c, (cplx<double>*)c->d_zu);
   if (!c->d_f64) {
     int r = dalloc(c, &c->d_f64, n);
     if (r) return r;
@@ -239,8 +238,8 @@ int gram_schmidt_dev(salmon_ctx* c) {
   hipLaunchKernelGGL((k_convert<float, double>), dim3(nb), dim3(256), 0, c->stream, (const cplx<float>*)c->d_zu,
                      c->d_f64, n);
   LAUNCHCHK(c);
-  hipLaunchKernelGGL(k_gram_schmidt, dim3(c->NK), dim3(256), 0, c->stream, c->d_f64, c->NB, c->g.NL, c->g.Hxyz);
-  LAUNCHCHK(c);
+  int r = gram_schmidt_multi_fwd(c, c->d_f64);
+  if (r) return r;
   hipLaunchKernelGGL((k_convert<double, float>), dim3(nb), dim3(256), 0, c->stream, (const cplx<double>*)c->d_f64,
                      (cplx<float>*)c->d_zu, n);
   LAUNCHCHK(c);
@@ -266,6 +265,14 @@ int read_sums(salmon_ctx* c, double out[8]) {
 #include "fused.hpp"
 #include "gs.hpp"
 #include "pc.hpp"
+
+namespace {
+int gram_schmidt_multi_fwd(salmon_ctx* c, cplx<double>* zu) {
+  int r = ensure_work(c);
+  if (r) return r;
+  return gram_schmidt_multi(c, zu);
+}
+}  // namespace
 
 namespace {
 
@@ -712,7 +719,7 @@ int salmon_gpu_subspace_diag(salmon_ctx* c, const double* kAc, double* esp) {
 }
 
 int salmon_gpu_ground_state(salmon_ctx* c, const double* kAc, int max_sweeps, int iter_cg_max, int precond,
-                            double tol, double* esp, double* esp_var, int* sweeps) {
+                            double tol, double* esp, double* esp_var, int* sweeps, double* history) {
   int r = gs_check(c, kAc);
   if (r) return r;
   if (max_sweeps < 1 || iter_cg_max < 1) return fail(c, SALMON_ERR_ARG, "max_sweeps and iter_cg_max must be >= 1");
@@ -743,6 +750,7 @@ int salmon_gpu_ground_state(salmon_ctx* c, const double* kAc, int max_sweeps, in
       prev_drop = drop;
     }
     prev_sum = sum;
+    if (history) std::copy(e1.begin(), e1.end(), history + (size_t)(n - 1) * c->NKB);
     e0.swap(e1);
     if (n > 1 && change < tol) break;
   }
@@ -762,6 +770,7 @@ int salmon_gpu_pcnet_create(unsigned seed, salmon_pcnet** out) {
   r |= pc_alloc(own, &n->m, NW);
   r |= pc_alloc(own, &n->v, NW);
   r |= pc_alloc(own, &n->grad, NW);
+  r |= pc_alloc(own, &n->bt, (size_t)kBt1 + kBt2 + kBt3);
   n->owned = own;
   if (r) {
     salmon_gpu_pcnet_finalize(n);
@@ -790,6 +799,7 @@ int salmon_gpu_pcnet_set_weights(salmon_pcnet* n, const float* w) {
   if (hipMemset(n->m, 0, sizeof(float) * NW) != hipSuccess || hipMemset(n->v, 0, sizeof(float) * NW) != hipSuccess)
     return SALMON_ERR_HIP;
   n->step = 0;
+  n->packed = false;
   return SALMON_OK;
 }
 
@@ -804,6 +814,21 @@ int salmon_gpu_pcnet_finalize(salmon_pcnet* n) {
   pcnet_free_buffers(n);
   for (void* b : n->owned) (void)hipFree(b);
   delete n;
+  return SALMON_OK;
+}
+
+int salmon_gpu_pcnet_apply(salmon_pcnet* n, salmon_ctx* c, const double* g, double* d) {
+  if (!n || !g || !d) return c ? fail(c, SALMON_ERR_ARG, "NULL argument") : SALMON_ERR_ARG;
+  int r = check_ready(c, false);
+  if (r) return r;
+  if (c->NK != 1) return fail(c, SALMON_ERR_UNSUPPORTED, "pcnet_apply on a context with one k point");
+  if ((r = ensure_work(c)) || (r = gs_ensure(c))) return r;
+  GsState* s = (GsState*)c->gs;
+  const size_t NL = c->g.NL;
+  HIPCHK(c, hipMemcpyAsync(s->w[W_G], g, sizeof(cplx<double>) * NL, hipMemcpyHostToDevice, c->stream));
+  if ((r = pc_infer(c, s, n, s->w[W_G], s->w[W_KG]))) return r;
+  HIPCHK(c, hipMemcpyAsync(d, s->w[W_KG], sizeof(cplx<double>) * NL, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
   return SALMON_OK;
 }
 

@@ -327,43 +327,6 @@ __global__ void __launch_bounds__(256) k_density_slices(const double* __restrict
   part2[(size_t)sl * NL + p] = r;
 }
 
-// FP64 Gram-Schmidt of NB bands of each k (one block per k), modified form of
-// Gram_Schmidt_ompk (ARTED/GS/Gram_Schmidt.f90:41-51).
-__global__ void __launch_bounds__(256) k_gram_schmidt(cplx<double>* __restrict__ zu, int NB, int NL, double Hxyz) {
-  __shared__ double smem[4 * 2];
-  __shared__ double bc[2];
-  cplx<double>* b0 = zu + (size_t)blockIdx.x * NB * NL;
-  for (int ib = 0; ib < NB; ++ib) {
-    cplx<double>* x = b0 + (size_t)ib * NL;
-    for (int ibt = 0; ibt < ib; ++ibt) {
-      const cplx<double>* y = b0 + (size_t)ibt * NL;
-      double v[2] = {0, 0};
-      for (int p = threadIdx.x; p < NL; p += 256) {
-        v[0] += y[p].re * x[p].re + y[p].im * x[p].im;
-        v[1] += y[p].re * x[p].im - y[p].im * x[p].re;
-      }
-      block_sum256<2>(v, smem);
-      if (threadIdx.x == 0) { bc[0] = v[0] * Hxyz; bc[1] = v[1] * Hxyz; }
-      __syncthreads();
-      const double zr = bc[0], zi = bc[1];
-      for (int p = threadIdx.x; p < NL; p += 256) {
-        const cplx<double> yy = y[p];
-        x[p].re -= yy.re * zr - yy.im * zi;
-        x[p].im -= yy.re * zi + yy.im * zr;
-      }
-      __syncthreads();
-    }
-    double v[1] = {0};
-    for (int p = threadIdx.x; p < NL; p += 256) v[0] += x[p].re * x[p].re + x[p].im * x[p].im;
-    block_sum256<1>(v, smem);
-    if (threadIdx.x == 0) bc[0] = sqrt(v[0] * Hxyz);
-    __syncthreads();
-    const double r = bc[0];
-    for (int p = threadIdx.x; p < NL; p += 256) { x[p].re /= r; x[p].im /= r; }
-    __syncthreads();
-  }
-}
-
 // <x|y> Hxyz over rows of (n, NL) pairs: dst(o) = sum conj(x) y * Hxyz (complex)
 template <typename T>
 __global__ void __launch_bounds__(256) k_dot_rows(const cplx<T>* __restrict__ x, const cplx<T>* __restrict__ y,

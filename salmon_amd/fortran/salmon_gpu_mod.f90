@@ -140,7 +140,7 @@ module salmon_gpu
     end function
 
     integer(c_int) function salmon_gpu_ground_state(ctx, kAc, max_sweeps, iter_cg_max, precond, tol, esp, esp_var, &
-        sweeps) bind(C, name="salmon_gpu_ground_state")
+        sweeps, history) bind(C, name="salmon_gpu_ground_state")
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: ctx
       real(c_double), intent(in) :: kAc(*)
@@ -148,6 +148,7 @@ module salmon_gpu
       real(c_double), value :: tol
       real(c_double), intent(out) :: esp(*), esp_var(*)
       integer(c_int), intent(out) :: sweeps
+      type(c_ptr), value :: history
     end function
 
     integer(c_int) function salmon_gpu_pcnet_create(seed, net) bind(C, name="salmon_gpu_pcnet_create")
@@ -171,6 +172,13 @@ module salmon_gpu
     integer(c_int) function salmon_gpu_pcnet_finalize(net) bind(C, name="salmon_gpu_pcnet_finalize")
       import :: c_int, c_ptr
       type(c_ptr), value :: net
+    end function
+
+    integer(c_int) function salmon_gpu_pcnet_apply(net, ctx, g, d) bind(C, name="salmon_gpu_pcnet_apply")
+      import :: c_int, c_ptr, c_double_complex
+      type(c_ptr), value :: net, ctx
+      complex(c_double_complex), intent(in) :: g(*)
+      complex(c_double_complex), intent(out) :: d(*)
     end function
 
     integer(c_int) function salmon_gpu_set_pcnet(ctx, net) bind(C, name="salmon_gpu_set_pcnet")

@@ -72,11 +72,12 @@ SIGNATURES = {
     "salmon_gpu_sp_energy": [_vp, _vp, _vp],
     "salmon_gpu_cg": [_vp, _vp, _ci, _ci, _vp, _vp],
     "salmon_gpu_subspace_diag": [_vp, _vp, _vp],
-    "salmon_gpu_ground_state": [_vp, _vp, _ci, _ci, _ci, _cd, _vp, _vp, _vp],
+    "salmon_gpu_ground_state": [_vp, _vp, _ci, _ci, _ci, _cd, _vp, _vp, _vp, _vp],
     "salmon_gpu_pcnet_create": [ctypes.c_uint, ctypes.POINTER(_vp)],
     "salmon_gpu_pcnet_set_weights": [_vp, _vp],
     "salmon_gpu_pcnet_get_weights": [_vp, _vp],
     "salmon_gpu_pcnet_finalize": [_vp],
+    "salmon_gpu_pcnet_apply": [_vp, _vp, _vp, _vp],
     "salmon_gpu_set_pcnet": [_vp, _vp],
     "salmon_gpu_pc_record": [_vp, _ci],
     "salmon_gpu_pc_samples": [_vp, _vp, _vp],
@@ -268,15 +269,16 @@ class Context:
 
     def ground_state(self, kAc, max_sweeps, iter_cg_max=4, precond=PC_KINETIC, tol=1e-10):
         """Sweeps of Gram-Schmidt, diag, Gram-Schmidt, CG, Gram-Schmidt until no band energy moves by tol
-        between two sweeps; returns dict(esp, esp_var, sweeps)."""
+        between two sweeps; returns dict(esp, esp_var, sweeps, history (sweeps, NK, NB))."""
         s = self.sys
         esp = np.zeros((s.NK, s.NB))
         ev = np.zeros((s.NK, s.NB))
+        hist = np.zeros((int(max_sweeps), s.NK, s.NB))
         n = ctypes.c_int()
         self._check(lib().salmon_gpu_ground_state(self._h, _p(_f64(kAc)), int(max_sweeps), int(iter_cg_max),
-                                                  int(precond), float(tol), _p(esp), _p(ev), ctypes.byref(n)),
-                    "ground_state")
-        return dict(esp=esp, esp_var=ev, sweeps=n.value)
+                                                  int(precond), float(tol), _p(esp), _p(ev), ctypes.byref(n),
+                                                  _p(hist)), "ground_state")
+        return dict(esp=esp, esp_var=ev, sweeps=n.value, history=hist[: n.value])
 
     # --- learned preconditioner: network, training samples -----------------
     def set_pcnet(self, net):
@@ -349,6 +351,13 @@ class PcNet:
         ctx._check(lib().salmon_gpu_pcnet_gradient(self._h, ctx._h, int(idx.size), _p(idx), ctypes.byref(loss), _p(g)),
                    "pcnet_gradient")
         return loss.value, g
+
+    def apply(self, ctx, g):
+        """The learned direction of one residual (BF16 matrix-core inference, as inside CG)."""
+        g = np.ascontiguousarray(g, dtype=np.complex128)
+        d = np.zeros_like(g)
+        ctx._check(lib().salmon_gpu_pcnet_apply(self._h, ctx._h, _p(g), _p(d)), "pcnet_apply")
+        return d
 
     def train(self, ctx, nsteps, batch=4, lr=1e-3, seed=0):
         loss = ctypes.c_double()

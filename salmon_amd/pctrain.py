@@ -1,0 +1,129 @@
+"""Training and evaluation of the learned CG preconditioner on the 32^3
+ground-state workload (L = 15 bohr, Gamma, 64 bands, 27 random-position
+atoms with the Gaussian KB projector model), all on the GPU through the C-ABI.
+
+Data: for every training seed, the kinetic-preconditioned sweep loop
+(ground_state.f90:136-140) is run to convergence to obtain the converged
+bands and their energies (the targets); then the same system restarts from its
+seeded random orbitals and the residuals of the first CG sweep are recorded
+(residual -> correction pairs: the network's correction of each residual is
+scored by the Rayleigh-Ritz objective against the converged band energy).
+
+Evaluation: on held-out seeds the kinetic and the learned solve start from the
+same orbitals; the count is the sweeps until every band energy is within
+`acc` (1e-8 Ha) of the reference, the reference being the kinetic solve
+continued to a 1e-12 Ha change per sweep.  One sweep runs iter_cg_max CG
+iterations per band.
+
+  python -m salmon_amd.pctrain train --train 200 --out weights.npy
+  python -m salmon_amd.pctrain eval --weights weights.npy --eval 50
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import sys
+import time
+
+import numpy as np
+
+from . import gpu, synth
+
+NB = 64
+NCG = 4
+TRAIN_SEED0 = 1000   # training potentials: seeds 1000..1000+n-1
+EVAL_SEED0 = 0       # held-out potentials: seeds 0..n-1
+
+
+def _log(msg):
+    print(msg, flush=True)
+
+
+def prepare(seed, nb=NB, target_tol=1e-9, max_sweeps=80, record=256):
+    """Context of one training system with its targets and recorded samples."""
+    s = synth.config5(seed=seed, NB=nb)
+    c = gpu.Context(s, gpu.FP64)
+    c.ground_state(s.kAc0, max_sweeps, iter_cg_max=NCG, precond=gpu.PC_KINETIC, tol=target_tol)
+    c.pc_set_targets(s.kAc0)
+    c.put_orbitals(s.zu)
+    c.pc_record(record)
+    c.ground_state(s.kAc0, 1, iter_cg_max=NCG, precond=gpu.PC_KINETIC, tol=0.0)
+    return c
+
+
+def train(n_train=200, epochs=6, steps=8, batch=8, lr=2e-3, seed=0, nb=NB, log=_log):
+    net = gpu.PcNet(seed=seed)
+    ctxs = []
+    t0 = time.time()
+    for i in range(n_train):
+        ctxs.append(prepare(TRAIN_SEED0 + i, nb=nb))
+        if i % 10 == 9 or i == n_train - 1:
+            log(f"[pctrain] prepared {i + 1}/{n_train} systems ({time.time() - t0:.0f} s)")
+    rng = np.random.default_rng(seed)
+    for ep in range(epochs):
+        losses = []
+        for j in rng.permutation(len(ctxs)):
+            losses.append(net.train(ctxs[j], steps, batch=batch, lr=lr * (0.5 ** (ep // 3)), seed=int(rng.integers(1 << 30))))
+        log(f"[pctrain] epoch {ep}: mean loss {np.mean(losses):.4f} ({time.time() - t0:.0f} s)")
+    w = net.weights()
+    for c in ctxs:
+        c.close()
+    return net, w
+
+
+def sweeps_to(history, ref, acc):
+    """First sweep (1-based) after which every band energy is within acc of ref."""
+    err = np.max(np.abs(history.reshape(history.shape[0], -1) - ref.ravel()[None, :]), axis=1)
+    hit = np.nonzero(err < acc)[0]
+    return int(hit[0]) + 1 if hit.size else None
+
+
+def evaluate(net, n_eval=50, acc=1e-8, max_sweeps=120, nb=NB, log=_log):
+    rows = []
+    for i in range(n_eval):
+        s = synth.config5(seed=EVAL_SEED0 + i, NB=nb)
+        with gpu.Context(s, gpu.FP64) as c:
+            rk = c.ground_state(s.kAc0, max_sweeps, iter_cg_max=NCG, precond=gpu.PC_KINETIC, tol=1e-12)
+            ref = rk["esp"]
+            c.put_orbitals(s.zu)
+            c.set_pcnet(net)
+            rl = c.ground_state(s.kAc0, max_sweeps, iter_cg_max=NCG, precond=gpu.PC_LEARNED, tol=1e-12)
+        nk, nl = sweeps_to(rk["history"], ref, acc), sweeps_to(rl["history"], ref, acc)
+        row = dict(seed=EVAL_SEED0 + i, kinetic=nk, learned=nl,
+                   learned_final_err=float(np.max(np.abs(rl["esp"] - ref))))
+        rows.append(row)
+        log(f"[pctrain] eval seed {row['seed']}: sweeps to {acc:g} Ha kinetic {nk} learned {nl} "
+            f"(final err {row['learned_final_err']:.1e})")
+    ok = [r for r in rows if r["kinetic"] and r["learned"]]
+    ratio = float(np.sum([r["kinetic"] for r in ok]) / max(1, np.sum([r["learned"] for r in ok])))
+    return dict(rows=rows, acc=acc, ncg=NCG, nb=nb, sweep_ratio=ratio,
+                converged_kinetic=sum(1 for r in rows if r["kinetic"]),
+                converged_learned=sum(1 for r in rows if r["learned"]))
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("mode", choices=["train", "eval"])
+    ap.add_argument("--train", type=int, default=200)
+    ap.add_argument("--eval", type=int, default=50)
+    ap.add_argument("--epochs", type=int, default=6)
+    ap.add_argument("--nb", type=int, default=NB)
+    ap.add_argument("--weights", default="pcnet_weights.npy")
+    ap.add_argument("--out", default=None)
+    a = ap.parse_args(argv)
+    if a.mode == "train":
+        _, w = train(a.train, epochs=a.epochs, nb=a.nb)
+        np.save(a.out or a.weights, w)
+        _log(f"[pctrain] weights written to {a.out or a.weights}")
+    else:
+        net = gpu.PcNet(weights=np.load(a.weights))
+        res = evaluate(net, a.eval, nb=a.nb)
+        _log(json.dumps({k: v for k, v in res.items() if k != "rows"}))
+        if a.out:
+            with open(a.out, "w") as f:
+                json.dump(res, f, indent=1)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

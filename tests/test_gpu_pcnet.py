@@ -122,3 +122,21 @@ def test_learned_requires_network():
     with gpu.Context(s, gpu.FP64) as c:
         with pytest.raises(gpu.SalmonGPUError, match="network"):
             c.cg(s.kAc0, 2, gpu.PC_LEARNED)
+
+
+def test_mfma_inference_matches_oracle(recorded):
+    """The BF16 matrix-core inference (as used inside CG) against the FP64
+    oracle network with the same weights: bf16 weights and activations over
+    three layers leave ~1e-2 relative error."""
+    s, c, Z, lam = recorded
+    net = gpu.PcNet(seed=5)
+    w = net.weights().astype(np.float64)
+    shape = (s.NLx, s.NLy, s.NLz)
+    for i in (0, 7, 19):
+        g = c.pc_get_sample(i)["g"]
+        d = net.apply(c, g)
+        ref = pcnet.direction(w, g, shape) * np.linalg.norm(g) / np.sqrt(s.NL)
+        err = np.linalg.norm(d - ref) / np.linalg.norm(ref)
+        print(f"[gpu] mfma inference sample {i}: rel err {err:.2e}", flush=True)
+        assert err < 3e-2
+    net.close()
