@@ -193,8 +193,9 @@ int salmon_gpu_pcnet_finalize(salmon_pcnet *net);
 int salmon_gpu_pcnet_apply(salmon_pcnet *net, salmon_ctx *ctx, const double *g, double *d);
 /* network used by SALMON_PC_LEARNED on ctx (NULL detaches; not owned) */
 int salmon_gpu_set_pcnet(salmon_ctx *ctx, salmon_pcnet *net);
-/* record up to max_samples residuals of the following CG calls (0 stops) */
-int salmon_gpu_pc_record(salmon_ctx *ctx, int max_samples);
+/* record up to max_samples residuals of the following CG calls, every
+ * stride-th CG iteration (max_samples = 0 stops and frees the store) */
+int salmon_gpu_pc_record(salmon_ctx *ctx, int max_samples, int stride);
 int salmon_gpu_pc_samples(salmon_ctx *ctx, int *stored, int *valid);
 int salmon_gpu_pc_get_sample(salmon_ctx *ctx, int i, double *g, double *x, int *band, double *theta, int *valid);
 int salmon_gpu_pc_set_targets(salmon_ctx *ctx, const double *kAc);

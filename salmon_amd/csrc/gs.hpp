@@ -42,7 +42,8 @@ struct GsState {
   // learned preconditioner: inference buffers, recorded training samples of
   // k = 0 (orthogonalised residual, iterate, band, <x|H|x>) and the targets
   PcInfer inf;
-  int smp_cap = 0, smp_n = 0;
+  int smp_cap = 0, smp_n = 0, smp_stride = 1;
+  long smp_seen = 0;  // iterations seen while recording
   bool recording = false;
   std::vector<void*> smp_bufs;
   cplx<double>* smp_g = nullptr;
@@ -54,7 +55,21 @@ struct GsState {
   std::vector<double> h_theta;
   cplx<double>* Zc = nullptr;  // (NB, NL) converged bands
   std::vector<double> lam;     // their energies
+  // captured launch sequences: a CG call (keyed by its arguments) and a
+  // Gram-Schmidt pass (keyed by the orbital array); the first call of a key
+  // runs eagerly, later ones capture once and replay
+  struct Graph {
+    hipGraphExec_t exec = nullptr;
+    long key[4] = {-1, -1, -1, -1};
+    long warm[4] = {-1, -1, -1, -1};
+  } g_cg, g_gs;
 };
+
+static void graph_free(GsState::Graph& g) {
+  if (g.exec) (void)hipGraphExecDestroy(g.exec);
+  g.exec = nullptr;
+  for (int i = 0; i < 4; ++i) g.key[i] = g.warm[i] = -1;
+}
 
 static int pc_infer(salmon_ctx* c, GsState* s, salmon_pcnet* net, const cplx<double>* g, cplx<double>* out);
 
@@ -66,6 +81,8 @@ static void gs_free(salmon_ctx* c) {
   for (void* p : s->bufs) (void)hipFree(p);
   for (void* p : s->smp_bufs) (void)hipFree(p);
   for (void* p : s->inf.bufs) (void)hipFree(p);
+  graph_free(s->g_cg);
+  graph_free(s->g_gs);
   delete s;
   c->gs = nullptr;
 }
@@ -219,7 +236,7 @@ static int cg_run(salmon_ctx* c, int ncg, int precond) {
       hipLaunchKernelGGL(k_cg_sub, gv, blk, 0, c->stream, S, (const cplx<double>*)zu, ib, resid, w[W_G],
                          (const cplx<double>*)nullptr, (int)(precond == SALMON_PC_NONE ? SUB_GRAD : SUB_NONE), it, ncg);
       LAUNCHCHK(c);
-      if (s->recording && s->smp_n < s->smp_cap && NK == 1) {
+      if (s->recording && s->smp_n < s->smp_cap && NK == 1 && (s->smp_seen++ % s->smp_stride) == 0) {
         const int slot = s->smp_n++;
         hipLaunchKernelGGL(k_pc_record, dim3((NL + 255) / 256), blk, 0, c->stream, S, (const cplx<double>*)w[W_G],
                            (const cplx<double>*)w[W_X], s->smp_g + (size_t)slot * NL, s->smp_x + (size_t)slot * NL,
@@ -300,6 +317,41 @@ static int gram_schmidt_multi(salmon_ctx* c, cplx<double>* zu) {
     hipLaunchKernelGGL(k_gs_scale, gv, blk, 0, c->stream, S, zu, ib);
     LAUNCHCHK(c);
   }
+  return SALMON_OK;
+}
+
+// Runs `enqueue` through the graph cache: eagerly on the first call of a key
+// (buffers are allocated there), captured and instantiated on the second,
+// replayed afterwards.  A failed capture leaves the key to eager launches.
+template <typename F>
+static int graph_run(salmon_ctx* c, GsState::Graph& g, const long key[4], F enqueue) {
+  const bool same = std::equal(key, key + 4, g.key);
+  if (g.exec && same) {
+    HIPCHK(c, hipGraphLaunch(g.exec, c->stream));
+    return SALMON_OK;
+  }
+  if (!std::equal(key, key + 4, g.warm)) {
+    std::copy(key, key + 4, g.warm);
+    return enqueue();
+  }
+  graph_free(g);
+  std::copy(key, key + 4, g.warm);
+  hipGraph_t graph = nullptr;
+  HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
+  const int r = enqueue();
+  const hipError_t e = hipStreamEndCapture(c->stream, &graph);
+  if (r) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return r;
+  }
+  if (e != hipSuccess || !graph) return fail(c, SALMON_ERR_HIP, "stream capture: %s", hipGetErrorString(e));
+  hipGraphExec_t exec = nullptr;
+  const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (ei != hipSuccess) return fail(c, SALMON_ERR_HIP, "graph instantiate: %s", hipGetErrorString(ei));
+  g.exec = exec;
+  std::copy(key, key + 4, g.key);
+  HIPCHK(c, hipGraphLaunch(g.exec, c->stream));
   return SALMON_OK;
 }
 

@@ -130,6 +130,20 @@ static int pc_wgrad(hipStream_t st, salmon_pcnet* n, const Grid& g, int ns, cons
   return hipGetLastError() == hipSuccess ? SALMON_OK : SALMON_ERR_HIP;
 }
 
+// BF16 copy of the network's current weights for the matrix-core inference.
+static int pc_pack(salmon_ctx* c, salmon_pcnet* net) {
+  if (net->packed) return SALMON_OK;
+  const hipStream_t st = c->stream;
+  hipLaunchKernelGGL((k_pc_pack_bf16<C0, C1>), dim3((kBt1 + 255) / 256), dim3(256), 0, st, (const float*)net->w, net->bt);
+  hipLaunchKernelGGL((k_pc_pack_bf16<C1, C2>), dim3((kBt2 + 255) / 256), dim3(256), 0, st, (const float*)(net->w + NW1),
+                     net->bt + kBt1);
+  hipLaunchKernelGGL((k_pc_pack_bf16<C2, C3>), dim3((kBt3 + 255) / 256), dim3(256), 0, st,
+                     (const float*)(net->w + NW1 + NW2), net->bt + kBt1 + kBt2);
+  LAUNCHCHK(c);
+  net->packed = true;
+  return SALMON_OK;
+}
+
 // Learned direction of n residual rows g (stride NL) into out, on the matrix
 // cores with BF16 weights and activations.
 static int pc_infer(salmon_ctx* c, GsState* s, salmon_pcnet* net, const cplx<double>* g, cplx<double>* out) {
@@ -151,15 +165,7 @@ static int pc_infer(salmon_ctx* c, GsState* s, salmon_pcnet* net, const cplx<dou
     I.NK = NK;
     I.NL = NL;
   }
-  if (!net->packed) {
-    hipLaunchKernelGGL((k_pc_pack_bf16<C0, C1>), dim3((kBt1 + 255) / 256), dim3(256), 0, st, (const float*)net->w, net->bt);
-    hipLaunchKernelGGL((k_pc_pack_bf16<C1, C2>), dim3((kBt2 + 255) / 256), dim3(256), 0, st, (const float*)(net->w + NW1),
-                       net->bt + kBt1);
-    hipLaunchKernelGGL((k_pc_pack_bf16<C2, C3>), dim3((kBt3 + 255) / 256), dim3(256), 0, st,
-                       (const float*)(net->w + NW1 + NW2), net->bt + kBt1 + kBt2);
-    LAUNCHCHK(c);
-    net->packed = true;
-  }
+  if (!net->packed) return fail(c, SALMON_ERR_STATE, "learned preconditioner weights not packed");
   const int nchunk = (NL + kRedChunk - 1) / kRedChunk;
   hipLaunchKernelGGL(k_pc_stats, dim3(nchunk, NK), dim3(kBlk), 0, st, g, NL, I.part);
   hipLaunchKernelGGL(k_pc_stats_fin, dim3((NK + 63) / 64), dim3(64), 0, st, (const double*)I.part, nchunk, NK, NL, I.stats);
@@ -265,4 +271,15 @@ static int pc_step(salmon_ctx* c, salmon_pcnet* n, const std::vector<int>& idx, 
   for (int i = 0; i < B; ++i) l += co[6 * (size_t)i];
   if (loss) *loss = l / B;
   return SALMON_OK;
+}
+
+// One CG call through the graph cache (eager while samples are recorded).
+static int cg_call(salmon_ctx* c, int ncg, int precond) {
+  GsState* s = (GsState*)c->gs;
+  salmon_pcnet* net = precond == SALMON_PC_LEARNED ? (salmon_pcnet*)c->pcnet : nullptr;
+  int r;
+  if (net && (r = pc_pack(c, net))) return r;
+  if (s->recording && s->smp_n < s->smp_cap) return cg_run(c, ncg, precond);
+  const long key[4] = {ncg, precond, (long)(uintptr_t)net, (long)(uintptr_t)c->d_zu};
+  return graph_run(c, s->g_cg, key, [&]() { return cg_run(c, ncg, precond); });
 }

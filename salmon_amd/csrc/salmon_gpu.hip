@@ -269,8 +269,10 @@ int read_sums(salmon_ctx* c, double out[8]) {
 namespace {
 int gram_schmidt_multi_fwd(salmon_ctx* c, cplx<double>* zu) {
   int r = ensure_work(c);
-  if (r) return r;
-  return gram_schmidt_multi(c, zu);
+  if (r || (r = gs_ensure(c))) return r;
+  GsState* s = (GsState*)c->gs;
+  const long key[4] = {(long)(uintptr_t)zu, c->NB, c->NK, 0};
+  return graph_run(c, s->g_gs, key, [&]() { return gram_schmidt_multi(c, zu); });
 }
 }  // namespace
 
@@ -708,7 +710,7 @@ int salmon_gpu_cg(salmon_ctx* c, const double* kAc, int iter_cg_max, int precond
   if (r) return r;
   if (iter_cg_max < 0) return fail(c, SALMON_ERR_ARG, "iter_cg_max < 0");
   if ((r = check_precond(c, precond))) return r;
-  if ((r = cg_run(c, iter_cg_max, precond))) return r;
+  if ((r = cg_call(c, iter_cg_max, precond))) return r;
   return cg_results(c, esp_var, iters);
 }
 
@@ -732,7 +734,7 @@ int salmon_gpu_ground_state(salmon_ctx* c, const double* kAc, int max_sweeps, in
     if ((r = gram_schmidt_dev(c))) return r;
     if ((r = subspace_diag(c, nullptr))) return r;
     if ((r = gram_schmidt_dev(c))) return r;
-    if ((r = cg_run(c, iter_cg_max, pc))) return r;
+    if ((r = cg_call(c, iter_cg_max, pc))) return r;
     if ((r = gram_schmidt_dev(c))) return r;
     if ((r = cg_results(c, var.data(), nullptr))) return r;
     if ((r = salmon_gpu_sp_energy(c, kAc, e1.data()))) return r;
@@ -826,7 +828,7 @@ int salmon_gpu_pcnet_apply(salmon_pcnet* n, salmon_ctx* c, const double* g, doub
   GsState* s = (GsState*)c->gs;
   const size_t NL = c->g.NL;
   HIPCHK(c, hipMemcpyAsync(s->w[W_G], g, sizeof(cplx<double>) * NL, hipMemcpyHostToDevice, c->stream));
-  if ((r = pc_infer(c, s, n, s->w[W_G], s->w[W_KG]))) return r;
+  if ((r = pc_pack(c, n)) || (r = pc_infer(c, s, n, s->w[W_G], s->w[W_KG]))) return r;
   HIPCHK(c, hipMemcpyAsync(d, s->w[W_KG], sizeof(cplx<double>) * NL, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return SALMON_OK;
@@ -838,16 +840,18 @@ int salmon_gpu_set_pcnet(salmon_ctx* c, salmon_pcnet* n) {
   return SALMON_OK;
 }
 
-int salmon_gpu_pc_record(salmon_ctx* c, int max_samples) {
+int salmon_gpu_pc_record(salmon_ctx* c, int max_samples, int stride) {
   int r = check_ready(c, true);
   if (r) return r;
   if (c->NK != 1) return fail(c, SALMON_ERR_UNSUPPORTED, "training samples are recorded for one k point");
-  if (max_samples < 0) return fail(c, SALMON_ERR_ARG, "max_samples < 0");
+  if (max_samples < 0 || stride < 1) return fail(c, SALMON_ERR_ARG, "max_samples < 0 or stride < 1");
   if ((r = ensure_work(c)) || (r = gs_ensure(c))) return r;
   GsState* s = (GsState*)c->gs;
   for (void* p : s->smp_bufs) (void)hipFree(p);
   s->smp_bufs.clear();
   s->smp_cap = s->smp_n = 0;
+  s->smp_stride = stride;
+  s->smp_seen = 0;
   s->recording = max_samples > 0;
   if (!s->recording) return SALMON_OK;
   const size_t NL = c->g.NL;

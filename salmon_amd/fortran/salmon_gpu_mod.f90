@@ -186,10 +186,10 @@ module salmon_gpu
       type(c_ptr), value :: ctx, net
     end function
 
-    integer(c_int) function salmon_gpu_pc_record(ctx, max_samples) bind(C, name="salmon_gpu_pc_record")
+    integer(c_int) function salmon_gpu_pc_record(ctx, max_samples, stride) bind(C, name="salmon_gpu_pc_record")
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx
-      integer(c_int), value :: max_samples
+      integer(c_int), value :: max_samples, stride
     end function
 
     integer(c_int) function salmon_gpu_pc_samples(ctx, stored, valid) bind(C, name="salmon_gpu_pc_samples")

@@ -79,7 +79,7 @@ SIGNATURES = {
     "salmon_gpu_pcnet_finalize": [_vp],
     "salmon_gpu_pcnet_apply": [_vp, _vp, _vp, _vp],
     "salmon_gpu_set_pcnet": [_vp, _vp],
-    "salmon_gpu_pc_record": [_vp, _ci],
+    "salmon_gpu_pc_record": [_vp, _ci, _ci],
     "salmon_gpu_pc_samples": [_vp, _vp, _vp],
     "salmon_gpu_pc_get_sample": [_vp, _ci, _vp, _vp, _vp, _vp, _vp],
     "salmon_gpu_pc_set_targets": [_vp, _vp],
@@ -285,8 +285,8 @@ class Context:
         self._pcnet = net
         self._check(lib().salmon_gpu_set_pcnet(self._h, net._h if net is not None else None), "set_pcnet")
 
-    def pc_record(self, max_samples):
-        self._check(lib().salmon_gpu_pc_record(self._h, int(max_samples)), "pc_record")
+    def pc_record(self, max_samples, stride=1):
+        self._check(lib().salmon_gpu_pc_record(self._h, int(max_samples), int(stride)), "pc_record")
 
     def pc_samples(self):
         st, va = ctypes.c_int(), ctypes.c_int()

@@ -3,11 +3,13 @@ ground-state workload (L = 15 bohr, Gamma, 64 bands, 27 random-position
 atoms with the Gaussian KB projector model), all on the GPU through the C-ABI.
 
 Data: for every training seed, the kinetic-preconditioned sweep loop
-(ground_state.f90:136-140) is run to convergence to obtain the converged
-bands and their energies (the targets); then the same system restarts from its
-seeded random orbitals and the residuals of the first CG sweep are recorded
-(residual -> correction pairs: the network's correction of each residual is
-scored by the Rayleigh-Ritz objective against the converged band energy).
+(ground_state.f90:136-140) runs `target_sweeps` sweeps to obtain the bands
+and their energies used as targets (the samples come from the first sweeps,
+where the distance to the target, 1e-2..10 Ha, dwarfs the targets' own
+error); then the same system restarts from its seeded random orbitals and
+every third CG iteration of the first three sweeps is recorded (residual ->
+correction pairs: the network's correction of each residual is scored by the
+Rayleigh-Ritz objective against the target band energy).
 
 Evaluation: on held-out seeds the kinetic and the learned solve start from the
 same orbitals; the count is the sweeps until every band energy is within
@@ -39,15 +41,15 @@ def _log(msg):
     print(msg, flush=True)
 
 
-def prepare(seed, nb=NB, target_tol=1e-9, max_sweeps=80, record=256):
+def prepare(seed, nb=NB, target_sweeps=30, record_sweeps=3, stride=3):
     """Context of one training system with its targets and recorded samples."""
     s = synth.config5(seed=seed, NB=nb)
     c = gpu.Context(s, gpu.FP64)
-    c.ground_state(s.kAc0, max_sweeps, iter_cg_max=NCG, precond=gpu.PC_KINETIC, tol=target_tol)
+    c.ground_state(s.kAc0, target_sweeps, iter_cg_max=NCG, precond=gpu.PC_KINETIC, tol=0.0)
     c.pc_set_targets(s.kAc0)
     c.put_orbitals(s.zu)
-    c.pc_record(record)
-    c.ground_state(s.kAc0, 1, iter_cg_max=NCG, precond=gpu.PC_KINETIC, tol=0.0)
+    c.pc_record(record_sweeps * nb * NCG // stride + 1, stride)
+    c.ground_state(s.kAc0, record_sweeps, iter_cg_max=NCG, precond=gpu.PC_KINETIC, tol=0.0)
     return c
 
 

@@ -145,3 +145,22 @@ def test_ground_state_rejects_mixed_and_bad_arguments():
     with _ctx(s) as c:
         with pytest.raises(gpu.SalmonGPUError, match="preconditioner"):
             c.cg(s.kAc0, 2, 7)
+
+
+def test_cg_and_gram_schmidt_graph_replays_match_eager():
+    """The first CG / Gram-Schmidt call of a configuration runs eagerly, the
+    second captures a graph, later ones replay it: every replay reproduces
+    the eager result bit for bit (eager work and synchronisations between)."""
+    s = synth.make_system(N=(12, 12, 12), L=7.0, nk3=(2, 1, 1), NB=6, seed=31, atoms="random", natoms=5,
+                          name="graph12")
+    with _ctx(s) as c:
+        outs = []
+        for rep in range(4):
+            c.put_orbitals(s.zu)
+            c.gram_schmidt()
+            zg = c.get_orbitals()
+            ev, it = c.cg(s.kAc0, 3, gpu.PC_KINETIC)
+            outs.append((zg, ev, it, c.get_orbitals()))
+            c.sp_energy(s.kAc0)  # eager work between replays
+        for o in outs[1:]:
+            assert all(np.array_equal(a, b) for a, b in zip(outs[0], o))

@@ -33,7 +33,7 @@ def recorded():
     lam = c.sp_energy(s.kAc0)[0]
     c.put_orbitals(s.zu)
     c.gram_schmidt()
-    c.pc_record(48)
+    c.pc_record(48, 1)
     c.cg(s.kAc0, 4, gpu.PC_KINETIC)
     c.cg(s.kAc0, 4, gpu.PC_KINETIC)
     stored, valid = c.pc_samples()
