@@ -218,6 +218,29 @@ int salmon_gpu_pcnet_train(salmon_pcnet *net, salmon_ctx *ctx, int nsteps, int b
 int salmon_gpu_rt_run(salmon_ctx *ctx, const double *ac, int nsteps, double dt, int reortho_every,
                       double *jt, double *et, double *norm, double *rho);
 
+/* ---- Batched independent systems (no reference counterpart) --------------
+ * A batch runs the RT loop of salmon_gpu_rt_run (middlepoint propagator,
+ * frozen potentials, current/energy/norm every step) for several independent
+ * set-up contexts at once -- different grids, band counts, k points and
+ * atoms -- with every phase of a step one launch over all systems and the
+ * step replayed from a captured HIP graph.  Contexts are referenced, not
+ * copied: their resident orbitals advance; they must not be used elsewhere
+ * while the batch runs and must outlive it.  All contexts share the batch's
+ * precision, A(t) and dt.  salmon_gpu_batch_tune times the stencil's z
+ * blocking (1, 2, 4 or 8 points per thread along z) per grid shape and keeps
+ * the fastest (rt_run tunes on first use); salmon_gpu_batch_zblock reports
+ * the choice.  Outputs: jt(nsys, nsteps, 3), et(nsys, nsteps) =
+ * Ekin+Enl+Eloc, norm(nsys, nsteps), row-major, each nullable. */
+typedef struct salmon_batch salmon_batch;
+int salmon_gpu_batch_create(int precision, salmon_batch **batch);
+int salmon_gpu_batch_add(salmon_batch *batch, salmon_ctx *ctx);
+int salmon_gpu_batch_tune(salmon_batch *batch);
+int salmon_gpu_batch_zblock(salmon_batch *batch, int sys);
+int salmon_gpu_batch_rt_run(salmon_batch *batch, const double *ac, int nsteps, double dt, double *jt, double *et,
+                            double *norm);
+const char *salmon_gpu_batch_last_error(const salmon_batch *batch);
+int salmon_gpu_batch_finalize(salmon_batch *batch);
+
 /* Kernel path of the RT step.  SALMON_PATH_AUTO uses the LDS-resident fused
  * step where the grid has a compiled shape (16^3) and the streamed kernels
  * otherwise; SALMON_PATH_STREAMED forces the streamed kernels.  Like the

@@ -1,0 +1,83 @@
+"""The 64-system batched workload: independent seeded systems with grids from
+{12^3, 16^3, 20x20x28, 24^3, 32^3}, bands U{8..64}, k points U{1..64}, atoms
+U{1..16} (synth.config4_system), propagated together with the batched step
+(one launch per phase over all systems, the step replayed from a captured HIP
+graph, z blocking tuned per grid shape), FP64 and complex64.  The same systems
+propagated one context after another give the unbatched rate beside it.
+
+  python -m salmon_amd.batchrun --steps 10 [--scale 1.0] [--out result.json]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import sys
+import time
+
+import numpy as np
+
+from . import gpu, synth
+
+
+def _log(msg):
+    print(msg, flush=True)
+
+
+def run(nsys=64, steps=10, scale=1.0, precisions=(gpu.FP64, gpu.MIXED), sequential=True, log=_log):
+    t0 = time.time()
+    systems = [synth.config4_system(i, scale=scale) for i in range(nsys)]
+    orbs = sum(s.NK * s.NB for s in systems)
+    pts = sum(s.NK * s.NB * s.NL for s in systems)
+    log(f"[batch] {nsys} systems, {orbs} orbitals, {pts * 16 / 1e9:.1f} GB in FP64 (built in {time.time() - t0:.0f} s)")
+    ac = synth.acos2_pulse(steps + 1, 0.08, 0.05, 0.057, 826.8)
+    out = dict(systems=nsys, orbitals=int(orbs), grid_points_x_orbitals=int(pts), steps=steps,
+               shapes=sorted({f"{s.NLx}x{s.NLy}x{s.NLz}" for s in systems}))
+    for prec in precisions:
+        name = "fp64" if prec == gpu.FP64 else "mixed"
+        ctxs = [gpu.Context(s, prec) for s in systems]
+        with gpu.Batch(ctxs, prec) as b:
+            tt = time.time()
+            zb = b.tune()
+            tune_s = time.time() - tt
+            b.rt_run(ac[:2], 1, 0.08)  # warm-up step (captures the graph)
+            tt = time.time()
+            r = b.rt_run(ac[1:], steps, 0.08)
+            dt = time.time() - tt
+        res = dict(ms_per_step=dt / steps * 1e3, orbital_steps_per_s=orbs * steps / dt,
+                   point_orbital_steps_per_s=pts * steps / dt, tune_s=tune_s,
+                   zblock={f"{s.NLx}x{s.NLy}x{s.NLz}": int(z) for s, z in zip(systems, zb)},
+                   finite=bool(np.all(np.isfinite(r["J"])) and np.all(np.isfinite(r["E"]))))
+        log(f"[batch] {name}: {res['ms_per_step']:.2f} ms/step, {res['orbital_steps_per_s']:.3e} orbital-steps/s "
+            f"(tune {tune_s:.1f} s, z blocking {res['zblock']})")
+        if sequential:
+            tt = time.time()
+            for c in ctxs:
+                c.set_path(gpu.PATH_STREAMED)
+                c.rt_run(ac[1:], steps, dt=0.08)
+            ds = time.time() - tt
+            res["sequential_ms_per_step"] = ds / steps * 1e3
+            log(f"[batch] {name}: one context after another {res['sequential_ms_per_step']:.2f} ms/step")
+        for c in ctxs:
+            c.close()
+        out[name] = res
+    return out
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--systems", type=int, default=64)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--scale", type=float, default=1.0)
+    ap.add_argument("--no-sequential", action="store_true")
+    ap.add_argument("--out", default=None)
+    a = ap.parse_args(argv)
+    res = run(a.systems, a.steps, a.scale, sequential=not a.no_sequential)
+    _log(json.dumps(res))
+    if a.out:
+        with open(a.out, "w") as f:
+            json.dump(res, f, indent=1)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

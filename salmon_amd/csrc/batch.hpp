@@ -1,0 +1,238 @@
+// Batched real-time step over independent systems of different grids, band
+// counts, k points and atoms.  Each phase of the step (KB projection and
+// stencil for each of the four Taylor terms, the Taylor fold, the
+// observables, their per-system sums) is one launch covering every system:
+// blocks are laid out system after system and a block finds its system by a
+// binary search over the systems' first-block offsets.  One step is captured
+// in a HIP graph and replayed; A(t) comes from a device table indexed by a
+// device step counter.  The stencil's z blocking (points per thread along z)
+// is chosen per grid shape by timing the candidates on the system's own data.
+#pragma once
+#include "streamed.hpp"
+
+namespace salmon {
+namespace batch {
+
+constexpr int kPhases = 4;  // first-block offsets: projection, stencil, fold, observables
+
+template <typename T>
+struct Sys {
+  Grid g;
+  KBTables kb;
+  cplx<T>* zu;
+  cplx<T>* h[4];
+  cplx<T>* uv;  // (NKB, Nlma)
+  const T* vloc;
+  const double* vloc64;
+  const double* kac0;
+  const double* occ;
+  double* rows;  // (NKB, 8)
+  int NK, NB, NKB, zb;
+  int pblk, hblk;            // projection blocks per orbital, stencil blocks per orbital
+  long first[kPhases + 1];   // first block of the system in each phase
+};
+
+template <typename T>
+__device__ __forceinline__ int find_sys(const Sys<T>* __restrict__ S, int n, int phase, long b) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (S[mid].first[phase] <= b) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+struct Step {
+  const double* ac;  // (nsteps+1, 3)
+  const int* step;   // device step counter
+  double dt;
+};
+
+__device__ __forceinline__ void step_fields(const Step& st, double amid[3], double anew[3]) {
+  const int n = *st.step;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    amid[d] = 0.5 * (st.ac[3 * n + d] + st.ac[3 * (n + 1) + d]);
+    anew[d] = st.ac[3 * (n + 1) + d];
+  }
+}
+
+// KB projection of Taylor term `term` (input zu for term 0, else h[term-1]).
+template <typename T>
+__global__ void __launch_bounds__(256) k_b_project(const Sys<T>* __restrict__ S, int nsys, int term, Step st) {
+  const long b = blockIdx.x;
+  const int s = find_sys(S, nsys, 0, b);
+  const Sys<T>& y = S[s];
+  const long lb = b - y.first[0];
+  const int o = (int)(lb / y.pblk);
+  const int ilma = (int)(lb % y.pblk) * 4 + (threadIdx.x / kWave);
+  if (o >= y.NKB || ilma >= y.kb.Nlma) return;
+  double am[3], an[3];
+  step_fields(st, am, an);
+  double kx, ky, kz;
+  orbital_k(y.kac0, y.NK, o / y.NB, am[0], am[1], am[2], kx, ky, kz);
+  const cplx<T>* in = (term == 0 ? y.zu : y.h[term - 1]) + (size_t)o * y.g.NL;
+  kb_project_wave<T>(in, ilma, kx, ky, kz, y.g, y.kb, y.uv + (size_t)o * y.kb.Nlma + ilma);
+}
+
+// ZB consecutive z points of one (ix, iy) line: the z neighbours come from a
+// register window of ZB+8 values; the y and x terms, the potential and the KB
+// term are added per point in the order of hpsi_point.
+template <typename T, int ZB>
+__device__ __forceinline__ void hpsi_zblock(const cplx<T>* __restrict__ E, int p0, double kx, double ky, double kz,
+                                            const T* __restrict__ vloc, const Grid& g, const KBTables& kb,
+                                            const cplx<T>* __restrict__ U, cplx<T>* __restrict__ out) {
+  const int NLy = g.NLy, NLz = g.NLz;
+  const int iz0 = p0 % NLz, iy = (p0 / NLz) % NLy, ix = p0 / (NLz * NLy);
+  const int line = p0 - iz0;
+  cplx<T> win[ZB + 8];
+#pragma unroll
+  for (int j = 0; j < ZB + 8; ++j) {
+    int z = iz0 - 4 + j;
+    z = z < 0 ? z + NLz : (z >= NLz ? z - NLz : z);
+    win[j] = E[line + z];
+  }
+  const T A = (T)((kx * kx + ky * ky + kz * kz - (g.lap[0][0] + g.lap[1][0] + g.lap[2][0])) * 0.5);
+  const double kk[3] = {kx, ky, kz};
+#pragma unroll
+  for (int q = 0; q < ZB; ++q) {
+    const int p = p0 + q;
+    T vr = 0, vi = 0, wr = 0, wi = 0;
+#pragma unroll
+    for (int n = 1; n <= 4; ++n) {
+      const cplx<T> ep = win[4 + q + n], em = win[4 + q - n];
+      const T C = (T)g.lap[2][n], D = (T)(kz * g.nab[2][n]);
+      vr += C * (ep.re + em.re);
+      vi += C * (ep.im + em.im);
+      wr += D * (ep.re - em.re);
+      wi += D * (ep.im - em.im);
+    }
+#pragma unroll
+    for (int ax3 = 1; ax3 >= 0; --ax3) {
+      const int n_ax = ax3 == 1 ? NLy : g.NLx;
+      const int c_ax = ax3 == 1 ? iy : ix;
+      const int stride = ax3 == 1 ? NLz : NLz * NLy;
+      const int base = p - c_ax * stride;
+#pragma unroll
+      for (int n = 1; n <= 4; ++n) {
+        int cp = c_ax + n;
+        if (cp >= n_ax) cp -= n_ax;
+        int cm = c_ax - n;
+        if (cm < 0) cm += n_ax;
+        const cplx<T> ep = E[base + cp * stride], em = E[base + cm * stride];
+        const T C = (T)g.lap[ax3][n], D = (T)(kk[ax3] * g.nab[ax3][n]);
+        vr += C * (ep.re + em.re);
+        vi += C * (ep.im + em.im);
+        wr += D * (ep.re - em.re);
+        wi += D * (ep.im - em.im);
+      }
+    }
+    const cplx<T> e = win[4 + q];
+    const T V = vloc[p];
+    cplx<T> f;
+    f.re = (V + A) * e.re - (T)0.5 * vr + wi;
+    f.im = (V + A) * e.im - (T)0.5 * vi - wr;
+    const int t = kb.pt2t[p];
+    if (t >= 0) {
+      for (int r = kb.t_start[t]; r < kb.t_start[t + 1]; ++r) {
+        const int il = kb.e_lma[r], jp = kb.e_jp[r];
+        const int a = kb.lma_atom[il];
+        const double kr = kx * kb.pos[3 * jp] + ky * kb.pos[3 * jp + 1] + kz * kb.pos[3 * jp + 2];
+        T sn, cs;
+        sincos_t((T)kr, &sn, &cs);
+        const T u = (T)kb.uv[(size_t)il * kb.Nps + (jp - kb.atom_start[a])];
+        const cplx<T> w = U[il];
+        f.re += u * (cs * w.re + sn * w.im);
+        f.im += u * (cs * w.im - sn * w.re);
+      }
+    }
+    out[p] = f;
+  }
+}
+
+// Stencil + local + KB of Taylor term `term`: h[term] = H (term == 0 ? zu : h[term-1]).
+template <typename T>
+__global__ void __launch_bounds__(256) k_b_hpsi(const Sys<T>* __restrict__ S, int nsys, int term, Step st) {
+  const long b = blockIdx.x;
+  const int s = find_sys(S, nsys, 1, b);
+  const Sys<T>& y = S[s];
+  const long lb = b - y.first[1];
+  const int o = (int)(lb / y.hblk);
+  const int p0 = ((int)(lb % y.hblk) * 256 + threadIdx.x) * y.zb;
+  if (o >= y.NKB || p0 >= y.g.NL) return;
+  double am[3], an[3];
+  step_fields(st, am, an);
+  double kx, ky, kz;
+  orbital_k(y.kac0, y.NK, o / y.NB, am[0], am[1], am[2], kx, ky, kz);
+  const cplx<T>* in = (term == 0 ? y.zu : y.h[term - 1]) + (size_t)o * y.g.NL;
+  cplx<T>* out = y.h[term] + (size_t)o * y.g.NL;
+  const cplx<T>* U = y.uv + (size_t)o * y.kb.Nlma;
+  switch (y.zb) {
+    case 1: hpsi_zblock<T, 1>(in, p0, kx, ky, kz, y.vloc, y.g, y.kb, U, out); break;
+    case 2: hpsi_zblock<T, 2>(in, p0, kx, ky, kz, y.vloc, y.g, y.kb, U, out); break;
+    case 4: hpsi_zblock<T, 4>(in, p0, kx, ky, kz, y.vloc, y.g, y.kb, U, out); break;
+    default: hpsi_zblock<T, 8>(in, p0, kx, ky, kz, y.vloc, y.g, y.kb, U, out); break;
+  }
+}
+
+// zu += sum_n zfac(n) h_n, zfac(n) = (-i dt)^n/n! (ARTED/RT/hamiltonian.f90:46-49,99-122).
+template <typename T>
+__global__ void __launch_bounds__(256) k_b_fold(const Sys<T>* __restrict__ S, int nsys, Step st) {
+  const long b = blockIdx.x;
+  const int s = find_sys(S, nsys, 2, b);
+  const Sys<T>& y = S[s];
+  const size_t i = (size_t)(b - y.first[2]) * 256 + threadIdx.x;
+  if (i >= (size_t)y.NKB * y.g.NL) return;
+  const double dt = st.dt;
+  const T z1i = (T)(-dt), z2r = (T)(-dt * dt / 2.0), z3i = (T)(dt * dt * dt / 6.0), z4r = (T)(dt * dt * dt * dt / 24.0);
+  cplx<T> z = y.zu[i];
+  const cplx<T> a = y.h[0][i], bb = y.h[1][i], c = y.h[2][i], d = y.h[3][i];
+  z.re += -z1i * a.im;
+  z.im += z1i * a.re;
+  z.re += z2r * bb.re;
+  z.im += z2r * bb.im;
+  z.re += -z3i * c.im;
+  z.im += z3i * c.re;
+  z.re += z4r * d.re;
+  z.im += z4r * d.im;
+  y.zu[i] = z;
+}
+
+// Observables of every orbital at kAc0 + A(t+dt): one block per orbital.
+template <typename T>
+__global__ void __launch_bounds__(256) k_b_observe(const Sys<T>* __restrict__ S, int nsys, Step st) {
+  const long b = blockIdx.x;
+  const int s = find_sys(S, nsys, 3, b);
+  const Sys<T>& y = S[s];
+  const int o = (int)(b - y.first[3]);
+  double am[3], an[3];
+  step_fields(st, am, an);
+  double kx, ky, kz;
+  orbital_k(y.kac0, y.NK, o / y.NB, an[0], an[1], an[2], kx, ky, kz);
+  observe_orbital<T>(y.zu + (size_t)o * y.g.NL, kx, ky, kz, y.occ[o], y.vloc64, y.g, y.kb, y.rows + (size_t)o * 8);
+}
+
+// Per-system sums of the rows (one block per system) into res(step, s, 0..7).
+template <typename T>
+__global__ void __launch_bounds__(256) k_b_sum(const Sys<T>* __restrict__ S, int nsys, int* step, double* res) {
+  __shared__ double smem[4 * 8];
+  const Sys<T>& y = S[blockIdx.x];
+  double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (int r = threadIdx.x; r < y.NKB; r += 256) {
+#pragma unroll
+    for (int i = 0; i < 8; ++i) v[i] += y.rows[(size_t)r * 8 + i];
+  }
+  block_sum256<8>(v, smem);
+  if (threadIdx.x == 0) {
+    const int n = *step;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) res[((size_t)n * nsys + blockIdx.x) * 8 + i] = v[i];
+  }
+}
+
+__global__ void k_b_tick(int* step) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *step += 1;
+}
+
+}  // namespace batch
+}  // namespace salmon

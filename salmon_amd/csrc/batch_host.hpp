@@ -1,0 +1,251 @@
+// Host side of the batched step: a batch references set-up contexts (their
+// orbitals, potentials, tables and k points stay where the contexts put them),
+// owns per-system work buffers, tunes the stencil's z blocking per grid shape
+// and runs the time loop as replays of one captured step.
+#pragma once
+#include <array>
+
+#include "batch.hpp"
+
+using namespace salmon::batch;
+
+struct salmon_batch {
+  int precision = SALMON_FP64;
+  hipStream_t stream = nullptr;
+  std::string err;
+  std::vector<salmon_ctx*> ctx;
+  std::vector<void*> bufs;        // work buffers (rebuilt by prepare)
+  void* d_sys = nullptr;          // Sys<T>[n]
+  std::vector<int> zb;            // chosen z blocking per system
+  std::vector<long> first[kPhases + 1];
+  bool ready = false, tuned = false;
+  int* d_step = nullptr;
+  double* d_ac = nullptr;
+  double* d_res = nullptr;
+  size_t cap_steps = 0;
+  hipGraphExec_t exec = nullptr;
+  double graph_dt = 0.0;
+};
+
+static int bfail(salmon_batch* b, int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  b->err = buf;
+  return code;
+}
+
+#define BHIP(b, x)                                                                                   \
+  do {                                                                                               \
+    hipError_t e_ = (x);                                                                             \
+    if (e_ != hipSuccess) return bfail((b), SALMON_ERR_HIP, "%s: %s", #x, hipGetErrorString(e_));    \
+  } while (0)
+
+static void batch_release(salmon_batch* b) {
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  b->exec = nullptr;
+  for (void* p : b->bufs) (void)hipFree(p);
+  b->bufs.clear();
+  b->d_sys = nullptr;
+  b->ready = false;
+}
+
+template <typename T>
+static int batch_build(salmon_batch* b) {
+  const int n = (int)b->ctx.size();
+  std::vector<Sys<T>> S(n);
+  long acc[kPhases] = {0, 0, 0, 0};
+  for (int i = 0; i < n; ++i) {
+    salmon_ctx* c = b->ctx[i];
+    Sys<T>& y = S[i];
+    y.g = c->g;
+    y.kb = c->kb;
+    y.zu = (cplx<T>*)c->d_zu;
+    y.vloc = vloc_ptr<T>(c);
+    y.vloc64 = c->d_vloc;
+    y.kac0 = c->d_kac0;
+    y.occ = c->d_occ;
+    y.NK = c->NK;
+    y.NB = c->NB;
+    y.NKB = c->NKB;
+    y.zb = b->zb[i];
+    const size_t nv = (size_t)c->NKB * c->g.NL;
+    for (int t = 0; t < 4; ++t) {
+      if (hipMalloc((void**)&y.h[t], nv * sizeof(cplx<T>)) != hipSuccess)
+        return bfail(b, SALMON_ERR_NOMEM, "batch work buffers (system %d)", i);
+      b->bufs.push_back(y.h[t]);
+    }
+    if (hipMalloc((void**)&y.uv, (size_t)c->NKB * std::max(1, c->kb.Nlma) * sizeof(cplx<T>)) != hipSuccess ||
+        hipMalloc((void**)&y.rows, (size_t)c->NKB * 8 * sizeof(double)) != hipSuccess)
+      return bfail(b, SALMON_ERR_NOMEM, "batch buffers (system %d)", i);
+    b->bufs.push_back(y.uv);
+    b->bufs.push_back(y.rows);
+    y.pblk = (std::max(1, c->kb.Nlma) + 3) / 4;
+    y.hblk = (c->g.NL / y.zb + 255) / 256;
+    const long nb[kPhases] = {(long)c->NKB * y.pblk, (long)c->NKB * y.hblk, (long)((nv + 255) / 256), (long)c->NKB};
+    for (int ph = 0; ph < kPhases; ++ph) {
+      y.first[ph] = acc[ph];
+      acc[ph] += nb[ph];
+    }
+    y.first[kPhases] = 0;
+  }
+  for (int ph = 0; ph < kPhases; ++ph) b->first[ph].assign(1, acc[ph]);
+  if (hipMalloc(&b->d_sys, sizeof(Sys<T>) * n) != hipSuccess) return bfail(b, SALMON_ERR_NOMEM, "batch descriptors");
+  b->bufs.push_back(b->d_sys);
+  BHIP(b, hipMemcpy(b->d_sys, S.data(), sizeof(Sys<T>) * n, hipMemcpyHostToDevice));
+  b->ready = true;
+  return SALMON_OK;
+}
+
+// One step's launches (captured by batch_run).
+template <typename T>
+static int batch_enqueue_step(salmon_batch* b, double dt) {
+  const int n = (int)b->ctx.size();
+  const Sys<T>* S = (const Sys<T>*)b->d_sys;
+  const Step st{b->d_ac, b->d_step, dt};
+  for (int t = 0; t < 4; ++t) {
+    hipLaunchKernelGGL(k_b_project<T>, dim3((unsigned)b->first[0][0]), dim3(256), 0, b->stream, S, n, t, st);
+    hipLaunchKernelGGL(k_b_hpsi<T>, dim3((unsigned)b->first[1][0]), dim3(256), 0, b->stream, S, n, t, st);
+  }
+  hipLaunchKernelGGL(k_b_fold<T>, dim3((unsigned)b->first[2][0]), dim3(256), 0, b->stream, S, n, st);
+  hipLaunchKernelGGL(k_b_observe<T>, dim3((unsigned)b->first[3][0]), dim3(256), 0, b->stream, S, n, st);
+  hipLaunchKernelGGL(k_b_sum<T>, dim3(n), dim3(256), 0, b->stream, S, n, b->d_step, b->d_res);
+  hipLaunchKernelGGL(k_b_tick, dim3(1), dim3(64), 0, b->stream, b->d_step);
+  BHIP(b, hipGetLastError());
+  return SALMON_OK;
+}
+
+// Times the stencil launch of system i alone for each z blocking that divides NLz.
+template <typename T>
+static int batch_tune_one(salmon_batch* b, int i, int* best) {
+  salmon_ctx* c = b->ctx[i];
+  std::vector<Sys<T>> one(1);
+  BHIP(b, hipMemcpy(one.data(), (Sys<T>*)b->d_sys + i, sizeof(Sys<T>), hipMemcpyDeviceToHost));
+  Sys<T>* d1 = nullptr;
+  BHIP(b, hipMalloc((void**)&d1, sizeof(Sys<T>)));
+  hipEvent_t e0, e1;
+  BHIP(b, hipEventCreate(&e0));
+  BHIP(b, hipEventCreate(&e1));
+  float bestt = 1e30f;
+  *best = 1;
+  const Step st{b->d_ac, b->d_step, 0.0};
+  for (int zb : {1, 2, 4, 8}) {
+    if (c->g.NLz % zb) continue;
+    Sys<T> y = one[0];
+    y.zb = zb;
+    y.hblk = (c->g.NL / zb + 255) / 256;
+    y.first[1] = 0;
+    BHIP(b, hipMemcpy(d1, &y, sizeof(Sys<T>), hipMemcpyHostToDevice));
+    const unsigned nblk = (unsigned)((long)y.NKB * y.hblk);
+    float tmin = 1e30f;
+    for (int rep = 0; rep < 4; ++rep) {
+      BHIP(b, hipEventRecord(e0, b->stream));
+      hipLaunchKernelGGL(k_b_hpsi<T>, dim3(nblk), dim3(256), 0, b->stream, (const Sys<T>*)d1, 1, 0, st);
+      BHIP(b, hipEventRecord(e1, b->stream));
+      BHIP(b, hipEventSynchronize(e1));
+      float ms = 0;
+      BHIP(b, hipEventElapsedTime(&ms, e0, e1));
+      if (rep > 0) tmin = std::min(tmin, ms);
+    }
+    if (tmin < bestt) {
+      bestt = tmin;
+      *best = zb;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipFree(d1);
+  return SALMON_OK;
+}
+
+template <typename T>
+static int batch_tune(salmon_batch* b) {
+  // build with z blocking 1 first (work buffers and descriptors for timing)
+  batch_release(b);
+  b->zb.assign(b->ctx.size(), 1);
+  int r = batch_build<T>(b);
+  if (r) return r;
+  if (!b->d_step) {
+    BHIP(b, hipMalloc((void**)&b->d_step, sizeof(int)));
+    BHIP(b, hipMemset(b->d_step, 0, sizeof(int)));
+  }
+  if (!b->d_ac) {
+    BHIP(b, hipMalloc((void**)&b->d_ac, sizeof(double) * 6));
+    BHIP(b, hipMemset(b->d_ac, 0, sizeof(double) * 6));
+    b->cap_steps = 1;
+  }
+  BHIP(b, hipMemset(b->d_step, 0, sizeof(int)));
+  std::vector<std::array<int, 4>> shapes;  // (NLx, NLy, NLz, zb)
+  std::vector<int> zb(b->ctx.size(), 1);
+  for (size_t i = 0; i < b->ctx.size(); ++i) {
+    const Grid& g = b->ctx[i]->g;
+    int found = -1;
+    for (size_t j = 0; j < shapes.size(); ++j)
+      if (shapes[j][0] == g.NLx && shapes[j][1] == g.NLy && shapes[j][2] == g.NLz) found = (int)j;
+    if (found < 0) {
+      int best = 1;
+      if ((r = batch_tune_one<T>(b, (int)i, &best))) return r;
+      shapes.push_back({g.NLx, g.NLy, g.NLz, best});
+      found = (int)shapes.size() - 1;
+    }
+    zb[i] = shapes[found][3];
+  }
+  batch_release(b);
+  b->zb = zb;
+  b->tuned = true;
+  return batch_build<T>(b);
+}
+
+template <typename T>
+static int batch_run(salmon_batch* b, const double* ac, int nsteps, double dt, double* jt, double* et, double* norm) {
+  const int n = (int)b->ctx.size();
+  int r;
+  if (!b->tuned && (r = batch_tune<T>(b))) return r;
+  if ((size_t)nsteps > b->cap_steps || !b->d_res) {
+    if (b->d_ac) (void)hipFree(b->d_ac);
+    if (b->d_res) (void)hipFree(b->d_res);
+    b->d_ac = nullptr;
+    b->d_res = nullptr;
+    BHIP(b, hipMalloc((void**)&b->d_ac, sizeof(double) * 3 * (nsteps + 1)));
+    BHIP(b, hipMalloc((void**)&b->d_res, sizeof(double) * 8 * (size_t)n * nsteps));
+    b->cap_steps = nsteps;
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);  // the graph holds the old table pointers
+    b->exec = nullptr;
+  }
+  BHIP(b, hipMemcpyAsync(b->d_ac, ac, sizeof(double) * 3 * (nsteps + 1), hipMemcpyHostToDevice, b->stream));
+  BHIP(b, hipMemsetAsync(b->d_step, 0, sizeof(int), b->stream));
+  if (b->exec && b->graph_dt != dt) {
+    (void)hipGraphExecDestroy(b->exec);
+    b->exec = nullptr;
+  }
+  if (!b->exec) {
+    hipGraph_t graph = nullptr;
+    BHIP(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeRelaxed));
+    r = batch_enqueue_step<T>(b, dt);
+    const hipError_t e = hipStreamEndCapture(b->stream, &graph);
+    if (r) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return r;
+    }
+    if (e != hipSuccess) return bfail(b, SALMON_ERR_HIP, "step capture: %s", hipGetErrorString(e));
+    const hipError_t ei = hipGraphInstantiate(&b->exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (ei != hipSuccess) return bfail(b, SALMON_ERR_HIP, "step graph: %s", hipGetErrorString(ei));
+    b->graph_dt = dt;
+  }
+  for (int s = 0; s < nsteps; ++s) BHIP(b, hipGraphLaunch(b->exec, b->stream));
+  std::vector<double> res(8 * (size_t)n * nsteps);
+  BHIP(b, hipMemcpyAsync(res.data(), b->d_res, sizeof(double) * res.size(), hipMemcpyDeviceToHost, b->stream));
+  BHIP(b, hipStreamSynchronize(b->stream));
+  for (int i = 0; i < n; ++i)
+    for (int s = 0; s < nsteps; ++s) {
+      const double* q = &res[((size_t)s * n + i) * 8];
+      if (jt)
+        for (int d = 0; d < 3; ++d) jt[((size_t)i * nsteps + s) * 3 + d] = q[d];
+      if (et) et[(size_t)i * nsteps + s] = q[3] + q[4] + q[5];
+      if (norm) norm[(size_t)i * nsteps + s] = q[6];
+    }
+  return SALMON_OK;
+}

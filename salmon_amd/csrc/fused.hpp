@@ -338,18 +338,19 @@ __global__ void __launch_bounds__(NY * NZ * XS, WPE) k_fused_step(FusedArgs<T> a
     __syncthreads();
     // observables at k + A(t+dt), FP64 sums; all neighbours from LDS (a rolled
     // loop keeps this sweep's registers low)
-    // Forward correlations C(a,n) = sum_i conj(psi_i) psi_{i + n e_a} (FP64):
-    // the kinetic energy (total_energy_stencil), the -i k.grad energy term and
-    // the stencil current (current_stencil, forward neighbours) are linear in
-    // them:  sum_i Re conj(psi)(-1/2 L psi) = -sum lap_n Re C,
-    //        sum_i Re conj(psi)(-i w)       = 2 sum_a k_a sum_n nab_n Im C,
-    //        current stencil term (axis a)  = sum_n nab_n Im C(a,n).
-    double n2 = 0, vs = 0;
-    double cr[3][4], ci[3][4];
+    // Forward correlations C(a,n) = conj(psi_i) psi_{i + n e_a} (FP64), summed
+    // over the grid, give the kinetic energy (total_energy_stencil), the
+    // -i k.grad energy term and the stencil current (current_stencil, forward
+    // neighbours):  sum_i Re conj(psi)(-1/2 L psi) = A0 |psi|^2 - sum lap_n Re C,
+    //               sum_i Re conj(psi)(-i w)       = 2 sum_a k_a sum_n nab_n Im C,
+    //               current stencil term (axis a)  = sum_n nab_n Im C(a,n);
+    // each point adds its correlations to those sums directly.
+    double ck[3][4];
 #pragma unroll
     for (int d = 0; d < 3; ++d)
 #pragma unroll
-      for (int nn = 0; nn < 4; ++nn) cr[d][nn] = ci[d][nn] = 0;
+      for (int nn = 0; nn < 4; ++nn) ck[d][nn] = 2.0 * kn[d] * g.nab[d][nn + 1];
+    double n2 = 0, vs = 0, te = 0, js[3] = {0, 0, 0};
 #pragma unroll 1
     for (int i = 0; i < PX; ++i) {
       const int x = x0 + i;
@@ -369,33 +370,25 @@ __global__ void __launch_bounds__(NY * NZ * XS, WPE) k_fused_step(FusedArgs<T> a
         const double q[3][2] = {{xp.re, xp.im}, {yp.re, yp.im}, {zp.re, zp.im}};
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
-          cr[d][nn - 1] += er * q[d][0] + ei * q[d][1];
-          ci[d][nn - 1] += er * q[d][1] - ei * q[d][0];
+          const double Cr = er * q[d][0] + ei * q[d][1];
+          const double Ci = er * q[d][1] - ei * q[d][0];
+          te += ck[d][nn - 1] * Ci - g.lap[d][nn] * Cr;
+          js[d] += g.nab[d][nn] * Ci;
         }
       }
     }
     {
       // this orbital's sums into the wave's accumulators
-      double te = A0 * wave_sum(n2), js[3];
-#pragma unroll
-      for (int d = 0; d < 3; ++d) {
-        js[d] = 0;
-#pragma unroll
-        for (int nn = 0; nn < 4; ++nn) {
-          const double Cr = wave_sum(cr[d][nn]), Ci = wave_sum(ci[d][nn]);
-          te += -g.lap[d][nn + 1] * Cr + 2.0 * kn[d] * g.nab[d][nn + 1] * Ci;
-          js[d] += g.nab[d][nn + 1] * Ci;
-        }
-      }
-      const double n2s = wave_sum(n2), vss = wave_sum(vs);
+      const double n2s = wave_sum(n2), vss = wave_sum(vs), tes = wave_sum(te);
+      const double j0 = wave_sum(js[0]), j1 = wave_sum(js[1]), j2 = wave_sum(js[2]);
       if (lane == 0) {
         red[wv * 10 + 0] += occ * n2s;
         red[wv * 10 + 1] += occ * vss;
-        red[wv * 10 + 2] += occ * te;
+        red[wv * 10 + 2] += occ * (A0 * n2s + tes);
         // reference: jx = jx + jxt*occ with jxt = 2*sum Im(...) (current.f90:206-210)
-        red[wv * 10 + 3] += occ * js[0];
-        red[wv * 10 + 4] += occ * js[1];
-        red[wv * 10 + 5] += occ * js[2];
+        red[wv * 10 + 3] += occ * j0;
+        red[wv * 10 + 4] += occ * j1;
+        red[wv * 10 + 5] += occ * j2;
       }
     }
     const int* jp_phase_n = opaque(a.jp_phase);

@@ -265,6 +265,7 @@ int read_sums(salmon_ctx* c, double out[8]) {
 #include "fused.hpp"
 #include "gs.hpp"
 #include "pc.hpp"
+#include "batch_host.hpp"
 
 namespace {
 int gram_schmidt_multi_fwd(salmon_ctx* c, cplx<double>* zu) {
@@ -995,6 +996,69 @@ int salmon_gpu_rt_run(salmon_ctx* c, const double* ac, int nsteps, double dt, in
   (void)hipStreamSynchronize(c->stream);
   (void)hipFree(d_res);
   return r;
+}
+
+int salmon_gpu_batch_create(int precision, salmon_batch** out) {
+  if (!out) return SALMON_ERR_ARG;
+  *out = nullptr;
+  if (precision != SALMON_FP64 && precision != SALMON_MIXED) return SALMON_ERR_ARG;
+  salmon_batch* b = new salmon_batch();
+  b->precision = precision;
+  if (hipStreamCreate(&b->stream) != hipSuccess) {
+    delete b;
+    return SALMON_ERR_HIP;
+  }
+  *out = b;
+  return SALMON_OK;
+}
+
+int salmon_gpu_batch_add(salmon_batch* b, salmon_ctx* c) {
+  if (!b) return SALMON_ERR_ARG;
+  if (!c) return bfail(b, SALMON_ERR_ARG, "context is NULL");
+  int r = check_ready(c, true);
+  if (r) return bfail(b, r, "context %zu not set up: %s", b->ctx.size(), c->err.c_str());
+  if (c->precision != b->precision) return bfail(b, SALMON_ERR_ARG, "context precision differs from the batch's");
+  if (c->g.NLz > 1 << 20) return bfail(b, SALMON_ERR_ARG, "grid too large");
+  (void)hipStreamSynchronize(c->stream);
+  batch_release(b);
+  b->tuned = false;
+  b->ctx.push_back(c);
+  return SALMON_OK;
+}
+
+int salmon_gpu_batch_tune(salmon_batch* b) {
+  if (!b) return SALMON_ERR_ARG;
+  if (b->ctx.empty()) return bfail(b, SALMON_ERR_STATE, "empty batch");
+  return b->precision == SALMON_FP64 ? batch_tune<double>(b) : batch_tune<float>(b);
+}
+
+int salmon_gpu_batch_zblock(salmon_batch* b, int i) {
+  if (!b || i < 0 || i >= (int)b->zb.size() || !b->tuned) return -1;
+  return b->zb[i];
+}
+
+int salmon_gpu_batch_rt_run(salmon_batch* b, const double* ac, int nsteps, double dt, double* jt, double* et,
+                            double* norm) {
+  if (!b) return SALMON_ERR_ARG;
+  if (b->ctx.empty()) return bfail(b, SALMON_ERR_STATE, "empty batch");
+  if (!ac || nsteps < 0) return bfail(b, SALMON_ERR_ARG, "ac is NULL or nsteps < 0");
+  if (nsteps == 0) return SALMON_OK;
+  return b->precision == SALMON_FP64 ? batch_run<double>(b, ac, nsteps, dt, jt, et, norm)
+                                     : batch_run<float>(b, ac, nsteps, dt, jt, et, norm);
+}
+
+const char* salmon_gpu_batch_last_error(const salmon_batch* b) { return b ? b->err.c_str() : "no batch"; }
+
+int salmon_gpu_batch_finalize(salmon_batch* b) {
+  if (!b) return SALMON_ERR_ARG;
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  batch_release(b);
+  if (b->d_step) (void)hipFree(b->d_step);
+  if (b->d_ac) (void)hipFree(b->d_ac);
+  if (b->d_res) (void)hipFree(b->d_res);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+  return SALMON_OK;
 }
 
 int salmon_gpu_synchronize(salmon_ctx* c) {

@@ -23,19 +23,11 @@ __device__ __forceinline__ void orbital_k(const double* kac0, int NK, int ik, do
 // the fly.  One wave per (projector, orbital); block = 4 waves.
 // ---------------------------------------------------------------------------
 template <typename T>
-__global__ void __launch_bounds__(256) k_kb_project(const cplx<T>* __restrict__ psi, int orb0, int NB, int NK,
-                                                    const double* __restrict__ kac0, double ax, double ay,
-                                                    double az, Grid g, KBTables kb, cplx<T>* __restrict__ uvpsi) {
+__device__ __forceinline__ void kb_project_wave(const cplx<T>* __restrict__ E, int ilma, double kx, double ky, double kz,
+                                                const Grid& g, const KBTables& kb, cplx<T>* __restrict__ dst) {
   const int lane = threadIdx.x & (kWave - 1);
-  const int ilma = blockIdx.x * 4 + (threadIdx.x / kWave);
-  const int o = blockIdx.y;
-  if (ilma >= kb.Nlma) return;
-  const int ikb = orb0 + o;
-  double kx, ky, kz;
-  orbital_k(kac0, NK, ikb / NB, ax, ay, az, kx, ky, kz);
   const int a = kb.lma_atom[ilma];
   const int j0 = kb.atom_start[a], m = kb.atom_start[a + 1] - j0;
-  const cplx<T>* E = psi + (size_t)o * g.NL;
   T sr = 0, si = 0;
   for (int j = lane; j < m; j += kWave) {
     const int jp = j0 + j;
@@ -52,8 +44,21 @@ __global__ void __launch_bounds__(256) k_kb_project(const cplx<T>* __restrict__ 
   si = wave_sum(si);
   if (lane == 0) {
     const T f = (T)(g.Hxyz * kb.lma_sign[ilma]);
-    uvpsi[(size_t)o * kb.Nlma + ilma] = cplx<T>{sr * f, si * f};
+    *dst = cplx<T>{sr * f, si * f};
   }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_kb_project(const cplx<T>* __restrict__ psi, int orb0, int NB, int NK,
+                                                    const double* __restrict__ kac0, double ax, double ay,
+                                                    double az, Grid g, KBTables kb, cplx<T>* __restrict__ uvpsi) {
+  const int ilma = blockIdx.x * 4 + (threadIdx.x / kWave);
+  const int o = blockIdx.y;
+  if (ilma >= kb.Nlma) return;
+  const int ikb = orb0 + o;
+  double kx, ky, kz;
+  orbital_k(kac0, NK, ikb / NB, ax, ay, az, kx, ky, kz);
+  kb_project_wave<T>(psi + (size_t)o * g.NL, ilma, kx, ky, kz, g, kb, uvpsi + (size_t)o * kb.Nlma + ilma);
 }
 
 // ---------------------------------------------------------------------------
@@ -66,21 +71,12 @@ __global__ void __launch_bounds__(256) k_kb_project(const cplx<T>* __restrict__ 
 // tt (nullable) gets F - V E before the KB term (hpsi.f90:131-133).
 // ---------------------------------------------------------------------------
 template <typename T>
-__global__ void __launch_bounds__(256) k_hpsi(const cplx<T>* __restrict__ in, cplx<T>* __restrict__ out,
-                                              cplx<T>* __restrict__ tt, int orb0, int NB, int NK,
-                                              const double* __restrict__ kac0, double ax, double ay, double az,
-                                              const T* __restrict__ vloc, Grid g, KBTables kb,
-                                              const cplx<T>* __restrict__ uvpsi) {
-  const int p = blockIdx.x * 256 + threadIdx.x;
-  const int o = blockIdx.y;
-  if (p >= g.NL) return;
-  const int ikb = orb0 + o;
-  double kx, ky, kz;
-  orbital_k(kac0, NK, ikb / NB, ax, ay, az, kx, ky, kz);
+__device__ __forceinline__ cplx<T> hpsi_point(const cplx<T>* __restrict__ E, int p, double kx, double ky, double kz,
+                                              const T* __restrict__ vloc, const Grid& g, const KBTables& kb,
+                                              const cplx<T>* __restrict__ U, cplx<T>* __restrict__ ttp) {
   const T A = (T)((kx * kx + ky * ky + kz * kz - (g.lap[0][0] + g.lap[1][0] + g.lap[2][0])) * 0.5);
   const int NLx = g.NLx, NLy = g.NLy, NLz = g.NLz;
   const int iz = p % NLz, iy = (p / NLz) % NLy, ix = p / (NLz * NLy);
-  const cplx<T>* E = in + (size_t)o * g.NL;
   const double kk[3] = {kx, ky, kz};
   T vr = 0, vi = 0, wr = 0, wi = 0;
   // z, y, x in the reference's order
@@ -109,10 +105,9 @@ __global__ void __launch_bounds__(256) k_hpsi(const cplx<T>* __restrict__ in, cp
   cplx<T> f;
   f.re = (V + A) * e.re - (T)0.5 * vr + wi;
   f.im = (V + A) * e.im - (T)0.5 * vi - wr;
-  if (tt) tt[(size_t)o * g.NL + p] = cplx<T>{f.re - V * e.re, f.im - V * e.im};
+  if (ttp) *ttp = cplx<T>{f.re - V * e.re, f.im - V * e.im};
   const int t = kb.pt2t[p];
   if (t >= 0) {
-    const cplx<T>* U = uvpsi + (size_t)o * kb.Nlma;
     for (int q = kb.t_start[t]; q < kb.t_start[t + 1]; ++q) {
       const int il = kb.e_lma[q], jp = kb.e_jp[q];
       const int a = kb.lma_atom[il];
@@ -126,7 +121,23 @@ __global__ void __launch_bounds__(256) k_hpsi(const cplx<T>* __restrict__ in, cp
       f.im += u * (c * w.im - s * w.re);
     }
   }
-  out[(size_t)o * g.NL + p] = f;
+  return f;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_hpsi(const cplx<T>* __restrict__ in, cplx<T>* __restrict__ out,
+                                              cplx<T>* __restrict__ tt, int orb0, int NB, int NK,
+                                              const double* __restrict__ kac0, double ax, double ay, double az,
+                                              const T* __restrict__ vloc, Grid g, KBTables kb,
+                                              const cplx<T>* __restrict__ uvpsi) {
+  const int p = blockIdx.x * 256 + threadIdx.x;
+  const int o = blockIdx.y;
+  if (p >= g.NL) return;
+  const int ikb = orb0 + o;
+  double kx, ky, kz;
+  orbital_k(kac0, NK, ikb / NB, ax, ay, az, kx, ky, kz);
+  out[(size_t)o * g.NL + p] = hpsi_point<T>(in + (size_t)o * g.NL, p, kx, ky, kz, vloc, g, kb,
+                                            uvpsi + (size_t)o * kb.Nlma, tt ? tt + (size_t)o * g.NL + p : nullptr);
 }
 
 // zu += z1 h1 + z2 h2 + z3 h3 + z4 h4 (update, ARTED/RT/hamiltonian.f90:99-122)
@@ -164,21 +175,13 @@ __global__ void __launch_bounds__(256) k_taylor_fold(cplx<T>* __restrict__ zu, c
 // out(ikb, 0..7) = Jx, Jy, Jz, Ekin, Enl, Eloc, Norm, 0
 // ---------------------------------------------------------------------------
 template <typename T>
-__global__ void __launch_bounds__(256) k_observe(const cplx<T>* __restrict__ zu, int orb0, int NB, int NK,
-                                                 const double* __restrict__ kac0, double ax, double ay, double az,
-                                                 const double* __restrict__ occv, const double* __restrict__ vloc,
-                                                 Grid g, KBTables kb, double* __restrict__ out) {
+__device__ __forceinline__ void observe_orbital(const cplx<T>* __restrict__ E, double kx, double ky, double kz,
+                                                double occ, const double* __restrict__ vloc, const Grid& g,
+                                                const KBTables& kb, double* __restrict__ r) {
   __shared__ double smem[4 * 8];
   __shared__ double nl[4][4];
-  const int o = blockIdx.x;
-  const int ikb = orb0 + o;
-  const int ik = ikb / NB;
-  double kx, ky, kz;
-  orbital_k(kac0, NK, ik, ax, ay, az, kx, ky, kz);
-  const double occ = occv[ikb];
   const double kk[3] = {kx, ky, kz};
   const double A0 = -(g.lap[0][0] + g.lap[1][0] + g.lap[2][0]) * 0.5;
-  const cplx<T>* E = zu + (size_t)o * g.NL;
   const int NLx = g.NLx, NLy = g.NLy, NLz = g.NLz;
   double n2 = 0, vs = 0, te = 0, jc[3] = {0, 0, 0};
   for (int p = threadIdx.x; p < g.NL; p += 256) {
@@ -260,7 +263,6 @@ __global__ void __launch_bounds__(256) k_observe(const cplx<T>* __restrict__ zu,
     double J[3], Enl = 0;
     for (int d = 0; d < 3; ++d) J[d] = (nl[0][d] + nl[1][d]) + (nl[2][d] + nl[3][d]);
     Enl = (nl[0][3] + nl[1][3]) + (nl[2][3] + nl[3][3]);
-    double* r = out + (size_t)ikb * 8;
     const double s = g.Hxyz / g.aLxyz;
     for (int d = 0; d < 3; ++d) r[d] = (occ * kk[d] * v[0] + 2.0 * v[3 + d] * occ) * s + J[d];
     r[3] = occ * v[2] * g.Hxyz + occ * (kx * kx + ky * ky + kz * kz) / 2.0;
@@ -269,6 +271,18 @@ __global__ void __launch_bounds__(256) k_observe(const cplx<T>* __restrict__ zu,
     r[6] = occ * v[0] * g.Hxyz;
     r[7] = 0.0;
   }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_observe(const cplx<T>* __restrict__ zu, int orb0, int NB, int NK,
+                                                 const double* __restrict__ kac0, double ax, double ay, double az,
+                                                 const double* __restrict__ occv, const double* __restrict__ vloc,
+                                                 Grid g, KBTables kb, double* __restrict__ out) {
+  const int o = blockIdx.x;
+  const int ikb = orb0 + o;
+  double kx, ky, kz;
+  orbital_k(kac0, NK, ikb / NB, ax, ay, az, kx, ky, kz);
+  observe_orbital<T>(zu + (size_t)o * g.NL, kx, ky, kz, occv[ikb], vloc, g, kb, out + (size_t)ikb * 8);
 }
 
 // Deterministic sum of per-orbital rows (NKB, 8) -> dst[0..7]; one block.

@@ -242,6 +242,48 @@ module salmon_gpu
       type(c_ptr), value :: jt, et, norm, rho
     end function
 
+    integer(c_int) function salmon_gpu_batch_create(precision, batch) bind(C, name="salmon_gpu_batch_create")
+      import :: c_int, c_ptr
+      integer(c_int), value :: precision
+      type(c_ptr), intent(out) :: batch
+    end function
+
+    integer(c_int) function salmon_gpu_batch_add(batch, ctx) bind(C, name="salmon_gpu_batch_add")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: batch, ctx
+    end function
+
+    integer(c_int) function salmon_gpu_batch_tune(batch) bind(C, name="salmon_gpu_batch_tune")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: batch
+    end function
+
+    integer(c_int) function salmon_gpu_batch_zblock(batch, sys) bind(C, name="salmon_gpu_batch_zblock")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: batch
+      integer(c_int), value :: sys
+    end function
+
+    integer(c_int) function salmon_gpu_batch_rt_run(batch, ac, nsteps, dt, jt, et, norm) &
+        bind(C, name="salmon_gpu_batch_rt_run")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: batch
+      real(c_double), intent(in) :: ac(*)
+      integer(c_int), value :: nsteps
+      real(c_double), value :: dt
+      type(c_ptr), value :: jt, et, norm
+    end function
+
+    type(c_ptr) function salmon_gpu_batch_last_error(batch) bind(C, name="salmon_gpu_batch_last_error")
+      import :: c_ptr
+      type(c_ptr), value :: batch
+    end function
+
+    integer(c_int) function salmon_gpu_batch_finalize(batch) bind(C, name="salmon_gpu_batch_finalize")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: batch
+    end function
+
     integer(c_int) function salmon_gpu_set_path(ctx, path) bind(C, name="salmon_gpu_set_path")
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx

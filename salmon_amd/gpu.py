@@ -86,6 +86,13 @@ SIGNATURES = {
     "salmon_gpu_pcnet_gradient": [_vp, _vp, _ci, _vp, _vp, _vp],
     "salmon_gpu_pcnet_train": [_vp, _vp, _ci, _ci, _cd, ctypes.c_uint, _vp],
     "salmon_gpu_rt_run": [_vp, _vp, _ci, _cd, _ci, _vp, _vp, _vp, _vp],
+    "salmon_gpu_batch_create": [_ci, ctypes.POINTER(_vp)],
+    "salmon_gpu_batch_add": [_vp, _vp],
+    "salmon_gpu_batch_tune": [_vp],
+    "salmon_gpu_batch_zblock": [_vp, _ci],
+    "salmon_gpu_batch_rt_run": [_vp, _vp, _ci, _cd, _vp, _vp, _vp],
+    "salmon_gpu_batch_last_error": [_vp],
+    "salmon_gpu_batch_finalize": [_vp],
     "salmon_gpu_set_path": [_vp, _ci],
     "salmon_gpu_path": [_vp],
     "salmon_gpu_synchronize": [_vp],
@@ -108,7 +115,7 @@ def lib():
         for name, argt in SIGNATURES.items():
             f = getattr(L, name)
             f.argtypes = argt
-            if name in ("salmon_gpu_errstr", "salmon_gpu_last_error"):
+            if name in ("salmon_gpu_errstr", "salmon_gpu_last_error", "salmon_gpu_batch_last_error"):
                 f.restype = ctypes.c_char_p
             elif name == "salmon_gpu_stream":
                 f.restype = ctypes.c_void_p
@@ -375,3 +382,53 @@ class PcNet:
             self.close()
         except Exception:
             pass
+
+
+class Batch:
+    """Independent set-up contexts run together (one launch per phase over all
+    systems, the step replayed from a captured graph)."""
+
+    def __init__(self, contexts, precision=FP64):
+        self._h = ctypes.c_void_p()
+        rc = lib().salmon_gpu_batch_create(int(precision), ctypes.byref(self._h))
+        if rc != 0:
+            raise SalmonGPUError(f"salmon_gpu_batch_create: {lib().salmon_gpu_errstr(rc).decode()}")
+        self.contexts = list(contexts)
+        for c in self.contexts:
+            self._check(lib().salmon_gpu_batch_add(self._h, c._h), "batch_add")
+
+    def _check(self, rc, what):
+        if rc != 0:
+            msg = lib().salmon_gpu_batch_last_error(self._h).decode() if self._h else ""
+            raise SalmonGPUError(f"{what}: {lib().salmon_gpu_errstr(rc).decode()}: {msg}")
+
+    def tune(self):
+        self._check(lib().salmon_gpu_batch_tune(self._h), "batch_tune")
+        return [lib().salmon_gpu_batch_zblock(self._h, i) for i in range(len(self.contexts))]
+
+    def rt_run(self, ac, nsteps, dt):
+        n = len(self.contexts)
+        ac = _f64(np.asarray(ac)[: nsteps + 1])
+        jt = np.zeros((n, nsteps, 3))
+        et = np.zeros((n, nsteps))
+        nt = np.zeros((n, nsteps))
+        self._check(lib().salmon_gpu_batch_rt_run(self._h, _p(ac), int(nsteps), float(dt), _p(jt), _p(et), _p(nt)),
+                    "batch_rt_run")
+        return dict(J=jt, E=et, norm=nt)
+
+    def close(self):
+        if self._h:
+            lib().salmon_gpu_batch_finalize(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()

@@ -403,3 +403,29 @@ def config5(seed=0, NB=64):
     """32^3, L=15, Gamma, 64 bands, 27 random-position atoms (ground-state CG)."""
     return make_system(N=(32, 32, 32), L=15.0, nk3=(1, 1, 1), NB=NB, seed=seed, atoms="random",
                        natoms=27, name="cfg5")
+
+
+CFG4_SHAPES = [(12, 12, 12), (16, 16, 16), (20, 20, 28), (24, 24, 24), (32, 32, 32)]
+CFG4_H = 10.26 / 16  # grid spacing of every system (keeps dt * rho(H) inside the Taylor-4 region)
+
+
+def config4_system(i, scale=1.0, orbitals=True):
+    """System i of the 64-system batch: grid from CFG4_SHAPES, bands
+    U{8..64}, k points U{1..64} (the first NK points of a 4x4x4
+    Monkhorst-Pack grid), atoms U{1..16} at random positions; seed i.
+    scale < 1 shrinks bands and k points (for quick checks)."""
+    rng = np.random.default_rng(10_000 + i)
+    N = CFG4_SHAPES[int(rng.integers(len(CFG4_SHAPES)))]
+    NB = max(1, int(round(int(rng.integers(8, 65)) * scale)))
+    NK = max(1, int(round(int(rng.integers(1, 65)) * scale)))
+    natoms = int(rng.integers(1, 17))
+    L = tuple(n * CFG4_H for n in N)
+    s = make_system(N=N, L=L, nk3=(4, 4, 4), NB=NB, seed=i, atoms="random", natoms=natoms, orbitals=False,
+                    name=f"cfg4_{i}")
+    s.kAc0 = np.ascontiguousarray(s.kAc0[:, :NK])
+    s.occ = np.full((NK, NB), 2.0 / NK)
+    if orbitals:
+        s.zu = random_orbitals(np.random.default_rng(20_000 + i), NK, NB, s.NL, s.Hxyz, orthonormalise=False)
+    else:
+        s.zu = np.zeros((NK, NB, s.NL), np.complex128)
+    return s

@@ -1,0 +1,84 @@
+"""Batched independent systems on the GPU: every system of a batch (different
+grids, bands, k points and atoms; one launch per phase over all systems, the
+step replayed from a captured graph, autotuned z blocking) against the same
+system run alone through the single-context streamed path.
+
+Tolerance: FP64 1e-12 relative on J(t), E(t) and the norms (the batched
+stencil keeps the single-context summation order; only FMA contraction may
+differ); complex64 1e-5 (max_t |dJ| / max_t |J|).
+"""
+import numpy as np
+import pytest
+
+from helpers import rel
+from salmon_amd import gpu, synth
+
+pytestmark = pytest.mark.gpu
+
+SYSTEMS = [0, 1, 2, 3, 5, 9]  # shapes 12^3, 20x20x28, 32^3, 16^3, 16^3, ...
+
+
+def _systems():
+    return [synth.config4_system(i, scale=0.12) for i in SYSTEMS]
+
+
+def _ac(n):
+    return synth.acos2_pulse(n, 0.08, 0.05, 0.057, 826.8)
+
+
+@pytest.mark.parametrize("prec", [gpu.FP64, gpu.MIXED])
+def test_batch_matches_single_contexts(prec):
+    nsteps = 6
+    ac = _ac(nsteps)
+    systems = _systems()
+    print(f"[gpu] batch prec={prec}: " + ", ".join(f"{s.NLx}x{s.NLy}x{s.NLz}/{s.NK}k/{s.NB}b" for s in systems),
+          flush=True)
+    singles = []
+    for s in systems:
+        with gpu.Context(s, prec) as c:
+            c.set_path(gpu.PATH_STREAMED)
+            singles.append(c.rt_run(ac, nsteps, dt=0.08))
+    ctxs = [gpu.Context(s, prec) for s in systems]
+    with gpu.Batch(ctxs, prec) as b:
+        zb = b.tune()
+        print(f"[gpu] batch z blocking {zb}", flush=True)
+        assert all(s.NLz % z == 0 for s, z in zip(systems, zb))
+        r1 = b.rt_run(ac[: 3 + 1], 3, 0.08)
+        r2 = b.rt_run(ac[3:], 3, 0.08)  # continues from the resident orbitals
+    J = np.concatenate([r1["J"], r2["J"]], axis=1)
+    E = np.concatenate([r1["E"], r2["E"]], axis=1)
+    N = np.concatenate([r1["norm"], r2["norm"]], axis=1)
+    for i, r in enumerate(singles):
+        tol = 1e-12 if prec == gpu.FP64 else 1e-5
+        ej = np.max(np.abs(J[i] - r["J"])) / np.max(np.abs(r["J"]))
+        print(f"[gpu] batch system {i}: J {ej:.2e} E {rel(E[i], r['E']):.2e} norm {rel(N[i], r['norm']):.2e}",
+              flush=True)
+        assert ej < tol
+        assert rel(E[i], r["E"]) < tol
+        assert rel(N[i], r["norm"]) < tol
+    for c in ctxs:
+        c.close()
+
+
+def test_batch_orbitals_match_single_context():
+    """The batch advances the contexts' resident orbitals exactly as a single
+    context does."""
+    nsteps = 2
+    ac = _ac(nsteps)
+    s = synth.config4_system(1, scale=0.12)
+    with gpu.Context(s, gpu.FP64) as c:
+        c.set_path(gpu.PATH_STREAMED)
+        c.rt_run(ac, nsteps, dt=0.08)
+        z1 = c.get_orbitals()
+    with gpu.Context(s, gpu.FP64) as c:
+        with gpu.Batch([c], gpu.FP64) as b:
+            b.rt_run(ac, nsteps, 0.08)
+        z2 = c.get_orbitals()
+    assert rel(z2, z1) < 1e-13
+
+
+def test_batch_rejects_mixed_precisions():
+    s = synth.config4_system(0, scale=0.1)
+    with gpu.Context(s, gpu.FP64) as c64, gpu.Context(s, gpu.MIXED) as c32:
+        with pytest.raises(gpu.SalmonGPUError, match="precision"):
+            gpu.Batch([c64, c32], gpu.FP64)

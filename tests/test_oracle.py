@@ -300,3 +300,21 @@ def test_kinetic_preconditioner_closed_form():
     assert rel(o.kinetic_precondition(tx, 2 * a - 3j * b), 2 * ka - 3j * kb) < 1e-13
     assert abs(np.vdot(b, ka) - np.conj(np.vdot(a, kb))) < 1e-12 * abs(np.vdot(b, ka))
     assert np.vdot(a, ka).real > 0
+
+
+def test_config4_generator():
+    """The 64-system batch: every system draws its grid from the five shapes,
+    bands in 8..64, k points in 1..64 and atoms in 1..16, deterministically;
+    the grid spacing keeps dt * rho(H) inside the Taylor-4 region."""
+    seen = set()
+    for i in range(64):
+        s = synth.config4_system(i, orbitals=False)
+        seen.add((s.NLx, s.NLy, s.NLz))
+        assert (s.NLx, s.NLy, s.NLz) in synth.CFG4_SHAPES
+        assert 8 <= s.NB <= 64 and 1 <= s.NK <= 64 and 1 <= s.NI <= 16
+        assert s.occ.shape == (s.NK, s.NB) and abs(s.occ.sum() - 2 * s.NB) < 1e-12
+    assert seen == set(synth.CFG4_SHAPES)
+    a, b = synth.config4_system(7, orbitals=False), synth.config4_system(7, orbitals=False)
+    assert np.array_equal(a.vloc, b.vloc) and a.NB == b.NB and a.NK == b.NK
+    s = synth.config4_system(2, orbitals=False)
+    assert s.dt * synth.spectral_bound(s) < 2 * math.sqrt(2)
