@@ -19,6 +19,7 @@ iterations per band.
 
   python -m salmon_amd.pctrain train --train 200 --out weights.npy
   python -m salmon_amd.pctrain eval --weights weights.npy --eval 50
+  python -m salmon_amd.pctrain solve --seed 0 --sweeps 5 --precond kinetic   (timing of a few sweeps)
 """
 from __future__ import annotations
 
@@ -53,8 +54,13 @@ def prepare(seed, nb=NB, target_sweeps=30, record_sweeps=3, stride=3):
     return c
 
 
-def train(n_train=200, epochs=6, steps=8, batch=8, lr=2e-3, seed=0, nb=NB, log=_log):
-    net = gpu.PcNet(seed=seed)
+def train(n_train=200, epochs=6, steps=8, batch=8, lr=2e-3, seed=0, nb=NB, dagger_epochs=0, weights=None,
+          log=_log):
+    """Train on residuals of kinetic-preconditioned CG; with dagger_epochs > 0,
+    then re-record every system's residuals along the learned solver's own
+    trajectory (its first three sweeps) and train on those as well (the
+    network meets the iterates it produces, not only the kinetic solver's)."""
+    net = gpu.PcNet(seed=seed, weights=weights)
     ctxs = []
     t0 = time.time()
     for i in range(n_train):
@@ -62,11 +68,25 @@ def train(n_train=200, epochs=6, steps=8, batch=8, lr=2e-3, seed=0, nb=NB, log=_
         if i % 10 == 9 or i == n_train - 1:
             log(f"[pctrain] prepared {i + 1}/{n_train} systems ({time.time() - t0:.0f} s)")
     rng = np.random.default_rng(seed)
-    for ep in range(epochs):
-        losses = []
-        for j in rng.permutation(len(ctxs)):
-            losses.append(net.train(ctxs[j], steps, batch=batch, lr=lr * (0.5 ** (ep // 3)), seed=int(rng.integers(1 << 30))))
-        log(f"[pctrain] epoch {ep}: mean loss {np.mean(losses):.4f} ({time.time() - t0:.0f} s)")
+
+    def run_epochs(n, lr0):
+        for ep in range(n):
+            losses = []
+            for j in rng.permutation(len(ctxs)):
+                losses.append(net.train(ctxs[j], steps, batch=batch, lr=lr0 * (0.5 ** (ep // 3)),
+                                        seed=int(rng.integers(1 << 30))))
+            log(f"[pctrain] epoch {ep}: mean loss {np.mean(losses):.4f} ({time.time() - t0:.0f} s)")
+
+    run_epochs(epochs, lr)
+    if dagger_epochs > 0:
+        for c in ctxs:
+            s = c.sys
+            c.set_pcnet(net)
+            c.put_orbitals(s.zu)
+            c.pc_record(3 * s.NB * NCG // 3 + 1, 3)
+            c.ground_state(s.kAc0, 3, iter_cg_max=NCG, precond=gpu.PC_LEARNED, tol=0.0)
+        log(f"[pctrain] residuals re-recorded along the learned trajectories ({time.time() - t0:.0f} s)")
+        run_epochs(dagger_epochs, lr / 4)
     w = net.weights()
     for c in ctxs:
         c.close()
@@ -105,16 +125,38 @@ def evaluate(net, n_eval=50, acc=1e-8, max_sweeps=120, nb=NB, log=_log):
 
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("mode", choices=["train", "eval"])
+    ap.add_argument("mode", choices=["train", "eval", "solve"])
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--sweeps", type=int, default=5)
+    ap.add_argument("--precond", choices=["none", "kinetic", "learned"], default="kinetic")
     ap.add_argument("--train", type=int, default=200)
     ap.add_argument("--eval", type=int, default=50)
     ap.add_argument("--epochs", type=int, default=6)
+    ap.add_argument("--dagger-epochs", type=int, default=0)
+    ap.add_argument("--init", default=None, help="start from these weights")
     ap.add_argument("--nb", type=int, default=NB)
-    ap.add_argument("--weights", default="pcnet_weights.npy")
+    ap.add_argument("--weights", default="salmon_amd/data/pcnet_cfg5.npy")
     ap.add_argument("--out", default=None)
     a = ap.parse_args(argv)
+    if a.mode == "solve":
+        pc = {"none": gpu.PC_NONE, "kinetic": gpu.PC_KINETIC, "learned": gpu.PC_LEARNED}[a.precond]
+        s = synth.config5(seed=a.seed, NB=a.nb)
+        with gpu.Context(s, gpu.FP64) as c:
+            net = None
+            if pc == gpu.PC_LEARNED:
+                net = gpu.PcNet(weights=np.load(a.weights))
+                c.set_pcnet(net)
+            c.ground_state(s.kAc0, 1, iter_cg_max=NCG, precond=pc, tol=0.0)  # warm-up sweep
+            c.put_orbitals(s.zu)
+            t = time.time()
+            r = c.ground_state(s.kAc0, a.sweeps, iter_cg_max=NCG, precond=pc, tol=0.0)
+            dt = time.time() - t
+        _log(json.dumps(dict(seed=a.seed, precond=a.precond, sweeps=r["sweeps"], ms_per_sweep=dt / r["sweeps"] * 1e3,
+                             lowest=float(r["esp"][0][0]), highest=float(r["esp"][0][-1]))))
+        return 0
     if a.mode == "train":
-        _, w = train(a.train, epochs=a.epochs, nb=a.nb)
+        _, w = train(a.train, epochs=a.epochs, nb=a.nb, dagger_epochs=a.dagger_epochs,
+                     weights=np.load(a.init) if a.init else None)
         np.save(a.out or a.weights, w)
         _log(f"[pctrain] weights written to {a.out or a.weights}")
     else:

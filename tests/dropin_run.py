@@ -17,7 +17,24 @@ from helpers import load_case  # noqa: E402
 from oracle import ref  # noqa: E402
 
 
+def ground_state(case, which, ncg):
+    """Gram_Schmidt then CG_omp(ncg) through the reference's harness."""
+    s, d = load_case(case)
+    path = ref.LIB_GPU if which == "gpu" else ref.LIB
+    R = ref.Ref(s, lib_path=path)
+    R.gram_schmidt()
+    zg = R.get()
+    ev = R.cg(s.kAc0, ncg)
+    z = R.get()
+    calls = ctypes.c_int.in_dll(ref.lib(), "arted_gpu_gs_calls").value if which == "gpu" else -1
+    print(json.dumps(dict(zg=[zg.real.ravel()[::97].tolist(), zg.imag.ravel()[::97].tolist()],
+                          z=[z.real.ravel()[::97].tolist(), z.imag.ravel()[::97].tolist()],
+                          esp_var=ev.ravel().tolist(), calls=calls)))
+
+
 def main(case, which, nsteps):
+    if which.endswith("-gs"):
+        return ground_state(case, which[:-3], nsteps)
     s, d = load_case(case)
     path = ref.LIB_GPU if which == "gpu" else ref.LIB
     R = ref.Ref(s, lib_path=path)
