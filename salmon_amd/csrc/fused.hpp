@@ -619,11 +619,10 @@ template <typename T, int NX, int NY, int NZ, int XS, int WPE, bool OV>
 static int fused_launch(salmon_ctx* c, const salmon::FusedArgs<T>& args, int nwg) {
   auto kern = salmon::k_fused_step<T, NX, NY, NZ, XS, WPE, OV>;
   const size_t lds = fused_lds_bytes<T, NX, NY, NZ, XS>();
-  static bool attr_set = false;
-  if (!attr_set) {
-    HIPCHK(c, hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    attr_set = true;
-  }
+  // set once per kernel instantiation (a thread-safe static initialisation)
+  static const hipError_t attr =
+      hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (attr != hipSuccess) return fail(c, SALMON_ERR_HIP, "hipFuncSetAttribute: %s", hipGetErrorString(attr));
   hipLaunchKernelGGL(kern, dim3(nwg), dim3(NY * NZ * XS), lds, c->stream, args);
   LAUNCHCHK(c);
   return SALMON_OK;

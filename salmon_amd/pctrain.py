@@ -27,6 +27,7 @@ import argparse
 import json
 import sys
 import time
+from concurrent.futures import ThreadPoolExecutor
 
 import numpy as np
 
@@ -34,6 +35,7 @@ from . import gpu, synth
 
 NB = 64
 NCG = 4
+WORKERS = 8          # contexts driven at once (one host thread and one HIP stream each)
 TRAIN_SEED0 = 1000   # training potentials: seeds 1000..1000+n-1
 EVAL_SEED0 = 0       # held-out potentials: seeds 0..n-1
 
@@ -60,13 +62,22 @@ def train(n_train=200, epochs=6, steps=8, batch=8, lr=2e-3, seed=0, nb=NB, dagge
     then re-record every system's residuals along the learned solver's own
     trajectory (its first three sweeps) and train on those as well (the
     network meets the iterates it produces, not only the kinetic solver's)."""
+    gpu.lib()
     net = gpu.PcNet(seed=seed, weights=weights)
-    ctxs = []
     t0 = time.time()
-    for i in range(n_train):
-        ctxs.append(prepare(TRAIN_SEED0 + i, nb=nb))
-        if i % 10 == 9 or i == n_train - 1:
-            log(f"[pctrain] prepared {i + 1}/{n_train} systems ({time.time() - t0:.0f} s)")
+    # several contexts (each on its own stream) are prepared at once: one
+    # band-sequential solve covers only a few dozen blocks of the GPU
+    done = [0]
+
+    def prep(i):
+        c = prepare(TRAIN_SEED0 + i, nb=nb)
+        done[0] += 1
+        if done[0] % 10 == 0:
+            log(f"[pctrain] prepared {done[0]}/{n_train} systems ({time.time() - t0:.0f} s)")
+        return c
+
+    with ThreadPoolExecutor(max_workers=WORKERS) as ex:
+        ctxs = list(ex.map(prep, range(n_train)))
     rng = np.random.default_rng(seed)
 
     def run_epochs(n, lr0):
@@ -79,12 +90,15 @@ def train(n_train=200, epochs=6, steps=8, batch=8, lr=2e-3, seed=0, nb=NB, dagge
 
     run_epochs(epochs, lr)
     if dagger_epochs > 0:
-        for c in ctxs:
+        def rerecord(c):
             s = c.sys
             c.set_pcnet(net)
             c.put_orbitals(s.zu)
             c.pc_record(3 * s.NB * NCG // 3 + 1, 3)
             c.ground_state(s.kAc0, 3, iter_cg_max=NCG, precond=gpu.PC_LEARNED, tol=0.0)
+
+        with ThreadPoolExecutor(max_workers=WORKERS) as ex:
+            list(ex.map(rerecord, ctxs))
         log(f"[pctrain] residuals re-recorded along the learned trajectories ({time.time() - t0:.0f} s)")
         run_epochs(dagger_epochs, lr / 4)
     w = net.weights()
@@ -101,8 +115,9 @@ def sweeps_to(history, ref, acc):
 
 
 def evaluate(net, n_eval=50, acc=1e-8, max_sweeps=120, nb=NB, log=_log):
-    rows = []
-    for i in range(n_eval):
+    gpu.lib()
+
+    def one(i):
         s = synth.config5(seed=EVAL_SEED0 + i, NB=nb)
         with gpu.Context(s, gpu.FP64) as c:
             rk = c.ground_state(s.kAc0, max_sweeps, iter_cg_max=NCG, precond=gpu.PC_KINETIC, tol=1e-12)
@@ -113,9 +128,12 @@ def evaluate(net, n_eval=50, acc=1e-8, max_sweeps=120, nb=NB, log=_log):
         nk, nl = sweeps_to(rk["history"], ref, acc), sweeps_to(rl["history"], ref, acc)
         row = dict(seed=EVAL_SEED0 + i, kinetic=nk, learned=nl,
                    learned_final_err=float(np.max(np.abs(rl["esp"] - ref))))
-        rows.append(row)
         log(f"[pctrain] eval seed {row['seed']}: sweeps to {acc:g} Ha kinetic {nk} learned {nl} "
             f"(final err {row['learned_final_err']:.1e})")
+        return row
+
+    with ThreadPoolExecutor(max_workers=WORKERS) as ex:
+        rows = list(ex.map(one, range(n_eval)))
     ok = [r for r in rows if r["kinetic"] and r["learned"]]
     ratio = float(np.sum([r["kinetic"] for r in ok]) / max(1, np.sum([r["learned"] for r in ok])))
     return dict(rows=rows, acc=acc, ncg=NCG, nb=nb, sweep_ratio=ratio,
