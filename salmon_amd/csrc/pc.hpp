@@ -17,7 +17,7 @@ struct salmon_pcnet {
   long step = 0;
   std::string err;
   // training buffers, sized for n samples of NL points
-  int cap_n = 0, cap_NL = 0, cap_NB = 0;
+  int cap_n = 0, cap_NL = 0, cap_NB = 0, cap_nlma = 0;
   std::vector<void*> bufs;
   float *u = nullptr, *a1 = nullptr, *a2 = nullptr, *y = nullptr, *gy = nullptr, *ga1 = nullptr, *ga2 = nullptr;
   float* wpart = nullptr;
@@ -43,7 +43,7 @@ static int pc_fail(salmon_pcnet* n, int code, const char* msg) {
 static void pcnet_free_buffers(salmon_pcnet* n) {
   for (void* b : n->bufs) (void)hipFree(b);
   n->bufs.clear();
-  n->cap_n = n->cap_NL = n->cap_NB = 0;
+  n->cap_n = n->cap_NL = n->cap_NB = n->cap_nlma = 0;
 }
 
 template <typename X>
@@ -55,7 +55,7 @@ static int pc_alloc(std::vector<void*>& bufs, X** p, size_t n) {
 }
 
 static int pcnet_buffers(salmon_pcnet* n, int ns, int NL, int NB, int Nlma) {
-  if (ns <= n->cap_n && NL == n->cap_NL && NB <= n->cap_NB) return SALMON_OK;
+  if (ns <= n->cap_n && NL == n->cap_NL && NB <= n->cap_NB && Nlma <= n->cap_nlma) return SALMON_OK;
   pcnet_free_buffers(n);
   const size_t v = (size_t)ns * NL;
   const int nchunk = (NL + kRedChunk - 1) / kRedChunk;
@@ -87,6 +87,7 @@ static int pcnet_buffers(salmon_pcnet* n, int ns, int NL, int NB, int Nlma) {
   n->cap_n = ns;
   n->cap_NL = NL;
   n->cap_NB = NB;
+  n->cap_nlma = std::max(1, Nlma);
   return SALMON_OK;
 }
 

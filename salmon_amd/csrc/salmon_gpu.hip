@@ -42,6 +42,7 @@ struct salmon_ctx {
   int chunk = 0;
   void* d_work[4] = {nullptr, nullptr, nullptr, nullptr};
   void* d_uvpsi = nullptr;
+  int uvpsi_nlma = 0;         // projectors d_uvpsi is sized for
   double* d_rows = nullptr;   // (NKB*8)
   double* d_sum = nullptr;    // (8)
   double* d_rho = nullptr;    // (NL)
@@ -107,7 +108,12 @@ void dfree(void** p) {
 size_t elem_size(const salmon_ctx* c) { return c->precision == SALMON_MIXED ? sizeof(cplx<float>) : sizeof(cplx<double>); }
 
 int ensure_work(salmon_ctx* c) {
-  if (c->d_work[0]) return SALMON_OK;
+  // the projection buffer follows the projector count (tables may be set after the first call)
+  if (c->d_work[0]) {
+    if (c->kb.Nlma <= c->uvpsi_nlma) return SALMON_OK;
+    c->uvpsi_nlma = std::max(1, c->kb.Nlma);
+    return dalloc(c, (char**)&c->d_uvpsi, (size_t)c->chunk * c->uvpsi_nlma * sizeof(cplx<double>));
+  }
   // work buffers hold at most ~1 GiB each
   size_t per = (size_t)c->g.NL * elem_size(c);
   size_t cap = std::max<size_t>(1, ((size_t)1 << 30) / per);
@@ -116,7 +122,8 @@ int ensure_work(salmon_ctx* c) {
     hipError_t e = hipMalloc(&c->d_work[i], (size_t)c->chunk * per);
     if (e != hipSuccess) return fail(c, SALMON_ERR_NOMEM, "work buffer: %s", hipGetErrorString(e));
   }
-  int r = dalloc(c, (char**)&c->d_uvpsi, (size_t)c->chunk * std::max(1, c->kb.Nlma) * sizeof(cplx<double>));
+  c->uvpsi_nlma = std::max(1, c->kb.Nlma);
+  int r = dalloc(c, (char**)&c->d_uvpsi, (size_t)c->chunk * c->uvpsi_nlma * sizeof(cplx<double>));
   if (r) return r;
   return SALMON_OK;
 }
@@ -635,8 +642,10 @@ int salmon_gpu_density(salmon_ctx* c, double* rho) {
 }
 
 int salmon_gpu_gram_schmidt(salmon_ctx* c) {
-  int r = check_ready(c, true);
-  if (r) return r;
+  // needs the orbitals only (no potential or projectors)
+  if (!c) return SALMON_ERR_ARG;
+  if (c->NKB == 0) return fail(c, SALMON_ERR_STATE, "salmon_gpu_set_kpoints not called");
+  if (!c->have_zu) return fail(c, SALMON_ERR_STATE, "salmon_gpu_put_orbitals not called");
   return gram_schmidt_dev(c);
 }
 
