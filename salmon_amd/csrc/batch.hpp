@@ -2,8 +2,8 @@
 // counts, k points and atoms.  Each phase of the step (KB projection and
 // stencil for each of the four Taylor terms, the Taylor fold, the
 // observables, their per-system sums) is one launch covering every system:
-// blocks are laid out system after system and a block finds its system by a
-// binary search over the systems' first-block offsets.  One step is captured
+// blocks are laid out system after system and a block finds its system in a
+// per-phase block -> system table (one load).  One step is captured
 // in a HIP graph and replayed; A(t) comes from a device table indexed by a
 // device step counter.  The stencil's z blocking (points per thread along z)
 // is chosen per grid shape by timing the candidates on the system's own data.
@@ -32,21 +32,14 @@ struct Sys {
   long first[kPhases + 1];   // first block of the system in each phase
 };
 
-template <typename T>
-__device__ __forceinline__ int find_sys(const Sys<T>* __restrict__ S, int n, int phase, long b) {
-  int lo = 0, hi = n - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (S[mid].first[phase] <= b) lo = mid; else hi = mid - 1;
-  }
-  return lo;
-}
-
 struct Step {
   const double* ac;  // (nsteps+1, 3)
   const int* step;   // device step counter
   double dt;
+  const unsigned short* owner[kPhases];  // system of every block, per phase
 };
+
+__device__ __forceinline__ int find_sys(const Step& st, int phase, long b) { return st.owner[phase][b]; }
 
 __device__ __forceinline__ void step_fields(const Step& st, double amid[3], double anew[3]) {
   const int n = *st.step;
@@ -61,7 +54,7 @@ __device__ __forceinline__ void step_fields(const Step& st, double amid[3], doub
 template <typename T>
 __global__ void __launch_bounds__(256) k_b_project(const Sys<T>* __restrict__ S, int nsys, int term, Step st) {
   const long b = blockIdx.x;
-  const int s = find_sys(S, nsys, 0, b);
+  const int s = find_sys(st, 0, b);
   const Sys<T>& y = S[s];
   const long lb = b - y.first[0];
   const int o = (int)(lb / y.pblk);
@@ -151,35 +144,30 @@ __device__ __forceinline__ void hpsi_zblock(const cplx<T>* __restrict__ E, int p
 }
 
 // Stencil + local + KB of Taylor term `term`: h[term] = H (term == 0 ? zu : h[term-1]).
-template <typename T>
-__global__ void __launch_bounds__(256) k_b_hpsi(const Sys<T>* __restrict__ S, int nsys, int term, Step st) {
-  const long b = blockIdx.x;
-  const int s = find_sys(S, nsys, 1, b);
+// One launch per z-blocking class (the systems tuned to ZB); block `base +
+// blockIdx.x` of the stencil phase.
+template <typename T, int ZB>
+__global__ void __launch_bounds__(256) k_b_hpsi(const Sys<T>* __restrict__ S, int term, Step st, long base) {
+  const long b = base + blockIdx.x;
+  const int s = find_sys(st, 1, b);
   const Sys<T>& y = S[s];
   const long lb = b - y.first[1];
   const int o = (int)(lb / y.hblk);
-  const int p0 = ((int)(lb % y.hblk) * 256 + threadIdx.x) * y.zb;
+  const int p0 = ((int)(lb % y.hblk) * 256 + threadIdx.x) * ZB;
   if (o >= y.NKB || p0 >= y.g.NL) return;
   double am[3], an[3];
   step_fields(st, am, an);
   double kx, ky, kz;
   orbital_k(y.kac0, y.NK, o / y.NB, am[0], am[1], am[2], kx, ky, kz);
   const cplx<T>* in = (term == 0 ? y.zu : y.h[term - 1]) + (size_t)o * y.g.NL;
-  cplx<T>* out = y.h[term] + (size_t)o * y.g.NL;
-  const cplx<T>* U = y.uv + (size_t)o * y.kb.Nlma;
-  switch (y.zb) {
-    case 1: hpsi_zblock<T, 1>(in, p0, kx, ky, kz, y.vloc, y.g, y.kb, U, out); break;
-    case 2: hpsi_zblock<T, 2>(in, p0, kx, ky, kz, y.vloc, y.g, y.kb, U, out); break;
-    case 4: hpsi_zblock<T, 4>(in, p0, kx, ky, kz, y.vloc, y.g, y.kb, U, out); break;
-    default: hpsi_zblock<T, 8>(in, p0, kx, ky, kz, y.vloc, y.g, y.kb, U, out); break;
-  }
+  hpsi_zblock<T, ZB>(in, p0, kx, ky, kz, y.vloc, y.g, y.kb, y.uv + (size_t)o * y.kb.Nlma, y.h[term] + (size_t)o * y.g.NL);
 }
 
 // zu += sum_n zfac(n) h_n, zfac(n) = (-i dt)^n/n! (ARTED/RT/hamiltonian.f90:46-49,99-122).
 template <typename T>
 __global__ void __launch_bounds__(256) k_b_fold(const Sys<T>* __restrict__ S, int nsys, Step st) {
   const long b = blockIdx.x;
-  const int s = find_sys(S, nsys, 2, b);
+  const int s = find_sys(st, 2, b);
   const Sys<T>& y = S[s];
   const size_t i = (size_t)(b - y.first[2]) * 256 + threadIdx.x;
   if (i >= (size_t)y.NKB * y.g.NL) return;
@@ -202,7 +190,7 @@ __global__ void __launch_bounds__(256) k_b_fold(const Sys<T>* __restrict__ S, in
 template <typename T>
 __global__ void __launch_bounds__(256) k_b_observe(const Sys<T>* __restrict__ S, int nsys, Step st) {
   const long b = blockIdx.x;
-  const int s = find_sys(S, nsys, 3, b);
+  const int s = find_sys(st, 3, b);
   const Sys<T>& y = S[s];
   const int o = (int)(b - y.first[3]);
   double am[3], an[3];

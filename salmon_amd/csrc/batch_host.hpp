@@ -18,6 +18,8 @@ struct salmon_batch {
   void* d_sys = nullptr;          // Sys<T>[n]
   std::vector<int> zb;            // chosen z blocking per system
   std::vector<long> first[kPhases + 1];
+  unsigned short* d_owner[kPhases] = {};  // block -> system tables (in bufs)
+  long zcls[4][2] = {};                   // stencil blocks of the z-blocking classes 1, 2, 4, 8: (first, count)
   bool ready = false, tuned = false;
   int* d_step = nullptr;
   double* d_ac = nullptr;
@@ -49,6 +51,7 @@ static void batch_release(salmon_batch* b) {
   for (void* p : b->bufs) (void)hipFree(p);
   b->bufs.clear();
   b->d_sys = nullptr;
+  for (int ph = 0; ph < kPhases; ++ph) b->d_owner[ph] = nullptr;
   b->ready = false;
 }
 
@@ -84,14 +87,40 @@ static int batch_build(salmon_batch* b) {
     b->bufs.push_back(y.rows);
     y.pblk = (std::max(1, c->kb.Nlma) + 3) / 4;
     y.hblk = (c->g.NL / y.zb + 255) / 256;
-    const long nb[kPhases] = {(long)c->NKB * y.pblk, (long)c->NKB * y.hblk, (long)((nv + 255) / 256), (long)c->NKB};
+    const long nb[kPhases] = {(long)c->NKB * y.pblk, 0, (long)((nv + 255) / 256), (long)c->NKB};
     for (int ph = 0; ph < kPhases; ++ph) {
+      if (ph == 1) continue;
       y.first[ph] = acc[ph];
       acc[ph] += nb[ph];
     }
     y.first[kPhases] = 0;
   }
-  for (int ph = 0; ph < kPhases; ++ph) b->first[ph].assign(1, acc[ph]);
+  // stencil blocks class by class, so that one launch per z blocking covers
+  // every system of the class
+  const int zbs[4] = {1, 2, 4, 8};
+  for (int k = 0; k < 4; ++k) {
+    b->zcls[k][0] = acc[1];
+    for (int i = 0; i < n; ++i) {
+      if (S[i].zb != zbs[k]) continue;
+      S[i].first[1] = acc[1];
+      acc[1] += (long)S[i].NKB * S[i].hblk;
+    }
+    b->zcls[k][1] = acc[1] - b->zcls[k][0];
+  }
+  for (int ph = 0; ph < kPhases; ++ph) {
+    b->first[ph].assign(1, acc[ph]);
+    std::vector<unsigned short> own((size_t)acc[ph]);
+    for (int i = 0; i < n; ++i) {
+      const long cnt = ph == 0 ? (long)S[i].NKB * S[i].pblk
+                     : ph == 1 ? (long)S[i].NKB * S[i].hblk
+                     : ph == 2 ? (long)(((size_t)S[i].NKB * S[i].g.NL + 255) / 256) : (long)S[i].NKB;
+      std::fill(own.begin() + S[i].first[ph], own.begin() + S[i].first[ph] + cnt, (unsigned short)i);
+    }
+    if (hipMalloc((void**)&b->d_owner[ph], sizeof(unsigned short) * std::max<size_t>(1, own.size())) != hipSuccess)
+      return bfail(b, SALMON_ERR_NOMEM, "batch block tables");
+    b->bufs.push_back(b->d_owner[ph]);
+    BHIP(b, hipMemcpy(b->d_owner[ph], own.data(), sizeof(unsigned short) * own.size(), hipMemcpyHostToDevice));
+  }
   if (hipMalloc(&b->d_sys, sizeof(Sys<T>) * n) != hipSuccess) return bfail(b, SALMON_ERR_NOMEM, "batch descriptors");
   b->bufs.push_back(b->d_sys);
   BHIP(b, hipMemcpy(b->d_sys, S.data(), sizeof(Sys<T>) * n, hipMemcpyHostToDevice));
@@ -104,10 +133,13 @@ template <typename T>
 static int batch_enqueue_step(salmon_batch* b, double dt) {
   const int n = (int)b->ctx.size();
   const Sys<T>* S = (const Sys<T>*)b->d_sys;
-  const Step st{b->d_ac, b->d_step, dt};
+  const Step st{b->d_ac, b->d_step, dt, {b->d_owner[0], b->d_owner[1], b->d_owner[2], b->d_owner[3]}};
   for (int t = 0; t < 4; ++t) {
     hipLaunchKernelGGL(k_b_project<T>, dim3((unsigned)b->first[0][0]), dim3(256), 0, b->stream, S, n, t, st);
-    hipLaunchKernelGGL(k_b_hpsi<T>, dim3((unsigned)b->first[1][0]), dim3(256), 0, b->stream, S, n, t, st);
+    if (b->zcls[0][1]) hipLaunchKernelGGL((k_b_hpsi<T, 1>), dim3((unsigned)b->zcls[0][1]), dim3(256), 0, b->stream, S, t, st, b->zcls[0][0]);
+    if (b->zcls[1][1]) hipLaunchKernelGGL((k_b_hpsi<T, 2>), dim3((unsigned)b->zcls[1][1]), dim3(256), 0, b->stream, S, t, st, b->zcls[1][0]);
+    if (b->zcls[2][1]) hipLaunchKernelGGL((k_b_hpsi<T, 4>), dim3((unsigned)b->zcls[2][1]), dim3(256), 0, b->stream, S, t, st, b->zcls[2][0]);
+    if (b->zcls[3][1]) hipLaunchKernelGGL((k_b_hpsi<T, 8>), dim3((unsigned)b->zcls[3][1]), dim3(256), 0, b->stream, S, t, st, b->zcls[3][0]);
   }
   hipLaunchKernelGGL(k_b_fold<T>, dim3((unsigned)b->first[2][0]), dim3(256), 0, b->stream, S, n, st);
   hipLaunchKernelGGL(k_b_observe<T>, dim3((unsigned)b->first[3][0]), dim3(256), 0, b->stream, S, n, st);
@@ -130,7 +162,12 @@ static int batch_tune_one(salmon_batch* b, int i, int* best) {
   BHIP(b, hipEventCreate(&e1));
   float bestt = 1e30f;
   *best = 1;
-  const Step st{b->d_ac, b->d_step, 0.0};
+  // the single-system launch below addresses blocks 0.. of a one-system table
+  std::vector<unsigned short> zeros((size_t)one[0].NKB * ((c->g.NL + 255) / 256) + 1, 0);
+  unsigned short* d_zero = nullptr;
+  BHIP(b, hipMalloc((void**)&d_zero, sizeof(unsigned short) * zeros.size()));
+  BHIP(b, hipMemcpy(d_zero, zeros.data(), sizeof(unsigned short) * zeros.size(), hipMemcpyHostToDevice));
+  const Step st{b->d_ac, b->d_step, 0.0, {d_zero, d_zero, d_zero, d_zero}};
   for (int zb : {1, 2, 4, 8}) {
     if (c->g.NLz % zb) continue;
     Sys<T> y = one[0];
@@ -142,7 +179,12 @@ static int batch_tune_one(salmon_batch* b, int i, int* best) {
     float tmin = 1e30f;
     for (int rep = 0; rep < 4; ++rep) {
       BHIP(b, hipEventRecord(e0, b->stream));
-      hipLaunchKernelGGL(k_b_hpsi<T>, dim3(nblk), dim3(256), 0, b->stream, (const Sys<T>*)d1, 1, 0, st);
+      switch (zb) {
+        case 1: hipLaunchKernelGGL((k_b_hpsi<T, 1>), dim3(nblk), dim3(256), 0, b->stream, (const Sys<T>*)d1, 0, st, 0L); break;
+        case 2: hipLaunchKernelGGL((k_b_hpsi<T, 2>), dim3(nblk), dim3(256), 0, b->stream, (const Sys<T>*)d1, 0, st, 0L); break;
+        case 4: hipLaunchKernelGGL((k_b_hpsi<T, 4>), dim3(nblk), dim3(256), 0, b->stream, (const Sys<T>*)d1, 0, st, 0L); break;
+        default: hipLaunchKernelGGL((k_b_hpsi<T, 8>), dim3(nblk), dim3(256), 0, b->stream, (const Sys<T>*)d1, 0, st, 0L); break;
+      }
       BHIP(b, hipEventRecord(e1, b->stream));
       BHIP(b, hipEventSynchronize(e1));
       float ms = 0;
@@ -157,6 +199,7 @@ static int batch_tune_one(salmon_batch* b, int i, int* best) {
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   (void)hipFree(d1);
+  (void)hipFree(d_zero);
   return SALMON_OK;
 }
 
