@@ -17,7 +17,10 @@
 namespace salmon {
 namespace cg {
 
-constexpr int kPPT = 4;              // points per thread per chunk
+#ifndef SALMON_CG_PPT
+#define SALMON_CG_PPT 4
+#endif
+constexpr int kPPT = SALMON_CG_PPT;  // points per thread per chunk
 constexpr int kChunk = 256 * kPPT;   // points per block
 constexpr int kJG = 8;               // lower bands per projection block
 
