@@ -86,6 +86,89 @@ __device__ __forceinline__ T wave_sum(T v) {
   return lane_read(v, 63);
 }
 
+// Wave sums of N values at once (N a power of two, 2 <= N <= 64), all 64
+// lanes active.  Stage s (s = 0..5) pairs the values across lane bit 5 - s:
+// the lanes with that bit 0 keep the pair sum of the even value of each pair,
+// the lanes with it 1 that of the odd value, so each stage halves the number
+// of values and the work per value.  Stages 0 and 1 move data with the
+// permlane32/permlane16 swaps (halves and odd/even 16-lane rows), stages 2..5
+// with DPP row_ror:8, row_half_mirror and quad_perm [2,3,0,1] / [1,0,3,2];
+// once one value is left, the remaining stages are plain pair sums.  The
+// total of value k ends in v[0] of the lanes whose bits 5, 4, ... equal bits
+// 0, 1, ... of k (wave_sums_lane(k)).  The summation order is fixed.
+namespace dpp {
+constexpr int kRowHalfMirror = 0x141, kQuad2301 = 0x4e, kQuad1032 = 0xb1;
+}  // namespace dpp
+
+__device__ __forceinline__ void perm_swap(int stage, float& x, float& y) {
+  const auto r = stage == 0 ? __builtin_amdgcn_permlane32_swap(__float_as_uint(x), __float_as_uint(y), false, false)
+                            : __builtin_amdgcn_permlane16_swap(__float_as_uint(x), __float_as_uint(y), false, false);
+  x = __uint_as_float(r[0]);
+  y = __uint_as_float(r[1]);
+}
+
+__device__ __forceinline__ void perm_swap(int stage, double& x, double& y) {
+  const unsigned long long bx = (unsigned long long)__double_as_longlong(x);
+  const unsigned long long by = (unsigned long long)__double_as_longlong(y);
+  const auto lo = stage == 0 ? __builtin_amdgcn_permlane32_swap((unsigned)bx, (unsigned)by, false, false)
+                             : __builtin_amdgcn_permlane16_swap((unsigned)bx, (unsigned)by, false, false);
+  const auto hi = stage == 0 ? __builtin_amdgcn_permlane32_swap((unsigned)(bx >> 32), (unsigned)(by >> 32), false, false)
+                             : __builtin_amdgcn_permlane16_swap((unsigned)(bx >> 32), (unsigned)(by >> 32), false, false);
+  x = __longlong_as_double((long long)(((unsigned long long)hi[0] << 32) | lo[0]));
+  y = __longlong_as_double((long long)(((unsigned long long)hi[1] << 32) | lo[1]));
+}
+
+template <int CTRL, typename T>
+__device__ __forceinline__ T dpp_pair(T p, T q, bool odd) {
+  const T t = odd ? p : q, u = odd ? q : p;
+  return u + dpp_mov<CTRL, 0xf>(t);
+}
+
+template <int S, typename T>
+__device__ __forceinline__ T dpp_stage(T p, T q, bool odd) {
+  if constexpr (S == 2) return dpp_pair<dpp::kRowRor8>(p, q, odd);
+  if constexpr (S == 3) return dpp_pair<dpp::kRowHalfMirror>(p, q, odd);
+  if constexpr (S == 4) return dpp_pair<dpp::kQuad2301>(p, q, odd);
+  return dpp_pair<dpp::kQuad1032>(p, q, odd);
+}
+
+template <int S, int NV, int N, typename T>
+__device__ __forceinline__ void wave_sums_stage(T (&v)[N], int lane) {
+  if constexpr (S < 6) {
+    if constexpr (NV > 1) {
+#pragma unroll
+      for (int i = 0; i < NV / 2; ++i) {
+        T x = v[2 * i], y = v[2 * i + 1];
+        if constexpr (S < 2) {
+          perm_swap(S, x, y);
+          v[i] = x + y;
+        } else {
+          v[i] = dpp_stage<S>(x, y, (lane >> (5 - S)) & 1);
+        }
+      }
+    } else {
+      T x = v[0], y = v[0];
+      if constexpr (S < 2) {
+        perm_swap(S, x, y);
+        v[0] = x + y;
+      } else {
+        v[0] = dpp_stage<S>(x, x, false);
+      }
+    }
+    wave_sums_stage<S + 1, (NV > 1 ? NV / 2 : 1)>(v, lane);
+  }
+}
+
+template <int N, typename T>
+__device__ __forceinline__ void wave_sums(T (&v)[N]) {
+  static_assert(N >= 2 && N <= 64 && (N & (N - 1)) == 0, "N a power of two");
+  wave_sums_stage<0, N>(v, (int)(threadIdx.x & (kWave - 1)));
+}
+
+__host__ __device__ constexpr int wave_sums_lane(int k) {
+  return ((k & 1) << 5) | ((k & 2) << 3) | ((k & 4) << 1) | ((k & 8) >> 1) | ((k & 16) >> 3) | ((k & 32) >> 5);
+}
+
 // Deterministic sum of NV values over a 256-thread block; the result is
 // valid in thread 0.  smem holds 4*NV doubles.
 template <int NV>

@@ -24,6 +24,7 @@ namespace salmon {
 
 constexpr int kFusedMaxNPI = 1152;      // sphere points per cell (LDS correction array)
 constexpr int kFusedMaxLmaPerAtom = 4;  // projectors per atom
+constexpr int kLA = kFusedMaxLmaPerAtom;
 
 template <typename T>
 struct FusedArgs {
@@ -48,11 +49,12 @@ struct FusedArgs {
   const int* jp_phase;   // (NPI) px | py << 10 | pz << 20 into the phase tables
   const double* jp_pos;  // (NPI*3) unwrapped coordinates
   const int* atom_start; // (NI+1)
-  const int* atom_lma;   // (NI+1) projector range of each atom
-  const double* lma_sign;
-  const T* uv;           // (Nlma*Nps)
-  const double* uv64;    // (Nlma*Nps)
-  int NI, Nps, Nlma, NPI;
+  // projectors padded to kFusedMaxLmaPerAtom per atom (zero rows, zero sign
+  // for an atom with fewer): row (at * kFusedMaxLmaPerAtom + l)
+  const double* sgnp;    // (NI*kFusedMaxLmaPerAtom) iuV
+  const T* uvp;          // (NI*kFusedMaxLmaPerAtom*Nps) uV(j, l) of atom at
+  const double* uvp64;   // the same in FP64
+  int NI, Nps, NPI;
   Grid g;
   double* wg_rows;       // (nwg*8)
   double* rho_part;      // (nwg*NL)
@@ -86,6 +88,52 @@ __device__ __forceinline__ cplx<T> win_at(const cplx<T> (&win)[W], int o) {
     return win[H + o];
   } else {
     return win[(o + NX) % NX];
+  }
+}
+
+// Neighbour sums of the stencil at own point i, added to lr/li and g:
+//   l += sum_a sum_n lapc_a(n) (psi+ + psi-),   g_a += sum_n nab_a(n) (psi+ - psi-)
+// (hpsi1_RT_stencil, stencil/F90/hpsi.f90:190, before the k factors); zb/yb
+// are the LDS bases of the z and y neighbours (+1..+4, -1..-4), x neighbours
+// come from the register window.
+template <typename T, int NX, int NYZ, int H, int XS, int W>
+__device__ __forceinline__ void nb_sums(const FusedArgs<T>& a, const cplx<T>* const (&zb)[8],
+                                        const cplx<T>* const (&yb)[8], const cplx<T> (&win)[W], int i, T& lr,
+                                        T& li, T (&gr)[3], T (&gi)[3]) {
+#pragma unroll
+  for (int nn = 1; nn <= 4; ++nn) {
+    const cplx<T> zp = zb[nn - 1][i * NYZ];
+    const cplx<T> zm = zb[nn + 3][i * NYZ];
+    lr += a.lapc[2][nn] * (zp.re + zm.re);
+    li += a.lapc[2][nn] * (zp.im + zm.im);
+    gr[2] += a.nab[2][nn] * (zp.re - zm.re);
+    gi[2] += a.nab[2][nn] * (zp.im - zm.im);
+    const cplx<T> yp = yb[nn - 1][i * NYZ];
+    const cplx<T> ym = yb[nn + 3][i * NYZ];
+    lr += a.lapc[1][nn] * (yp.re + ym.re);
+    li += a.lapc[1][nn] * (yp.im + ym.im);
+    gr[1] += a.nab[1][nn] * (yp.re - ym.re);
+    gi[1] += a.nab[1][nn] * (yp.im - ym.im);
+    const cplx<T> xp = win_at<T, NX, H, XS, W>(win, i + nn);
+    const cplx<T> xm = win_at<T, NX, H, XS, W>(win, i - nn);
+    lr += a.lapc[0][nn] * (xp.re + xm.re);
+    li += a.lapc[0][nn] * (xp.im + xm.im);
+    gr[0] += a.nab[0][nn] * (xp.re - xm.re);
+    gi[0] += a.nab[0][nn] * (xp.im - xm.im);
+  }
+}
+
+// LDS bases of the z and y neighbours of own point 0 (x0, iy, iz) in buf
+template <typename T, int NY, int NZ>
+__device__ __forceinline__ void nb_bases(const cplx<T>* buf, int x0, int iy, int iz, const cplx<T>* (&zb)[8],
+                                         const cplx<T>* (&yb)[8]) {
+#pragma unroll
+  for (int nn = 1; nn <= 4; ++nn) {
+    const cplx<T>* base = buf + x0 * NY * NZ;
+    zb[nn - 1] = base + iy * NZ + (iz + nn) % NZ;
+    zb[nn + 3] = base + iy * NZ + (iz - nn + NZ) % NZ;
+    yb[nn - 1] = base + ((iy + nn) % NY) * NZ + iz;
+    yb[nn + 3] = base + ((iy - nn + NY) % NY) * NZ + iz;
   }
 }
 
@@ -191,46 +239,44 @@ __global__ void __launch_bounds__(NY * NZ * XS, WPE) k_fused_step(FusedArgs<T> a
       }
       const int* jp_phase = opaque(a.jp_phase);
       const int* jp_grid = opaque(a.jp_grid);
-      const T* uvt = opaque(a.uv);
+      const T* uvt = opaque(a.uvp);
       // KB term of pseudo_pt (hpsi.f90:176-206) per atom in one wave:
       //   uVpsi(l) = Hxyz iuV(l) sum_j uV(j,l) e^{ik.r_j} psi(J_j)       (projection)
       //   dps(j)   = e^{-ik.r_j} sum_l uV(j,l) uVpsi(l)                  (back-projection)
-      // the wave sum leaves uVpsi in every lane, so no barrier separates the two
+      // the wave sums leave uVpsi in every lane, so no barrier separates the two
       for (int at = wv; at < a.NI; at += NW) {
         const int j0 = a.atom_start[at], m = a.atom_start[at + 1] - j0;
-        const int l0 = a.atom_lma[at], nl = a.atom_lma[at + 1] - l0;
-        T sr[kFusedMaxLmaPerAtom], si[kFusedMaxLmaPerAtom];
+        const T* ua = uvt + (size_t)at * kLA * a.Nps;
+        T q[2 * kLA];
 #pragma unroll
-        for (int l = 0; l < kFusedMaxLmaPerAtom; ++l) { sr[l] = 0; si[l] = 0; }
+        for (int l = 0; l < 2 * kLA; ++l) q[l] = 0;
         for (int j = lane; j < m; j += kWave) {
           const int jp = j0 + j;
           const cplx<T> e = phase_at<T, NX, NY, NZ>(phm, jp_phase[jp]);
           const cplx<T> w = cmul(e, in[jp_grid[jp]]);
 #pragma unroll
-          for (int l = 0; l < kFusedMaxLmaPerAtom; ++l) {
-            if (l < nl) {
-              const T u = uvt[(size_t)(l0 + l) * a.Nps + j];
-              sr[l] += u * w.re;
-              si[l] += u * w.im;
-            }
+          for (int l = 0; l < kLA; ++l) {
+            const T u = ua[l * a.Nps + j];
+            q[2 * l] += u * w.re;
+            q[2 * l + 1] += u * w.im;
           }
         }
+        wave_sums(q);
+        T sr[kLA], si[kLA];
 #pragma unroll
-        for (int l = 0; l < kFusedMaxLmaPerAtom; ++l) {
-          const T f = l < nl ? (T)(g.Hxyz * a.lma_sign[l0 + l]) : (T)0;
-          sr[l] = wave_sum(sr[l]) * f;
-          si[l] = wave_sum(si[l]) * f;
+        for (int l = 0; l < kLA; ++l) {
+          const T f = (T)(g.Hxyz * a.sgnp[at * kLA + l]);
+          sr[l] = lane_read(q[0], wave_sums_lane(2 * l)) * f;
+          si[l] = lane_read(q[0], wave_sums_lane(2 * l + 1)) * f;
         }
         for (int j = lane; j < m; j += kWave) {
           const int jp = j0 + j;
           T tr = 0, ti = 0;
 #pragma unroll
-          for (int l = 0; l < kFusedMaxLmaPerAtom; ++l) {
-            if (l < nl) {
-              const T u = uvt[(size_t)(l0 + l) * a.Nps + j];
-              tr += u * sr[l];
-              ti += u * si[l];
-            }
+          for (int l = 0; l < kLA; ++l) {
+            const T u = ua[l * a.Nps + j];
+            tr += u * sr[l];
+            ti += u * si[l];
           }
           const cplx<T> e = phase_at<T, NX, NY, NZ>(phm, jp_phase[jp]);
           dps[jp] = cplx<T>{e.re * tr + e.im * ti, e.re * ti - e.im * tr};  // conj(e) * t
@@ -257,14 +303,7 @@ __global__ void __launch_bounds__(NY * NZ * XS, WPE) k_fused_step(FusedArgs<T> a
       // point is a compile-time constant that folds into the ds_read offset.
       const cplx<T>* zb[8];
       const cplx<T>* yb[8];
-#pragma unroll
-      for (int nn = 1; nn <= 4; ++nn) {
-        const cplx<T>* base = in + x0 * NYZ;
-        zb[nn - 1] = base + iy * NZ + (iz + nn) % NZ;
-        zb[nn + 3] = base + iy * NZ + (iz - nn + NZ) % NZ;
-        yb[nn - 1] = base + ((iy + nn) % NY) * NZ + iz;
-        yb[nn + 3] = base + ((iy - nn + NY) % NY) * NZ + iz;
-      }
+      nb_bases<T, NY, NZ>(in, x0, iy, iz, zb, yb);
       const T* vrow = opaque(a.vloc) + x0 * NYZ + yz;
       const T zr = (n == 1) ? a.z2r : ((n == 3) ? a.z4r : (T)0);
       const T zi = (n == 0) ? a.z1i : ((n == 2) ? a.z3i : (T)0);
@@ -278,34 +317,14 @@ __global__ void __launch_bounds__(NY * NZ * XS, WPE) k_fused_step(FusedArgs<T> a
         const cplx<T> e0 = win[H + i];
         const T vAi = vrow[i * NYZ] + A;
         T lr = vAi * e0.re, li = vAi * e0.im;
-        T gzr = 0, gzi = 0, gyr = 0, gyi = 0, gxr = 0, gxi = 0;
-#pragma unroll
-        for (int nn = 1; nn <= 4; ++nn) {
-          const cplx<T> zp = zb[nn - 1][i * NYZ];
-          const cplx<T> zm = zb[nn + 3][i * NYZ];
-          lr += a.lapc[2][nn] * (zp.re + zm.re);
-          li += a.lapc[2][nn] * (zp.im + zm.im);
-          gzr += a.nab[2][nn] * (zp.re - zm.re);
-          gzi += a.nab[2][nn] * (zp.im - zm.im);
-          const cplx<T> yp = yb[nn - 1][i * NYZ];
-          const cplx<T> ym = yb[nn + 3][i * NYZ];
-          lr += a.lapc[1][nn] * (yp.re + ym.re);
-          li += a.lapc[1][nn] * (yp.im + ym.im);
-          gyr += a.nab[1][nn] * (yp.re - ym.re);
-          gyi += a.nab[1][nn] * (yp.im - ym.im);
-          const cplx<T> xp = win_at<T, NX, H, XS, W>(win, i + nn);
-          const cplx<T> xm = win_at<T, NX, H, XS, W>(win, i - nn);
-          lr += a.lapc[0][nn] * (xp.re + xm.re);
-          li += a.lapc[0][nn] * (xp.im + xm.im);
-          gxr += a.nab[0][nn] * (xp.re - xm.re);
-          gxi += a.nab[0][nn] * (xp.im - xm.im);
-        }
+        T gr[3] = {0, 0, 0}, gi[3] = {0, 0, 0};
+        nb_sums<T, NX, NYZ, H, XS, W>(a, zb, yb, win, i, lr, li, gr, gi);
         // KB term of the sphere point covering this point (branch-free:
         // uncovered points read the zero entry dps[NPI])
         const cplx<T> d = dps[(prim[i >> 1] >> (16 * (i & 1))) & 0xffff];
         // -i * (k.G)
-        h[i].re = lr + (kmx * gxi + kmy * gyi + kmz * gzi) + d.re;
-        h[i].im = li - (kmx * gxr + kmy * gyr + kmz * gzr) + d.im;
+        h[i].re = lr + (kmx * gi[0] + kmy * gi[1] + kmz * gi[2]) + d.re;
+        h[i].im = li - (kmx * gr[0] + kmy * gr[1] + kmz * gr[2]) + d.im;
         out[(x0 + i) * NYZ + yz] = h[i];
         acc[i].re += zr * h[i].re - zi * h[i].im;
         acc[i].im += zr * h[i].im + zi * h[i].re;
@@ -328,53 +347,52 @@ __global__ void __launch_bounds__(NY * NZ * XS, WPE) k_fused_step(FusedArgs<T> a
       in = out;
       out = tmp;
     }
-    // psi(t+dt): to HBM and to the free LDS buffer
+    // psi(t+dt): to HBM, to the free LDS buffer and into the x window
 #pragma unroll
     for (int i = 0; i < PX; ++i) {
       const int p = (x0 + i) * NYZ + yz;
       zo[p] = acc[i];
       out[p] = acc[i];
+      win[H + i] = acc[i];
     }
     __syncthreads();
-    // observables at k + A(t+dt), FP64 sums; all neighbours from LDS (a rolled
-    // loop keeps this sweep's registers low)
-    // Forward correlations C(a,n) = conj(psi_i) psi_{i + n e_a} (FP64), summed
-    // over the grid, give the kinetic energy (total_energy_stencil), the
-    // -i k.grad energy term and the stencil current (current_stencil, forward
-    // neighbours):  sum_i Re conj(psi)(-1/2 L psi) = A0 |psi|^2 - sum lap_n Re C,
-    //               sum_i Re conj(psi)(-i w)       = 2 sum_a k_a sum_n nab_n Im C,
-    //               current stencil term (axis a)  = sum_n nab_n Im C(a,n);
-    // each point adds its correlations to those sums directly.
-    double ck[3][4];
+    if constexpr (XS > 1) {
 #pragma unroll
-    for (int d = 0; d < 3; ++d)
-#pragma unroll
-      for (int nn = 0; nn < 4; ++nn) ck[d][nn] = 2.0 * kn[d] * g.nab[d][nn + 1];
+      for (int i = 0; i < H; ++i) {
+        win[i] = out[((x0 - H + i + NX) % NX) * NYZ + yz];
+        win[H + PX + i] = out[((x0 + PX + i) % NX) * NYZ + yz];
+      }
+    }
+    // observables at k + A(t+dt), FP64 sums.  The stencil's neighbour sums on
+    // psi(t+dt), L = sum_a sum_n (-lap_n/2)(psi+ + psi-) and
+    // G_a = sum_n nab_n (psi+ - psi-), give per point the kinetic term
+    // Re conj(psi) L of total_energy_stencil (the centre term A0 |psi|^2 is
+    // added per orbital) and the stencil current Im conj(psi) G_a; the
+    // gradient energy sum_i Re conj(psi)(-i k.G) is sum_a k_a * current_a per
+    // orbital.  Over the periodic grid, sum_i Im conj(psi_i)(psi_{i+n} -
+    // psi_{i-n}) = 2 sum_i Im conj(psi_i) psi_{i+n}: the central sums are
+    // twice the forward sums of current_stencil (stencil/F90/current.f90:112-114).
+    // Products and sums are FP64; L and G are in the propagation precision.
     double n2 = 0, vs = 0, te = 0, js[3] = {0, 0, 0};
-#pragma unroll 1
-    for (int i = 0; i < PX; ++i) {
-      const int x = x0 + i;
-      const int p = x * NYZ + yz;
-      const cplx<T> e0 = out[p];
-      const double er = (double)e0.re, ei = (double)e0.im;
-      const double a2 = er * er + ei * ei;
-      n2 += a2;
-      vs += a.vloc64[p] * a2;
-      rho[p] += occ * a2;
-      const cplx<T>* rowp = out + x * NYZ;
+    {
+      const cplx<T>* zb[8];
+      const cplx<T>* yb[8];
+      nb_bases<T, NY, NZ>(out, x0, iy, iz, zb, yb);
+      const double* vrow = opaque(a.vloc64) + x0 * NYZ + yz;
+      double* rrow = rho + x0 * NYZ + yz;
 #pragma unroll
-      for (int nn = 1; nn <= 4; ++nn) {
-        const cplx<T> zp = rowp[iy * NZ + (iz + nn) % NZ];
-        const cplx<T> yp = rowp[((iy + nn) % NY) * NZ + iz];
-        const cplx<T> xp = out[((x + nn) % NX) * NYZ + yz];
-        const double q[3][2] = {{xp.re, xp.im}, {yp.re, yp.im}, {zp.re, zp.im}};
+      for (int i = 0; i < PX; ++i) {
+        T lr = 0, li = 0;
+        T gr[3] = {0, 0, 0}, gi[3] = {0, 0, 0};
+        nb_sums<T, NX, NYZ, H, XS, W>(a, zb, yb, win, i, lr, li, gr, gi);
+        const double er = (double)win[H + i].re, ei = (double)win[H + i].im;
+        const double a2 = er * er + ei * ei;
+        n2 += a2;
+        vs += vrow[i * NYZ] * a2;
+        rrow[i * NYZ] += occ * a2;
+        te += er * (double)lr + ei * (double)li;
 #pragma unroll
-        for (int d = 0; d < 3; ++d) {
-          const double Cr = er * q[d][0] + ei * q[d][1];
-          const double Ci = er * q[d][1] - ei * q[d][0];
-          te += ck[d][nn - 1] * Ci - g.lap[d][nn] * Cr;
-          js[d] += g.nab[d][nn] * Ci;
-        }
+        for (int d = 0; d < 3; ++d) js[d] += er * (double)gi[d] - ei * (double)gr[d];
       }
     }
     {
@@ -384,8 +402,9 @@ __global__ void __launch_bounds__(NY * NZ * XS, WPE) k_fused_step(FusedArgs<T> a
       if (lane == 0) {
         red[wv * 10 + 0] += occ * n2s;
         red[wv * 10 + 1] += occ * vss;
-        red[wv * 10 + 2] += occ * (A0 * n2s + tes);
-        // reference: jx = jx + jxt*occ with jxt = 2*sum Im(...) (current.f90:206-210)
+        red[wv * 10 + 2] += occ * (A0 * n2s + tes + kn[0] * j0 + kn[1] * j1 + kn[2] * j2);
+        // reference: jx = jx + jxt*occ with jxt = 2*sum Im(...) over forward
+        // neighbours (current.f90:206-210) = the central sum
         red[wv * 10 + 3] += occ * j0;
         red[wv * 10 + 4] += occ * j1;
         red[wv * 10 + 5] += occ * j2;
@@ -393,38 +412,50 @@ __global__ void __launch_bounds__(NY * NZ * XS, WPE) k_fused_step(FusedArgs<T> a
     }
     const int* jp_phase_n = opaque(a.jp_phase);
     const int* jp_grid_n = opaque(a.jp_grid);
-    const double* uv64 = opaque(a.uv64);
+    const double* uv64 = opaque(a.uvp64);
     const double* jp_pos = opaque(a.jp_pos);
     // nonlocal current and energy (current.f90:247-269, total_energy.f90:191-199):
-    // waves over (atom, projector)
+    // one wave per atom; per projector l the projection U = sum_j uV e psi and
+    // its first moments X_d = sum_j uV r_d e psi, all in FP64
     for (int at = wv; at < a.NI; at += NW) {
       const int j0 = a.atom_start[at], m = a.atom_start[at + 1] - j0;
-      for (int l = a.atom_lma[at]; l < a.atom_lma[at + 1]; ++l) {
-        double ur = 0, ui = 0, xr0 = 0, xi0 = 0, xr1 = 0, xi1 = 0, xr2 = 0, xi2 = 0;
-        for (int j = lane; j < m; j += kWave) {
-          const int jp = j0 + j;
-          const cplx<T> e = phase_at<T, NX, NY, NZ>(phn, jp_phase_n[jp]);
-          const cplx<T> w = cmul(e, out[jp_grid_n[jp]]);
-          const double u = uv64[(size_t)l * a.Nps + j];
+      const double* ua = uv64 + (size_t)at * kLA * a.Nps;
+      double q[8 * kLA];  // per l: Ur, Ui, Xr_x, Xi_x, Xr_y, Xi_y, Xr_z, Xi_z
+#pragma unroll
+      for (int l = 0; l < 8 * kLA; ++l) q[l] = 0;
+      for (int j = lane; j < m; j += kWave) {
+        const int jp = j0 + j;
+        const cplx<T> e = phase_at<T, NX, NY, NZ>(phn, jp_phase_n[jp]);
+        const cplx<T> w = cmul(e, out[jp_grid_n[jp]]);
+        const double px = jp_pos[3 * jp], py = jp_pos[3 * jp + 1], pz = jp_pos[3 * jp + 2];
+#pragma unroll
+        for (int l = 0; l < kLA; ++l) {
+          const double u = ua[l * a.Nps + j];
           const double tr = u * (double)w.re, ti = u * (double)w.im;
-          const double px = jp_pos[3 * jp], py = jp_pos[3 * jp + 1], pz = jp_pos[3 * jp + 2];
-          ur += tr; ui += ti;
-          xr0 += px * tr; xi0 += px * ti;
-          xr1 += py * tr; xi1 += py * ti;
-          xr2 += pz * tr; xi2 += pz * ti;
+          q[8 * l + 0] += tr;
+          q[8 * l + 1] += ti;
+          q[8 * l + 2] += px * tr;
+          q[8 * l + 3] += px * ti;
+          q[8 * l + 4] += py * tr;
+          q[8 * l + 5] += py * ti;
+          q[8 * l + 6] += pz * tr;
+          q[8 * l + 7] += pz * ti;
         }
-        ur = wave_sum(ur); ui = wave_sum(ui);
-        xr0 = wave_sum(xr0); xi0 = wave_sum(xi0);
-        xr1 = wave_sum(xr1); xi1 = wave_sum(xi1);
-        xr2 = wave_sum(xr2); xi2 = wave_sum(xi2);
+      }
+      wave_sums(q);
+      const double hx = g.Hxyz, f = occ / g.aLxyz * 2.0 * hx;
+#pragma unroll
+      for (int l = 0; l < kLA; ++l) {
+        double r[8];
+#pragma unroll
+        for (int c = 0; c < 8; ++c) r[c] = lane_read(q[0], wave_sums_lane(8 * l + c));
         if (lane == 0) {
-          const double sg = a.lma_sign[l], hx = g.Hxyz;
-          const double Ur = ur * hx * sg, Ui = ui * hx * sg;
-          const double f = occ / g.aLxyz * 2.0 * hx;
-          red[wv * 10 + 6] += f * (xr0 * Ui - xi0 * Ur);
-          red[wv * 10 + 7] += f * (xr1 * Ui - xi1 * Ur);
-          red[wv * 10 + 8] += f * (xr2 * Ui - xi2 * Ur);
-          red[wv * 10 + 9] += occ * (ur * ur + ui * ui) * hx * hx * sg;
+          const double sg = a.sgnp[at * kLA + l];
+          const double Ur = r[0] * hx * sg, Ui = r[1] * hx * sg;
+          red[wv * 10 + 6] += f * (r[2] * Ui - r[3] * Ur);
+          red[wv * 10 + 7] += f * (r[4] * Ui - r[5] * Ur);
+          red[wv * 10 + 8] += f * (r[6] * Ui - r[7] * Ur);
+          red[wv * 10 + 9] += occ * (r[0] * r[0] + r[1] * r[1]) * hx * hx * sg;
         }
       }
     }
@@ -445,7 +476,7 @@ __global__ void __launch_bounds__(NY * NZ * XS, WPE) k_fused_step(FusedArgs<T> a
     double* r = a.wg_rows + (size_t)blockIdx.x * 8;
     const double f = g.Hxyz / g.aLxyz;
 #pragma unroll
-    for (int d = 0; d < 3; ++d) r[d] = (kn[d] * s[0] + 2.0 * s[3 + d]) * f + s[6 + d];
+    for (int d = 0; d < 3; ++d) r[d] = (kn[d] * s[0] + s[3 + d]) * f + s[6 + d];
     r[3] = s[2] * g.Hxyz + occsum * (kn[0] * kn[0] + kn[1] * kn[1] + kn[2] * kn[2]) / 2.0;
     r[4] = s[9];
     r[5] = s[1] * g.Hxyz;
@@ -484,10 +515,11 @@ struct FusedState {
   int shape = 0;  // 1: 16^3
   std::vector<void*> bufs;
   const int *pt_prim = nullptr, *ov_start = nullptr, *ov_jp = nullptr, *jp_atom = nullptr, *jp_j = nullptr,
-            *jp_grid = nullptr, *jp_phase = nullptr, *atom_lma = nullptr;
+            *jp_grid = nullptr, *jp_phase = nullptr;
   int NOV = 0;
   const double* jp_pos = nullptr;
-  float* uv32 = nullptr;
+  const double *sgnp = nullptr, *uvp64 = nullptr;
+  const float* uvp32 = nullptr;
   double* wg_rows = nullptr;
   double* rho_part = nullptr;
   int nwg_alloc = 0;
@@ -510,7 +542,8 @@ static void fused_free_tables(FusedState* f) {
   for (void* p : f->bufs) (void)hipFree(p);
   f->bufs.clear();
   f->tables = false;
-  f->uv32 = nullptr;
+  f->sgnp = f->uvp64 = nullptr;
+  f->uvp32 = nullptr;
 }
 
 static void fused_free(salmon_ctx* c) {
@@ -538,18 +571,24 @@ static int fused_set_tables(salmon_ctx* c) {
   const salmon::Grid& g = c->g;
   if (kb.NPI + 1 > salmon::kFusedMaxNPI) return SALMON_OK;  // the streamed path serves this case
   std::vector<int> start(kb.NI + 1), pidx(kb.NPI), lma_atom(kb.Nlma);
-  std::vector<double> pos(3 * (size_t)kb.NPI), uv((size_t)kb.Nlma * kb.Nps);
+  std::vector<double> pos(3 * (size_t)kb.NPI), uv((size_t)kb.Nlma * kb.Nps), sign(kb.Nlma);
   if (hipMemcpy(start.data(), kb.atom_start, sizeof(int) * start.size(), hipMemcpyDeviceToHost) != hipSuccess ||
       (kb.NPI && hipMemcpy(pidx.data(), kb.pidx, sizeof(int) * pidx.size(), hipMemcpyDeviceToHost) != hipSuccess) ||
       (kb.NPI && hipMemcpy(pos.data(), kb.pos, sizeof(double) * pos.size(), hipMemcpyDeviceToHost) != hipSuccess) ||
       (kb.Nlma && hipMemcpy(lma_atom.data(), kb.lma_atom, sizeof(int) * lma_atom.size(), hipMemcpyDeviceToHost) != hipSuccess) ||
-      (kb.Nlma && hipMemcpy(uv.data(), kb.uv, sizeof(double) * uv.size(), hipMemcpyDeviceToHost) != hipSuccess))
+      (kb.Nlma && hipMemcpy(uv.data(), kb.uv, sizeof(double) * uv.size(), hipMemcpyDeviceToHost) != hipSuccess) ||
+      (kb.Nlma && hipMemcpy(sign.data(), kb.lma_sign, sizeof(double) * sign.size(), hipMemcpyDeviceToHost) != hipSuccess))
     return fail(c, SALMON_ERR_HIP, "fused tables: device read-back failed");
-  std::vector<int> atom_lma(kb.NI + 1, 0);
-  for (int l = 0; l < kb.Nlma; ++l) atom_lma[lma_atom[l] + 1]++;
-  for (int at = 0; at < kb.NI; ++at) {
-    if (atom_lma[at + 1] > salmon::kFusedMaxLmaPerAtom) return SALMON_OK;  // streamed path
-    atom_lma[at + 1] += atom_lma[at];
+  // projectors padded to kLA rows per atom, in their order within the atom
+  constexpr int kLA = salmon::kLA;
+  std::vector<int> nla(kb.NI, 0);
+  std::vector<double> sgnp((size_t)kb.NI * kLA, 0.0), uvp64((size_t)kb.NI * kLA * kb.Nps, 0.0);
+  for (int l = 0; l < kb.Nlma; ++l) {
+    const int at = lma_atom[l];
+    if (nla[at] == kLA) return SALMON_OK;  // more projectors per atom: the streamed path serves this case
+    const size_t row = (size_t)at * kLA + nla[at]++;
+    sgnp[row] = sign[l];
+    for (int j = 0; j < kb.Nps; ++j) uvp64[row * kb.Nps + j] = uv[(size_t)l * kb.Nps + j];
   }
   std::vector<int> jp_atom(kb.NPI), jp_j(kb.NPI), jp_phase(kb.NPI);
   const int N[3] = {g.NLx, g.NLy, g.NLz};
@@ -582,8 +621,8 @@ static int fused_set_tables(salmon_ctx* c) {
     }
   }
   f->NOV = (int)ov_start.size() - 1;
-  std::vector<float> uv32(uv.size());
-  for (size_t i = 0; i < uv.size(); ++i) uv32[i] = (float)uv[i];
+  std::vector<float> uvp32(uvp64.size());
+  for (size_t i = 0; i < uvp64.size(); ++i) uvp32[i] = (float)uvp64[i];
   auto up = [&](const void* src, size_t bytes, void** dst) -> int {
     void* d = nullptr;
     if (hipMalloc(&d, bytes ? bytes : 4) != hipSuccess) return fail(c, SALMON_ERR_NOMEM, "fused tables");
@@ -601,8 +640,9 @@ static int fused_set_tables(salmon_ctx* c) {
       (r = up(jp_j.data(), sizeof(int) * jp_j.size(), (void**)&f->jp_j)) ||
       (r = up(pidx.data(), sizeof(int) * pidx.size(), (void**)&f->jp_grid)) ||
       (r = up(jp_phase.data(), sizeof(int) * jp_phase.size(), (void**)&f->jp_phase)) ||
-      (r = up(atom_lma.data(), sizeof(int) * atom_lma.size(), (void**)&f->atom_lma)) ||
-      (r = up(uv32.data(), sizeof(float) * uv32.size(), (void**)&f->uv32)))
+      (r = up(sgnp.data(), sizeof(double) * sgnp.size(), (void**)&f->sgnp)) ||
+      (r = up(uvp64.data(), sizeof(double) * uvp64.size(), (void**)&f->uvp64)) ||
+      (r = up(uvp32.data(), sizeof(float) * uvp32.size(), (void**)&f->uvp32)))
     return r;
   f->jp_pos = kb.pos;
   f->tables = true;
@@ -676,11 +716,12 @@ static int fused_step(salmon_ctx* c, const double amid[3], const double anew[3],
       a.nab[d][n] = (T)c->g.nab[d][n];
     }
   a.pt_prim = f->pt_prim; a.ov_start = f->ov_start; a.ov_jp = f->ov_jp; a.NOV = f->NOV; a.jp_atom = f->jp_atom; a.jp_j = f->jp_j; a.jp_grid = f->jp_grid;
-  a.jp_phase = f->jp_phase; a.jp_pos = f->jp_pos; a.atom_start = c->kb.atom_start; a.atom_lma = f->atom_lma;
-  a.lma_sign = c->kb.lma_sign;
-  a.uv = sizeof(T) == 8 ? (const T*)c->kb.uv : (const T*)f->uv32;
-  a.uv64 = c->kb.uv;
-  a.NI = c->kb.NI; a.Nps = c->kb.Nps; a.Nlma = c->kb.Nlma; a.NPI = c->kb.NPI;
+  a.jp_phase = f->jp_phase; a.jp_pos = f->jp_pos; a.atom_start = c->kb.atom_start;
+  a.sgnp = f->sgnp;
+  if constexpr (sizeof(T) == 8) a.uvp = (const T*)f->uvp64;
+  else a.uvp = (const T*)f->uvp32;
+  a.uvp64 = f->uvp64;
+  a.NI = c->kb.NI; a.Nps = c->kb.Nps; a.NPI = c->kb.NPI;
   a.g = c->g;
   a.wg_rows = f->wg_rows;
   a.rho_part = f->rho_part;
