@@ -34,6 +34,8 @@ struct FusedArgs {
   const T* vloc;
   const double* vloc64;
   int NK, NB, bpw, nbg;
+  int prop;              // 1: Taylor step at k0 + am
+  int obs;               // 1: observables of the input orbitals at k0 + an
   double am[3], an[3];
   T z1i, z2r, z3i, z4r;
   T lapc[3][5];          // -lap(n)/2 per axis (propagation) -- uniform, kept in SGPRs
@@ -74,6 +76,11 @@ __device__ __forceinline__ cplx<T> phase_at(const cplx<T>* ph, int code) {
 // Returns p unchanged but opaque to the optimiser, so loads through it are
 // not hoisted out of the orbital and H loops (hoisted table values would stay
 // live in registers for the whole kernel).
+template <bool B>
+struct BoolC {
+  static constexpr bool value = B;
+};
+
 template <typename P>
 __device__ __forceinline__ P* opaque(P* p) {
   asm volatile("" : "+s"(p));
@@ -123,6 +130,37 @@ __device__ __forceinline__ void nb_sums(const FusedArgs<T>& a, const cplx<T>* co
   }
 }
 
+// The same sums with the gradient term applied at once, as the reference's
+// stencil does with D = kAc nab (common/hpsi.f90:122-124, stencil/F90/hpsi.f90:190):
+//   h.re += sum lapc (psi+ + psi-).re + D (psi+ - psi-).im
+//   h.im += sum lapc (psi+ + psi-).im - D (psi+ - psi-).re
+template <typename T, int NX, int NYZ, int H, int XS, int W>
+__device__ __forceinline__ void nb_sums_k(const FusedArgs<T>& a, const T (&D)[3][4], const cplx<T>* const (&zb)[8],
+                                          const cplx<T>* const (&yb)[8], const cplx<T> (&win)[W], int i, T& lr,
+                                          T& li) {
+#pragma unroll
+  for (int nn = 1; nn <= 4; ++nn) {
+    const cplx<T> zp = zb[nn - 1][i * NYZ];
+    const cplx<T> zm = zb[nn + 3][i * NYZ];
+    lr += a.lapc[2][nn] * (zp.re + zm.re);
+    li += a.lapc[2][nn] * (zp.im + zm.im);
+    lr += D[2][nn - 1] * (zp.im - zm.im);
+    li -= D[2][nn - 1] * (zp.re - zm.re);
+    const cplx<T> yp = yb[nn - 1][i * NYZ];
+    const cplx<T> ym = yb[nn + 3][i * NYZ];
+    lr += a.lapc[1][nn] * (yp.re + ym.re);
+    li += a.lapc[1][nn] * (yp.im + ym.im);
+    lr += D[1][nn - 1] * (yp.im - ym.im);
+    li -= D[1][nn - 1] * (yp.re - ym.re);
+    const cplx<T> xp = win_at<T, NX, H, XS, W>(win, i + nn);
+    const cplx<T> xm = win_at<T, NX, H, XS, W>(win, i - nn);
+    lr += a.lapc[0][nn] * (xp.re + xm.re);
+    li += a.lapc[0][nn] * (xp.im + xm.im);
+    lr += D[0][nn - 1] * (xp.im - xm.im);
+    li -= D[0][nn - 1] * (xp.re - xm.re);
+  }
+}
+
 // LDS bases of the z and y neighbours of own point 0 (x0, iy, iz) in buf
 template <typename T, int NY, int NZ>
 __device__ __forceinline__ void nb_bases(const cplx<T>* buf, int x0, int iy, int iz, const cplx<T>* (&zb)[8],
@@ -135,6 +173,26 @@ __device__ __forceinline__ void nb_bases(const cplx<T>* buf, int x0, int iy, int
     yb[nn - 1] = base + ((iy + nn) % NY) * NZ + iz;
     yb[nn + 3] = base + ((iy - nn + NY) % NY) * NZ + iz;
   }
+}
+
+// The complex64 kernel with one workgroup per CU (XS >= 2) has the LDS to
+// hold per-workgroup sphere-point tables (phases at both k, grid index,
+// projector values) and the density accumulator; the others read the tables
+// from global memory and accumulate the density there.
+template <typename T, int XS>
+__host__ __device__ constexpr bool fused_lds_tables() {
+  return sizeof(T) == 4 && XS >= 2;
+}
+
+// The complex64 kernel evaluates the observables of its input orbitals inside
+// its first H application; the FP64 kernel, short of registers for that,
+// evaluates those of psi(t+dt) after the step.
+#ifndef SALMON_FUSED_DEFER
+#define SALMON_FUSED_DEFER 1
+#endif
+template <typename T, int XS>
+__host__ __device__ constexpr bool fused_defer() {
+  return SALMON_FUSED_DEFER && sizeof(T) == 4;
 }
 
 template <typename T, int NX, int NY, int NZ, int XS, int WPE, bool OV>
@@ -154,7 +212,17 @@ __global__ void __launch_bounds__(NY * NZ * XS, WPE) k_fused_step(FusedArgs<T> a
   cplx<T>* phm = bufB + NL;      // phase tables at the midpoint k
   cplx<T>* phn = phm + NPH;      // phase tables at k + A(t+dt)
   cplx<T>* dps = phn + NPH;      // KB term at each sphere point of the current H
-  double* red = reinterpret_cast<double*>(dps + kFusedMaxNPI);
+  constexpr bool LT = fused_lds_tables<T, XS>();
+  // observables: of the input orbitals during the first H application
+  // (DEFER), or of psi(t+dt) after the step
+  constexpr bool DEFER = fused_defer<T, XS>();
+  constexpr int MT = LT ? kFusedMaxNPI : 0;
+  cplx<T>* em = dps + kFusedMaxNPI;  // e^{i km.(r_j - T_j)} per sphere point
+  cplx<T>* en = em + MT;             // e^{i kn.(r_j - T_j)}
+  T* ug = reinterpret_cast<T*>(en + MT);                    // uV(j, l) at [l * kFusedMaxNPI + jp]
+  int* gix = reinterpret_cast<int*>(ug + kLA * MT);         // grid index of each sphere point
+  double* rhol = reinterpret_cast<double*>(gix + MT);       // density of the workgroup's bands
+  double* red = rhol + (LT ? NL : 0);
 
   const int t = threadIdx.x;
   const int lane = t & (kWave - 1), wv = t / kWave;
@@ -187,8 +255,13 @@ __global__ void __launch_bounds__(NY * NZ * XS, WPE) k_fused_step(FusedArgs<T> a
     sincos(kn[ax] * pos, &s, &c);
     phn[q] = cplx<T>{(T)c, (T)s};
   }
-  // the k dependence of the gradient term is applied after the per-axis sums
+  // gradient coefficients at the midpoint k: D(a, n) = k_a nab_a(n)
   const T kmx = (T)km[0], kmy = (T)km[1], kmz = (T)km[2];
+  T D[3][4];
+#pragma unroll
+  for (int d = 0; d < 3; ++d)
+#pragma unroll
+    for (int nn = 0; nn < 4; ++nn) D[d][nn] = (T)(km[d] * g.nab[d][nn + 1]);
   const T A = (T)((km[0] * km[0] + km[1] * km[1] + km[2] * km[2] - (g.lap[0][0] + g.lap[1][0] + g.lap[2][0])) * 0.5);
   // the (first) sphere point covering each own grid point, or the zero entry
   // NPI; two 16-bit indices per register
@@ -208,9 +281,22 @@ __global__ void __launch_bounds__(NY * NZ * XS, WPE) k_fused_step(FusedArgs<T> a
   // density partial of this workgroup: each thread owns its points, so the
   // slice is updated without atomics
   double* rho = a.rho_part + (size_t)blockIdx.x * NL;
+  double* rho_acc = LT ? rhol : rho;
 #pragma unroll
-  for (int i = 0; i < PX; ++i) rho[(x0 + i) * NYZ + yz] = 0.0;
+  for (int i = 0; i < PX; ++i) rho_acc[(x0 + i) * NYZ + yz] = 0.0;
   __syncthreads();
+  if constexpr (LT) {
+    for (int q = t; q < a.NPI; q += NT) {
+      const int code = a.jp_phase[q];
+      em[q] = phase_at<T, NX, NY, NZ>(phm, code);
+      en[q] = phase_at<T, NX, NY, NZ>(phn, code);
+      gix[q] = a.jp_grid[q];
+      const T* ur = a.uvp + (size_t)a.jp_atom[q] * kLA * a.Nps + a.jp_j[q];
+#pragma unroll
+      for (int l = 0; l < kLA; ++l) ug[l * kFusedMaxNPI + q] = ur[(size_t)l * a.Nps];
+    }
+    __syncthreads();
+  }
 
   for (int ib = b0; ib < b1; ++ib) {
     const size_t ikb = (size_t)ik * a.NB + ib;
@@ -230,6 +316,7 @@ __global__ void __launch_bounds__(NY * NZ * XS, WPE) k_fused_step(FusedArgs<T> a
     cplx<T>* out = bufB;
 #pragma unroll 1
     for (int n = 0; n < 4; ++n) {
+      const bool obs = DEFER && n == 0 && a.obs;
       if constexpr (XS > 1) {
 #pragma unroll
         for (int i = 0; i < H; ++i) {
@@ -252,11 +339,12 @@ __global__ void __launch_bounds__(NY * NZ * XS, WPE) k_fused_step(FusedArgs<T> a
         for (int l = 0; l < 2 * kLA; ++l) q[l] = 0;
         for (int j = lane; j < m; j += kWave) {
           const int jp = j0 + j;
-          const cplx<T> e = phase_at<T, NX, NY, NZ>(phm, jp_phase[jp]);
-          const cplx<T> w = cmul(e, in[jp_grid[jp]]);
+          cplx<T> w;
+          if constexpr (LT) w = cmul(em[jp], in[gix[jp]]);
+          else w = cmul(phase_at<T, NX, NY, NZ>(phm, jp_phase[jp]), in[jp_grid[jp]]);
 #pragma unroll
           for (int l = 0; l < kLA; ++l) {
-            const T u = ua[l * a.Nps + j];
+            const T u = LT ? ug[l * kFusedMaxNPI + jp] : ua[l * a.Nps + j];
             q[2 * l] += u * w.re;
             q[2 * l + 1] += u * w.im;
           }
@@ -274,12 +362,69 @@ __global__ void __launch_bounds__(NY * NZ * XS, WPE) k_fused_step(FusedArgs<T> a
           T tr = 0, ti = 0;
 #pragma unroll
           for (int l = 0; l < kLA; ++l) {
-            const T u = ua[l * a.Nps + j];
+            const T u = LT ? ug[l * kFusedMaxNPI + jp] : ua[l * a.Nps + j];
             tr += u * sr[l];
             ti += u * si[l];
           }
-          const cplx<T> e = phase_at<T, NX, NY, NZ>(phm, jp_phase[jp]);
+          const cplx<T> e = LT ? em[jp] : phase_at<T, NX, NY, NZ>(phm, jp_phase[jp]);
           dps[jp] = cplx<T>{e.re * tr + e.im * ti, e.re * ti - e.im * tr};  // conj(e) * t
+        }
+      }
+      if (DEFER && obs) {
+        // nonlocal current and energy of this orbital (current.f90:247-269,
+        // total_energy.f90:191-199) at k_obs: one wave per atom; per projector
+        // l the projection U = sum_j uV e psi and its first moments
+        // X_d = sum_j uV r_d e psi, all in FP64
+        const double* uv64 = opaque(a.uvp64);
+        const double* jp_pos = opaque(a.jp_pos);
+        // projectors per pass: all four in complex64; one at a time in FP64,
+        // where the x window and the Taylor sum hold more registers
+        constexpr int LB = sizeof(T) == 4 ? kLA : 1;
+        for (int at = wv; at < a.NI; at += NW) {
+          const int j0 = a.atom_start[at], m = a.atom_start[at + 1] - j0;
+          const double hx = g.Hxyz, f = occ / g.aLxyz * 2.0 * hx;
+#pragma unroll 1
+          for (int lb = 0; lb < kLA; lb += LB) {
+            const double* ua = uv64 + ((size_t)at * kLA + lb) * a.Nps;
+            double q[8 * LB];  // per l: Ur, Ui, Xr_x, Xi_x, Xr_y, Xi_y, Xr_z, Xi_z
+#pragma unroll
+            for (int l = 0; l < 8 * LB; ++l) q[l] = 0;
+            for (int j = lane; j < m; j += kWave) {
+              const int jp = j0 + j;
+              cplx<T> w;
+              if constexpr (LT) w = cmul(en[jp], in[gix[jp]]);
+              else w = cmul(phase_at<T, NX, NY, NZ>(phn, jp_phase[jp]), in[jp_grid[jp]]);
+              const double px = jp_pos[3 * jp], py = jp_pos[3 * jp + 1], pz = jp_pos[3 * jp + 2];
+#pragma unroll
+              for (int l = 0; l < LB; ++l) {
+                const double u = ua[l * a.Nps + j];
+                const double tr = u * (double)w.re, ti = u * (double)w.im;
+                q[8 * l + 0] += tr;
+                q[8 * l + 1] += ti;
+                q[8 * l + 2] += px * tr;
+                q[8 * l + 3] += px * ti;
+                q[8 * l + 4] += py * tr;
+                q[8 * l + 5] += py * ti;
+                q[8 * l + 6] += pz * tr;
+                q[8 * l + 7] += pz * ti;
+              }
+            }
+            wave_sums(q);
+#pragma unroll
+            for (int l = 0; l < LB; ++l) {
+              double r[8];
+#pragma unroll
+              for (int c = 0; c < 8; ++c) r[c] = lane_read(q[0], wave_sums_lane(8 * l + c));
+              if (lane == 0) {
+                const double sg = a.sgnp[at * kLA + lb + l];
+                const double Ur = r[0] * hx * sg, Ui = r[1] * hx * sg;
+                red[wv * 10 + 6] += f * (r[2] * Ui - r[3] * Ur);
+                red[wv * 10 + 7] += f * (r[4] * Ui - r[5] * Ur);
+                red[wv * 10 + 8] += f * (r[6] * Ui - r[7] * Ur);
+                red[wv * 10 + 9] += occ * (r[0] * r[0] + r[1] * r[1]) * hx * hx * sg;
+              }
+            }
+          }
         }
       }
       __syncthreads();  // KB terms complete
@@ -311,42 +456,111 @@ __global__ void __launch_bounds__(NY * NZ * XS, WPE) k_fused_step(FusedArgs<T> a
       // its input in the x window only when no later own point needs that
       // input (a delay of 4 points; the first 4 of a periodic column wait to
       // the end), which keeps few results live in registers.
-      cplx<T> h[PX];
+      auto sweep = [&](auto obs_c) {
+        constexpr bool OBS = decltype(obs_c)::value;
+        // observables of the input orbital at k_obs, FP64 sums.  The
+        // stencil's neighbour sums L = sum_a sum_n (-lap_n/2)(psi+ + psi-) and
+        // G_a = sum_n nab_n (psi+ - psi-) give per point the kinetic term
+        // Re conj(psi) L of total_energy_stencil (the centre term A0 |psi|^2
+        // is added per orbital) and the stencil current Im conj(psi) G_a; the
+        // gradient energy sum_i Re conj(psi)(-i k.G) is sum_a k_a current_a
+        // per orbital.  Over the periodic grid, sum_i Im conj(psi_i)
+        // (psi_{i+n} - psi_{i-n}) = 2 sum_i Im conj(psi_i) psi_{i+n}: the
+        // central sums are twice the forward sums of current_stencil
+        // (stencil/F90/current.f90:112-114).  L and G are in the propagation
+        // precision, products and sums in FP64.
+        double n2 = 0, vs = 0, te = 0, js[3] = {0, 0, 0};
+        const double* v64 = opaque(a.vloc64) + x0 * NYZ + yz;
+        cplx<T> h[PX];
 #pragma unroll
-      for (int i = 0; i < PX; ++i) {
-        const cplx<T> e0 = win[H + i];
-        const T vAi = vrow[i * NYZ] + A;
-        T lr = vAi * e0.re, li = vAi * e0.im;
-        T gr[3] = {0, 0, 0}, gi[3] = {0, 0, 0};
-        nb_sums<T, NX, NYZ, H, XS, W>(a, zb, yb, win, i, lr, li, gr, gi);
-        // KB term of the sphere point covering this point (branch-free:
-        // uncovered points read the zero entry dps[NPI])
-        const cplx<T> d = dps[(prim[i >> 1] >> (16 * (i & 1))) & 0xffff];
-        // -i * (k.G)
-        h[i].re = lr + (kmx * gi[0] + kmy * gi[1] + kmz * gi[2]) + d.re;
-        h[i].im = li - (kmx * gr[0] + kmy * gr[1] + kmz * gr[2]) + d.im;
-        out[(x0 + i) * NYZ + yz] = h[i];
-        acc[i].re += zr * h[i].re - zi * h[i].im;
-        acc[i].im += zr * h[i].im + zi * h[i].re;
-        if constexpr (XS > 1) {
-          if (i >= 4) win[H + i - 4] = h[i - 4];
-        } else {
-          if (i >= 8 && i - 4 < NX - 4) win[i - 4] = h[i - 4];
-        }
-      }
+        for (int i = 0; i < PX; ++i) {
+          const cplx<T> e0 = win[H + i];
+          const T vAi = vrow[i * NYZ] + A;
+          T lr, li;
+          if constexpr (OBS) {
+            T Lr = 0, Li = 0;
+            T gr[3] = {0, 0, 0}, gi[3] = {0, 0, 0};
+            nb_sums<T, NX, NYZ, H, XS, W>(a, zb, yb, win, i, Lr, Li, gr, gi);
+            const double er = (double)e0.re, ei = (double)e0.im;
+            const double a2 = er * er + ei * ei;
+            n2 += a2;
+            vs += v64[i * NYZ] * a2;
+            rho_acc[(x0 + i) * NYZ + yz] += occ * a2;
+            te += er * (double)Lr + ei * (double)Li;
 #pragma unroll
-      for (int i = 0; i < PX; ++i) {
-        if constexpr (XS > 1) {
-          if (i >= PX - 4) win[H + i] = h[i];
-        } else {
-          if (i < 4 || i >= NX - 4) win[i] = h[i];
+            for (int d = 0; d < 3; ++d) js[d] += er * (double)gi[d] - ei * (double)gr[d];
+            lr = vAi * e0.re + Lr + (kmx * gi[0] + kmy * gi[1] + kmz * gi[2]);
+            li = vAi * e0.im + Li - (kmx * gr[0] + kmy * gr[1] + kmz * gr[2]);
+          } else if constexpr (!DEFER) {
+            lr = vAi * e0.re;
+            li = vAi * e0.im;
+            nb_sums_k<T, NX, NYZ, H, XS, W>(a, D, zb, yb, win, i, lr, li);
+          } else {
+            // the same arithmetic as the observables sweep, so that all four
+            // H applications of a step use one Hamiltonian to the last bit
+            T Lr = 0, Li = 0;
+            T gr[3] = {0, 0, 0}, gi[3] = {0, 0, 0};
+            nb_sums<T, NX, NYZ, H, XS, W>(a, zb, yb, win, i, Lr, Li, gr, gi);
+            lr = vAi * e0.re + Lr + (kmx * gi[0] + kmy * gi[1] + kmz * gi[2]);
+            li = vAi * e0.im + Li - (kmx * gr[0] + kmy * gr[1] + kmz * gr[2]);
+          }
+          // KB term of the sphere point covering this point (branch-free:
+          // uncovered points read the zero entry dps[NPI])
+          const cplx<T> d = dps[(prim[i >> 1] >> (16 * (i & 1))) & 0xffff];
+          h[i].re = lr + d.re;
+          h[i].im = li + d.im;
+          out[(x0 + i) * NYZ + yz] = h[i];
+          acc[i].re += zr * h[i].re - zi * h[i].im;
+          acc[i].im += zr * h[i].im + zi * h[i].re;
+          if constexpr (XS > 1) {
+            if (i >= 4) win[H + i - 4] = h[i - 4];
+          } else {
+            if (i >= 8 && i - 4 < NX - 4) win[i - 4] = h[i - 4];
+          }
         }
+#pragma unroll
+        for (int i = 0; i < PX; ++i) {
+          if constexpr (XS > 1) {
+            if (i >= PX - 4) win[H + i] = h[i];
+          } else {
+            if (i < 4 || i >= NX - 4) win[i] = h[i];
+          }
+        }
+        if constexpr (OBS) {
+          // this orbital's sums into the wave's accumulators
+          const double n2s = wave_sum(n2), vss = wave_sum(vs), tes = wave_sum(te);
+          const double j0 = wave_sum(js[0]), j1 = wave_sum(js[1]), j2 = wave_sum(js[2]);
+          if (lane == 0) {
+            red[wv * 10 + 0] += occ * n2s;
+            red[wv * 10 + 1] += occ * vss;
+            red[wv * 10 + 2] += occ * (A0 * n2s + tes + kn[0] * j0 + kn[1] * j1 + kn[2] * j2);
+            // reference: jx = jx + jxt*occ with jxt = 2*sum Im(...) over
+            // forward neighbours (current.f90:206-210) = the central sum
+            red[wv * 10 + 3] += occ * j0;
+            red[wv * 10 + 4] += occ * j1;
+            red[wv * 10 + 5] += occ * j2;
+          }
+        }
+      };
+      if constexpr (DEFER) {
+        if (obs) sweep(BoolC<true>{});
+        else sweep(BoolC<false>{});
+      } else {
+        sweep(BoolC<false>{});
       }
       __syncthreads();
       cplx<T>* tmp = in;
       in = out;
       out = tmp;
+      if (!a.prop) break;  // observables only
     }
+    if constexpr (DEFER) {
+      // psi(t+dt) to HBM
+      if (a.prop) {
+#pragma unroll
+        for (int i = 0; i < PX; ++i) zo[(x0 + i) * NYZ + yz] = acc[i];
+      }
+    } else {
     // psi(t+dt): to HBM, to the free LDS buffer and into the x window
 #pragma unroll
     for (int i = 0; i < PX; ++i) {
@@ -459,8 +673,11 @@ __global__ void __launch_bounds__(NY * NZ * XS, WPE) k_fused_step(FusedArgs<T> a
         }
       }
     }
-    // the next orbital overwrites bufA only; bufB (psi(t+dt)) is read above
-    // and is next written after the barrier that follows the next load.
+    }
+  }
+  if constexpr (LT) {
+#pragma unroll
+    for (int i = 0; i < PX; ++i) rho[(x0 + i) * NYZ + yz] = rhol[(x0 + i) * NYZ + yz];
   }
   // per-workgroup sums of the per-wave accumulators, in wave order
   __syncthreads();
@@ -496,7 +713,7 @@ __global__ void __launch_bounds__(NY * NZ * XS, WPE) k_fused_step(FusedArgs<T> a
 #define SALMON_FUSED_XS64 2
 #endif
 #ifndef SALMON_FUSED_XS32
-#define SALMON_FUSED_XS32 1
+#define SALMON_FUSED_XS32 2
 #endif
 // waves per SIMD the register allocation targets: the FP64 kernel's 154 KiB of
 // LDS allows one workgroup per CU, the complex64 kernel's 77 KiB two
@@ -651,7 +868,10 @@ static int fused_set_tables(salmon_ctx* c) {
 
 template <typename T, int NX, int NY, int NZ, int XS>
 static size_t fused_lds_bytes() {
-  return sizeof(salmon::cplx<T>) * (2 * NX * NY * NZ + 2 * 5 * (NX + NY + NZ) + salmon::kFusedMaxNPI) +
+  constexpr bool LT = salmon::fused_lds_tables<T, XS>();
+  constexpr size_t MT = LT ? salmon::kFusedMaxNPI : 0;
+  return sizeof(salmon::cplx<T>) * (2 * NX * NY * NZ + 2 * 5 * (NX + NY + NZ) + salmon::kFusedMaxNPI + 2 * MT) +
+         sizeof(T) * salmon::kLA * MT + sizeof(int) * MT + sizeof(double) * (LT ? NX * NY * NZ : 0) +
          sizeof(double) * 10 * ((NY * NZ * XS + 63) / 64);
 }
 
@@ -668,10 +888,13 @@ static int fused_launch(salmon_ctx* c, const salmon::FusedArgs<T>& args, int nwg
   return SALMON_OK;
 }
 
-// workgroups resident per CU for a launch (1 for the FP64 16^3 kernel, whose
-// LDS is 141 KiB; 2 for the complex64 kernels)
+// One launch over all (k, band): with amid, the Taylor-4 step at kAc0 + amid;
+// with aobs, the observables at kAc0 + aobs (density into d_rho, the 8
+// result rows into res_dev) -- of the orbitals as they are on entry for the
+// complex64 kernel (fused_defer), of psi(t+dt) for the FP64 kernel, which is
+// always called with both.  For complex64 either may be null (not both).
 template <typename T>
-static int fused_step(salmon_ctx* c, const double amid[3], const double anew[3], double dt, double* res_dev) {
+static int fused_step(salmon_ctx* c, const double* amid, const double* aobs, double dt, double* res_dev) {
   FusedState* f = (FusedState*)c->fz;
   if (!f || !f->tables) return fail(c, SALMON_ERR_UNSUPPORTED, "fused tables not built");
   int ncu = 256;
@@ -708,7 +931,12 @@ static int fused_step(salmon_ctx* c, const double amid[3], const double anew[3],
   a.vloc = vloc_ptr<T>(c);
   a.vloc64 = c->d_vloc;
   a.NK = c->NK; a.NB = c->NB; a.bpw = bpw; a.nbg = nbg;
-  for (int d = 0; d < 3; ++d) { a.am[d] = amid[d]; a.an[d] = anew[d]; }
+  a.prop = amid != nullptr;
+  a.obs = aobs != nullptr;
+  for (int d = 0; d < 3; ++d) {
+    a.am[d] = amid ? amid[d] : 0.0;
+    a.an[d] = aobs ? aobs[d] : 0.0;
+  }
   a.z1i = (T)(-dt); a.z2r = (T)(-dt * dt / 2.0); a.z3i = (T)(dt * dt * dt / 6.0); a.z4r = (T)(dt * dt * dt * dt / 24.0);
   for (int d = 0; d < 3; ++d)
     for (int n = 0; n < 5; ++n) {
@@ -735,9 +963,8 @@ static int fused_step(salmon_ctx* c, const double amid[3], const double anew[3],
     r = a.NOV > 0 ? fused_launch<T, 16, 16, 16, kFusedXS32, kFusedWPE32, true>(c, a, nwg)
                   : fused_launch<T, 16, 16, 16, kFusedXS32, kFusedWPE32, false>(c, a, nwg);
   }
-  if (r) return r;
+  if (r || !aobs) return r;
   hipLaunchKernelGGL(salmon::k_sum_rows, dim3(1), dim3(256), 0, c->stream, f->wg_rows, nwg, res_dev);
   LAUNCHCHK(c);
   return density_reduce(c, f->rho_part, nwg);
-  return SALMON_OK;
 }

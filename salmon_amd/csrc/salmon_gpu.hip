@@ -286,18 +286,57 @@ int gram_schmidt_multi_fwd(salmon_ctx* c, cplx<double>* zu) {
 
 namespace {
 
-// One RT step on resident orbitals: Taylor-4 at kAc0 + amid, density, then
-// observables at kAc0 + anew into res[0..7].
+// One RT step on resident orbitals (streamed path): Taylor-4 at kAc0 + amid,
+// density, then observables at kAc0 + anew into res[0..7].
 template <typename T>
 int rt_step(salmon_ctx* c, const double amid[3], const double anew[3], double dt, double* res_dev) {
   int r;
-  if (fused_ready(c)) return fused_step<T>(c, amid, anew, dt, res_dev);
   if ((r = taylor_streamed<T>(c, c->d_kac0, amid[0], amid[1], amid[2], dt))) return r;
   if ((r = density_dev<T>(c))) return r;
   if ((r = observe_rows<T>(c, c->d_kac0, anew[0], anew[1], anew[2]))) return r;
   hipLaunchKernelGGL(k_sum_rows, dim3(1), dim3(256), 0, c->stream, c->d_rows, c->NKB, res_dev);
   LAUNCHCHK(c);
   return SALMON_OK;
+}
+
+// nsteps RT steps on the fused path.  The observables of step n (at
+// kAc0 + A(t_{n+1}), of psi(t_{n+1})) are evaluated by the launch of step
+// n+1 during its first H application, which reads psi(t_{n+1}) anyway; an
+// observables-only launch closes the run, and one runs before each
+// re-orthonormalisation, so every result is of the same orbitals as in the
+// step-by-step order (step, observables, re-orthonormalisation).
+template <typename T>
+int rt_run_fused(salmon_ctx* c, const double* ac, int nsteps, double dt, int reortho_every, double* d_res) {
+  int r = SALMON_OK;
+  if constexpr (!salmon::fused_defer<T, kFusedXS64>()) {
+    // observables of psi(t+dt) in the launch of the step itself
+    for (int n = 0; n < nsteps && r == SALMON_OK; ++n) {
+      double amid[3], anew[3];
+      for (int d = 0; d < 3; ++d) {
+        amid[d] = 0.5 * (ac[3 * n + d] + ac[3 * (n + 1) + d]);
+        anew[d] = ac[3 * (n + 1) + d];
+      }
+      r = fused_step<T>(c, amid, anew, dt, d_res + 8 * (size_t)n);
+      if (r == SALMON_OK && reortho_every > 0 && (n + 1) % reortho_every == 0) r = gram_schmidt_dev(c);
+    }
+    return r;
+  }
+  double aobs[3];
+  bool pending = false;
+  for (int n = 0; n < nsteps && r == SALMON_OK; ++n) {
+    double amid[3];
+    for (int d = 0; d < 3; ++d) amid[d] = 0.5 * (ac[3 * n + d] + ac[3 * (n + 1) + d]);
+    r = fused_step<T>(c, amid, pending ? aobs : nullptr, dt, pending ? d_res + 8 * (size_t)(n - 1) : nullptr);
+    for (int d = 0; d < 3; ++d) aobs[d] = ac[3 * (n + 1) + d];
+    pending = true;
+    if (r == SALMON_OK && reortho_every > 0 && (n + 1) % reortho_every == 0) {
+      r = fused_step<T>(c, nullptr, aobs, dt, d_res + 8 * (size_t)n);
+      pending = false;
+      if (r == SALMON_OK) r = gram_schmidt_dev(c);
+    }
+  }
+  if (r == SALMON_OK && pending) r = fused_step<T>(c, nullptr, aobs, dt, d_res + 8 * (size_t)(nsteps - 1));
+  return r;
 }
 
 }  // namespace
@@ -973,7 +1012,11 @@ int salmon_gpu_rt_run(salmon_ctx* c, const double* ac, int nsteps, double dt, in
   if (nsteps == 0) return SALMON_OK;
   double* d_res = nullptr;
   if ((r = dalloc(c, &d_res, 8 * (size_t)nsteps))) return r;
-  for (int n = 0; n < nsteps && r == SALMON_OK; ++n) {
+  if (fused_ready(c)) {
+    r = c->precision == SALMON_FP64 ? rt_run_fused<double>(c, ac, nsteps, dt, reortho_every, d_res)
+                                    : rt_run_fused<float>(c, ac, nsteps, dt, reortho_every, d_res);
+  }
+  for (int n = 0; n < nsteps && r == SALMON_OK && !fused_ready(c); ++n) {
     double amid[3], anew[3];
     for (int d = 0; d < 3; ++d) {
       amid[d] = 0.5 * (ac[3 * n + d] + ac[3 * (n + 1) + d]);
