@@ -198,8 +198,9 @@ def main():
     traffic = measured_traffic()
     roof = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
             "traffic": traffic.get("mixed"),
-            "kernel": "k_fused_step<float> (one launch per RT step; duration = timed region / steps, which "
-                      "also holds two small reduction kernels per step)",
+            "kernel": "k_fused_step<float> (one launch per RT step, which also evaluates the previous step's "
+                      "observables, plus one observables-only launch closing the run; duration = timed region / "
+                      "steps, which also holds the small row and density reduction kernels)",
             "algorithmic_bytes_per_launch": B32,
             "fp64": {"kernel": "k_fused_step<double>", "achieved": B64 / (ms64 * 1e-3) / 1e9,
                      "frac": B64 / (ms64 * 1e-3) / 1e9 / HBM_PEAK_GBS, "algorithmic_bytes_per_launch": B64,
