@@ -115,6 +115,29 @@ def test_cfg3_mixed_precision_error_against_fp64():
     assert nerr <= 1e-5
 
 
+@pytest.mark.parametrize("prec", [gpu.FP64, gpu.MIXED])
+def test_fused_run_split_matches_single_run(prec):
+    """The complex64 step launch evaluates the previous step's observables and
+    an observables-only launch closes each run: a run split into two calls
+    gives the same J(t), E(t), norms, density and orbitals, bit for bit, as
+    one call (FP64 evaluates them in the step's own launch)."""
+    s, d = load_case("cfg1")
+    ac = d["ac"][:7]
+    with _ctx(s, prec) as c:
+        assert c.path() == gpu.PATH_FUSED
+        r = c.rt_run(ac, 6)
+        z = c.get_orbitals()
+    with _ctx(s, prec) as c:
+        r1 = c.rt_run(ac[:4], 3)
+        r2 = c.rt_run(ac[3:], 3)
+        z2 = c.get_orbitals()
+    for key in ("J", "E", "norm"):
+        both = np.concatenate([r1[key], r2[key]])
+        assert np.array_equal(both, r[key]), key
+    assert np.array_equal(z2, z)
+    assert np.array_equal(r2["rho"], r["rho"])
+
+
 def test_mixed_hpsi_close_to_fp64():
     s, d = load_case("small_k")
     with _ctx(s, gpu.MIXED) as c:
