@@ -22,6 +22,7 @@ struct Sys {
   cplx<T>* zu;
   cplx<T>* h[4];
   cplx<T>* uv;  // (NKB, Nlma)
+  cplx<T>* ph;  // (NK, NPI) KB phases at this step's midpoint k
   const T* vloc;
   const double* vloc64;
   const double* kac0;
@@ -65,7 +66,25 @@ __global__ void __launch_bounds__(256) k_b_project(const Sys<T>* __restrict__ S,
   double kx, ky, kz;
   orbital_k(y.kac0, y.NK, o / y.NB, am[0], am[1], am[2], kx, ky, kz);
   const cplx<T>* in = (term == 0 ? y.zu : y.h[term - 1]) + (size_t)o * y.g.NL;
-  kb_project_wave<T>(in, ilma, kx, ky, kz, y.g, y.kb, y.uv + (size_t)o * y.kb.Nlma + ilma);
+  kb_project_wave<T>(in, ilma, kx, ky, kz, y.g, y.kb, y.uv + (size_t)o * y.kb.Nlma + ilma,
+                     y.ph + (size_t)(o / y.NB) * y.kb.NPI);
+}
+
+// The KB phases of every system's k points at this step's midpoint k (one
+// grid row per system), shared by the step's four projections and stencils.
+template <typename T>
+__global__ void __launch_bounds__(256) k_b_phase(const Sys<T>* __restrict__ S, Step st) {
+  const Sys<T>& y = S[blockIdx.y];
+  const long i = (long)blockIdx.x * 256 + threadIdx.x;
+  if (i >= (long)y.NK * y.kb.NPI) return;
+  const int ik = (int)(i / y.kb.NPI), jp = (int)(i - (long)ik * y.kb.NPI);
+  double am[3], an[3];
+  step_fields(st, am, an);
+  double kx, ky, kz;
+  orbital_k(y.kac0, y.NK, ik, am[0], am[1], am[2], kx, ky, kz);
+  double s, c;
+  sincos(kx * y.kb.pos[3 * jp] + ky * y.kb.pos[3 * jp + 1] + kz * y.kb.pos[3 * jp + 2], &s, &c);
+  y.ph[i] = cplx<T>{(T)c, (T)s};
 }
 
 // ZB consecutive z points of one (ix, iy) line: the z neighbours come from a
@@ -74,7 +93,8 @@ __global__ void __launch_bounds__(256) k_b_project(const Sys<T>* __restrict__ S,
 template <typename T, int ZB>
 __device__ __forceinline__ void hpsi_zblock(const cplx<T>* __restrict__ E, int p0, double kx, double ky, double kz,
                                             const T* __restrict__ vloc, const Grid& g, const KBTables& kb,
-                                            const cplx<T>* __restrict__ U, cplx<T>* __restrict__ out) {
+                                            const cplx<T>* __restrict__ U, cplx<T>* __restrict__ out,
+                                            const cplx<T>* __restrict__ ph) {
   const int NLy = g.NLy, NLz = g.NLz;
   const int iz0 = p0 % NLz, iy = (p0 / NLz) % NLy, ix = p0 / (NLz * NLy);
   const int line = p0 - iz0;
@@ -130,9 +150,7 @@ __device__ __forceinline__ void hpsi_zblock(const cplx<T>* __restrict__ E, int p
       for (int r = kb.t_start[t]; r < kb.t_start[t + 1]; ++r) {
         const int il = kb.e_lma[r], jp = kb.e_jp[r];
         const int a = kb.lma_atom[il];
-        const double kr = kx * kb.pos[3 * jp] + ky * kb.pos[3 * jp + 1] + kz * kb.pos[3 * jp + 2];
-        T sn, cs;
-        sincos_t((T)kr, &sn, &cs);
+        const T cs = ph[jp].re, sn = ph[jp].im;
         const T u = (T)kb.uv[(size_t)il * kb.Nps + (jp - kb.atom_start[a])];
         const cplx<T> w = U[il];
         f.re += u * (cs * w.re + sn * w.im);
@@ -160,7 +178,8 @@ __global__ void __launch_bounds__(256) k_b_hpsi(const Sys<T>* __restrict__ S, in
   double kx, ky, kz;
   orbital_k(y.kac0, y.NK, o / y.NB, am[0], am[1], am[2], kx, ky, kz);
   const cplx<T>* in = (term == 0 ? y.zu : y.h[term - 1]) + (size_t)o * y.g.NL;
-  hpsi_zblock<T, ZB>(in, p0, kx, ky, kz, y.vloc, y.g, y.kb, y.uv + (size_t)o * y.kb.Nlma, y.h[term] + (size_t)o * y.g.NL);
+  hpsi_zblock<T, ZB>(in, p0, kx, ky, kz, y.vloc, y.g, y.kb, y.uv + (size_t)o * y.kb.Nlma, y.h[term] + (size_t)o * y.g.NL,
+                     y.ph + (size_t)(o / y.NB) * y.kb.NPI);
 }
 
 // zu += sum_n zfac(n) h_n, zfac(n) = (-i dt)^n/n! (ARTED/RT/hamiltonian.f90:46-49,99-122).

@@ -25,6 +25,7 @@ struct salmon_batch {
   double* d_ac = nullptr;
   double* d_res = nullptr;
   size_t cap_steps = 0;
+  long max_ph = 0;                // largest NK * NPI over the systems
   hipGraphExec_t exec = nullptr;
   double graph_dt = 0.0;
 };
@@ -59,6 +60,7 @@ template <typename T>
 static int batch_build(salmon_batch* b) {
   const int n = (int)b->ctx.size();
   std::vector<Sys<T>> S(n);
+  b->max_ph = 0;
   long acc[kPhases] = {0, 0, 0, 0};
   for (int i = 0; i < n; ++i) {
     salmon_ctx* c = b->ctx[i];
@@ -81,10 +83,13 @@ static int batch_build(salmon_batch* b) {
       b->bufs.push_back(y.h[t]);
     }
     if (hipMalloc((void**)&y.uv, (size_t)c->NKB * std::max(1, c->kb.Nlma) * sizeof(cplx<T>)) != hipSuccess ||
+        hipMalloc((void**)&y.ph, (size_t)c->NK * std::max(1, c->kb.NPI) * sizeof(cplx<T>)) != hipSuccess ||
         hipMalloc((void**)&y.rows, (size_t)c->NKB * 8 * sizeof(double)) != hipSuccess)
       return bfail(b, SALMON_ERR_NOMEM, "batch buffers (system %d)", i);
     b->bufs.push_back(y.uv);
+    b->bufs.push_back(y.ph);
     b->bufs.push_back(y.rows);
+    b->max_ph = std::max(b->max_ph, (long)c->NK * c->kb.NPI);
     y.pblk = (std::max(1, c->kb.Nlma) + 3) / 4;
     y.hblk = (c->g.NL / y.zb + 255) / 256;
     const long nb[kPhases] = {(long)c->NKB * y.pblk, 0, (long)((nv + 255) / 256), (long)c->NKB};
@@ -134,6 +139,8 @@ static int batch_enqueue_step(salmon_batch* b, double dt) {
   const int n = (int)b->ctx.size();
   const Sys<T>* S = (const Sys<T>*)b->d_sys;
   const Step st{b->d_ac, b->d_step, dt, {b->d_owner[0], b->d_owner[1], b->d_owner[2], b->d_owner[3]}};
+  if (b->max_ph > 0)
+    hipLaunchKernelGGL(k_b_phase<T>, dim3((unsigned)((b->max_ph + 255) / 256), n), dim3(256), 0, b->stream, S, st);
   for (int t = 0; t < 4; ++t) {
     hipLaunchKernelGGL(k_b_project<T>, dim3((unsigned)b->first[0][0]), dim3(256), 0, b->stream, S, n, t, st);
     if (b->zcls[0][1]) hipLaunchKernelGGL((k_b_hpsi<T, 1>), dim3((unsigned)b->zcls[0][1]), dim3(256), 0, b->stream, S, t, st, b->zcls[0][0]);
@@ -175,6 +182,9 @@ static int batch_tune_one(salmon_batch* b, int i, int* best) {
     y.hblk = (c->g.NL / zb + 255) / 256;
     y.first[1] = 0;
     BHIP(b, hipMemcpy(d1, &y, sizeof(Sys<T>), hipMemcpyHostToDevice));
+    if ((long)y.NK * y.kb.NPI > 0)
+      hipLaunchKernelGGL(k_b_phase<T>, dim3((unsigned)(((long)y.NK * y.kb.NPI + 255) / 256), 1), dim3(256), 0, b->stream,
+                         (const Sys<T>*)d1, st);
     const unsigned nblk = (unsigned)((long)y.NKB * y.hblk);
     float tmin = 1e30f;
     for (int rep = 0; rep < 4; ++rep) {

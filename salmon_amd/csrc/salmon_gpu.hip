@@ -43,6 +43,8 @@ struct salmon_ctx {
   void* d_work[4] = {nullptr, nullptr, nullptr, nullptr};
   void* d_uvpsi = nullptr;
   int uvpsi_nlma = 0;         // projectors d_uvpsi is sized for
+  void* d_ph = nullptr;       // KB phases of the streamed Taylor step (NK, NPI)
+  size_t ph_bytes = 0;
   double* d_rows = nullptr;   // (NKB*8)
   double* d_sum = nullptr;    // (8)
   double* d_rho = nullptr;    // (NL)
@@ -147,17 +149,18 @@ const float* vloc_ptr<float>(const salmon_ctx* c) { return c->d_vloc_f; }
 // H applied to nOrb orbitals (rows of NL) from `in` to `out` (optionally tt).
 template <typename T>
 int apply_h(salmon_ctx* c, const cplx<T>* in, cplx<T>* out, cplx<T>* tt, int orb0, int nOrb, int NB, int NK,
-            const double* kbase, double ax, double ay, double az, void* uvbuf = nullptr) {
+            const double* kbase, double ax, double ay, double az, void* uvbuf = nullptr,
+            const cplx<T>* ph = nullptr) {
   cplx<T>* uv = (cplx<T>*)(uvbuf ? uvbuf : c->d_uvpsi);
   dim3 gp((c->kb.Nlma + 3) / 4, nOrb);
   if (c->kb.Nlma > 0) {
     hipLaunchKernelGGL(k_kb_project<T>, gp, dim3(256), 0, c->stream, in, orb0, NB, NK, kbase, ax, ay, az, c->g,
-                       c->kb, uv);
+                       c->kb, uv, ph);
     LAUNCHCHK(c);
   }
   dim3 gh((c->g.NL + 255) / 256, nOrb);
   hipLaunchKernelGGL(k_hpsi<T>, gh, dim3(256), 0, c->stream, in, out, tt, orb0, NB, NK, kbase, ax, ay, az,
-                     vloc_ptr<T>(c), c->g, c->kb, (const cplx<T>*)uv);
+                     vloc_ptr<T>(c), c->g, c->kb, (const cplx<T>*)uv, ph);
   LAUNCHCHK(c);
   return SALMON_OK;
 }
@@ -169,15 +172,31 @@ int taylor_streamed(salmon_ctx* c, const double* kbase, double ax, double ay, do
   // zfac(n) = (-i dt)^n / n!  (hamiltonian.f90:46-49)
   const double z1i = -dt, z2r = -dt * dt / 2.0, z3i = dt * dt * dt / 6.0, z4r = dt * dt * dt * dt / 24.0;
   const size_t NL = c->g.NL;
+  // KB phases of every k point at this step's k, shared by the four H applications
+  const size_t nph = (size_t)c->NK * c->kb.NPI;
+  cplx<T>* ph = nullptr;
+  if (nph > 0) {
+    if (nph * sizeof(cplx<T>) > c->ph_bytes) {
+      if (c->d_ph) (void)hipFree(c->d_ph);
+      c->d_ph = nullptr;
+      c->ph_bytes = 0;
+      if (hipMalloc(&c->d_ph, nph * sizeof(cplx<T>)) != hipSuccess) return fail(c, SALMON_ERR_NOMEM, "phase table");
+      c->ph_bytes = nph * sizeof(cplx<T>);
+    }
+    ph = (cplx<T>*)c->d_ph;
+    hipLaunchKernelGGL(k_phase_table<T>, dim3((unsigned)((nph + 255) / 256)), dim3(256), 0, c->stream, kbase, c->NK, ax,
+                       ay, az, c->kb, ph);
+    LAUNCHCHK(c);
+  }
   for (int o0 = 0; o0 < c->NKB; o0 += c->chunk) {
     const int n = std::min(c->chunk, c->NKB - o0);
     cplx<T>* zu = (cplx<T>*)c->d_zu + (size_t)o0 * NL;
     cplx<T>* h[4];
     for (int i = 0; i < 4; ++i) h[i] = (cplx<T>*)c->d_work[i];
-    if ((r = apply_h<T>(c, zu, h[0], nullptr, o0, n, c->NB, c->NK, kbase, ax, ay, az))) return r;
-    if ((r = apply_h<T>(c, h[0], h[1], nullptr, o0, n, c->NB, c->NK, kbase, ax, ay, az))) return r;
-    if ((r = apply_h<T>(c, h[1], h[2], nullptr, o0, n, c->NB, c->NK, kbase, ax, ay, az))) return r;
-    if ((r = apply_h<T>(c, h[2], h[3], nullptr, o0, n, c->NB, c->NK, kbase, ax, ay, az))) return r;
+    if ((r = apply_h<T>(c, zu, h[0], nullptr, o0, n, c->NB, c->NK, kbase, ax, ay, az, nullptr, ph))) return r;
+    if ((r = apply_h<T>(c, h[0], h[1], nullptr, o0, n, c->NB, c->NK, kbase, ax, ay, az, nullptr, ph))) return r;
+    if ((r = apply_h<T>(c, h[1], h[2], nullptr, o0, n, c->NB, c->NK, kbase, ax, ay, az, nullptr, ph))) return r;
+    if ((r = apply_h<T>(c, h[2], h[3], nullptr, o0, n, c->NB, c->NK, kbase, ax, ay, az, nullptr, ph))) return r;
     const size_t tot = (size_t)n * NL;
     hipLaunchKernelGGL(k_taylor_fold<T>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, c->stream, zu, h[0], h[1],
                        h[2], h[3], tot, (T)z1i, (T)z2r, (T)z3i, (T)z4r);
@@ -1142,7 +1161,7 @@ int salmon_gpu_finalize(salmon_ctx* c) {
   dfree((void**)&c->d_rho_part2);
   dfree((void**)&c->d_kac0); dfree((void**)&c->d_kac); dfree((void**)&c->d_occ); dfree((void**)&c->d_rows);
   dfree((void**)&c->d_sum); dfree((void**)&c->d_kone); dfree((void**)&c->d_oneocc); dfree(&c->d_zu);
-  dfree((void**)&c->d_f64); dfree(&c->d_uvpsi);
+  dfree((void**)&c->d_f64); dfree(&c->d_uvpsi); dfree(&c->d_ph);
   for (int i = 0; i < 4; ++i) dfree(&c->d_work[i]);
   for (int i = 0; i < 3; ++i) dfree(&c->d_stage[i]);
   for (void* p : c->kb_bufs) (void)hipFree(p);

@@ -24,16 +24,22 @@ __device__ __forceinline__ void orbital_k(const double* kac0, int NK, int ik, do
 // ---------------------------------------------------------------------------
 template <typename T>
 __device__ __forceinline__ void kb_project_wave(const cplx<T>* __restrict__ E, int ilma, double kx, double ky, double kz,
-                                                const Grid& g, const KBTables& kb, cplx<T>* __restrict__ dst) {
+                                                const Grid& g, const KBTables& kb, cplx<T>* __restrict__ dst,
+                                                const cplx<T>* __restrict__ ph = nullptr) {
   const int lane = threadIdx.x & (kWave - 1);
   const int a = kb.lma_atom[ilma];
   const int j0 = kb.atom_start[a], m = kb.atom_start[a + 1] - j0;
   T sr = 0, si = 0;
   for (int j = lane; j < m; j += kWave) {
     const int jp = j0 + j;
-    const double kr = kx * kb.pos[3 * jp] + ky * kb.pos[3 * jp + 1] + kz * kb.pos[3 * jp + 2];
     T s, c;
-    sincos_t((T)kr, &s, &c);
+    if (ph) {
+      c = ph[jp].re;
+      s = ph[jp].im;
+    } else {
+      const double kr = kx * kb.pos[3 * jp] + ky * kb.pos[3 * jp + 1] + kz * kb.pos[3 * jp + 2];
+      sincos_t((T)kr, &s, &c);
+    }
     const T u = (T)kb.uv[(size_t)ilma * kb.Nps + j];
     const cplx<T> e = E[kb.pidx[jp]];
     // (u c + i u s) * (e.re + i e.im)
@@ -51,14 +57,33 @@ __device__ __forceinline__ void kb_project_wave(const cplx<T>* __restrict__ E, i
 template <typename T>
 __global__ void __launch_bounds__(256) k_kb_project(const cplx<T>* __restrict__ psi, int orb0, int NB, int NK,
                                                     const double* __restrict__ kac0, double ax, double ay,
-                                                    double az, Grid g, KBTables kb, cplx<T>* __restrict__ uvpsi) {
+                                                    double az, Grid g, KBTables kb, cplx<T>* __restrict__ uvpsi,
+                                                    const cplx<T>* __restrict__ ph) {
   const int ilma = blockIdx.x * 4 + (threadIdx.x / kWave);
   const int o = blockIdx.y;
   if (ilma >= kb.Nlma) return;
   const int ikb = orb0 + o;
   double kx, ky, kz;
   orbital_k(kac0, NK, ikb / NB, ax, ay, az, kx, ky, kz);
-  kb_project_wave<T>(psi + (size_t)o * g.NL, ilma, kx, ky, kz, g, kb, uvpsi + (size_t)o * kb.Nlma + ilma);
+  kb_project_wave<T>(psi + (size_t)o * g.NL, ilma, kx, ky, kz, g, kb, uvpsi + (size_t)o * kb.Nlma + ilma,
+                     ph ? ph + (size_t)(ikb / NB) * kb.NPI : nullptr);
+}
+
+// e^{i k.(r_j - T_j)} of every sphere point for every k point at kAc0 + a
+// (NK, NPI): the phases of the KB projection and apply, shared by the four H
+// applications of a Taylor step and by all bands of a k point.  FP64 sincos,
+// rounded to the propagation precision.
+template <typename T>
+__global__ void __launch_bounds__(256) k_phase_table(const double* __restrict__ kac0, int NK, double ax, double ay,
+                                                     double az, KBTables kb, cplx<T>* __restrict__ ph) {
+  const long i = (long)blockIdx.x * 256 + threadIdx.x;
+  if (i >= (long)NK * kb.NPI) return;
+  const int ik = (int)(i / kb.NPI), jp = (int)(i - (long)ik * kb.NPI);
+  double kx, ky, kz;
+  orbital_k(kac0, NK, ik, ax, ay, az, kx, ky, kz);
+  double s, c;
+  sincos(kx * kb.pos[3 * jp] + ky * kb.pos[3 * jp + 1] + kz * kb.pos[3 * jp + 2], &s, &c);
+  ph[i] = cplx<T>{(T)c, (T)s};
 }
 
 // ---------------------------------------------------------------------------
@@ -73,7 +98,8 @@ __global__ void __launch_bounds__(256) k_kb_project(const cplx<T>* __restrict__ 
 template <typename T>
 __device__ __forceinline__ cplx<T> hpsi_point(const cplx<T>* __restrict__ E, int p, double kx, double ky, double kz,
                                               const T* __restrict__ vloc, const Grid& g, const KBTables& kb,
-                                              const cplx<T>* __restrict__ U, cplx<T>* __restrict__ ttp) {
+                                              const cplx<T>* __restrict__ U, cplx<T>* __restrict__ ttp,
+                                              const cplx<T>* __restrict__ ph = nullptr) {
   const T A = (T)((kx * kx + ky * ky + kz * kz - (g.lap[0][0] + g.lap[1][0] + g.lap[2][0])) * 0.5);
   const int NLx = g.NLx, NLy = g.NLy, NLz = g.NLz;
   const int iz = p % NLz, iy = (p / NLz) % NLy, ix = p / (NLz * NLy);
@@ -111,9 +137,14 @@ __device__ __forceinline__ cplx<T> hpsi_point(const cplx<T>* __restrict__ E, int
     for (int q = kb.t_start[t]; q < kb.t_start[t + 1]; ++q) {
       const int il = kb.e_lma[q], jp = kb.e_jp[q];
       const int a = kb.lma_atom[il];
-      const double kr = kx * kb.pos[3 * jp] + ky * kb.pos[3 * jp + 1] + kz * kb.pos[3 * jp + 2];
       T s, c;
-      sincos_t((T)kr, &s, &c);
+      if (ph) {
+        c = ph[jp].re;
+        s = ph[jp].im;
+      } else {
+        const double kr = kx * kb.pos[3 * jp] + ky * kb.pos[3 * jp + 1] + kz * kb.pos[3 * jp + 2];
+        sincos_t((T)kr, &s, &c);
+      }
       const T u = (T)kb.uv[(size_t)il * kb.Nps + (jp - kb.atom_start[a])];
       // zproj * uVpsi = u (c - i s) (U.re + i U.im)
       const cplx<T> w = U[il];
@@ -129,7 +160,7 @@ __global__ void __launch_bounds__(256) k_hpsi(const cplx<T>* __restrict__ in, cp
                                               cplx<T>* __restrict__ tt, int orb0, int NB, int NK,
                                               const double* __restrict__ kac0, double ax, double ay, double az,
                                               const T* __restrict__ vloc, Grid g, KBTables kb,
-                                              const cplx<T>* __restrict__ uvpsi) {
+                                              const cplx<T>* __restrict__ uvpsi, const cplx<T>* __restrict__ ph) {
   const int p = blockIdx.x * 256 + threadIdx.x;
   const int o = blockIdx.y;
   if (p >= g.NL) return;
@@ -137,7 +168,8 @@ __global__ void __launch_bounds__(256) k_hpsi(const cplx<T>* __restrict__ in, cp
   double kx, ky, kz;
   orbital_k(kac0, NK, ikb / NB, ax, ay, az, kx, ky, kz);
   out[(size_t)o * g.NL + p] = hpsi_point<T>(in + (size_t)o * g.NL, p, kx, ky, kz, vloc, g, kb,
-                                            uvpsi + (size_t)o * kb.Nlma, tt ? tt + (size_t)o * g.NL + p : nullptr);
+                                            uvpsi + (size_t)o * kb.Nlma, tt ? tt + (size_t)o * g.NL + p : nullptr,
+                                            ph ? ph + (size_t)(ikb / NB) * kb.NPI : nullptr);
 }
 
 // zu += z1 h1 + z2 h2 + z3 h3 + z4 h4 (update, ARTED/RT/hamiltonian.f90:99-122)

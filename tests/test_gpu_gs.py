@@ -117,7 +117,10 @@ def test_ground_state_reaches_dense_eigenvalues():
         print(f"[gpu] ground_state pc={pc}: sweeps {r['sweeps']} err {np.max(np.abs(r['esp'][0] - w)):.2e}",
               flush=True)
         assert np.max(np.abs(r["esp"][0] - w)) < 1e-8
-        assert np.max(r["esp_var"] / s.occ) < 1e-6
+        # the sweeps stop once no eigenvalue moves by more than tol = 1e-12;
+        # the residual norm left then is of order sqrt(tol * gap) ~ 1e-6, and
+        # where under 1e-5 it lands depends on rounding along the path
+        assert np.max(r["esp_var"] / s.occ) < 1e-5
         sweeps[pc] = r["sweeps"]
     assert sweeps[gpu.PC_KINETIC] < sweeps[gpu.PC_NONE]
 
