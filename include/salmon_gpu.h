@@ -193,6 +193,13 @@ int salmon_gpu_pcnet_finalize(salmon_pcnet *net);
 int salmon_gpu_pcnet_apply(salmon_pcnet *net, salmon_ctx *ctx, const double *g, double *d);
 /* network used by SALMON_PC_LEARNED on ctx (NULL detaches; not owned) */
 int salmon_gpu_set_pcnet(salmon_ctx *ctx, salmon_pcnet *net);
+/* No counterpart in the reference.  CG and Gram-Schmidt calls run through a
+ * graph cache (captured on the second call with the same arguments, replayed
+ * after); enable = 0 runs them eagerly.  While a context's stream is being
+ * captured, a HIP call of another host thread that touches the legacy stream
+ * (a plain hipMemcpy, hipStreamCreate) fails, so programs that drive several
+ * contexts from several threads disable the cache on every context. */
+int salmon_gpu_set_graphs(salmon_ctx *ctx, int enable);
 /* record up to max_samples residuals of the following CG calls, every
  * stride-th CG iteration (max_samples = 0 stops and frees the store) */
 int salmon_gpu_pc_record(salmon_ctx *ctx, int max_samples, int stride);
@@ -261,6 +268,7 @@ int salmon_gpu_synchronize(salmon_ctx *ctx);
 void *salmon_gpu_stream(salmon_ctx *ctx);
 
 const char *salmon_gpu_errstr(int code);
+/* with ctx NULL: the message of this thread's last failed salmon_gpu_init */
 const char *salmon_gpu_last_error(const salmon_ctx *ctx);
 int salmon_gpu_finalize(salmon_ctx *ctx);
 

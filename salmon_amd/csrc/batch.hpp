@@ -13,7 +13,7 @@
 namespace salmon {
 namespace batch {
 
-constexpr int kPhases = 4;  // first-block offsets: projection, stencil, fold, observables
+constexpr int kPhases = 5;  // first-block offsets: projection, stencil, fold, observables, KB apply
 
 template <typename T>
 struct Sys {
@@ -23,13 +23,14 @@ struct Sys {
   cplx<T>* h[4];
   cplx<T>* uv;  // (NKB, Nlma)
   cplx<T>* ph;  // (NK, NPI) KB phases at this step's midpoint k
+  cplx<T>* kbv; // (NKB, NT) KB term of every touched point of the current H application
   const T* vloc;
   const double* vloc64;
   const double* kac0;
   const double* occ;
   double* rows;  // (NKB, 8)
   int NK, NB, NKB, zb;
-  int pblk, hblk;            // projection blocks per orbital, stencil blocks per orbital
+  int pblk, hblk, kblk;      // projection, stencil and KB-apply blocks per orbital
   long first[kPhases + 1];   // first block of the system in each phase
 };
 
@@ -87,14 +88,45 @@ __global__ void __launch_bounds__(256) k_b_phase(const Sys<T>* __restrict__ S, S
   y.ph[i] = cplx<T>{(T)c, (T)s};
 }
 
+// KB term of every touched grid point of every orbital (pseudo_pt phase 2,
+// common/hpsi.f90:195-206, gathered through the transposed table): one
+// thread per (orbital, touched point) sums its table entries, so the stencil
+// adds one value per touched point instead of looping over entries with most
+// lanes of the wave idle.
+template <typename T>
+__global__ void __launch_bounds__(256) k_b_kbapply(const Sys<T>* __restrict__ S, Step st) {
+  const long b = blockIdx.x;
+  const int s = find_sys(st, 4, b);
+  const Sys<T>& y = S[s];
+  const long lb = b - y.first[4];
+  const int o = (int)(lb / y.kblk);
+  const int t = (int)(lb % y.kblk) * 256 + threadIdx.x;
+  const KBTables& kb = y.kb;
+  if (o >= y.NKB || t >= kb.NT) return;
+  const cplx<T>* ph = y.ph + (size_t)(o / y.NB) * kb.NPI;
+  const cplx<T>* U = y.uv + (size_t)o * kb.Nlma;
+  T fr = 0, fi = 0;
+  for (int r = kb.t_start[t]; r < kb.t_start[t + 1]; ++r) {
+    const int il = kb.e_lma[r], jp = kb.e_jp[r];
+    const int a = kb.lma_atom[il];
+    const T cs = ph[jp].re, sn = ph[jp].im;
+    const T u = (T)kb.uv[(size_t)il * kb.Nps + (jp - kb.atom_start[a])];
+    // zproj * uVpsi = u (c - i s) (U.re + i U.im)
+    const cplx<T> w = U[il];
+    fr += u * (cs * w.re + sn * w.im);
+    fi += u * (cs * w.im - sn * w.re);
+  }
+  y.kbv[(size_t)o * kb.NT + t] = cplx<T>{fr, fi};
+}
+
 // ZB consecutive z points of one (ix, iy) line: the z neighbours come from a
-// register window of ZB+8 values; the y and x terms, the potential and the KB
-// term are added per point in the order of hpsi_point.
+// register window of ZB+8 values; the y and x terms and the potential are
+// added per point in the order of hpsi_point, then the touched point's KB term
+// (k_b_kbapply).
 template <typename T, int ZB>
 __device__ __forceinline__ void hpsi_zblock(const cplx<T>* __restrict__ E, int p0, double kx, double ky, double kz,
                                             const T* __restrict__ vloc, const Grid& g, const KBTables& kb,
-                                            const cplx<T>* __restrict__ U, cplx<T>* __restrict__ out,
-                                            const cplx<T>* __restrict__ ph) {
+                                            const cplx<T>* __restrict__ kbv, cplx<T>* __restrict__ out) {
   const int NLy = g.NLy, NLz = g.NLz;
   const int iz0 = p0 % NLz, iy = (p0 / NLz) % NLy, ix = p0 / (NLz * NLy);
   const int line = p0 - iz0;
@@ -147,15 +179,9 @@ __device__ __forceinline__ void hpsi_zblock(const cplx<T>* __restrict__ E, int p
     f.im = (V + A) * e.im - (T)0.5 * vi - wr;
     const int t = kb.pt2t[p];
     if (t >= 0) {
-      for (int r = kb.t_start[t]; r < kb.t_start[t + 1]; ++r) {
-        const int il = kb.e_lma[r], jp = kb.e_jp[r];
-        const int a = kb.lma_atom[il];
-        const T cs = ph[jp].re, sn = ph[jp].im;
-        const T u = (T)kb.uv[(size_t)il * kb.Nps + (jp - kb.atom_start[a])];
-        const cplx<T> w = U[il];
-        f.re += u * (cs * w.re + sn * w.im);
-        f.im += u * (cs * w.im - sn * w.re);
-      }
+      const cplx<T> v = kbv[t];
+      f.re += v.re;
+      f.im += v.im;
     }
     out[p] = f;
   }
@@ -178,8 +204,7 @@ __global__ void __launch_bounds__(256) k_b_hpsi(const Sys<T>* __restrict__ S, in
   double kx, ky, kz;
   orbital_k(y.kac0, y.NK, o / y.NB, am[0], am[1], am[2], kx, ky, kz);
   const cplx<T>* in = (term == 0 ? y.zu : y.h[term - 1]) + (size_t)o * y.g.NL;
-  hpsi_zblock<T, ZB>(in, p0, kx, ky, kz, y.vloc, y.g, y.kb, y.uv + (size_t)o * y.kb.Nlma, y.h[term] + (size_t)o * y.g.NL,
-                     y.ph + (size_t)(o / y.NB) * y.kb.NPI);
+  hpsi_zblock<T, ZB>(in, p0, kx, ky, kz, y.vloc, y.g, y.kb, y.kbv + (size_t)o * y.kb.NT, y.h[term] + (size_t)o * y.g.NL);
 }
 
 // zu += sum_n zfac(n) h_n, zfac(n) = (-i dt)^n/n! (ARTED/RT/hamiltonian.f90:46-49,99-122).

@@ -61,7 +61,7 @@ static int batch_build(salmon_batch* b) {
   const int n = (int)b->ctx.size();
   std::vector<Sys<T>> S(n);
   b->max_ph = 0;
-  long acc[kPhases] = {0, 0, 0, 0};
+  long acc[kPhases] = {0, 0, 0, 0, 0};
   for (int i = 0; i < n; ++i) {
     salmon_ctx* c = b->ctx[i];
     Sys<T>& y = S[i];
@@ -84,15 +84,19 @@ static int batch_build(salmon_batch* b) {
     }
     if (hipMalloc((void**)&y.uv, (size_t)c->NKB * std::max(1, c->kb.Nlma) * sizeof(cplx<T>)) != hipSuccess ||
         hipMalloc((void**)&y.ph, (size_t)c->NK * std::max(1, c->kb.NPI) * sizeof(cplx<T>)) != hipSuccess ||
+        hipMalloc((void**)&y.kbv, (size_t)c->NKB * std::max(1, c->kb.NT) * sizeof(cplx<T>)) != hipSuccess ||
         hipMalloc((void**)&y.rows, (size_t)c->NKB * 8 * sizeof(double)) != hipSuccess)
       return bfail(b, SALMON_ERR_NOMEM, "batch buffers (system %d)", i);
     b->bufs.push_back(y.uv);
     b->bufs.push_back(y.ph);
+    b->bufs.push_back(y.kbv);
     b->bufs.push_back(y.rows);
     b->max_ph = std::max(b->max_ph, (long)c->NK * c->kb.NPI);
     y.pblk = (std::max(1, c->kb.Nlma) + 3) / 4;
     y.hblk = (c->g.NL / y.zb + 255) / 256;
-    const long nb[kPhases] = {(long)c->NKB * y.pblk, 0, (long)((nv + 255) / 256), (long)c->NKB};
+    y.kblk = (c->kb.NT + 255) / 256;
+    const long nb[kPhases] = {(long)c->NKB * y.pblk, 0, (long)((nv + 255) / 256), (long)c->NKB,
+                              (long)c->NKB * y.kblk};
     for (int ph = 0; ph < kPhases; ++ph) {
       if (ph == 1) continue;
       y.first[ph] = acc[ph];
@@ -118,7 +122,8 @@ static int batch_build(salmon_batch* b) {
     for (int i = 0; i < n; ++i) {
       const long cnt = ph == 0 ? (long)S[i].NKB * S[i].pblk
                      : ph == 1 ? (long)S[i].NKB * S[i].hblk
-                     : ph == 2 ? (long)(((size_t)S[i].NKB * S[i].g.NL + 255) / 256) : (long)S[i].NKB;
+                     : ph == 2 ? (long)(((size_t)S[i].NKB * S[i].g.NL + 255) / 256)
+                     : ph == 3 ? (long)S[i].NKB : (long)S[i].NKB * S[i].kblk;
       std::fill(own.begin() + S[i].first[ph], own.begin() + S[i].first[ph] + cnt, (unsigned short)i);
     }
     if (hipMalloc((void**)&b->d_owner[ph], sizeof(unsigned short) * std::max<size_t>(1, own.size())) != hipSuccess)
@@ -138,11 +143,13 @@ template <typename T>
 static int batch_enqueue_step(salmon_batch* b, double dt) {
   const int n = (int)b->ctx.size();
   const Sys<T>* S = (const Sys<T>*)b->d_sys;
-  const Step st{b->d_ac, b->d_step, dt, {b->d_owner[0], b->d_owner[1], b->d_owner[2], b->d_owner[3]}};
+  const Step st{b->d_ac, b->d_step, dt, {b->d_owner[0], b->d_owner[1], b->d_owner[2], b->d_owner[3], b->d_owner[4]}};
   if (b->max_ph > 0)
     hipLaunchKernelGGL(k_b_phase<T>, dim3((unsigned)((b->max_ph + 255) / 256), n), dim3(256), 0, b->stream, S, st);
   for (int t = 0; t < 4; ++t) {
     hipLaunchKernelGGL(k_b_project<T>, dim3((unsigned)b->first[0][0]), dim3(256), 0, b->stream, S, n, t, st);
+    if (b->first[4][0] > 0)
+      hipLaunchKernelGGL(k_b_kbapply<T>, dim3((unsigned)b->first[4][0]), dim3(256), 0, b->stream, S, st);
     if (b->zcls[0][1]) hipLaunchKernelGGL((k_b_hpsi<T, 1>), dim3((unsigned)b->zcls[0][1]), dim3(256), 0, b->stream, S, t, st, b->zcls[0][0]);
     if (b->zcls[1][1]) hipLaunchKernelGGL((k_b_hpsi<T, 2>), dim3((unsigned)b->zcls[1][1]), dim3(256), 0, b->stream, S, t, st, b->zcls[1][0]);
     if (b->zcls[2][1]) hipLaunchKernelGGL((k_b_hpsi<T, 4>), dim3((unsigned)b->zcls[2][1]), dim3(256), 0, b->stream, S, t, st, b->zcls[2][0]);
@@ -174,7 +181,7 @@ static int batch_tune_one(salmon_batch* b, int i, int* best) {
   unsigned short* d_zero = nullptr;
   BHIP(b, hipMalloc((void**)&d_zero, sizeof(unsigned short) * zeros.size()));
   BHIP(b, hipMemcpy(d_zero, zeros.data(), sizeof(unsigned short) * zeros.size(), hipMemcpyHostToDevice));
-  const Step st{b->d_ac, b->d_step, 0.0, {d_zero, d_zero, d_zero, d_zero}};
+  const Step st{b->d_ac, b->d_step, 0.0, {d_zero, d_zero, d_zero, d_zero, d_zero}};
   for (int zb : {1, 2, 4, 8}) {
     if (c->g.NLz % zb) continue;
     Sys<T> y = one[0];

@@ -43,6 +43,7 @@ struct salmon_ctx {
   void* d_work[4] = {nullptr, nullptr, nullptr, nullptr};
   void* d_uvpsi = nullptr;
   int uvpsi_nlma = 0;         // projectors d_uvpsi is sized for
+  bool graphs = true;         // CG / Gram-Schmidt through the graph cache
   void* d_ph = nullptr;       // KB phases of the streamed Taylor step (NK, NPI)
   size_t ph_bytes = 0;
   double* d_rows = nullptr;   // (NKB*8)
@@ -374,17 +375,31 @@ const char* salmon_gpu_errstr(int code) {
   }
 }
 
-const char* salmon_gpu_last_error(const salmon_ctx* c) { return c ? c->err.c_str() : "no context"; }
+// message of this thread's last failed salmon_gpu_init (no context holds it)
+static thread_local std::string tl_init_err;
+
+const char* salmon_gpu_last_error(const salmon_ctx* c) {
+  if (c) return c->err.c_str();
+  return tl_init_err.empty() ? "no context" : tl_init_err.c_str();
+}
 
 int salmon_gpu_init(const salmon_grid_t* grid, int precision, salmon_ctx** out) {
   if (!grid || !out) return SALMON_ERR_ARG;
   *out = nullptr;
+  tl_init_err.clear();
   if (grid->NLx < 9 || grid->NLy < 9 || grid->NLz < 9) return SALMON_ERR_ARG;  // Nd=4 stencil needs > 2*Nd points
   if (precision != SALMON_FP64 && precision != SALMON_MIXED) return SALMON_ERR_ARG;
   if (!(grid->aLx > 0 && grid->aLy > 0 && grid->aLz > 0)) return SALMON_ERR_ARG;
   salmon_ctx* c = new salmon_ctx();
   c->precision = precision;
-  if (hipGetDevice(&c->device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess) {
+  hipError_t e = hipGetDevice(&c->device);
+  if (e == hipSuccess) {
+    e = hipStreamCreate(&c->stream);
+    if (e != hipSuccess) tl_init_err = std::string("hipStreamCreate: ") + hipGetErrorString(e);
+  } else {
+    tl_init_err = std::string("hipGetDevice: ") + hipGetErrorString(e);
+  }
+  if (e != hipSuccess) {
     delete c;
     return SALMON_ERR_HIP;
   }
@@ -899,6 +914,13 @@ int salmon_gpu_pcnet_apply(salmon_pcnet* n, salmon_ctx* c, const double* g, doub
   if ((r = pc_pack(c, n)) || (r = pc_infer(c, s, n, s->w[W_G], s->w[W_KG]))) return r;
   HIPCHK(c, hipMemcpyAsync(d, s->w[W_KG], sizeof(cplx<double>) * NL, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
+  return SALMON_OK;
+}
+
+int salmon_gpu_set_graphs(salmon_ctx* c, int enable) {
+  if (!c) return SALMON_ERR_ARG;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  c->graphs = enable != 0;
   return SALMON_OK;
 }
 

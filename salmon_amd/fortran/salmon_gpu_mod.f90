@@ -181,6 +181,12 @@ module salmon_gpu
       complex(c_double_complex), intent(out) :: d(*)
     end function
 
+    integer(c_int) function salmon_gpu_set_graphs(ctx, enable) bind(C, name="salmon_gpu_set_graphs")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: enable
+    end function
+
     integer(c_int) function salmon_gpu_set_pcnet(ctx, net) bind(C, name="salmon_gpu_set_pcnet")
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx, net

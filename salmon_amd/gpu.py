@@ -78,6 +78,7 @@ SIGNATURES = {
     "salmon_gpu_pcnet_get_weights": [_vp, _vp],
     "salmon_gpu_pcnet_finalize": [_vp],
     "salmon_gpu_pcnet_apply": [_vp, _vp, _vp, _vp],
+    "salmon_gpu_set_graphs": [_vp, _ci],
     "salmon_gpu_set_pcnet": [_vp, _vp],
     "salmon_gpu_pc_record": [_vp, _ci, _ci],
     "salmon_gpu_pc_samples": [_vp, _vp, _vp],
@@ -151,7 +152,8 @@ class Context:
             setattr(g, name, (ctypes.c_double * 5)(*[float(v) for v in getattr(sys, name)]))
         rc = lib().salmon_gpu_init(ctypes.byref(g), int(precision), ctypes.byref(self._h))
         if rc != 0:
-            raise SalmonGPUError(f"salmon_gpu_init: {lib().salmon_gpu_errstr(rc).decode()}")
+            raise SalmonGPUError(f"salmon_gpu_init: {lib().salmon_gpu_errstr(rc).decode()} "
+                                 f"({lib().salmon_gpu_last_error(None).decode()})")
         self._check(lib().salmon_gpu_set_potential(self._h, _p(_f64(sys.vloc))), "set_potential")
         keep = [_i32(sys.Mps), _i32(sys.Jxyz), _i32(sys.Jxx), _i32(sys.Jyy), _i32(sys.Jzz), _i32(sys.a_tbl),
                 _f64(sys.uV), _i32(sys.iuV)]
@@ -299,6 +301,11 @@ class Context:
         st, va = ctypes.c_int(), ctypes.c_int()
         self._check(lib().salmon_gpu_pc_samples(self._h, ctypes.byref(st), ctypes.byref(va)), "pc_samples")
         return st.value, va.value
+
+    def set_graphs(self, enable):
+        """CG / Gram-Schmidt through the graph cache (default) or eagerly; a
+        program driving several contexts from several threads turns it off."""
+        self._check(lib().salmon_gpu_set_graphs(self._h, int(bool(enable))), "set_graphs")
 
     def pc_get_sample(self, i):
         g = np.zeros(self.sys.NL, np.complex128)

@@ -48,6 +48,7 @@ def prepare(seed, nb=NB, target_sweeps=30, record_sweeps=3, stride=3):
     """Context of one training system with its targets and recorded samples."""
     s = synth.config5(seed=seed, NB=nb)
     c = gpu.Context(s, gpu.FP64)
+    c.set_graphs(False)  # contexts are driven from several threads at once
     c.ground_state(s.kAc0, target_sweeps, iter_cg_max=NCG, precond=gpu.PC_KINETIC, tol=0.0)
     c.pc_set_targets(s.kAc0)
     c.put_orbitals(s.zu)
@@ -120,6 +121,7 @@ def evaluate(net, n_eval=50, acc=1e-8, max_sweeps=120, nb=NB, log=_log):
     def one(i):
         s = synth.config5(seed=EVAL_SEED0 + i, NB=nb)
         with gpu.Context(s, gpu.FP64) as c:
+            c.set_graphs(False)  # contexts are driven from several threads at once
             rk = c.ground_state(s.kAc0, max_sweeps, iter_cg_max=NCG, precond=gpu.PC_KINETIC, tol=1e-12)
             ref = rk["esp"]
             c.put_orbitals(s.zu)

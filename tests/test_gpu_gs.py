@@ -167,3 +167,12 @@ def test_cg_and_gram_schmidt_graph_replays_match_eager():
             c.sp_energy(s.kAc0)  # eager work between replays
         for o in outs[1:]:
             assert all(np.array_equal(a, b) for a, b in zip(outs[0], o))
+    # with the cache off every call is eager, with the same results
+    with _ctx(s) as c:
+        c.set_graphs(False)
+        for rep in range(3):
+            c.put_orbitals(s.zu)
+            c.gram_schmidt()
+            zg = c.get_orbitals()
+            ev, it = c.cg(s.kAc0, 3, gpu.PC_KINETIC)
+            assert all(np.array_equal(a, b) for a, b in zip(outs[0], (zg, ev, it, c.get_orbitals())))
