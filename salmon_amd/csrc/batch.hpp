@@ -71,10 +71,11 @@ __global__ void __launch_bounds__(256) k_b_project(const Sys<T>* __restrict__ S,
                      y.ph + (size_t)(o / y.NB) * y.kb.NPI);
 }
 
-// The KB phases of every system's k points at this step's midpoint k (one
-// grid row per system), shared by the step's four projections and stencils.
+// The KB phases of every system's k points (one grid row per system) at this
+// step's midpoint k, shared by the step's four projections and stencils, or
+// (at_new) at k + A(t+dt) for the observables.
 template <typename T>
-__global__ void __launch_bounds__(256) k_b_phase(const Sys<T>* __restrict__ S, Step st) {
+__global__ void __launch_bounds__(256) k_b_phase(const Sys<T>* __restrict__ S, Step st, int at_new) {
   const Sys<T>& y = S[blockIdx.y];
   const long i = (long)blockIdx.x * 256 + threadIdx.x;
   if (i >= (long)y.NK * y.kb.NPI) return;
@@ -82,7 +83,8 @@ __global__ void __launch_bounds__(256) k_b_phase(const Sys<T>* __restrict__ S, S
   double am[3], an[3];
   step_fields(st, am, an);
   double kx, ky, kz;
-  orbital_k(y.kac0, y.NK, ik, am[0], am[1], am[2], kx, ky, kz);
+  const double* a = at_new ? an : am;
+  orbital_k(y.kac0, y.NK, ik, a[0], a[1], a[2], kx, ky, kz);
   double s, c;
   sincos(kx * y.kb.pos[3 * jp] + ky * y.kb.pos[3 * jp + 1] + kz * y.kb.pos[3 * jp + 2], &s, &c);
   y.ph[i] = cplx<T>{(T)c, (T)s};
@@ -241,7 +243,8 @@ __global__ void __launch_bounds__(256) k_b_observe(const Sys<T>* __restrict__ S,
   step_fields(st, am, an);
   double kx, ky, kz;
   orbital_k(y.kac0, y.NK, o / y.NB, an[0], an[1], an[2], kx, ky, kz);
-  observe_orbital<T>(y.zu + (size_t)o * y.g.NL, kx, ky, kz, y.occ[o], y.vloc64, y.g, y.kb, y.rows + (size_t)o * 8);
+  observe_orbital<T>(y.zu + (size_t)o * y.g.NL, kx, ky, kz, y.occ[o], y.vloc64, y.g, y.kb, y.rows + (size_t)o * 8,
+                     y.ph + (size_t)(o / y.NB) * y.kb.NPI);
 }
 
 // Per-system sums of the rows (one block per system) into res(step, s, 0..7).

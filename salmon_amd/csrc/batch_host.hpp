@@ -145,7 +145,7 @@ static int batch_enqueue_step(salmon_batch* b, double dt) {
   const Sys<T>* S = (const Sys<T>*)b->d_sys;
   const Step st{b->d_ac, b->d_step, dt, {b->d_owner[0], b->d_owner[1], b->d_owner[2], b->d_owner[3], b->d_owner[4]}};
   if (b->max_ph > 0)
-    hipLaunchKernelGGL(k_b_phase<T>, dim3((unsigned)((b->max_ph + 255) / 256), n), dim3(256), 0, b->stream, S, st);
+    hipLaunchKernelGGL(k_b_phase<T>, dim3((unsigned)((b->max_ph + 255) / 256), n), dim3(256), 0, b->stream, S, st, 0);
   for (int t = 0; t < 4; ++t) {
     hipLaunchKernelGGL(k_b_project<T>, dim3((unsigned)b->first[0][0]), dim3(256), 0, b->stream, S, n, t, st);
     if (b->first[4][0] > 0)
@@ -156,6 +156,9 @@ static int batch_enqueue_step(salmon_batch* b, double dt) {
     if (b->zcls[3][1]) hipLaunchKernelGGL((k_b_hpsi<T, 8>), dim3((unsigned)b->zcls[3][1]), dim3(256), 0, b->stream, S, t, st, b->zcls[3][0]);
   }
   hipLaunchKernelGGL(k_b_fold<T>, dim3((unsigned)b->first[2][0]), dim3(256), 0, b->stream, S, n, st);
+  // the observables' phases at k + A(t+dt) (the step's projections are done)
+  if (b->max_ph > 0)
+    hipLaunchKernelGGL(k_b_phase<T>, dim3((unsigned)((b->max_ph + 255) / 256), n), dim3(256), 0, b->stream, S, st, 1);
   hipLaunchKernelGGL(k_b_observe<T>, dim3((unsigned)b->first[3][0]), dim3(256), 0, b->stream, S, n, st);
   hipLaunchKernelGGL(k_b_sum<T>, dim3(n), dim3(256), 0, b->stream, S, n, b->d_step, b->d_res);
   hipLaunchKernelGGL(k_b_tick, dim3(1), dim3(64), 0, b->stream, b->d_step);
@@ -191,7 +194,7 @@ static int batch_tune_one(salmon_batch* b, int i, int* best) {
     BHIP(b, hipMemcpy(d1, &y, sizeof(Sys<T>), hipMemcpyHostToDevice));
     if ((long)y.NK * y.kb.NPI > 0)
       hipLaunchKernelGGL(k_b_phase<T>, dim3((unsigned)(((long)y.NK * y.kb.NPI + 255) / 256), 1), dim3(256), 0, b->stream,
-                         (const Sys<T>*)d1, st);
+                         (const Sys<T>*)d1, st, 0);
     const unsigned nblk = (unsigned)((long)y.NKB * y.hblk);
     float tmin = 1e30f;
     for (int rep = 0; rep < 4; ++rep) {

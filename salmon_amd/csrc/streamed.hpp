@@ -209,49 +209,55 @@ __global__ void __launch_bounds__(256) k_taylor_fold(cplx<T>* __restrict__ zu, c
 template <typename T>
 __device__ __forceinline__ void observe_orbital(const cplx<T>* __restrict__ E, double kx, double ky, double kz,
                                                 double occ, const double* __restrict__ vloc, const Grid& g,
-                                                const KBTables& kb, double* __restrict__ r) {
+                                                const KBTables& kb, double* __restrict__ r,
+                                                const cplx<T>* __restrict__ ph = nullptr) {
   __shared__ double smem[4 * 8];
   __shared__ double nl[4][4];
   const double kk[3] = {kx, ky, kz};
   const double A0 = -(g.lap[0][0] + g.lap[1][0] + g.lap[2][0]) * 0.5;
   const int NLx = g.NLx, NLy = g.NLy, NLz = g.NLz;
+  // Per point the stencil's neighbour sums, in the orbitals' precision:
+  //   L = sum_a sum_n lap_n (psi+ + psi-),  G_a = sum_n nab_n (psi+ - psi-);
+  // Re conj(psi) L gives the kinetic term of total_energy_stencil
+  // (total_energy.f90:172-190), Im conj(psi) G_a the stencil current; summed
+  // over the periodic grid the central sum is twice the forward sum of
+  // current_stencil (stencil/F90/current.f90:112-114), and the -i k.grad
+  // energy is sum_a k_a current_a.  Products and sums in FP64.
   double n2 = 0, vs = 0, te = 0, jc[3] = {0, 0, 0};
   for (int p = threadIdx.x; p < g.NL; p += 256) {
     const int iz = p % NLz, iy = (p / NLz) % NLy, ix = p / (NLz * NLy);
-    const double er = (double)E[p].re, ei = (double)E[p].im;
+    const cplx<T> e0 = E[p];
+    const double er = (double)e0.re, ei = (double)e0.im;
     const double a2 = er * er + ei * ei;
     n2 += a2;
     vs += vloc[p] * a2;
-    double vr = 0, vi = 0, wr = 0, wi = 0;
+    T Lr = 0, Li = 0, Gr[3] = {0, 0, 0}, Gi[3] = {0, 0, 0};
 #pragma unroll
     for (int ax3 = 2; ax3 >= 0; --ax3) {
       const int n_ax = ax3 == 2 ? NLz : (ax3 == 1 ? NLy : NLx);
       const int c_ax = ax3 == 2 ? iz : (ax3 == 1 ? iy : ix);
       const int stride = ax3 == 2 ? 1 : (ax3 == 1 ? NLz : NLz * NLy);
       const int base = p - c_ax * stride;
-      double fr = 0, fi = 0;
 #pragma unroll
       for (int n = 1; n <= 4; ++n) {
         int cp = c_ax + n;
         if (cp >= n_ax) cp -= n_ax;
         int cm = c_ax - n;
         if (cm < 0) cm += n_ax;
-        const double pr = (double)E[base + cp * stride].re, pi = (double)E[base + cp * stride].im;
-        const double mr = (double)E[base + cm * stride].re, mi = (double)E[base + cm * stride].im;
-        const double nb = g.nab[ax3][n], C = g.lap[ax3][n], D = kk[ax3] * nb;
-        fr += nb * pr;
-        fi += nb * pi;
-        vr += C * (pr + mr);
-        vi += C * (pi + mi);
-        wr += D * (pr - mr);
-        wi += D * (pi - mi);
+        const cplx<T> ep = E[base + cp * stride], em = E[base + cm * stride];
+        const T C = (T)g.lap[ax3][n], nb = (T)g.nab[ax3][n];
+        Lr += C * (ep.re + em.re);
+        Li += C * (ep.im + em.im);
+        Gr[ax3] += nb * (ep.re - em.re);
+        Gi[ax3] += nb * (ep.im - em.im);
       }
-      jc[ax3] += er * fi - ei * fr;  // Im(conj(psi) * f)
     }
-    // z = A0 psi - 0.5 v - i w ; Re(conj(psi) z)
-    const double zr = A0 * er - 0.5 * vr + wi, zi = A0 * ei - 0.5 * vi - wr;
-    te += er * zr + ei * zi;
+    te += er * (double)Lr + ei * (double)Li;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) jc[d] += er * (double)Gi[d] - ei * (double)Gr[d];
   }
+  // Re conj(psi) (A0 psi - L/2 - i sum_a k_a G_a) summed = A0 n2 - te/2 + sum_a k_a jc_a
+  te = -0.5 * te;
   double v[6] = {n2, vs, te, jc[0], jc[1], jc[2]};
   block_sum256<6>(v, smem);
   // nonlocal: wave w takes projectors w, w+4, ...
@@ -265,7 +271,12 @@ __device__ __forceinline__ void observe_orbital(const cplx<T>* __restrict__ E, d
       const int jp = j0 + j;
       const double px = kb.pos[3 * jp], py = kb.pos[3 * jp + 1], pz = kb.pos[3 * jp + 2];
       double s, c;
-      sincos(kx * px + ky * py + kz * pz, &s, &c);
+      if (ph) {
+        c = (double)ph[jp].re;
+        s = (double)ph[jp].im;
+      } else {
+        sincos(kx * px + ky * py + kz * pz, &s, &c);
+      }
       const double u = kb.uv[(size_t)il * kb.Nps + j];
       const cplx<T> e = E[kb.pidx[jp]];
       const double tr = u * (c * (double)e.re - s * (double)e.im);
@@ -296,8 +307,9 @@ __device__ __forceinline__ void observe_orbital(const cplx<T>* __restrict__ E, d
     for (int d = 0; d < 3; ++d) J[d] = (nl[0][d] + nl[1][d]) + (nl[2][d] + nl[3][d]);
     Enl = (nl[0][3] + nl[1][3]) + (nl[2][3] + nl[3][3]);
     const double s = g.Hxyz / g.aLxyz;
-    for (int d = 0; d < 3; ++d) r[d] = (occ * kk[d] * v[0] + 2.0 * v[3 + d] * occ) * s + J[d];
-    r[3] = occ * v[2] * g.Hxyz + occ * (kx * kx + ky * ky + kz * kz) / 2.0;
+    for (int d = 0; d < 3; ++d) r[d] = (occ * kk[d] * v[0] + v[3 + d] * occ) * s + J[d];
+    r[3] = occ * (A0 * v[0] + v[2] + kx * v[3] + ky * v[4] + kz * v[5]) * g.Hxyz +
+           occ * (kx * kx + ky * ky + kz * kz) / 2.0;
     r[4] = Enl;
     r[5] = occ * v[1] * g.Hxyz;
     r[6] = occ * v[0] * g.Hxyz;
