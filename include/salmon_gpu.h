@@ -259,6 +259,17 @@ enum { SALMON_PATH_AUTO = 0, SALMON_PATH_STREAMED = 1, SALMON_PATH_FUSED = 2 };
 int salmon_gpu_set_path(salmon_ctx *ctx, int path);
 int salmon_gpu_path(salmon_ctx *ctx);
 
+/* Propagator of salmon_gpu_rt_run: the input variable propagator and its
+ * dispatch in dt_evolve_KB (ARTED/RT/dt_evolve.f90:28-59).  MIDPOINT: one
+ * Taylor-4 step over dt at kAc0 + (A_n + A_{n+1})/2 (the default).  ETRS
+ * (dt_evolve_etrs_omp_KB, :218-348, with the frozen potential, whose
+ * extrapolation 3V - 3V_old1 + V_old2 is then V): Taylor-4 over dt/2 at
+ * kAc0 + A_n, then over dt/2 at kAc0 + A_{n+1}.  Observables are at
+ * kAc0 + A_{n+1} either way.  A Fortran host running the reference's own
+ * dt_evolve_KB reaches both through the hamiltonian drop-in instead. */
+enum { SALMON_PROP_MIDPOINT = 0, SALMON_PROP_ETRS = 1 };
+int salmon_gpu_set_propagator(salmon_ctx *ctx, int propagator);
+
 /* Waits for all work of the context. */
 int salmon_gpu_synchronize(salmon_ctx *ctx);
 

@@ -569,6 +569,9 @@ __global__ void __launch_bounds__(NY * NZ * XS, WPE) k_fused_step(FusedArgs<T> a
       out[p] = acc[i];
       win[H + i] = acc[i];
     }
+    // its observables unless the launch is the first half of an ETRS step
+    // (a uniform branch: the barriers inside are taken by all threads or none)
+    if (a.obs) {
     __syncthreads();
     if constexpr (XS > 1) {
 #pragma unroll
@@ -673,6 +676,7 @@ __global__ void __launch_bounds__(NY * NZ * XS, WPE) k_fused_step(FusedArgs<T> a
         }
       }
     }
+    }  // a.obs
     }
   }
   if constexpr (LT) {

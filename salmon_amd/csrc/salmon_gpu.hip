@@ -44,6 +44,7 @@ struct salmon_ctx {
   void* d_uvpsi = nullptr;
   int uvpsi_nlma = 0;         // projectors d_uvpsi is sized for
   bool graphs = true;         // CG / Gram-Schmidt through the graph cache
+  int propagator = SALMON_PROP_MIDPOINT;  // of rt_run
   void* d_ph = nullptr;       // KB phases of the streamed Taylor step (NK, NPI)
   size_t ph_bytes = 0;
   double* d_rows = nullptr;   // (NKB*8)
@@ -306,12 +307,23 @@ int gram_schmidt_multi_fwd(salmon_ctx* c, cplx<double>* zu) {
 
 namespace {
 
-// One RT step on resident orbitals (streamed path): Taylor-4 at kAc0 + amid,
-// density, then observables at kAc0 + anew into res[0..7].
+// One RT step n on resident orbitals (streamed path), A(t_n) = a0,
+// A(t_{n+1}) = a1: middlepoint (dt_evolve.f90:43-51) Taylor-4 at
+// kAc0 + (a0 + a1)/2, or ETRS (dt_evolve.f90:53-59,218-348 with a frozen
+// potential, whose extrapolation 3V - 3V_old + V_old2 is then V) Taylor-4 over
+// dt/2 at kAc0 + a0 and again at kAc0 + a1; then the density and the
+// observables at kAc0 + a1 into res[0..7].
 template <typename T>
-int rt_step(salmon_ctx* c, const double amid[3], const double anew[3], double dt, double* res_dev) {
+int rt_step(salmon_ctx* c, const double a0[3], const double a1[3], double dt, double* res_dev) {
   int r;
-  if ((r = taylor_streamed<T>(c, c->d_kac0, amid[0], amid[1], amid[2], dt))) return r;
+  const double* anew = a1;
+  if (c->propagator == SALMON_PROP_ETRS) {
+    if ((r = taylor_streamed<T>(c, c->d_kac0, a0[0], a0[1], a0[2], 0.5 * dt))) return r;
+    if ((r = taylor_streamed<T>(c, c->d_kac0, a1[0], a1[1], a1[2], 0.5 * dt))) return r;
+  } else {
+    const double amid[3] = {0.5 * (a0[0] + a1[0]), 0.5 * (a0[1] + a1[1]), 0.5 * (a0[2] + a1[2])};
+    if ((r = taylor_streamed<T>(c, c->d_kac0, amid[0], amid[1], amid[2], dt))) return r;
+  }
   if ((r = density_dev<T>(c))) return r;
   if ((r = observe_rows<T>(c, c->d_kac0, anew[0], anew[1], anew[2]))) return r;
   hipLaunchKernelGGL(k_sum_rows, dim3(1), dim3(256), 0, c->stream, c->d_rows, c->NKB, res_dev);
@@ -328,15 +340,25 @@ int rt_step(salmon_ctx* c, const double amid[3], const double anew[3], double dt
 template <typename T>
 int rt_run_fused(salmon_ctx* c, const double* ac, int nsteps, double dt, int reortho_every, double* d_res) {
   int r = SALMON_OK;
+  const bool etrs = c->propagator == SALMON_PROP_ETRS;
+  // the Taylor launches of step n: middlepoint, one at (A_n + A_{n+1})/2 over
+  // dt; ETRS, one at A_n and one at A_{n+1}, each over dt/2 (see rt_step)
+  auto kpoint = [&](int n, int part, double a[3]) {
+    for (int d = 0; d < 3; ++d)
+      a[d] = etrs ? ac[3 * (n + part) + d] : 0.5 * (ac[3 * n + d] + ac[3 * (n + 1) + d]);
+  };
+  const double dts = etrs ? 0.5 * dt : dt;
   if constexpr (!salmon::fused_defer<T, kFusedXS64>()) {
-    // observables of psi(t+dt) in the launch of the step itself
+    // observables of psi(t+dt) in the (last) launch of the step itself
     for (int n = 0; n < nsteps && r == SALMON_OK; ++n) {
-      double amid[3], anew[3];
-      for (int d = 0; d < 3; ++d) {
-        amid[d] = 0.5 * (ac[3 * n + d] + ac[3 * (n + 1) + d]);
-        anew[d] = ac[3 * (n + 1) + d];
+      double a[3], anew[3];
+      for (int d = 0; d < 3; ++d) anew[d] = ac[3 * (n + 1) + d];
+      if (etrs) {
+        kpoint(n, 0, a);
+        if ((r = fused_step<T>(c, a, nullptr, dts, nullptr))) break;
       }
-      r = fused_step<T>(c, amid, anew, dt, d_res + 8 * (size_t)n);
+      kpoint(n, 1, a);
+      r = fused_step<T>(c, a, anew, dts, d_res + 8 * (size_t)n);
       if (r == SALMON_OK && reortho_every > 0 && (n + 1) % reortho_every == 0) r = gram_schmidt_dev(c);
     }
     return r;
@@ -344,9 +366,13 @@ int rt_run_fused(salmon_ctx* c, const double* ac, int nsteps, double dt, int reo
   double aobs[3];
   bool pending = false;
   for (int n = 0; n < nsteps && r == SALMON_OK; ++n) {
-    double amid[3];
-    for (int d = 0; d < 3; ++d) amid[d] = 0.5 * (ac[3 * n + d] + ac[3 * (n + 1) + d]);
-    r = fused_step<T>(c, amid, pending ? aobs : nullptr, dt, pending ? d_res + 8 * (size_t)(n - 1) : nullptr);
+    double a[3];
+    kpoint(n, 0, a);
+    r = fused_step<T>(c, a, pending ? aobs : nullptr, dts, pending ? d_res + 8 * (size_t)(n - 1) : nullptr);
+    if (r == SALMON_OK && etrs) {
+      kpoint(n, 1, a);
+      r = fused_step<T>(c, a, nullptr, dts, nullptr);
+    }
     for (int d = 0; d < 3; ++d) aobs[d] = ac[3 * (n + 1) + d];
     pending = true;
     if (r == SALMON_OK && reortho_every > 0 && (n + 1) % reortho_every == 0) {
@@ -917,6 +943,14 @@ int salmon_gpu_pcnet_apply(salmon_pcnet* n, salmon_ctx* c, const double* g, doub
   return SALMON_OK;
 }
 
+int salmon_gpu_set_propagator(salmon_ctx* c, int propagator) {
+  if (!c) return SALMON_ERR_ARG;
+  if (propagator != SALMON_PROP_MIDPOINT && propagator != SALMON_PROP_ETRS)
+    return fail(c, SALMON_ERR_ARG, "propagator %d is neither SALMON_PROP_MIDPOINT nor SALMON_PROP_ETRS", propagator);
+  c->propagator = propagator;
+  return SALMON_OK;
+}
+
 int salmon_gpu_set_graphs(salmon_ctx* c, int enable) {
   if (!c) return SALMON_ERR_ARG;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
@@ -1058,13 +1092,10 @@ int salmon_gpu_rt_run(salmon_ctx* c, const double* ac, int nsteps, double dt, in
                                     : rt_run_fused<float>(c, ac, nsteps, dt, reortho_every, d_res);
   }
   for (int n = 0; n < nsteps && r == SALMON_OK && !fused_ready(c); ++n) {
-    double amid[3], anew[3];
-    for (int d = 0; d < 3; ++d) {
-      amid[d] = 0.5 * (ac[3 * n + d] + ac[3 * (n + 1) + d]);
-      anew[d] = ac[3 * (n + 1) + d];
-    }
-    r = c->precision == SALMON_FP64 ? rt_step<double>(c, amid, anew, dt, d_res + 8 * (size_t)n)
-                                    : rt_step<float>(c, amid, anew, dt, d_res + 8 * (size_t)n);
+    const double* a0 = ac + 3 * n;
+    const double* a1 = ac + 3 * (n + 1);
+    r = c->precision == SALMON_FP64 ? rt_step<double>(c, a0, a1, dt, d_res + 8 * (size_t)n)
+                                    : rt_step<float>(c, a0, a1, dt, d_res + 8 * (size_t)n);
     if (r == SALMON_OK && reortho_every > 0 && (n + 1) % reortho_every == 0) r = gram_schmidt_dev(c);
   }
   if (r == SALMON_OK) {

@@ -20,6 +20,8 @@ module salmon_gpu
   integer(c_int), parameter :: SALMON_PC_NONE = 0
   integer(c_int), parameter :: SALMON_PC_KINETIC = 1
   integer(c_int), parameter :: SALMON_PC_LEARNED = 2
+  integer(c_int), parameter :: SALMON_PROP_MIDPOINT = 0
+  integer(c_int), parameter :: SALMON_PROP_ETRS = 1
 
   type, bind(C) :: salmon_grid_t
     integer(c_int) :: NLx, NLy, NLz
@@ -179,6 +181,12 @@ module salmon_gpu
       type(c_ptr), value :: net, ctx
       complex(c_double_complex), intent(in) :: g(*)
       complex(c_double_complex), intent(out) :: d(*)
+    end function
+
+    integer(c_int) function salmon_gpu_set_propagator(ctx, propagator) bind(C, name="salmon_gpu_set_propagator")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: propagator
     end function
 
     integer(c_int) function salmon_gpu_set_graphs(ctx, enable) bind(C, name="salmon_gpu_set_graphs")

@@ -38,6 +38,7 @@ LIB_PATH = os.path.join(HERE, "lib", "libsalmon_gpu.so")
 FP64 = 0
 MIXED = 1
 PATH_AUTO, PATH_STREAMED, PATH_FUSED = 0, 1, 2
+PROP_MIDPOINT, PROP_ETRS = 0, 1
 PC_NONE, PC_KINETIC, PC_LEARNED = 0, 1, 2
 PCNET_WEIGHTS = 27 * 2 * 16 + 27 * 16 * 16 + 27 * 16 * 2
 
@@ -79,6 +80,7 @@ SIGNATURES = {
     "salmon_gpu_pcnet_finalize": [_vp],
     "salmon_gpu_pcnet_apply": [_vp, _vp, _vp, _vp],
     "salmon_gpu_set_graphs": [_vp, _ci],
+    "salmon_gpu_set_propagator": [_vp, _ci],
     "salmon_gpu_set_pcnet": [_vp, _vp],
     "salmon_gpu_pc_record": [_vp, _ci, _ci],
     "salmon_gpu_pc_samples": [_vp, _vp, _vp],
@@ -301,6 +303,12 @@ class Context:
         st, va = ctypes.c_int(), ctypes.c_int()
         self._check(lib().salmon_gpu_pc_samples(self._h, ctypes.byref(st), ctypes.byref(va)), "pc_samples")
         return st.value, va.value
+
+    def set_propagator(self, propagator):
+        """PROP_MIDPOINT (default) or PROP_ETRS for rt_run (dt_evolve.f90:28-59)."""
+        if isinstance(propagator, str):
+            propagator = {"middlepoint": PROP_MIDPOINT, "etrs": PROP_ETRS}[propagator]
+        self._check(lib().salmon_gpu_set_propagator(self._h, int(propagator)), "set_propagator")
 
     def set_graphs(self, enable):
         """CG / Gram-Schmidt through the graph cache (default) or eagerly; a

@@ -138,6 +138,63 @@ def test_fused_run_split_matches_single_run(prec):
     assert np.array_equal(r2["rho"], r["rho"])
 
 
+def _etrs_oracle(s, ac, nsteps):
+    """ETRS with the frozen potential (dt_evolve.f90:218-348): Taylor-4 over
+    dt/2 at kAc0 + A_n, then over dt/2 at kAc0 + A_{n+1} (the oracle's
+    hamiltonian, pinned against the reference's); J and E at kAc0 + A_{n+1}."""
+    o = Oracle(s)
+    z = s.zu.copy()
+    J, E = [], []
+    for n in range(nsteps):
+        z = o.hamiltonian(kac_at(s, ac[n]), z, 0.5 * s.dt)
+        z = o.hamiltonian(kac_at(s, ac[n + 1]), z, 0.5 * s.dt)
+        k1 = kac_at(s, ac[n + 1])
+        J.append(o.current(k1, z))
+        E.append(sum(o.energy(k1, z)))
+    return z, np.array(J), np.array(E)
+
+
+@pytest.mark.parametrize("name,path", [("cfg1", gpu.PATH_AUTO), ("small_k", gpu.PATH_AUTO),
+                                       ("cfg1", gpu.PATH_STREAMED)])
+def test_etrs_matches_oracle(name, path):
+    """The ETRS propagator on the fused (16^3) and the streamed paths, FP64,
+    against the oracle's two half steps per step."""
+    s, d = load_case(name)
+    nsteps = 4
+    ac = synth.acos2_pulse(nsteps + 1, s.dt, 0.05, 0.3, 2.0)
+    zr, Jr, Er = _etrs_oracle(s, ac, nsteps)
+    with _ctx(s) as c:
+        c.set_path(path)
+        c.set_propagator("etrs")
+        r = c.rt_run(ac, nsteps)
+        z = c.get_orbitals()
+    print(f"[gpu] etrs {name} path={path}: zu {rel(z, zr):.2e} J {rel(r['J'], Jr):.2e} E {rel(r['E'], Er):.2e}",
+          flush=True)
+    assert rel(z, zr) < 1e-12
+    assert rel(r["J"], Jr) < 1e-10
+    assert rel(r["E"], Er) < 1e-11
+
+
+def test_etrs_mixed_close_to_fp64_and_differs_from_midpoint():
+    s, d = load_case("cfg1")
+    nsteps = 20
+    ac = synth.acos2_pulse(nsteps + 1, s.dt, 0.05, 0.3, 2.0)
+    out = {}
+    for prec, prop in ((gpu.FP64, "etrs"), (gpu.MIXED, "etrs"), (gpu.FP64, "middlepoint")):
+        with _ctx(s, prec) as c:
+            c.set_propagator(prop)
+            out[(prec, prop)] = c.rt_run(ac, nsteps)
+    J64 = out[(gpu.FP64, "etrs")]["J"]
+    jerr = np.max(np.abs(out[(gpu.MIXED, "etrs")]["J"] - J64)) / np.max(np.abs(J64))
+    dmid = np.max(np.abs(out[(gpu.FP64, "middlepoint")]["J"] - J64)) / np.max(np.abs(J64))
+    print(f"[gpu] etrs mixed J err {jerr:.2e}; etrs vs middlepoint {dmid:.2e}", flush=True)
+    assert jerr < 1e-5
+    assert dmid > 1e-8  # a different propagator, not the same call twice
+    with gpu.Context(s, gpu.FP64) as c:
+        with pytest.raises(gpu.SalmonGPUError, match="propagator"):
+            c.set_propagator(7)
+
+
 def test_mixed_hpsi_close_to_fp64():
     s, d = load_case("small_k")
     with _ctx(s, gpu.MIXED) as c:
