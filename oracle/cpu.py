@@ -61,6 +61,9 @@ def lib():
         L.or_kinetic_precondition.argtypes = [P, ctypes.c_double, vp]
         L.or_rt_run.argtypes = [P, vp, vp, ctypes.c_int, ctypes.c_double, vp, vp, vp, vp, vp, ctypes.c_int]
         L.or_num_threads.restype = ctypes.c_int
+        ci, cd = ctypes.c_int, ctypes.c_double
+        L.or_hartree.argtypes = [ci, ci, ci, cd, cd, cd, vp, vp]
+        L.or_pz.argtypes = [ci, vp, vp, vp]
         _lib = L
     return _lib
 
@@ -190,6 +193,22 @@ class Oracle:
         if want_rho:
             out["rho"] = rho
         return out
+
+
+def hartree(sys, rho):
+    """V_H of the density rho(NL) on the system's orthorhombic cell."""
+    rho = np.ascontiguousarray(rho, dtype=np.float64)
+    vh = np.zeros_like(rho)
+    lib().or_hartree(sys.NLx, sys.NLy, sys.NLz, float(sys.aLx), float(sys.aLy), float(sys.aLz), _p(rho), _p(vh))
+    return vh
+
+
+def pz(rho):
+    """(v_xc, e_xc) of the spin-unpolarised PZ81 LDA at the density rho."""
+    rho = np.ascontiguousarray(rho, dtype=np.float64)
+    v, e = np.zeros_like(rho), np.zeros_like(rho)
+    lib().or_pz(rho.size, _p(rho), _p(v), _p(e))
+    return v, e
 
 
 def midpoint_kac(sys, ac, n):

@@ -19,6 +19,9 @@
  *   or_cg                ARTED/GS/CG.f90:34-139 (CG_ompk, skip_gsortho='n');
  *                        the optional kinetic preconditioner has no reference
  *                        counterpart
+ *   or_hartree           ARTED/common/Hartree.f90:20-128 (orthorhombic cell)
+ *   or_pz                src/xc/builtin_pz.f90:24-80 (exc_cor_pz / PZxc) as
+ *                        called by src/xc/salmon_xc.f90:499-511 (rho_s = rho/2)
  *   or_rt_run            ARTED/control/control_sc.f90:150-222 +
  *                        ARTED/RT/dt_evolve.f90:43-51,106-216 (middlepoint,
  *                        frozen potential: no_update_func='y')
@@ -764,6 +767,82 @@ void or_rt_run(const or_system *s, const double *kAc0, const double *ac, int nst
   }
   if (!rho) free(rtmp);
   free(kAc);
+}
+
+/* Hartree potential of a density on the periodic grid: rho(G) = sum_r rho(r)
+ * e^{-iG.r} / NL by one DFT pass per axis (z, y, x as the reference),
+ * V_H(G) = 4 pi rho(G) / |G|^2 with the G = 0 term dropped, V_H(r) =
+ * sum_G V_H(G) e^{iG.r}; G_a = 2 pi n_a / aL_a, n_a in -N_a/2 .. N_a-1-N_a/2. */
+static void or_dft_axis(int NLx, int NLy, int NLz, int axis, int sgn, const zc *in, zc *out)
+{
+  const int N = axis == 0 ? NLx : (axis == 1 ? NLy : NLz);
+  const int es = axis == 0 ? NLy * NLz : (axis == 1 ? NLz : 1);
+  const int NL = NLx * NLy * NLz;
+#pragma omp parallel for
+  for (int p = 0; p < NL; ++p) {
+    const int l = (p / es) % N;
+    const int base = p - l * es;
+    double re = 0.0, im = 0.0;
+    for (int m = 0; m < N; ++m) {
+      const double a = sgn * 2.0 * M_PI * (double)(((long)l * m) % N) / N;
+      const double c = cos(a), sn = sin(a);
+      const zc v = in[base + m * es];
+      re += c * v.re - sn * v.im;
+      im += c * v.im + sn * v.re;
+    }
+    out[p] = zmk(re, im);
+  }
+}
+
+void or_hartree(int NLx, int NLy, int NLz, double aLx, double aLy, double aLz, const double *rho, double *vh)
+{
+  const int NL = NLx * NLy * NLz;
+  zc *A = (zc *)malloc(sizeof(zc) * NL), *B = (zc *)malloc(sizeof(zc) * NL);
+  for (int p = 0; p < NL; ++p) A[p] = zmk(rho[p], 0.0);
+  or_dft_axis(NLx, NLy, NLz, 2, -1, A, B);
+  or_dft_axis(NLx, NLy, NLz, 1, -1, B, A);
+  or_dft_axis(NLx, NLy, NLz, 0, -1, A, B);
+  for (int p = 0; p < NL; ++p) {
+    const int ix = p / (NLy * NLz), iy = (p / NLz) % NLy, iz = p % NLz;
+    const int nx = ix < NLx - NLx / 2 ? ix : ix - NLx;
+    const int ny = iy < NLy - NLy / 2 ? iy : iy - NLy;
+    const int nz = iz < NLz - NLz / 2 ? iz : iz - NLz;
+    const double gx = 2.0 * M_PI * nx / aLx, gy = 2.0 * M_PI * ny / aLy, gz = 2.0 * M_PI * nz / aLz;
+    const double G2 = gx * gx + gy * gy + gz * gz;
+    const double f = (nx == 0 && ny == 0 && nz == 0) ? 0.0 : 4.0 * M_PI / G2 / NL;
+    B[p] = zscale(f, B[p]);
+  }
+  or_dft_axis(NLx, NLy, NLz, 2, +1, B, A);
+  or_dft_axis(NLx, NLy, NLz, 1, +1, A, B);
+  or_dft_axis(NLx, NLy, NLz, 0, +1, B, A);
+  for (int p = 0; p < NL; ++p) vh[p] = A[p].re;
+  free(A);
+  free(B);
+}
+
+/* Perdew-Zunger (1981) LDA exchange-correlation of the spin-unpolarised
+ * density rho per point: e_xc(rho) and v_xc = e_xc + rho de_xc/drho, with the
+ * reference's regularisation rho + 1e-10 and its branch at rs = 1. */
+void or_pz(int n, const double *rho, double *vxc, double *exc)
+{
+  const double gU = -0.1423, b1U = 1.0529, b2U = 0.3334, AU = 0.0311, BU = -0.048, CU = 0.002, DU = -0.0116;
+  const double cst = 0.75 * pow(3.0 / (2.0 * M_PI), 2.0 / 3.0);
+  for (int i = 0; i < n; ++i) {
+    const double t = rho[i] + 1e-10;
+    const double rs = pow(3.0 / (4.0 * M_PI * t), 1.0 / 3.0);
+    double e = -cst / rs, de = e / (3.0 * t);
+    if (rs > 1.0) {
+      const double q = sqrt(rs), den = 1.0 + b1U * q + b2U * rs;
+      e += gU / den;
+      de += gU * (0.5 * b1U * q + b2U * rs) / (3.0 * t) / (den * den);
+    } else {
+      const double ln = log(rs);
+      e += AU * ln + BU + CU * rs * ln + DU * rs;
+      de -= rs / (3.0 * t) * (AU / rs + CU * (ln + 1.0) + DU);
+    }
+    if (exc) exc[i] = e;
+    vxc[i] = e + rho[i] * de;
+  }
 }
 
 int or_num_threads(void)

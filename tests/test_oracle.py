@@ -318,3 +318,48 @@ def test_config4_generator():
     assert np.array_equal(a.vloc, b.vloc) and a.NB == b.NB and a.NK == b.NK
     s = synth.config4_system(2, orbitals=False)
     assert s.dt * synth.spectral_bound(s) < 2 * math.sqrt(2)
+
+
+def test_pz_matches_reference_and_its_derivative():
+    """The oracle's PZ81 LDA against the reference's own exc_cor_pz (built in
+    oracle/_ref, called with rho_s = rho/2 as salmon_xc.f90:499-511 does), and
+    v_xc = d(rho e_xc)/drho by central differences (both sides of rs = 1)."""
+    from oracle import cpu
+    rho = np.concatenate([np.geomspace(1e-6, 0.2, 40), np.geomspace(0.25, 30.0, 40)])
+    v, e = cpu.pz(rho)
+    h = 1e-6 * rho
+    vp, ep = cpu.pz(rho + h)
+    vm, em = cpu.pz(rho - h)
+    fd = ((rho + h) * ep - (rho - h) * em) / (2 * h)
+    assert np.max(np.abs(fd - v) / np.abs(v)) < 1e-7
+    from oracle import ref
+    if not ref.available():
+        pytest.skip("reference build (oracle/_ref) not present")
+    import ctypes
+    L = ctypes.CDLL(ref.LIB)
+    f = L._QMbuiltin_pzPexc_cor_pz
+    n = ctypes.c_int(rho.size)
+    rs = np.ascontiguousarray(rho * 0.5)
+    exc, eexc, vexc = np.zeros_like(rho), np.zeros_like(rho), np.zeros_like(rho)
+    p = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+    f(ctypes.byref(n), p(rs), p(exc), p(eexc), p(vexc))
+    assert np.max(np.abs(vexc - v)) < 1e-14 * np.max(np.abs(v))
+    assert np.max(np.abs(exc - e)) < 1e-14 * np.max(np.abs(e))
+
+
+def test_hartree_closed_form():
+    """V_H of a plane-wave density cos(G.r) is 4 pi/|G|^2 cos(G.r); a constant
+    density (G = 0 only) gives zero; linear in rho."""
+    from oracle import cpu
+    s = synth.make_system(N=(12, 10, 14), L=(6.0, 5.5, 7.2), nk3=(1, 1, 1), NB=1, seed=2, name="hart",
+                          orbitals=False)
+    ix, iy, iz = np.meshgrid(np.arange(s.NLx), np.arange(s.NLy), np.arange(s.NLz), indexing="ij")
+    for n in ((1, 0, 0), (2, -3, 1), (0, 5, -7)):
+        G = 2 * np.pi * np.array(n) / np.array([s.aLx, s.aLy, s.aLz])
+        ph = (G[0] * ix * s.aLx / s.NLx + G[1] * iy * s.aLy / s.NLy + G[2] * iz * s.aLz / s.NLz).ravel()
+        vh = cpu.hartree(s, np.cos(ph))
+        assert np.max(np.abs(vh - 4 * np.pi / (G @ G) * np.cos(ph))) < 1e-12 * 4 * np.pi / (G @ G)
+    assert np.max(np.abs(cpu.hartree(s, np.full(s.NL, 0.7)))) < 1e-13
+    rng = np.random.default_rng(1)
+    a, b = rng.random(s.NL), rng.random(s.NL)
+    assert rel(cpu.hartree(s, 2 * a - b), 2 * cpu.hartree(s, a) - cpu.hartree(s, b)) < 1e-12
