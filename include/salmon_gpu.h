@@ -270,6 +270,23 @@ int salmon_gpu_path(salmon_ctx *ctx);
 enum { SALMON_PROP_MIDPOINT = 0, SALMON_PROP_ETRS = 1 };
 int salmon_gpu_set_propagator(salmon_ctx *ctx, int propagator);
 
+/* Self-consistent local potential (dt_evolve.f90:185-207 with
+ * no_update_func = 'n'): after each Taylor step of salmon_gpu_rt_run the
+ * density rho(t+dt) gives Vloc = V_H + vpsl + V_xc -- the Hartree potential
+ * of ARTED/common/Hartree.f90:20-128 (orthorhombic cell, up to 256 points per
+ * axis) and, with SALMON_XC_PZ, the PZ81 LDA of src/xc/builtin_pz.f90 -- and
+ * the observables are taken with the new potential.  salmon_gpu_set_scf sets
+ * Vloc(0) from the resident orbitals' density at once; vpsl NULL returns to
+ * the frozen potential set by salmon_gpu_set_potential (the last Vloc stays).
+ * Middlepoint propagator only.  The energy's local term is then
+ * sum Vloc rho Hxyz with the total Vloc (no Hartree double-counting
+ * correction; the reference's G-space total energy is outside this path).
+ * salmon_gpu_hartree_xc returns V_H and V_xc (xc = SALMON_XC_NONE: zeros) of
+ * any density rho(NL) on the context's grid without changing its potential. */
+enum { SALMON_XC_NONE = 0, SALMON_XC_PZ = 1 };
+int salmon_gpu_set_scf(salmon_ctx *ctx, const double *vpsl, int xc);
+int salmon_gpu_hartree_xc(salmon_ctx *ctx, const double *rho, int xc, double *vh, double *vxc);
+
 /* Waits for all work of the context. */
 int salmon_gpu_synchronize(salmon_ctx *ctx);
 

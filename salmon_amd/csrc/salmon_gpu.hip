@@ -45,6 +45,12 @@ struct salmon_ctx {
   int uvpsi_nlma = 0;         // projectors d_uvpsi is sized for
   bool graphs = true;         // CG / Gram-Schmidt through the graph cache
   int propagator = SALMON_PROP_MIDPOINT;  // of rt_run
+  // self-consistent potential (salmon_gpu_set_scf): Vloc = V_H[rho] + Vpsl + V_xc[rho]
+  bool scf = false;
+  int scf_xc = 0;
+  double* d_vpsl = nullptr;
+  double* d_scf_tw = nullptr;                 // DFT twiddles, 3 x kMaxN (cos, sin)
+  void* d_scf_buf[2] = {nullptr, nullptr};    // two complex grids
   void* d_ph = nullptr;       // KB phases of the streamed Taylor step (NK, NPI)
   size_t ph_bytes = 0;
   double* d_rows = nullptr;   // (NKB*8)
@@ -291,6 +297,7 @@ int read_sums(salmon_ctx* c, double out[8]) {
 }  // namespace
 
 #include "fused.hpp"
+#include "scf.hpp"
 #include "gs.hpp"
 #include "pc.hpp"
 #include "batch_host.hpp"
@@ -306,6 +313,53 @@ int gram_schmidt_multi_fwd(salmon_ctx* c, cplx<double>* zu) {
 }  // namespace
 
 namespace {
+
+// V_H and V_xc of the device density rho: Hartree by per-axis DFTs and the
+// Coulomb kernel, then Vloc = V_H + Vpsl + V_xc into the context's potential
+// (set_v) and/or the parts into vh, vxc (device, nullable).
+int scf_potential(salmon_ctx* c, const double* rho, bool set_v, int xc, double* vh, double* vxc) {
+  const Grid& g = c->g;
+  if (g.NLx > salmon::scf::kMaxN || g.NLy > salmon::scf::kMaxN || g.NLz > salmon::scf::kMaxN)
+    return fail(c, SALMON_ERR_UNSUPPORTED, "Hartree solve limited to %d points per axis", salmon::scf::kMaxN);
+  int r;
+  if (!c->d_scf_tw) {
+    std::vector<double> tw((size_t)3 * 2 * salmon::scf::kMaxN, 0.0);
+    const int N[3] = {g.NLx, g.NLy, g.NLz};
+    for (int a = 0; a < 3; ++a)
+      for (int j = 0; j < N[a]; ++j) {
+        const double ang = 2.0 * M_PI * j / N[a];
+        tw[(size_t)a * 2 * salmon::scf::kMaxN + 2 * j] = std::cos(ang);
+        tw[(size_t)a * 2 * salmon::scf::kMaxN + 2 * j + 1] = std::sin(ang);
+      }
+    if ((r = dalloc(c, &c->d_scf_tw, tw.size()))) return r;
+    HIPCHK(c, hipMemcpy(c->d_scf_tw, tw.data(), sizeof(double) * tw.size(), hipMemcpyHostToDevice));
+  }
+  for (int i = 0; i < 2; ++i)
+    if (!c->d_scf_buf[i] && (r = dalloc(c, (cplx<double>**)&c->d_scf_buf[i], (size_t)g.NL))) return r;
+  cplx<double>* A = (cplx<double>*)c->d_scf_buf[0];
+  cplx<double>* B = (cplx<double>*)c->d_scf_buf[1];
+  const dim3 gr((g.NL + 255) / 256), bl(256);
+  hipLaunchKernelGGL(salmon::scf::k_real_to_cplx, gr, bl, 0, c->stream, rho, A, g.NL);
+  hipLaunchKernelGGL(salmon::scf::k_dft_axis, gr, bl, 0, c->stream, (const cplx<double>*)A, B, g, 2, -1, c->d_scf_tw);
+  hipLaunchKernelGGL(salmon::scf::k_dft_axis, gr, bl, 0, c->stream, (const cplx<double>*)B, A, g, 1, -1, c->d_scf_tw);
+  hipLaunchKernelGGL(salmon::scf::k_dft_axis, gr, bl, 0, c->stream, (const cplx<double>*)A, B, g, 0, -1, c->d_scf_tw);
+  hipLaunchKernelGGL(salmon::scf::k_coulomb, gr, bl, 0, c->stream, B, g);
+  hipLaunchKernelGGL(salmon::scf::k_dft_axis, gr, bl, 0, c->stream, (const cplx<double>*)B, A, g, 2, +1, c->d_scf_tw);
+  hipLaunchKernelGGL(salmon::scf::k_dft_axis, gr, bl, 0, c->stream, (const cplx<double>*)A, B, g, 1, +1, c->d_scf_tw);
+  hipLaunchKernelGGL(salmon::scf::k_dft_axis, gr, bl, 0, c->stream, (const cplx<double>*)B, A, g, 0, +1, c->d_scf_tw);
+  LAUNCHCHK(c);
+  if (set_v) {
+    hipLaunchKernelGGL(salmon::scf::k_vloc, gr, bl, 0, c->stream, (const cplx<double>*)A, rho, (const double*)c->d_vpsl,
+                       xc, g.NL, c->d_vloc, c->d_vloc_f, vh, vxc);
+  } else {
+    // parts only: Vpsl = 0 into a scratch potential (the second grid)
+    double* scratch = (double*)B;
+    hipLaunchKernelGGL(salmon::scf::k_vloc, gr, bl, 0, c->stream, (const cplx<double>*)A, rho, (const double*)nullptr,
+                       xc, g.NL, scratch, (float*)nullptr, vh, vxc);
+  }
+  LAUNCHCHK(c);
+  return SALMON_OK;
+}
 
 // One RT step n on resident orbitals (streamed path), A(t_n) = a0,
 // A(t_{n+1}) = a1: middlepoint (dt_evolve.f90:43-51) Taylor-4 at
@@ -382,6 +436,30 @@ int rt_run_fused(salmon_ctx* c, const double* ac, int nsteps, double dt, int reo
     }
   }
   if (r == SALMON_OK && pending) r = fused_step<T>(c, nullptr, aobs, dt, d_res + 8 * (size_t)(nsteps - 1));
+  return r;
+}
+
+// nsteps RT steps with the self-consistent potential, in the reference's
+// order (dt_evolve.f90:106-216 with no_update_func = 'n', then
+// control_sc.f90:203-222): the Taylor-4 step with Vloc(t) (fused or
+// streamed, no observables inside), rho(t+dt), Vloc(t+dt) = V_H + Vpsl + V_xc,
+// then current and energy at kAc0 + A(t+dt) with the new potential.
+template <typename T>
+int rt_run_scf(salmon_ctx* c, const double* ac, int nsteps, double dt, int reortho_every, double* d_res) {
+  int r = SALMON_OK;
+  for (int n = 0; n < nsteps && r == SALMON_OK; ++n) {
+    double amid[3];
+    const double* anew = ac + 3 * (n + 1);
+    for (int d = 0; d < 3; ++d) amid[d] = 0.5 * (ac[3 * n + d] + anew[d]);
+    if (fused_ready(c)) r = fused_step<T>(c, amid, nullptr, dt, nullptr);
+    else r = taylor_streamed<T>(c, c->d_kac0, amid[0], amid[1], amid[2], dt);
+    if (r || (r = density_dev<T>(c))) break;
+    if ((r = scf_potential(c, c->d_rho, true, c->scf_xc, nullptr, nullptr))) break;
+    if ((r = observe_rows<T>(c, c->d_kac0, anew[0], anew[1], anew[2]))) break;
+    hipLaunchKernelGGL(k_sum_rows, dim3(1), dim3(256), 0, c->stream, c->d_rows, c->NKB, d_res + 8 * (size_t)n);
+    LAUNCHCHK(c);
+    if (reortho_every > 0 && (n + 1) % reortho_every == 0) r = gram_schmidt_dev(c);
+  }
   return r;
 }
 
@@ -943,6 +1021,46 @@ int salmon_gpu_pcnet_apply(salmon_pcnet* n, salmon_ctx* c, const double* g, doub
   return SALMON_OK;
 }
 
+int salmon_gpu_set_scf(salmon_ctx* c, const double* vpsl, int xc) {
+  if (!c) return SALMON_ERR_ARG;
+  if (!vpsl) {
+    c->scf = false;
+    return SALMON_OK;
+  }
+  if (xc != SALMON_XC_NONE && xc != SALMON_XC_PZ) return fail(c, SALMON_ERR_ARG, "xc %d is neither NONE nor PZ", xc);
+  int r = check_ready(c, true);
+  if (r) return r;
+  if ((r = dalloc(c, &c->d_vpsl, (size_t)c->g.NL))) return r;
+  HIPCHK(c, hipMemcpy(c->d_vpsl, vpsl, sizeof(double) * c->g.NL, hipMemcpyHostToDevice));
+  c->scf_xc = xc;
+  // Vloc(0) from the resident orbitals' density
+  if ((r = c->precision == SALMON_FP64 ? density_dev<double>(c) : density_dev<float>(c))) return r;
+  if ((r = scf_potential(c, c->d_rho, true, xc, nullptr, nullptr))) return r;
+  c->scf = true;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return SALMON_OK;
+}
+
+int salmon_gpu_hartree_xc(salmon_ctx* c, const double* rho, int xc, double* vh, double* vxc) {
+  if (!c || !rho) return c ? fail(c, SALMON_ERR_ARG, "rho is NULL") : SALMON_ERR_ARG;
+  if (xc != SALMON_XC_NONE && xc != SALMON_XC_PZ) return fail(c, SALMON_ERR_ARG, "xc %d is neither NONE nor PZ", xc);
+  const size_t NL = c->g.NL;
+  double *d_in = nullptr, *d_vh = nullptr, *d_vx = nullptr;
+  int r;
+  if ((r = dalloc(c, &d_in, 3 * NL))) return r;
+  d_vh = d_in + NL;
+  d_vx = d_in + 2 * NL;
+  HIPCHK(c, hipMemcpy(d_in, rho, sizeof(double) * NL, hipMemcpyHostToDevice));
+  r = scf_potential(c, d_in, false, xc, d_vh, d_vx);
+  if (r == SALMON_OK) {
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (vh) HIPCHK(c, hipMemcpy(vh, d_vh, sizeof(double) * NL, hipMemcpyDeviceToHost));
+    if (vxc) HIPCHK(c, hipMemcpy(vxc, d_vx, sizeof(double) * NL, hipMemcpyDeviceToHost));
+  }
+  (void)hipFree(d_in);
+  return r;
+}
+
 int salmon_gpu_set_propagator(salmon_ctx* c, int propagator) {
   if (!c) return SALMON_ERR_ARG;
   if (propagator != SALMON_PROP_MIDPOINT && propagator != SALMON_PROP_ETRS)
@@ -1087,11 +1205,17 @@ int salmon_gpu_rt_run(salmon_ctx* c, const double* ac, int nsteps, double dt, in
   if (nsteps == 0) return SALMON_OK;
   double* d_res = nullptr;
   if ((r = dalloc(c, &d_res, 8 * (size_t)nsteps))) return r;
-  if (fused_ready(c)) {
+  if (c->scf) {
+    if (c->propagator != SALMON_PROP_MIDPOINT)
+      r = fail(c, SALMON_ERR_UNSUPPORTED, "the self-consistent potential is built for the middlepoint propagator");
+    else
+      r = c->precision == SALMON_FP64 ? rt_run_scf<double>(c, ac, nsteps, dt, reortho_every, d_res)
+                                      : rt_run_scf<float>(c, ac, nsteps, dt, reortho_every, d_res);
+  } else if (fused_ready(c)) {
     r = c->precision == SALMON_FP64 ? rt_run_fused<double>(c, ac, nsteps, dt, reortho_every, d_res)
                                     : rt_run_fused<float>(c, ac, nsteps, dt, reortho_every, d_res);
   }
-  for (int n = 0; n < nsteps && r == SALMON_OK && !fused_ready(c); ++n) {
+  for (int n = 0; n < nsteps && r == SALMON_OK && !fused_ready(c) && !c->scf; ++n) {
     const double* a0 = ac + 3 * n;
     const double* a1 = ac + 3 * (n + 1);
     r = c->precision == SALMON_FP64 ? rt_step<double>(c, a0, a1, dt, d_res + 8 * (size_t)n)
@@ -1215,6 +1339,7 @@ int salmon_gpu_finalize(salmon_ctx* c) {
   dfree((void**)&c->d_kac0); dfree((void**)&c->d_kac); dfree((void**)&c->d_occ); dfree((void**)&c->d_rows);
   dfree((void**)&c->d_sum); dfree((void**)&c->d_kone); dfree((void**)&c->d_oneocc); dfree(&c->d_zu);
   dfree((void**)&c->d_f64); dfree(&c->d_uvpsi); dfree(&c->d_ph);
+  dfree((void**)&c->d_vpsl); dfree((void**)&c->d_scf_tw); dfree(&c->d_scf_buf[0]); dfree(&c->d_scf_buf[1]);
   for (int i = 0; i < 4; ++i) dfree(&c->d_work[i]);
   for (int i = 0; i < 3; ++i) dfree(&c->d_stage[i]);
   for (void* p : c->kb_bufs) (void)hipFree(p);

@@ -22,6 +22,8 @@ module salmon_gpu
   integer(c_int), parameter :: SALMON_PC_LEARNED = 2
   integer(c_int), parameter :: SALMON_PROP_MIDPOINT = 0
   integer(c_int), parameter :: SALMON_PROP_ETRS = 1
+  integer(c_int), parameter :: SALMON_XC_NONE = 0
+  integer(c_int), parameter :: SALMON_XC_PZ = 1
 
   type, bind(C) :: salmon_grid_t
     integer(c_int) :: NLx, NLy, NLz
@@ -187,6 +189,21 @@ module salmon_gpu
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx
       integer(c_int), value :: propagator
+    end function
+
+    integer(c_int) function salmon_gpu_set_scf(ctx, vpsl, xc) bind(C, name="salmon_gpu_set_scf")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: vpsl(*)
+      integer(c_int), value :: xc
+    end function
+
+    integer(c_int) function salmon_gpu_hartree_xc(ctx, rho, xc, vh, vxc) bind(C, name="salmon_gpu_hartree_xc")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: rho(*)
+      integer(c_int), value :: xc
+      real(c_double), intent(out) :: vh(*), vxc(*)
     end function
 
     integer(c_int) function salmon_gpu_set_graphs(ctx, enable) bind(C, name="salmon_gpu_set_graphs")

@@ -39,6 +39,7 @@ FP64 = 0
 MIXED = 1
 PATH_AUTO, PATH_STREAMED, PATH_FUSED = 0, 1, 2
 PROP_MIDPOINT, PROP_ETRS = 0, 1
+XC_NONE, XC_PZ = 0, 1
 PC_NONE, PC_KINETIC, PC_LEARNED = 0, 1, 2
 PCNET_WEIGHTS = 27 * 2 * 16 + 27 * 16 * 16 + 27 * 16 * 2
 
@@ -81,6 +82,8 @@ SIGNATURES = {
     "salmon_gpu_pcnet_apply": [_vp, _vp, _vp, _vp],
     "salmon_gpu_set_graphs": [_vp, _ci],
     "salmon_gpu_set_propagator": [_vp, _ci],
+    "salmon_gpu_set_scf": [_vp, _vp, _ci],
+    "salmon_gpu_hartree_xc": [_vp, _vp, _ci, _vp, _vp],
     "salmon_gpu_set_pcnet": [_vp, _vp],
     "salmon_gpu_pc_record": [_vp, _ci, _ci],
     "salmon_gpu_pc_samples": [_vp, _vp, _vp],
@@ -309,6 +312,22 @@ class Context:
         if isinstance(propagator, str):
             propagator = {"middlepoint": PROP_MIDPOINT, "etrs": PROP_ETRS}[propagator]
         self._check(lib().salmon_gpu_set_propagator(self._h, int(propagator)), "set_propagator")
+
+    def set_scf(self, vpsl, xc=XC_PZ):
+        """Self-consistent Vloc = V_H[rho] + vpsl + V_xc[rho] after every RT
+        step (vpsl None: back to the frozen potential)."""
+        if vpsl is None:
+            self._check(lib().salmon_gpu_set_scf(self._h, None, int(xc)), "set_scf")
+            return
+        v = _f64(vpsl)
+        self._check(lib().salmon_gpu_set_scf(self._h, _p(v), int(xc)), "set_scf")
+
+    def hartree_xc(self, rho, xc=XC_PZ):
+        """(V_H, V_xc) of the density rho(NL) on this context's grid."""
+        r = _f64(rho)
+        vh, vx = np.zeros(self.sys.NL), np.zeros(self.sys.NL)
+        self._check(lib().salmon_gpu_hartree_xc(self._h, _p(r), int(xc), _p(vh), _p(vx)), "hartree_xc")
+        return vh, vx
 
     def set_graphs(self, enable):
         """CG / Gram-Schmidt through the graph cache (default) or eagerly; a
