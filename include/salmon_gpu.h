@@ -278,7 +278,9 @@ int salmon_gpu_set_propagator(salmon_ctx *ctx, int propagator);
  * the observables are taken with the new potential.  salmon_gpu_set_scf sets
  * Vloc(0) from the resident orbitals' density at once; vpsl NULL returns to
  * the frozen potential set by salmon_gpu_set_potential (the last Vloc stays).
- * Middlepoint propagator only.  The energy's local term is then
+ * With SALMON_PROP_ETRS the second half step uses the extrapolated potential
+ * 3 Vloc - 3 Vloc_old1 + Vloc_old2 (dt_evolve.f90:247-251), the history
+ * starting at Vloc(0) when salmon_gpu_set_scf is called.  The energy's local term is then
  * sum Vloc rho Hxyz with the total Vloc (no Hartree double-counting
  * correction; the reference's G-space total energy is outside this path).
  * salmon_gpu_hartree_xc returns V_H and V_xc (xc = SALMON_XC_NONE: zeros) of

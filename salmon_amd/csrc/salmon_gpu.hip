@@ -51,6 +51,7 @@ struct salmon_ctx {
   double* d_vpsl = nullptr;
   double* d_scf_tw = nullptr;                 // DFT twiddles, 3 x kMaxN (cos, sin)
   void* d_scf_buf[2] = {nullptr, nullptr};    // two complex grids
+  double* d_vold = nullptr;                   // ETRS potential history V_old1, V_old2 (2 NL)
   void* d_ph = nullptr;       // KB phases of the streamed Taylor step (NK, NPI)
   size_t ph_bytes = 0;
   double* d_rows = nullptr;   // (NKB*8)
@@ -443,16 +444,32 @@ int rt_run_fused(salmon_ctx* c, const double* ac, int nsteps, double dt, int reo
 // order (dt_evolve.f90:106-216 with no_update_func = 'n', then
 // control_sc.f90:203-222): the Taylor-4 step with Vloc(t) (fused or
 // streamed, no observables inside), rho(t+dt), Vloc(t+dt) = V_H + Vpsl + V_xc,
-// then current and energy at kAc0 + A(t+dt) with the new potential.
+// then current and energy at kAc0 + A(t+dt) with the new potential.  ETRS
+// (dt_evolve.f90:218-348): Taylor-4 over dt/2 at kAc0 + A_n with Vloc(t),
+// the extrapolated potential 3 V - 3 V_old1 + V_old2, Taylor-4 over dt/2 at
+// kAc0 + A_{n+1} with it, then the same update from rho(t+dt).
 template <typename T>
 int rt_run_scf(salmon_ctx* c, const double* ac, int nsteps, double dt, int reortho_every, double* d_res) {
   int r = SALMON_OK;
+  const bool etrs = c->propagator == SALMON_PROP_ETRS;
+  auto taylor = [&](const double a[3], double h) {
+    if (fused_ready(c)) return fused_step<T>(c, a, nullptr, h, nullptr);
+    return taylor_streamed<T>(c, c->d_kac0, a[0], a[1], a[2], h);
+  };
+  const dim3 gr((c->g.NL + 255) / 256), bl(256);
   for (int n = 0; n < nsteps && r == SALMON_OK; ++n) {
     double amid[3];
     const double* anew = ac + 3 * (n + 1);
-    for (int d = 0; d < 3; ++d) amid[d] = 0.5 * (ac[3 * n + d] + anew[d]);
-    if (fused_ready(c)) r = fused_step<T>(c, amid, nullptr, dt, nullptr);
-    else r = taylor_streamed<T>(c, c->d_kac0, amid[0], amid[1], amid[2], dt);
+    if (etrs) {
+      if ((r = taylor(ac + 3 * n, 0.5 * dt))) break;
+      hipLaunchKernelGGL(salmon::scf::k_vloc_extrapolate, gr, bl, 0, c->stream, c->d_vloc, c->d_vloc_f, c->d_vold,
+                         c->d_vold + c->g.NL, c->g.NL);
+      LAUNCHCHK(c);
+      r = taylor(anew, 0.5 * dt);
+    } else {
+      for (int d = 0; d < 3; ++d) amid[d] = 0.5 * (ac[3 * n + d] + anew[d]);
+      r = taylor(amid, dt);
+    }
     if (r || (r = density_dev<T>(c))) break;
     if ((r = scf_potential(c, c->d_rho, true, c->scf_xc, nullptr, nullptr))) break;
     if ((r = observe_rows<T>(c, c->d_kac0, anew[0], anew[1], anew[2]))) break;
@@ -1033,9 +1050,14 @@ int salmon_gpu_set_scf(salmon_ctx* c, const double* vpsl, int xc) {
   if ((r = dalloc(c, &c->d_vpsl, (size_t)c->g.NL))) return r;
   HIPCHK(c, hipMemcpy(c->d_vpsl, vpsl, sizeof(double) * c->g.NL, hipMemcpyHostToDevice));
   c->scf_xc = xc;
-  // Vloc(0) from the resident orbitals' density
+  // Vloc(0) from the resident orbitals' density; the ETRS history starts at
+  // V_old1 = V_old2 = Vloc(0) (control_sc.f90:82)
   if ((r = c->precision == SALMON_FP64 ? density_dev<double>(c) : density_dev<float>(c))) return r;
   if ((r = scf_potential(c, c->d_rho, true, xc, nullptr, nullptr))) return r;
+  if (!c->d_vold && (r = dalloc(c, &c->d_vold, 2 * (size_t)c->g.NL))) return r;
+  HIPCHK(c, hipMemcpyAsync(c->d_vold, c->d_vloc, sizeof(double) * c->g.NL, hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->d_vold + c->g.NL, c->d_vloc, sizeof(double) * c->g.NL, hipMemcpyDeviceToDevice,
+                           c->stream));
   c->scf = true;
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return SALMON_OK;
@@ -1206,11 +1228,8 @@ int salmon_gpu_rt_run(salmon_ctx* c, const double* ac, int nsteps, double dt, in
   double* d_res = nullptr;
   if ((r = dalloc(c, &d_res, 8 * (size_t)nsteps))) return r;
   if (c->scf) {
-    if (c->propagator != SALMON_PROP_MIDPOINT)
-      r = fail(c, SALMON_ERR_UNSUPPORTED, "the self-consistent potential is built for the middlepoint propagator");
-    else
-      r = c->precision == SALMON_FP64 ? rt_run_scf<double>(c, ac, nsteps, dt, reortho_every, d_res)
-                                      : rt_run_scf<float>(c, ac, nsteps, dt, reortho_every, d_res);
+    r = c->precision == SALMON_FP64 ? rt_run_scf<double>(c, ac, nsteps, dt, reortho_every, d_res)
+                                    : rt_run_scf<float>(c, ac, nsteps, dt, reortho_every, d_res);
   } else if (fused_ready(c)) {
     r = c->precision == SALMON_FP64 ? rt_run_fused<double>(c, ac, nsteps, dt, reortho_every, d_res)
                                     : rt_run_fused<float>(c, ac, nsteps, dt, reortho_every, d_res);
@@ -1339,7 +1358,7 @@ int salmon_gpu_finalize(salmon_ctx* c) {
   dfree((void**)&c->d_kac0); dfree((void**)&c->d_kac); dfree((void**)&c->d_occ); dfree((void**)&c->d_rows);
   dfree((void**)&c->d_sum); dfree((void**)&c->d_kone); dfree((void**)&c->d_oneocc); dfree(&c->d_zu);
   dfree((void**)&c->d_f64); dfree(&c->d_uvpsi); dfree(&c->d_ph);
-  dfree((void**)&c->d_vpsl); dfree((void**)&c->d_scf_tw); dfree(&c->d_scf_buf[0]); dfree(&c->d_scf_buf[1]);
+  dfree((void**)&c->d_vpsl); dfree((void**)&c->d_vold); dfree((void**)&c->d_scf_tw); dfree(&c->d_scf_buf[0]); dfree(&c->d_scf_buf[1]);
   for (int i = 0; i < 4; ++i) dfree(&c->d_work[i]);
   for (int i = 0; i < 3; ++i) dfree(&c->d_stage[i]);
   for (void* p : c->kb_bufs) (void)hipFree(p);

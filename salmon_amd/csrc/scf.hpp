@@ -97,5 +97,21 @@ __global__ void __launch_bounds__(256) k_vloc(const cplx<double>* __restrict__ v
   if (vxc) vxc[p] = x;
 }
 
+// ETRS potential extrapolation (dt_evolve.f90:247-251): Vnew = 3 V - 3 V_old1 +
+// V_old2, the history shifts (old2 <- old1, old1 <- V) and V <- Vnew with its
+// FP32 copy
+__global__ void __launch_bounds__(256) k_vloc_extrapolate(double* __restrict__ v64, float* __restrict__ v32,
+                                                          double* __restrict__ old1, double* __restrict__ old2,
+                                                          int NL) {
+  const int p = blockIdx.x * 256 + threadIdx.x;
+  if (p >= NL) return;
+  const double v = v64[p], o1 = old1[p];
+  const double vn = 3.0 * v - 3.0 * o1 + old2[p];
+  old2[p] = o1;
+  old1[p] = v;
+  v64[p] = vn;
+  v32[p] = (float)vn;
+}
+
 }  // namespace scf
 }  // namespace salmon

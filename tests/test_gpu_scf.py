@@ -36,49 +36,63 @@ def test_hartree_xc_matches_oracle(name):
     assert np.array_equal(vh0, vh) and not np.any(vx0)
 
 
-def _scf_oracle(s, vpsl, ac, nsteps):
+def _scf_oracle(s, vpsl, ac, nsteps, etrs=False):
+    """The RT loop with the potential refreshed after every step; ETRS as
+    dt_evolve.f90:218-348 (history seeded with V(0), control_sc.f90:82)."""
     z = s.zu.copy()
 
     def potential(z):
         rho = Oracle(s).density(z)
         return cpu.hartree(s, rho) + vpsl + cpu.pz(rho)[0]
 
+    def with_v(v):
+        return Oracle(dataclasses.replace(s, vloc=v))
+
     v = potential(z)
+    old1, old2 = v.copy(), v.copy()
     J, E = [], []
     for n in range(nsteps):
-        o = Oracle(dataclasses.replace(s, vloc=v))
-        z = o.hamiltonian(midpoint_kac(s, ac, n), z, s.dt)
+        if etrs:
+            z = with_v(v).hamiltonian(kac_at(s, ac[n]), z, 0.5 * s.dt)
+            v, old1, old2 = 3 * v - 3 * old1 + old2, v, old1
+            z = with_v(v).hamiltonian(kac_at(s, ac[n + 1]), z, 0.5 * s.dt)
+        else:
+            z = with_v(v).hamiltonian(midpoint_kac(s, ac, n), z, s.dt)
         v = potential(z)
-        o = Oracle(dataclasses.replace(s, vloc=v))
+        o = with_v(v)
         k1 = kac_at(s, ac[n + 1])
         J.append(o.current(k1, z))
         E.append(sum(o.energy(k1, z)))
     return z, np.array(J), np.array(E)
 
 
+@pytest.mark.parametrize("prop", ["middlepoint", "etrs"])
 @pytest.mark.parametrize("name,path", [("cfg1", gpu.PATH_AUTO), ("small_k", gpu.PATH_AUTO),
                                        ("cfg1", gpu.PATH_STREAMED)])
-def test_scf_rt_matches_oracle(name, path):
+def test_scf_rt_matches_oracle(name, path, prop):
     """RT steps with Vloc = V_H + Vpsl + V_xc refreshed after each step, on
-    the fused (16^3) and the streamed paths, FP64."""
+    the fused (16^3) and the streamed paths, FP64, both propagators (ETRS
+    over four steps so that the extrapolation sees a full history)."""
     s, _ = load_case(name)
     vpsl = s.vloc.copy()
-    nsteps = 3
+    nsteps = 4 if prop == "etrs" else 3
     ac = synth.acos2_pulse(nsteps + 1, s.dt, 0.05, 0.3, 2.0)
-    zr, Jr, Er = _scf_oracle(s, vpsl, ac, nsteps)
+    zr, Jr, Er = _scf_oracle(s, vpsl, ac, nsteps, etrs=prop == "etrs")
     with gpu.Context(s, gpu.FP64) as c:
         c.set_path(path)
+        c.set_propagator(prop)
         c.set_scf(vpsl, gpu.XC_PZ)
         r = c.rt_run(ac, nsteps)
         z = c.get_orbitals()
-    print(f"[gpu] scf {name} path={path}: zu {rel(z, zr):.2e} J {rel(r['J'], Jr):.2e} E {rel(r['E'], Er):.2e}",
+    print(f"[gpu] scf {name} path={path} {prop}: zu {rel(z, zr):.2e} J {rel(r['J'], Jr):.2e} E {rel(r['E'], Er):.2e}",
           flush=True)
     assert rel(z, zr) < 1e-12
     assert rel(r["J"], Jr) < 1e-10
     assert rel(r["E"], Er) < 1e-11
 
 
-def test_scf_mixed_close_to_fp64_and_errors():
+@pytest.mark.parametrize("prop", ["middlepoint", "etrs"])
+def test_scf_mixed_close_to_fp64(prop):
     s, _ = load_case("cfg1")
     vpsl = s.vloc.copy()
     nsteps = 20
@@ -86,20 +100,34 @@ def test_scf_mixed_close_to_fp64_and_errors():
     out = {}
     for prec in (gpu.FP64, gpu.MIXED):
         with gpu.Context(s, prec) as c:
+            c.set_propagator(prop)
             c.set_scf(vpsl, gpu.XC_PZ)
             out[prec] = c.rt_run(ac, nsteps)
     with gpu.Context(s, gpu.FP64) as c:
+        c.set_propagator(prop)
         frozen = c.rt_run(ac, nsteps)
     J64 = out[gpu.FP64]["J"]
     jerr = np.max(np.abs(out[gpu.MIXED]["J"] - J64)) / np.max(np.abs(J64))
     dfro = np.max(np.abs(frozen["J"] - J64)) / np.max(np.abs(J64))
-    print(f"[gpu] scf mixed J err {jerr:.2e}; scf vs frozen {dfro:.2e}", flush=True)
+    print(f"[gpu] scf {prop} mixed J err {jerr:.2e}; scf vs frozen {dfro:.2e}", flush=True)
     assert jerr < 1e-5
     assert dfro > 1e-6  # the potential does move
+
+
+def test_scf_errors_and_switch_off():
+    s, _ = load_case("small_k")
     with gpu.Context(s, gpu.FP64) as c:
         with pytest.raises(gpu.SalmonGPUError, match="xc"):
-            c.set_scf(vpsl, 5)
-        c.set_scf(vpsl, gpu.XC_PZ)
-        c.set_propagator("etrs")
-        with pytest.raises(gpu.SalmonGPUError, match="middlepoint"):
-            c.rt_run(ac, 2)
+            c.set_scf(s.vloc, 5)
+        with pytest.raises(gpu.SalmonGPUError, match="xc"):
+            c.hartree_xc(np.ones(s.NL), -1)
+        # switched off again: the frozen-potential loop with the last Vloc,
+        # here V(0) = V_H[rho0] + Vpsl + V_xc[rho0]
+        c.set_scf(s.vloc, gpu.XC_PZ)
+        c.set_scf(None)
+        ac = synth.acos2_pulse(3, s.dt, 0.05, 0.3, 2.0)
+        r = c.rt_run(ac, 2)
+    rho0 = Oracle(s).density(s.zu)
+    v0 = cpu.hartree(s, rho0) + s.vloc + cpu.pz(rho0)[0]
+    ref = Oracle(dataclasses.replace(s, vloc=v0)).rt_run(ac, 2)
+    assert rel(r["J"], ref["J"]) < 1e-10
