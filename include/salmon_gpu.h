@@ -245,6 +245,14 @@ int salmon_gpu_batch_tune(salmon_batch *batch);
 int salmon_gpu_batch_zblock(salmon_batch *batch, int sys);
 int salmon_gpu_batch_rt_run(salmon_batch *batch, const double *ac, int nsteps, double dt, double *jt, double *et,
                             double *norm);
+/* The same with one A(t) per system, ac(nsys, nsteps+1, 3) row-major: the
+ * macro-points of the multiscale Maxwell-TDDFT loop, each stepped from its
+ * own Ac_m to Ac_new_m (replaces the per-macro-point dt_evolve_KB_MS and
+ * current_RT_MS calls of ARTED/control/control_ms.f90:302-323; the
+ * reference runs them one after another).  nsteps = 1 per call is the
+ * coupled loop, where A(t+dt) depends on the currents. */
+int salmon_gpu_batch_rt_run_ms(salmon_batch *batch, const double *ac, int nsteps, double dt, double *jt,
+                               double *et, double *norm);
 const char *salmon_gpu_batch_last_error(const salmon_batch *batch);
 int salmon_gpu_batch_finalize(salmon_batch *batch);
 

@@ -35,20 +35,25 @@ struct Sys {
 };
 
 struct Step {
-  const double* ac;  // (nsteps+1, 3)
+  const double* ac;  // (nsteps+1, 3), or (nsys, nsteps+1, 3) with ac_stride = 3 (nsteps+1)
   const int* step;   // device step counter
   double dt;
   const unsigned short* owner[kPhases];  // system of every block, per phase
+  long ac_stride;    // doubles between the A(t) tables of consecutive systems (0: one shared table)
 };
 
 __device__ __forceinline__ int find_sys(const Step& st, int phase, long b) { return st.owner[phase][b]; }
 
-__device__ __forceinline__ void step_fields(const Step& st, double amid[3], double anew[3]) {
+// A at the step's midpoint and at t+dt for system s (control_ms.f90 gives
+// every macro-point its own Ac_m, Ac_new_m: dt_evolve.f90:85 for the
+// midpoint, control_ms.f90:316-318 for the current)
+__device__ __forceinline__ void step_fields(const Step& st, int s, double amid[3], double anew[3]) {
   const int n = *st.step;
+  const double* ac = st.ac + (size_t)s * st.ac_stride;
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
-    amid[d] = 0.5 * (st.ac[3 * n + d] + st.ac[3 * (n + 1) + d]);
-    anew[d] = st.ac[3 * (n + 1) + d];
+    amid[d] = 0.5 * (ac[3 * n + d] + ac[3 * (n + 1) + d]);
+    anew[d] = ac[3 * (n + 1) + d];
   }
 }
 
@@ -63,7 +68,7 @@ __global__ void __launch_bounds__(256) k_b_project(const Sys<T>* __restrict__ S,
   const int ilma = (int)(lb % y.pblk) * 4 + (threadIdx.x / kWave);
   if (o >= y.NKB || ilma >= y.kb.Nlma) return;
   double am[3], an[3];
-  step_fields(st, am, an);
+  step_fields(st, s, am, an);
   double kx, ky, kz;
   orbital_k(y.kac0, y.NK, o / y.NB, am[0], am[1], am[2], kx, ky, kz);
   const cplx<T>* in = (term == 0 ? y.zu : y.h[term - 1]) + (size_t)o * y.g.NL;
@@ -81,7 +86,7 @@ __global__ void __launch_bounds__(256) k_b_phase(const Sys<T>* __restrict__ S, S
   if (i >= (long)y.NK * y.kb.NPI) return;
   const int ik = (int)(i / y.kb.NPI), jp = (int)(i - (long)ik * y.kb.NPI);
   double am[3], an[3];
-  step_fields(st, am, an);
+  step_fields(st, blockIdx.y, am, an);
   double kx, ky, kz;
   const double* a = at_new ? an : am;
   orbital_k(y.kac0, y.NK, ik, a[0], a[1], a[2], kx, ky, kz);
@@ -202,7 +207,7 @@ __global__ void __launch_bounds__(256) k_b_hpsi(const Sys<T>* __restrict__ S, in
   const int p0 = ((int)(lb % y.hblk) * 256 + threadIdx.x) * ZB;
   if (o >= y.NKB || p0 >= y.g.NL) return;
   double am[3], an[3];
-  step_fields(st, am, an);
+  step_fields(st, s, am, an);
   double kx, ky, kz;
   orbital_k(y.kac0, y.NK, o / y.NB, am[0], am[1], am[2], kx, ky, kz);
   const cplx<T>* in = (term == 0 ? y.zu : y.h[term - 1]) + (size_t)o * y.g.NL;
@@ -240,7 +245,7 @@ __global__ void __launch_bounds__(256) k_b_observe(const Sys<T>* __restrict__ S,
   const Sys<T>& y = S[s];
   const int o = (int)(b - y.first[3]);
   double am[3], an[3];
-  step_fields(st, am, an);
+  step_fields(st, s, am, an);
   double kx, ky, kz;
   orbital_k(y.kac0, y.NK, o / y.NB, an[0], an[1], an[2], kx, ky, kz);
   observe_orbital<T>(y.zu + (size_t)o * y.g.NL, kx, ky, kz, y.occ[o], y.vloc64, y.g, y.kb, y.rows + (size_t)o * 8,

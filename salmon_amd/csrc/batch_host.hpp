@@ -25,9 +25,11 @@ struct salmon_batch {
   double* d_ac = nullptr;
   double* d_res = nullptr;
   size_t cap_steps = 0;
+  size_t cap_ac = 0;              // doubles in d_ac
   long max_ph = 0;                // largest NK * NPI over the systems
   hipGraphExec_t exec = nullptr;
   double graph_dt = 0.0;
+  long graph_stride = 0;          // the A(t) table stride the graph was captured with
 };
 
 static int bfail(salmon_batch* b, int code, const char* fmt, ...) {
@@ -140,10 +142,11 @@ static int batch_build(salmon_batch* b) {
 
 // One step's launches (captured by batch_run).
 template <typename T>
-static int batch_enqueue_step(salmon_batch* b, double dt) {
+static int batch_enqueue_step(salmon_batch* b, double dt, long ac_stride) {
   const int n = (int)b->ctx.size();
   const Sys<T>* S = (const Sys<T>*)b->d_sys;
-  const Step st{b->d_ac, b->d_step, dt, {b->d_owner[0], b->d_owner[1], b->d_owner[2], b->d_owner[3], b->d_owner[4]}};
+  const Step st{b->d_ac, b->d_step, dt, {b->d_owner[0], b->d_owner[1], b->d_owner[2], b->d_owner[3], b->d_owner[4]},
+                ac_stride};
   if (b->max_ph > 0)
     hipLaunchKernelGGL(k_b_phase<T>, dim3((unsigned)((b->max_ph + 255) / 256), n), dim3(256), 0, b->stream, S, st, 0);
   for (int t = 0; t < 4; ++t) {
@@ -184,7 +187,7 @@ static int batch_tune_one(salmon_batch* b, int i, int* best) {
   unsigned short* d_zero = nullptr;
   BHIP(b, hipMalloc((void**)&d_zero, sizeof(unsigned short) * zeros.size()));
   BHIP(b, hipMemcpy(d_zero, zeros.data(), sizeof(unsigned short) * zeros.size(), hipMemcpyHostToDevice));
-  const Step st{b->d_ac, b->d_step, 0.0, {d_zero, d_zero, d_zero, d_zero, d_zero}};
+  const Step st{b->d_ac, b->d_step, 0.0, {d_zero, d_zero, d_zero, d_zero, d_zero}, 0};
   for (int zb : {1, 2, 4, 8}) {
     if (c->g.NLz % zb) continue;
     Sys<T> y = one[0];
@@ -237,7 +240,7 @@ static int batch_tune(salmon_batch* b) {
   if (!b->d_ac) {
     BHIP(b, hipMalloc((void**)&b->d_ac, sizeof(double) * 6));
     BHIP(b, hipMemset(b->d_ac, 0, sizeof(double) * 6));
-    b->cap_steps = 1;
+    b->cap_ac = 6;
   }
   BHIP(b, hipMemset(b->d_step, 0, sizeof(int)));
   std::vector<std::array<int, 4>> shapes;  // (NLx, NLy, NLz, zb)
@@ -261,32 +264,42 @@ static int batch_tune(salmon_batch* b) {
   return batch_build<T>(b);
 }
 
+// nsteps steps; ac is one A(t) table (nsteps+1, 3) for every system, or
+// (per_sys) one per system, (nsys, nsteps+1, 3)
 template <typename T>
-static int batch_run(salmon_batch* b, const double* ac, int nsteps, double dt, double* jt, double* et, double* norm) {
+static int batch_run(salmon_batch* b, const double* ac, int nsteps, double dt, double* jt, double* et, double* norm,
+                     bool per_sys) {
   const int n = (int)b->ctx.size();
   int r;
   if (!b->tuned && (r = batch_tune<T>(b))) return r;
-  if ((size_t)nsteps > b->cap_steps || !b->d_res) {
+  const size_t need_ac = (size_t)3 * (nsteps + 1) * (per_sys ? n : 1);
+  const long stride = per_sys ? 3L * (nsteps + 1) : 0L;
+  if (need_ac > b->cap_ac || !b->d_ac) {
     if (b->d_ac) (void)hipFree(b->d_ac);
-    if (b->d_res) (void)hipFree(b->d_res);
     b->d_ac = nullptr;
-    b->d_res = nullptr;
-    BHIP(b, hipMalloc((void**)&b->d_ac, sizeof(double) * 3 * (nsteps + 1)));
-    BHIP(b, hipMalloc((void**)&b->d_res, sizeof(double) * 8 * (size_t)n * nsteps));
-    b->cap_steps = nsteps;
-    if (b->exec) (void)hipGraphExecDestroy(b->exec);  // the graph holds the old table pointers
+    BHIP(b, hipMalloc((void**)&b->d_ac, sizeof(double) * need_ac));
+    b->cap_ac = need_ac;
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);  // the graph holds the old table pointer
     b->exec = nullptr;
   }
-  BHIP(b, hipMemcpyAsync(b->d_ac, ac, sizeof(double) * 3 * (nsteps + 1), hipMemcpyHostToDevice, b->stream));
+  if ((size_t)nsteps > b->cap_steps || !b->d_res) {
+    if (b->d_res) (void)hipFree(b->d_res);
+    b->d_res = nullptr;
+    BHIP(b, hipMalloc((void**)&b->d_res, sizeof(double) * 8 * (size_t)n * nsteps));
+    b->cap_steps = nsteps;
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    b->exec = nullptr;
+  }
+  BHIP(b, hipMemcpyAsync(b->d_ac, ac, sizeof(double) * need_ac, hipMemcpyHostToDevice, b->stream));
   BHIP(b, hipMemsetAsync(b->d_step, 0, sizeof(int), b->stream));
-  if (b->exec && b->graph_dt != dt) {
+  if (b->exec && (b->graph_dt != dt || b->graph_stride != stride)) {
     (void)hipGraphExecDestroy(b->exec);
     b->exec = nullptr;
   }
   if (!b->exec) {
     hipGraph_t graph = nullptr;
     BHIP(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeRelaxed));
-    r = batch_enqueue_step<T>(b, dt);
+    r = batch_enqueue_step<T>(b, dt, stride);
     const hipError_t e = hipStreamEndCapture(b->stream, &graph);
     if (r) {
       if (graph) (void)hipGraphDestroy(graph);
@@ -297,6 +310,7 @@ static int batch_run(salmon_batch* b, const double* ac, int nsteps, double dt, d
     (void)hipGraphDestroy(graph);
     if (ei != hipSuccess) return bfail(b, SALMON_ERR_HIP, "step graph: %s", hipGetErrorString(ei));
     b->graph_dt = dt;
+    b->graph_stride = stride;
   }
   for (int s = 0; s < nsteps; ++s) BHIP(b, hipGraphLaunch(b->exec, b->stream));
   std::vector<double> res(8 * (size_t)n * nsteps);

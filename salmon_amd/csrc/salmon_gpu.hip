@@ -1310,8 +1310,18 @@ int salmon_gpu_batch_rt_run(salmon_batch* b, const double* ac, int nsteps, doubl
   if (b->ctx.empty()) return bfail(b, SALMON_ERR_STATE, "empty batch");
   if (!ac || nsteps < 0) return bfail(b, SALMON_ERR_ARG, "ac is NULL or nsteps < 0");
   if (nsteps == 0) return SALMON_OK;
-  return b->precision == SALMON_FP64 ? batch_run<double>(b, ac, nsteps, dt, jt, et, norm)
-                                     : batch_run<float>(b, ac, nsteps, dt, jt, et, norm);
+  return b->precision == SALMON_FP64 ? batch_run<double>(b, ac, nsteps, dt, jt, et, norm, false)
+                                     : batch_run<float>(b, ac, nsteps, dt, jt, et, norm, false);
+}
+
+int salmon_gpu_batch_rt_run_ms(salmon_batch* b, const double* ac, int nsteps, double dt, double* jt, double* et,
+                               double* norm) {
+  if (!b) return SALMON_ERR_ARG;
+  if (b->ctx.empty()) return bfail(b, SALMON_ERR_STATE, "empty batch");
+  if (!ac || nsteps < 0) return bfail(b, SALMON_ERR_ARG, "ac is NULL or nsteps < 0");
+  if (nsteps == 0) return SALMON_OK;
+  return b->precision == SALMON_FP64 ? batch_run<double>(b, ac, nsteps, dt, jt, et, norm, true)
+                                     : batch_run<float>(b, ac, nsteps, dt, jt, et, norm, true);
 }
 
 const char* salmon_gpu_batch_last_error(const salmon_batch* b) { return b ? b->err.c_str() : "no batch"; }

@@ -305,6 +305,17 @@ module salmon_gpu
       type(c_ptr), value :: jt, et, norm
     end function
 
+    ! ac(3, nsteps+1, nsys): one A(t) per macro-point
+    integer(c_int) function salmon_gpu_batch_rt_run_ms(batch, ac, nsteps, dt, jt, et, norm) &
+        bind(C, name="salmon_gpu_batch_rt_run_ms")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: batch
+      real(c_double), intent(in) :: ac(*)
+      integer(c_int), value :: nsteps
+      real(c_double), value :: dt
+      type(c_ptr), value :: jt, et, norm
+    end function
+
     type(c_ptr) function salmon_gpu_batch_last_error(batch) bind(C, name="salmon_gpu_batch_last_error")
       import :: c_ptr
       type(c_ptr), value :: batch

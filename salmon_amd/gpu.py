@@ -97,6 +97,7 @@ SIGNATURES = {
     "salmon_gpu_batch_tune": [_vp],
     "salmon_gpu_batch_zblock": [_vp, _ci],
     "salmon_gpu_batch_rt_run": [_vp, _vp, _ci, _cd, _vp, _vp, _vp],
+    "salmon_gpu_batch_rt_run_ms": [_vp, _vp, _ci, _cd, _vp, _vp, _vp],
     "salmon_gpu_batch_last_error": [_vp],
     "salmon_gpu_batch_finalize": [_vp],
     "salmon_gpu_set_path": [_vp, _ci],
@@ -448,6 +449,19 @@ class Batch:
         nt = np.zeros((n, nsteps))
         self._check(lib().salmon_gpu_batch_rt_run(self._h, _p(ac), int(nsteps), float(dt), _p(jt), _p(et), _p(nt)),
                     "batch_rt_run")
+        return dict(J=jt, E=et, norm=nt)
+
+    def rt_run_ms(self, ac, nsteps, dt):
+        """nsteps steps with one A(t) per system: ac (nsys, nsteps+1, 3)."""
+        n = len(self.contexts)
+        ac = _f64(np.asarray(ac)[:, : nsteps + 1])
+        if ac.shape != (n, nsteps + 1, 3):
+            raise SalmonGPUError(f"ac must have shape (nsys, nsteps+1, 3) = {(n, nsteps + 1, 3)}")
+        jt = np.zeros((n, nsteps, 3))
+        et = np.zeros((n, nsteps))
+        nt = np.zeros((n, nsteps))
+        self._check(lib().salmon_gpu_batch_rt_run_ms(self._h, _p(ac), int(nsteps), float(dt), _p(jt), _p(et),
+                                                     _p(nt)), "batch_rt_run_ms")
         return dict(J=jt, E=et, norm=nt)
 
     def close(self):

@@ -23,8 +23,8 @@ def test_gs_wfn_in_rt_data_out(tmp_path):
     s2 = dataclasses.replace(s, zu=zu, occ=occ)
     nsteps = 12
     # A on iterations -1..nsteps+1, as the reference's Ac arrays
-    ac_full = np.vstack([np.zeros((1, 3)), synth.acos2_pulse(nsteps + 2, s.dt, 0.05, 0.3, 2.0)])
-    ac = ac_full[1:]
+    ac_full = np.vstack([np.zeros((1, 3)), synth.acos2_pulse(nsteps + 1, s.dt, 0.05, 0.3, 2.0)])
+    ac = ac_full[1:]  # iterations 0..nsteps+1
     print(f"[gpu] formats small_k nsteps={nsteps}", flush=True)
     with gpu.Context(s2, gpu.FP64) as c:
         j0 = c.current(kac_at(s, ac[0]))
@@ -43,6 +43,6 @@ def test_gs_wfn_in_rt_data_out(tmp_path):
     print(f"[gpu] formats: J {rel(data[:, 13:16], jr):.2e} dE {rel(de[:, 2], er[::4] - er[0]):.2e}", flush=True)
     assert data.shape == (nsteps + 1, 16)
     assert rel(data[:, 13:16], jr) < 1e-10
-    assert np.allclose(data[:, 9:12], ac, rtol=1e-14, atol=0)
+    assert np.allclose(data[:, 7:10], ac[: nsteps + 1], rtol=1e-14, atol=0)  # Ac_tot x, y, z
     assert np.allclose(de[:, 1], er[::4], rtol=1e-11, atol=0)
     assert np.max(np.abs(de[:, 2] - (er[::4] - er[0]))) < 1e-11 * np.max(np.abs(er))
