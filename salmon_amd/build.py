@@ -12,7 +12,8 @@ FLANG = os.environ.get("FLANG", "/opt/rocm/lib/llvm/bin/flang")
 ARCH = "gfx950"
 
 HIP_SOURCES = ["salmon_gpu.hip"]
-HIP_DEPS = ["common.hpp", "streamed.hpp", "fused.hpp", "cg.hpp", "gs.hpp"]
+HIP_DEPS = ["common.hpp", "streamed.hpp", "fused.hpp", "cg.hpp", "gs.hpp", "scf.hpp", "pc.hpp", "pcnet.hpp",
+            "batch.hpp", "batch_host.hpp"]
 HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-shared", f"--offload-arch={ARCH}",
              # keep hipcc from forming packed-FP32 instructions
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
