@@ -94,9 +94,11 @@ def workload(seed=0, nsteps=2000):
     return s, ac
 
 
-def run_gpu(s, ac, prec, warmup, steps, ev):
+def run_gpu(s, ac, prec, warmup, steps, ev, scf=False):
     from salmon_amd import gpu
     with gpu.Context(s, prec) as c:
+        if scf:  # Vloc = V_H + Vpsl + V_xc (PZ81) refreshed after every step
+            c.set_scf(s.vloc, gpu.XC_PZ)
         c.rt_run(ac[: warmup + 1], warmup)
         c.synchronize()
         st = c.stream()
@@ -205,6 +207,14 @@ def main():
             "fp64": {"kernel": "k_fused_step<double>", "achieved": B64 / (ms64 * 1e-3) / 1e9,
                      "frac": B64 / (ms64 * 1e-3) / 1e9 / HBM_PEAK_GBS, "algorithmic_bytes_per_launch": B64,
                      "traffic": traffic.get("fp64")}}
+    # the same workload with the self-consistent potential (Hartree + LDA
+    # after every step), a shorter run
+    nscf = min(steps, 25)
+    s64 = run_gpu(s, ac, gpu.FP64, args.warmup, nscf, ev, scf=True)
+    s32 = run_gpu(s, ac, gpu.MIXED, args.warmup, nscf, ev, scf=True)
+    scf = {"steps": nscf, "ms_per_step": s32["ms"] / nscf, "value": nkb * nscf / (s32["ms"] * 1e-3),
+           "fp64": {"ms_per_step": s64["ms"] / nscf, "value": nkb * nscf / (s64["ms"] * 1e-3)},
+           "J_rel_err": float(np.max(np.abs(s32["J"] - s64["J"])) / np.max(np.abs(s64["J"])))}
     cpu = None
     if not args.no_cpu:
         try:
@@ -223,6 +233,7 @@ def main():
         "mixed_vs_fp64_speedup": v32 / v64,
         "J_rel_err": jerr, "E_rel_err": eerr,
         "roofline": roof,
+        "scf": scf,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))

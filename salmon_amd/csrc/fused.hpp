@@ -36,6 +36,8 @@ struct FusedArgs {
   int NK, NB, bpw, nbg;
   int prop;              // 1: Taylor step at k0 + am
   int obs;               // 1: observables of the input orbitals at k0 + an
+  int rho_post;          // 1: the density of psi(t+dt), summed where it is stored
+                         // (the observables sweep then leaves the density alone)
   double am[3], an[3];
   T z1i, z2r, z3i, z4r;
   T lapc[3][5];          // -lap(n)/2 per axis (propagation) -- uniform, kept in SGPRs
@@ -195,7 +197,10 @@ __host__ __device__ constexpr bool fused_defer() {
   return SALMON_FUSED_DEFER && sizeof(T) == 4;
 }
 
-template <typename T, int NX, int NY, int NZ, int XS, int WPE, bool OV>
+// RP: the density of psi(t+dt) is summed where psi(t+dt) is stored (a.rho_post
+// set by the host); a separate instantiation, so the plain step keeps its
+// register budget.
+template <typename T, int NX, int NY, int NZ, int XS, int WPE, bool OV, bool RP>
 __global__ void __launch_bounds__(NY * NZ * XS, WPE) k_fused_step(FusedArgs<T> a) {
   constexpr int NYZ = NY * NZ, NT = NYZ * XS, NL = NX * NYZ;
   constexpr int PX = NX / XS;                 // own x points per thread
@@ -485,7 +490,7 @@ __global__ void __launch_bounds__(NY * NZ * XS, WPE) k_fused_step(FusedArgs<T> a
             const double a2 = er * er + ei * ei;
             n2 += a2;
             vs += v64[i * NYZ] * a2;
-            rho_acc[(x0 + i) * NYZ + yz] += occ * a2;
+            if constexpr (!RP) rho_acc[(x0 + i) * NYZ + yz] += occ * a2;
             te += er * (double)Lr + ei * (double)Li;
 #pragma unroll
             for (int d = 0; d < 3; ++d) js[d] += er * (double)gi[d] - ei * (double)gr[d];
@@ -559,6 +564,13 @@ __global__ void __launch_bounds__(NY * NZ * XS, WPE) k_fused_step(FusedArgs<T> a
       if (a.prop) {
 #pragma unroll
         for (int i = 0; i < PX; ++i) zo[(x0 + i) * NYZ + yz] = acc[i];
+        if constexpr (RP) {
+#pragma unroll
+          for (int i = 0; i < PX; ++i) {
+            const double er = (double)acc[i].re, ei = (double)acc[i].im;
+            rho_acc[(x0 + i) * NYZ + yz] += occ * (er * er + ei * ei);
+          }
+        }
       }
     } else {
     // psi(t+dt): to HBM, to the free LDS buffer and into the x window
@@ -568,6 +580,13 @@ __global__ void __launch_bounds__(NY * NZ * XS, WPE) k_fused_step(FusedArgs<T> a
       zo[p] = acc[i];
       out[p] = acc[i];
       win[H + i] = acc[i];
+    }
+    if constexpr (RP) {
+#pragma unroll
+      for (int i = 0; i < PX; ++i) {
+        const double er = (double)acc[i].re, ei = (double)acc[i].im;
+        rho_acc[(x0 + i) * NYZ + yz] += occ * (er * er + ei * ei);
+      }
     }
     // its observables unless the launch is the first half of an ETRS step
     // (a uniform branch: the barriers inside are taken by all threads or none)
@@ -606,7 +625,7 @@ __global__ void __launch_bounds__(NY * NZ * XS, WPE) k_fused_step(FusedArgs<T> a
         const double a2 = er * er + ei * ei;
         n2 += a2;
         vs += vrow[i * NYZ] * a2;
-        rrow[i * NYZ] += occ * a2;
+        if constexpr (!RP) rrow[i * NYZ] += occ * a2;
         te += er * (double)lr + ei * (double)li;
 #pragma unroll
         for (int d = 0; d < 3; ++d) js[d] += er * (double)gi[d] - ei * (double)gr[d];
@@ -879,9 +898,9 @@ static size_t fused_lds_bytes() {
          sizeof(double) * 10 * ((NY * NZ * XS + 63) / 64);
 }
 
-template <typename T, int NX, int NY, int NZ, int XS, int WPE, bool OV>
-static int fused_launch(salmon_ctx* c, const salmon::FusedArgs<T>& args, int nwg) {
-  auto kern = salmon::k_fused_step<T, NX, NY, NZ, XS, WPE, OV>;
+template <typename T, int NX, int NY, int NZ, int XS, int WPE, bool OV, bool RP>
+static int fused_launch_rp(salmon_ctx* c, const salmon::FusedArgs<T>& args, int nwg) {
+  auto kern = salmon::k_fused_step<T, NX, NY, NZ, XS, WPE, OV, RP>;
   const size_t lds = fused_lds_bytes<T, NX, NY, NZ, XS>();
   // set once per kernel instantiation (a thread-safe static initialisation)
   static const hipError_t attr =
@@ -892,13 +911,22 @@ static int fused_launch(salmon_ctx* c, const salmon::FusedArgs<T>& args, int nwg
   return SALMON_OK;
 }
 
+template <typename T, int NX, int NY, int NZ, int XS, int WPE, bool OV>
+static int fused_launch(salmon_ctx* c, const salmon::FusedArgs<T>& args, int nwg) {
+  if (args.rho_post) return fused_launch_rp<T, NX, NY, NZ, XS, WPE, OV, true>(c, args, nwg);
+  return fused_launch_rp<T, NX, NY, NZ, XS, WPE, OV, false>(c, args, nwg);
+}
+
 // One launch over all (k, band): with amid, the Taylor-4 step at kAc0 + amid;
 // with aobs, the observables at kAc0 + aobs (density into d_rho, the 8
 // result rows into res_dev) -- of the orbitals as they are on entry for the
 // complex64 kernel (fused_defer), of psi(t+dt) for the FP64 kernel, which is
 // always called with both.  For complex64 either may be null (not both).
+// rho_post (with amid): the density of psi(t+dt) into d_rho instead of that
+// of the entry orbitals.
 template <typename T>
-static int fused_step(salmon_ctx* c, const double* amid, const double* aobs, double dt, double* res_dev) {
+static int fused_step(salmon_ctx* c, const double* amid, const double* aobs, double dt, double* res_dev,
+                      bool rho_post = false) {
   FusedState* f = (FusedState*)c->fz;
   if (!f || !f->tables) return fail(c, SALMON_ERR_UNSUPPORTED, "fused tables not built");
   int ncu = 256;
@@ -937,6 +965,7 @@ static int fused_step(salmon_ctx* c, const double* amid, const double* aobs, dou
   a.NK = c->NK; a.NB = c->NB; a.bpw = bpw; a.nbg = nbg;
   a.prop = amid != nullptr;
   a.obs = aobs != nullptr;
+  a.rho_post = rho_post && amid != nullptr;
   for (int d = 0; d < 3; ++d) {
     a.am[d] = amid ? amid[d] : 0.0;
     a.an[d] = aobs ? aobs[d] : 0.0;
@@ -967,8 +996,10 @@ static int fused_step(salmon_ctx* c, const double* amid, const double* aobs, dou
     r = a.NOV > 0 ? fused_launch<T, 16, 16, 16, kFusedXS32, kFusedWPE32, true>(c, a, nwg)
                   : fused_launch<T, 16, 16, 16, kFusedXS32, kFusedWPE32, false>(c, a, nwg);
   }
-  if (r || !aobs) return r;
-  hipLaunchKernelGGL(salmon::k_sum_rows, dim3(1), dim3(256), 0, c->stream, f->wg_rows, nwg, res_dev);
-  LAUNCHCHK(c);
+  if (r || !(aobs || a.rho_post)) return r;
+  if (aobs) {
+    hipLaunchKernelGGL(salmon::k_sum_rows, dim3(1), dim3(256), 0, c->stream, f->wg_rows, nwg, res_dev);
+    LAUNCHCHK(c);
+  }
   return density_reduce(c, f->rho_part, nwg);
 }

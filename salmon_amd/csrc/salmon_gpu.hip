@@ -448,35 +448,62 @@ int rt_run_fused(salmon_ctx* c, const double* ac, int nsteps, double dt, int reo
 // (dt_evolve.f90:218-348): Taylor-4 over dt/2 at kAc0 + A_n with Vloc(t),
 // the extrapolated potential 3 V - 3 V_old1 + V_old2, Taylor-4 over dt/2 at
 // kAc0 + A_{n+1} with it, then the same update from rho(t+dt).
+//
+// On the fused path the step's launch also sums rho(t+dt) as it stores
+// psi(t+dt).  The complex64 kernel's observables of step n are those it
+// evaluates at the start of the next launch (or of an observables-only launch
+// before a re-orthonormalisation and at the end), which reads psi(t_{n+1})
+// with Vloc(t_{n+1}) already in place: the same order as above without a
+// separate pass over the orbitals.
 template <typename T>
 int rt_run_scf(salmon_ctx* c, const double* ac, int nsteps, double dt, int reortho_every, double* d_res) {
   int r = SALMON_OK;
   const bool etrs = c->propagator == SALMON_PROP_ETRS;
-  auto taylor = [&](const double a[3], double h) {
-    if (fused_ready(c)) return fused_step<T>(c, a, nullptr, h, nullptr);
-    return taylor_streamed<T>(c, c->d_kac0, a[0], a[1], a[2], h);
-  };
+  const bool fused = fused_ready(c);
+  const bool defer = fused && salmon::fused_defer<T, kFusedXS64>();
   const dim3 gr((c->g.NL + 255) / 256), bl(256);
+  double aobs[3];
+  bool pending = false;  // observables of the previous step not yet evaluated (deferred form)
+  // one Taylor launch; `last` marks the step's final one (it sums the density)
+  auto taylor = [&](const double a[3], double h, bool last, int n) {
+    if (!fused) return taylor_streamed<T>(c, c->d_kac0, a[0], a[1], a[2], h);
+    const bool with_obs = defer && pending;
+    const int q = fused_step<T>(c, a, with_obs ? aobs : nullptr, h, with_obs ? d_res + 8 * (size_t)(n - 1) : nullptr,
+                                last);
+    if (with_obs) pending = false;
+    return q;
+  };
   for (int n = 0; n < nsteps && r == SALMON_OK; ++n) {
     double amid[3];
     const double* anew = ac + 3 * (n + 1);
     if (etrs) {
-      if ((r = taylor(ac + 3 * n, 0.5 * dt))) break;
+      if ((r = taylor(ac + 3 * n, 0.5 * dt, false, n))) break;
       hipLaunchKernelGGL(salmon::scf::k_vloc_extrapolate, gr, bl, 0, c->stream, c->d_vloc, c->d_vloc_f, c->d_vold,
                          c->d_vold + c->g.NL, c->g.NL);
       LAUNCHCHK(c);
-      r = taylor(anew, 0.5 * dt);
+      r = taylor(anew, 0.5 * dt, true, n);
     } else {
       for (int d = 0; d < 3; ++d) amid[d] = 0.5 * (ac[3 * n + d] + anew[d]);
-      r = taylor(amid, dt);
+      r = taylor(amid, dt, true, n);
     }
-    if (r || (r = density_dev<T>(c))) break;
+    if (r || (!fused && (r = density_dev<T>(c)))) break;
     if ((r = scf_potential(c, c->d_rho, true, c->scf_xc, nullptr, nullptr))) break;
+    if (defer) {
+      for (int d = 0; d < 3; ++d) aobs[d] = anew[d];
+      pending = true;
+      if (reortho_every > 0 && (n + 1) % reortho_every == 0) {
+        if ((r = fused_step<T>(c, nullptr, aobs, dt, d_res + 8 * (size_t)n))) break;
+        pending = false;
+        r = gram_schmidt_dev(c);
+      }
+      continue;
+    }
     if ((r = observe_rows<T>(c, c->d_kac0, anew[0], anew[1], anew[2]))) break;
     hipLaunchKernelGGL(k_sum_rows, dim3(1), dim3(256), 0, c->stream, c->d_rows, c->NKB, d_res + 8 * (size_t)n);
     LAUNCHCHK(c);
     if (reortho_every > 0 && (n + 1) % reortho_every == 0) r = gram_schmidt_dev(c);
   }
+  if (r == SALMON_OK && pending) r = fused_step<T>(c, nullptr, aobs, dt, d_res + 8 * (size_t)(nsteps - 1));
   return r;
 }
 

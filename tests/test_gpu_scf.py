@@ -112,6 +112,18 @@ def test_scf_mixed_close_to_fp64(prop):
     print(f"[gpu] scf {prop} mixed J err {jerr:.2e}; scf vs frozen {dfro:.2e}", flush=True)
     assert jerr < 1e-5
     assert dfro > 1e-6  # the potential does move
+    # with re-orthonormalisation every 7 steps (the observables of those
+    # steps are taken before it) both precisions agree the same way
+    re = {}
+    for prec in (gpu.FP64, gpu.MIXED):
+        with gpu.Context(s, prec) as c:
+            c.set_propagator(prop)
+            c.set_scf(vpsl, gpu.XC_PZ)
+            re[prec] = c.rt_run(ac, nsteps, reortho_every=7)
+    jerr_re = np.max(np.abs(re[gpu.MIXED]["J"] - re[gpu.FP64]["J"])) / np.max(np.abs(re[gpu.FP64]["J"]))
+    print(f"[gpu] scf {prop} reortho 7: mixed J err {jerr_re:.2e}", flush=True)
+    assert jerr_re < 1e-5
+    assert rel(re[gpu.FP64]["J"][:7], J64[:7]) < 1e-13  # identical up to the first re-orthonormalisation
 
 
 def test_scf_errors_and_switch_off():
