@@ -38,6 +38,8 @@ struct State {
   unsigned* cnt;     // (NK) arrival counters of the last-block reductions
   cplx<double>* zs;  // (NK*NB) projections on the lower bands
   cplx<double>* gram;  // (NK*NB*NB) G(j,i) = Hxyz <B_j|B_i>, i < j, of the stored bands
+  cplx<double>* minv;  // (NK*NB*NB) rows of (I + L)^-1, L(j,i) = G(j,i) for i < j
+  double* minvr;       // (NK*NB*NB) rows of (I + Re L)^-1
   const double* occ; // (NK*NB)
   int NK, NB, NL, nchunk;
   int pstride;       // doubles of part per k: nchunk*ceil(NB/kJG)*kPartQ
@@ -102,10 +104,16 @@ struct Src {
 // bands, which the lower bands need not make diagonal (the start vector's
 // overlap keeps only its real part, CG.f90:46,70, so a band may keep an
 // imaginary overlap with the bands below it).  k_cg_proj forms b in one
-// parallel pass and solves the triangle in the last block:
-//   PROJ_START  real coefficients (real(8) :: s, CG.f90:46,70)
+// parallel pass and solves the triangle in the last block, as c = M b with
+// M = (I + L)^-1 (L the strictly lower part of G): G does not change while a
+// band iterates, so M gains one row per stored band and each solve is a
+// parallel matrix-vector product instead of ib dependent steps.
+//   PROJ_START  real coefficients (real(8) :: s, CG.f90:46,70): the
+//               recurrence keeps c real, so c = (I + Re L)^-1 Re b
 //   PROJ_VEC    complex coefficients (CG.f90:86-89)
-//   PROJ_GRAM   stores row ib of G from the stored band ib (no solve).
+//   PROJ_GRAM   stores row ib of G from the stored band ib, and row ib of
+//               both inverses: M(ib, l) = -sum_{l<=q<ib} G(ib, q) M(q, l)
+//               (unit diagonal implicit).
 // Grid: (nchunk, ceil(ib/kJG), NK).
 enum : int { PROJ_START = 0, PROJ_VEC = 1, PROJ_GRAM = 2, PROJ_PLAIN = 3 };
 constexpr int kCgMaxNB = 1024;
@@ -167,36 +175,43 @@ __global__ void __launch_bounds__(256) k_cg_proj(State S, const cplx<double>* __
   if (threadIdx.x == 0) __hip_atomic_store(&S.cnt[k], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   __syncthreads();
   cplx<double>* gram = S.gram + (size_t)k * S.NB * S.NB;
+  cplx<double>* M = S.minv + (size_t)k * S.NB * S.NB;
+  double* MR = S.minvr + (size_t)k * S.NB * S.NB;
   if (mode == PROJ_GRAM) {
     // raw_j = Hxyz <B_j|B_ib>, so G(ib, j) = conj(raw_j)
     for (int j = threadIdx.x; j < ib; j += 256) gram[(size_t)ib * S.NB + j] = cplx<double>{zsh[j].re, -zsh[j].im};
+    // the unit diagonal of M is implicit (band 0 has no projection launch)
+    for (int l = threadIdx.x; l < ib; l += 256) {
+      double sr = zsh[l].re, si = -zsh[l].im, rr = zsh[l].re;  // q = l: G(ib, l) M(l, l)
+      for (int q = l + 1; q < ib; ++q) {
+        const double gr = zsh[q].re, gi = -zsh[q].im;
+        const cplx<double> m = M[(size_t)q * S.NB + l];
+        sr += gr * m.re - gi * m.im;
+        si += gr * m.im + gi * m.re;
+        rr += gr * MR[(size_t)q * S.NB + l];
+      }
+      M[(size_t)ib * S.NB + l] = cplx<double>{-sr, -si};
+      MR[(size_t)ib * S.NB + l] = -rr;
+    }
     return;
   }
   if (mode == PROJ_PLAIN) {  // coefficients against the unmodified vector
     for (int j = threadIdx.x; j < ib; j += 256) S.zs[(size_t)k * S.NB + j] = zsh[j];
     return;
   }
-  if (threadIdx.x < kWave) {
-    const int lane = threadIdx.x;
-    for (int j = 0; j < ib; ++j) {
-      double sr = 0.0, si = 0.0;
-      for (int l = lane; l < j; l += kWave) {
-        const cplx<double> G = gram[(size_t)j * S.NB + l], z = zsh[l];
-        sr += G.re * z.re - G.im * z.im;
-        si += G.re * z.im + G.im * z.re;
+  for (int j = threadIdx.x; j < ib; j += 256) {
+    double cr = zsh[j].re, ci = mode == PROJ_START ? 0.0 : zsh[j].im;
+    if (mode == PROJ_START) {
+      for (int l = 0; l < j; ++l) cr += MR[(size_t)j * S.NB + l] * zsh[l].re;
+    } else {
+      for (int l = 0; l < j; ++l) {
+        const cplx<double> m = M[(size_t)j * S.NB + l], z = zsh[l];
+        cr += m.re * z.re - m.im * z.im;
+        ci += m.re * z.im + m.im * z.re;
       }
-      sr = wave_sum(sr);
-      si = wave_sum(si);
-      const cplx<double> b = zsh[j];
-      __builtin_amdgcn_wave_barrier();
-      if (lane == 0) zsh[j] = cplx<double>{b.re - sr, mode == PROJ_START ? 0.0 : b.im - si};
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
+    S.zs[(size_t)k * S.NB + j] = cplx<double>{cr, ci};
   }
-  __syncthreads();
-  for (int j = threadIdx.x; j < ib; j += 256) S.zs[(size_t)k * S.NB + j] = zsh[j];
 }
 
 // Roles of the subtraction kernel's reduction.

@@ -30,6 +30,8 @@ struct GsState {
   unsigned* cnt = nullptr;
   cplx<double>* zs = nullptr;
   cplx<double>* gram = nullptr;
+  cplx<double>* minv = nullptr;  // (I + L)^-1 rows of the lower-band triangle
+  double* minvr = nullptr;       // (I + Re L)^-1 rows
   double* tw = nullptr;    // DFT twiddles (3*kDftMax complex)
   double* symb = nullptr;  // kinetic symbol per axis (3*kDftMax)
   int pstride = 0;
@@ -141,6 +143,8 @@ static int gs_ensure(salmon_ctx* c) {
   if ((r = gs_alloc(c, s, &s->cnt, (size_t)s->NK))) return r;
   if ((r = gs_alloc(c, s, &s->zs, (size_t)s->NK * s->NB))) return r;
   if ((r = gs_alloc(c, s, &s->gram, (size_t)s->NK * s->NB * s->NB))) return r;
+  if ((r = gs_alloc(c, s, &s->minv, (size_t)s->NK * s->NB * s->NB))) return r;
+  if ((r = gs_alloc(c, s, &s->minvr, (size_t)s->NK * s->NB * s->NB))) return r;
   if ((r = gs_alloc(c, s, &s->tw, (size_t)2 * 3 * kDftMax))) return r;
   if ((r = gs_alloc(c, s, &s->symb, (size_t)3 * kDftMax))) return r;
   HIPCHK(c, hipMemsetAsync(s->cnt, 0, sizeof(unsigned) * s->NK, c->stream));
@@ -164,6 +168,8 @@ static State gs_view(salmon_ctx* c, GsState* s) {
   S.cnt = s->cnt;
   S.zs = s->zs;
   S.gram = s->gram;
+  S.minv = s->minv;
+  S.minvr = s->minvr;
   S.occ = c->d_occ;
   S.NK = s->NK;
   S.NB = s->NB;

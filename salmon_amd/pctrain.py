@@ -157,11 +157,14 @@ def main(argv=None):
     ap.add_argument("--nb", type=int, default=NB)
     ap.add_argument("--weights", default="salmon_amd/data/pcnet_cfg5.npy")
     ap.add_argument("--out", default=None)
+    ap.add_argument("--no-graphs", action="store_true", help="solve: run CG eagerly (no captured graphs)")
     a = ap.parse_args(argv)
     if a.mode == "solve":
         pc = {"none": gpu.PC_NONE, "kinetic": gpu.PC_KINETIC, "learned": gpu.PC_LEARNED}[a.precond]
         s = synth.config5(seed=a.seed, NB=a.nb)
         with gpu.Context(s, gpu.FP64) as c:
+            if a.no_graphs:
+                c.set_graphs(False)
             net = None
             if pc == gpu.PC_LEARNED:
                 net = gpu.PcNet(weights=np.load(a.weights))
