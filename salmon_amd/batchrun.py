@@ -23,6 +23,9 @@ def _log(msg):
     print(msg, flush=True)
 
 
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+
+
 def run(nsys=64, steps=10, scale=1.0, precisions=(gpu.FP64, gpu.MIXED), sequential=True, log=_log):
     t0 = time.time()
     systems = [synth.config4_system(i, scale=scale) for i in range(nsys)]
@@ -47,8 +50,15 @@ def run(nsys=64, steps=10, scale=1.0, precisions=(gpu.FP64, gpu.MIXED), sequenti
                    point_orbital_steps_per_s=pts * steps / dt, tune_s=tune_s,
                    zblock={f"{s.NLx}x{s.NLy}x{s.NLz}": int(z) for s, z in zip(systems, zb)},
                    finite=bool(np.all(np.isfinite(r["J"])) and np.all(np.isfinite(r["E"]))))
+        # HBM roofline with the algorithmic bytes of the streamed step, 12 S NL
+        # per orbital-step (four H applications and the observables pass)
+        S = 16 if prec == gpu.FP64 else 8
+        res["algorithmic_bytes_per_step"] = 12 * S * pts
+        res["roofline_gbs"] = res["algorithmic_bytes_per_step"] / (res["ms_per_step"] * 1e-3) / 1e9
+        res["roofline_frac"] = res["roofline_gbs"] / HBM_PEAK_GBS
         log(f"[batch] {name}: {res['ms_per_step']:.2f} ms/step, {res['orbital_steps_per_s']:.3e} orbital-steps/s "
-            f"(tune {tune_s:.1f} s, z blocking {res['zblock']})")
+            f"(tune {tune_s:.1f} s, z blocking {res['zblock']}); {res['roofline_gbs']:.0f} GB/s of algorithmic "
+            f"bytes = {res['roofline_frac']:.3f} of {HBM_PEAK_GBS:.0f} GB/s")
         if sequential:
             # the same kernels one context at a time (each context frees its
             # work buffers before the next one runs)
