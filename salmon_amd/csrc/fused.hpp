@@ -321,6 +321,11 @@ __global__ void __launch_bounds__(NY * NZ * XS, WPE) k_fused_step(FusedArgs<T> a
     cplx<T>* out = bufB;
 #pragma unroll 1
     for (int n = 0; n < 4; ++n) {
+      // the FP64 kernel's observables-only launch (RP instantiation, a.prop
+      // unset): no H application; the observables below read the entry orbital
+      if constexpr (RP && !DEFER) {
+        if (!a.prop) break;
+      }
       const bool obs = DEFER && n == 0 && a.obs;
       if constexpr (XS > 1) {
 #pragma unroll
@@ -577,7 +582,7 @@ __global__ void __launch_bounds__(NY * NZ * XS, WPE) k_fused_step(FusedArgs<T> a
 #pragma unroll
     for (int i = 0; i < PX; ++i) {
       const int p = (x0 + i) * NYZ + yz;
-      zo[p] = acc[i];
+      if (!RP || a.prop) zo[p] = acc[i];
       out[p] = acc[i];
       win[H + i] = acc[i];
     }
@@ -913,15 +918,19 @@ static int fused_launch_rp(salmon_ctx* c, const salmon::FusedArgs<T>& args, int 
 
 template <typename T, int NX, int NY, int NZ, int XS, int WPE, bool OV>
 static int fused_launch(salmon_ctx* c, const salmon::FusedArgs<T>& args, int nwg) {
-  if (args.rho_post) return fused_launch_rp<T, NX, NY, NZ, XS, WPE, OV, true>(c, args, nwg);
+  // the self-consistent run's launches: steps that sum rho(t+dt), and the
+  // FP64 kernel's observables-only launches
+  if (args.rho_post || (sizeof(T) == 8 && !args.prop))
+    return fused_launch_rp<T, NX, NY, NZ, XS, WPE, OV, true>(c, args, nwg);
   return fused_launch_rp<T, NX, NY, NZ, XS, WPE, OV, false>(c, args, nwg);
 }
 
 // One launch over all (k, band): with amid, the Taylor-4 step at kAc0 + amid;
 // with aobs, the observables at kAc0 + aobs (density into d_rho, the 8
 // result rows into res_dev) -- of the orbitals as they are on entry for the
-// complex64 kernel (fused_defer), of psi(t+dt) for the FP64 kernel, which is
-// always called with both.  For complex64 either may be null (not both).
+// complex64 kernel (fused_defer), of psi(t+dt) for the FP64 kernel, or of
+// the entry orbitals for an FP64 launch without amid (no density then).
+// Either may be null (not both).
 // rho_post (with amid): the density of psi(t+dt) into d_rho instead of that
 // of the entry orbitals.
 template <typename T>
@@ -1001,5 +1010,7 @@ static int fused_step(salmon_ctx* c, const double* amid, const double* aobs, dou
     hipLaunchKernelGGL(salmon::k_sum_rows, dim3(1), dim3(256), 0, c->stream, f->wg_rows, nwg, res_dev);
     LAUNCHCHK(c);
   }
+  // the FP64 observables-only launch sums no density (d_rho keeps rho(t+dt))
+  if (sizeof(T) == 8 && !a.prop) return SALMON_OK;
   return density_reduce(c, f->rho_part, nwg);
 }

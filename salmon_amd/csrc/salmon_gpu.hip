@@ -498,9 +498,14 @@ int rt_run_scf(salmon_ctx* c, const double* ac, int nsteps, double dt, int reort
       }
       continue;
     }
-    if ((r = observe_rows<T>(c, c->d_kac0, anew[0], anew[1], anew[2]))) break;
-    hipLaunchKernelGGL(k_sum_rows, dim3(1), dim3(256), 0, c->stream, c->d_rows, c->NKB, d_res + 8 * (size_t)n);
-    LAUNCHCHK(c);
+    if (fused) {
+      // FP64: an observables-only launch with Vloc(t+dt) in place
+      if ((r = fused_step<T>(c, nullptr, anew, dt, d_res + 8 * (size_t)n))) break;
+    } else {
+      if ((r = observe_rows<T>(c, c->d_kac0, anew[0], anew[1], anew[2]))) break;
+      hipLaunchKernelGGL(k_sum_rows, dim3(1), dim3(256), 0, c->stream, c->d_rows, c->NKB, d_res + 8 * (size_t)n);
+      LAUNCHCHK(c);
+    }
     if (reortho_every > 0 && (n + 1) % reortho_every == 0) r = gram_schmidt_dev(c);
   }
   if (r == SALMON_OK && pending) r = fused_step<T>(c, nullptr, aobs, dt, d_res + 8 * (size_t)(nsteps - 1));
