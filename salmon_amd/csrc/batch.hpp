@@ -130,6 +130,36 @@ __global__ void __launch_bounds__(256) k_b_kbapply(const Sys<T>* __restrict__ S,
 // register window of ZB+8 values; the y and x terms and the potential are
 // added per point in the order of hpsi_point, then the touched point's KB term
 // (k_b_kbapply).
+// ZB consecutive values from src; complex64 runs of an even length as 16-byte
+// loads (the callers' runs start at even indices of 16-byte aligned orbitals)
+template <typename T, int ZB>
+__device__ __forceinline__ void load_run(const cplx<T>* __restrict__ src, cplx<T> (&v)[ZB]) {
+  if constexpr (sizeof(T) == 4 && ZB % 2 == 0) {
+    const float4* s4 = reinterpret_cast<const float4*>(src);
+#pragma unroll
+    for (int h = 0; h < ZB / 2; ++h) {
+      const float4 f = s4[h];
+      v[2 * h] = cplx<T>{f.x, f.y};
+      v[2 * h + 1] = cplx<T>{f.z, f.w};
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < ZB; ++q) v[q] = src[q];
+  }
+}
+
+template <typename T, int ZB>
+__device__ __forceinline__ void store_run(cplx<T>* __restrict__ dst, const cplx<T> (&v)[ZB]) {
+  if constexpr (sizeof(T) == 4 && ZB % 2 == 0) {
+    float4* d4 = reinterpret_cast<float4*>(dst);
+#pragma unroll
+    for (int h = 0; h < ZB / 2; ++h) d4[h] = float4{v[2 * h].re, v[2 * h].im, v[2 * h + 1].re, v[2 * h + 1].im};
+  } else {
+#pragma unroll
+    for (int q = 0; q < ZB; ++q) dst[q] = v[q];
+  }
+}
+
 template <typename T, int ZB>
 __device__ __forceinline__ void hpsi_zblock(const cplx<T>* __restrict__ E, int p0, double kx, double ky, double kz,
                                             const T* __restrict__ vloc, const Grid& g, const KBTables& kb,
@@ -146,52 +176,62 @@ __device__ __forceinline__ void hpsi_zblock(const cplx<T>* __restrict__ E, int p
   }
   const T A = (T)((kx * kx + ky * ky + kz * kz - (g.lap[0][0] + g.lap[1][0] + g.lap[2][0])) * 0.5);
   const double kk[3] = {kx, ky, kz};
+  // per own point: z neighbours (window), then y, then x, each n = 1..4
+  T vr[ZB], vi[ZB], wr[ZB], wi[ZB];
 #pragma unroll
   for (int q = 0; q < ZB; ++q) {
-    const int p = p0 + q;
-    T vr = 0, vi = 0, wr = 0, wi = 0;
+    vr[q] = vi[q] = wr[q] = wi[q] = 0;
 #pragma unroll
     for (int n = 1; n <= 4; ++n) {
       const cplx<T> ep = win[4 + q + n], em = win[4 + q - n];
       const T C = (T)g.lap[2][n], D = (T)(kz * g.nab[2][n]);
-      vr += C * (ep.re + em.re);
-      vi += C * (ep.im + em.im);
-      wr += D * (ep.re - em.re);
-      wi += D * (ep.im - em.im);
+      vr[q] += C * (ep.re + em.re);
+      vi[q] += C * (ep.im + em.im);
+      wr[q] += D * (ep.re - em.re);
+      wi[q] += D * (ep.im - em.im);
     }
+  }
 #pragma unroll
-    for (int ax3 = 1; ax3 >= 0; --ax3) {
-      const int n_ax = ax3 == 1 ? NLy : g.NLx;
-      const int c_ax = ax3 == 1 ? iy : ix;
-      const int stride = ax3 == 1 ? NLz : NLz * NLy;
-      const int base = p - c_ax * stride;
+  for (int ax3 = 1; ax3 >= 0; --ax3) {
+    const int n_ax = ax3 == 1 ? NLy : g.NLx;
+    const int c_ax = ax3 == 1 ? iy : ix;
+    const int stride = ax3 == 1 ? NLz : NLz * NLy;
+    const int base = p0 - c_ax * stride;
 #pragma unroll
-      for (int n = 1; n <= 4; ++n) {
-        int cp = c_ax + n;
-        if (cp >= n_ax) cp -= n_ax;
-        int cm = c_ax - n;
-        if (cm < 0) cm += n_ax;
-        const cplx<T> ep = E[base + cp * stride], em = E[base + cm * stride];
-        const T C = (T)g.lap[ax3][n], D = (T)(kk[ax3] * g.nab[ax3][n]);
-        vr += C * (ep.re + em.re);
-        vi += C * (ep.im + em.im);
-        wr += D * (ep.re - em.re);
-        wi += D * (ep.im - em.im);
+    for (int n = 1; n <= 4; ++n) {
+      int cp = c_ax + n;
+      if (cp >= n_ax) cp -= n_ax;
+      int cm = c_ax - n;
+      if (cm < 0) cm += n_ax;
+      cplx<T> ep[ZB], em[ZB];
+      load_run<T, ZB>(E + base + cp * stride, ep);
+      load_run<T, ZB>(E + base + cm * stride, em);
+      const T C = (T)g.lap[ax3][n], D = (T)(kk[ax3] * g.nab[ax3][n]);
+#pragma unroll
+      for (int q = 0; q < ZB; ++q) {
+        vr[q] += C * (ep[q].re + em[q].re);
+        vi[q] += C * (ep[q].im + em[q].im);
+        wr[q] += D * (ep[q].re - em[q].re);
+        wi[q] += D * (ep[q].im - em[q].im);
       }
     }
+  }
+  cplx<T> f[ZB];
+#pragma unroll
+  for (int q = 0; q < ZB; ++q) {
+    const int p = p0 + q;
     const cplx<T> e = win[4 + q];
     const T V = vloc[p];
-    cplx<T> f;
-    f.re = (V + A) * e.re - (T)0.5 * vr + wi;
-    f.im = (V + A) * e.im - (T)0.5 * vi - wr;
+    f[q].re = (V + A) * e.re - (T)0.5 * vr[q] + wi[q];
+    f[q].im = (V + A) * e.im - (T)0.5 * vi[q] - wr[q];
     const int t = kb.pt2t[p];
     if (t >= 0) {
       const cplx<T> v = kbv[t];
-      f.re += v.re;
-      f.im += v.im;
+      f[q].re += v.re;
+      f[q].im += v.im;
     }
-    out[p] = f;
   }
+  store_run<T, ZB>(out + p0, f);
 }
 
 // Stencil + local + KB of Taylor term `term`: h[term] = H (term == 0 ? zu : h[term-1]).
