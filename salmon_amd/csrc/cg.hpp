@@ -22,6 +22,13 @@ namespace cg {
 #endif
 constexpr int kPPT = SALMON_CG_PPT;  // points per thread per chunk
 constexpr int kChunk = 256 * kPPT;   // points per block
+// the lower-band subtractions (k_cg_sub, k_gs_sub) read ib bands per point and
+// run on finer chunks: more workgroups keep more of those loads in flight
+#ifndef SALMON_CG_SUB_PPT
+#define SALMON_CG_SUB_PPT 1
+#endif
+constexpr int kSubPPT = SALMON_CG_SUB_PPT;
+constexpr int kSubChunk = 256 * kSubPPT;
 constexpr int kJG = 8;               // lower bands per projection block
 
 // per-k scalars (double slots)
@@ -42,6 +49,7 @@ struct State {
   double* minvr;       // (NK*NB*NB) rows of (I + Re L)^-1
   const double* occ; // (NK*NB)
   int NK, NB, NL, nchunk;
+  int nsub;          // chunks of the subtraction kernels (kSubChunk points)
   int pstride;       // doubles of part per k: nchunk*ceil(NB/kJG)*kPartQ
   double Hxyz;
 };
@@ -68,6 +76,7 @@ __device__ bool last_block_sum(double (&v)[NQ], const State& S, int nblk, int bi
   if (threadIdx.x == 0) {
 #pragma unroll
     for (int q = 0; q < NQ; ++q) tot[q] = 0.0;
+#pragma unroll 8
     for (int b = 0; b < nblk; ++b) {
 #pragma unroll
       for (int q = 0; q < NQ; ++q) tot[q] += __hip_atomic_load(&part[b * NQ + q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -165,6 +174,7 @@ __global__ void __launch_bounds__(256) k_cg_proj(State S, const cplx<double>* __
   for (int j = threadIdx.x; j < ib; j += 256) {
     const int grp = j / kJG, jj = j % kJG;
     double sr = 0.0, si = 0.0;
+#pragma unroll 8
     for (int cc = 0; cc < S.nchunk; ++cc) {
       const size_t b = (size_t)(grp * S.nchunk + cc) * kPartQ;
       sr += __hip_atomic_load(&part[b + 2 * jj], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -228,7 +238,7 @@ enum : int {
 // out = src - sum_{j<ib} zs(j) B_j  (CG.f90:69-71, 86-89), then the role's
 // reduction.  For SUB_GRAD / SUB_PCDIR the finisher forms the conjugation
 // ratio uk = s/gkgk (CG.f90:92-98; iteration 1 takes p = direction).
-// Grid (nchunk, NK).
+// Grid (nsub, NK).
 __global__ void __launch_bounds__(256) k_cg_sub(State S, const cplx<double>* __restrict__ zu, int ib, Src src,
                                                 cplx<double>* out, const cplx<double>* __restrict__ other,
                                                 int role, int it, int ncg) {
@@ -241,10 +251,11 @@ __global__ void __launch_bounds__(256) k_cg_sub(State S, const cplx<double>* __r
   const cplx<double>* zs = S.zs + (size_t)k * S.NB;
   double v[1] = {0.0};
 #pragma unroll
-  for (int i = 0; i < kPPT; ++i) {
-    const int p = c * kChunk + i * 256 + threadIdx.x;
+  for (int i = 0; i < kSubPPT; ++i) {
+    const int p = c * kSubChunk + i * 256 + threadIdx.x;
     if (p < S.NL) {
       cplx<double> e = src.at(k, S.NL, p, xhx);
+#pragma unroll 8
       for (int j = 0; j < ib; ++j) {
         const cplx<double> b = B[(size_t)j * S.NL + p], z = zs[j];
         e.re -= b.re * z.re - b.im * z.im;
@@ -265,7 +276,7 @@ __global__ void __launch_bounds__(256) k_cg_sub(State S, const cplx<double>* __r
     return;
   }
   double tot[1];
-  if (!last_block_sum<1>(v, S, S.nchunk, c, tot)) return;
+  if (!last_block_sum<1>(v, S, S.nsub, c, tot)) return;
   if (threadIdx.x != 0) return;
   double* sc = S.sc + k * kNS;
   if (role == SUB_START) {
@@ -667,6 +678,7 @@ __global__ void __launch_bounds__(256) k_sub_matrix_sum(const double* __restrict
   const int idx = blockIdx.x * 256 + threadIdx.x;
   if (idx >= NB * NB * 2) return;
   double s = 0.0;
+#pragma unroll 8
   for (int sl = 0; sl < nslice; ++sl) s += part[((size_t)k * nslice + sl) * NB * NB * 2 + idx];
   M[(size_t)k * NB * NB * 2 + idx] = s * scale;
 }
@@ -704,7 +716,7 @@ __global__ void __launch_bounds__(256) k_rotate_bands(const cplx<double>* __rest
 
 // Gram-Schmidt step of band ib in place (Gram_Schmidt_ompk,
 // ARTED/GS/Gram_Schmidt.f90:43-50): v -= sum_{j<ib} zs_j B_j; with_norm ends
-// with SN = 1/sqrt(|v|^2 Hxyz).  Grid (nchunk, NK).
+// with SN = 1/sqrt(|v|^2 Hxyz).  Grid (nsub, NK).
 __global__ void __launch_bounds__(256) k_gs_sub(State S, cplx<double>* __restrict__ zu, int ib, int with_norm) {
   const int k = blockIdx.y, c = blockIdx.x;
   cplx<double>* B = zu + (size_t)k * S.NB * S.NL;
@@ -712,10 +724,11 @@ __global__ void __launch_bounds__(256) k_gs_sub(State S, cplx<double>* __restric
   const cplx<double>* zs = S.zs + (size_t)k * S.NB;
   double a[1] = {0.0};
 #pragma unroll
-  for (int i = 0; i < kPPT; ++i) {
-    const int p = c * kChunk + i * 256 + threadIdx.x;
+  for (int i = 0; i < kSubPPT; ++i) {
+    const int p = c * kSubChunk + i * 256 + threadIdx.x;
     if (p < S.NL) {
       cplx<double> e = v[p];
+#pragma unroll 8
       for (int j = 0; j < ib; ++j) {
         const cplx<double> b = B[(size_t)j * S.NL + p], z = zs[j];
         e.re -= b.re * z.re - b.im * z.im;
@@ -727,7 +740,7 @@ __global__ void __launch_bounds__(256) k_gs_sub(State S, cplx<double>* __restric
   }
   if (!with_norm) return;
   double tot[1];
-  if (!last_block_sum<1>(a, S, S.nchunk, c, tot)) return;
+  if (!last_block_sum<1>(a, S, S.nsub, c, tot)) return;
   if (threadIdx.x == 0) S.sc[k * kNS + SN] = 1.0 / sqrt(tot[0] * S.Hxyz);
 }
 

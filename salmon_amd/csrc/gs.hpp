@@ -129,7 +129,7 @@ static int gs_ensure(salmon_ctx* c) {
   s->Nlma = c->kb.Nlma;
   s->nchunk = (s->NL + kChunk - 1) / kChunk;
   s->ngroup = (s->NB + kJG - 1) / kJG;
-  s->pstride = s->nchunk * s->ngroup * kPartQ;
+  s->pstride = std::max(s->nchunk * s->ngroup * kPartQ, (s->NL + kSubChunk - 1) / kSubChunk);
   const size_t nv = (size_t)s->NK * s->NL;
   int r;
   for (int i = 0; i < 11; ++i)
@@ -175,6 +175,7 @@ static State gs_view(salmon_ctx* c, GsState* s) {
   S.NB = s->NB;
   S.NL = s->NL;
   S.nchunk = s->nchunk;
+  S.nsub = (s->NL + kSubChunk - 1) / kSubChunk;
   S.pstride = s->pstride;
   S.Hxyz = c->g.Hxyz;
   return S;
@@ -209,7 +210,7 @@ static int cg_run(salmon_ctx* c, int ncg, int precond) {
   cplx<double>* zu = (cplx<double>*)c->d_zu;
   cplx<double>** w = s->w;
   const int NK = s->NK, NB = s->NB, NL = s->NL;
-  const dim3 gv(s->nchunk, NK), blk(256);
+  const dim3 gv(s->nchunk, NK), blk(256), gsub(S.nsub, NK);
   const size_t bstride = (size_t)NB * NL;
   int r;
   auto H = [&](const cplx<double>* in, cplx<double>* out, cplx<double>* tt) {
@@ -227,7 +228,7 @@ static int cg_run(salmon_ctx* c, int ncg, int precond) {
     const Src start{zu + (size_t)ib * NL, bstride, nullptr, nullptr};
     // start vector orthogonalised to the lower bands and normalised (CG.f90:67-74)
     if ((r = proj(ib, start, PROJ_START, 0))) return r;
-    hipLaunchKernelGGL(k_cg_sub, gv, blk, 0, c->stream, S, (const cplx<double>*)zu, ib, start, w[W_X],
+    hipLaunchKernelGGL(k_cg_sub, gsub, blk, 0, c->stream, S, (const cplx<double>*)zu, ib, start, w[W_X],
                        (const cplx<double>*)nullptr, (int)SUB_START, 0, ncg);
     LAUNCHCHK(c);
     hipLaunchKernelGGL(k_cg_scale, gv, blk, 0, c->stream, S, w[W_X], (cplx<double>*)nullptr, (size_t)0, 0);
@@ -239,7 +240,7 @@ static int cg_run(salmon_ctx* c, int ncg, int precond) {
       // residual orthogonalised to the lower bands (CG.f90:85-90)
       if ((r = proj(ib, resid, PROJ_VEC, 1))) return r;
       const cplx<double>* dir = w[W_G];
-      hipLaunchKernelGGL(k_cg_sub, gv, blk, 0, c->stream, S, (const cplx<double>*)zu, ib, resid, w[W_G],
+      hipLaunchKernelGGL(k_cg_sub, gsub, blk, 0, c->stream, S, (const cplx<double>*)zu, ib, resid, w[W_G],
                          (const cplx<double>*)nullptr, (int)(precond == SALMON_PC_NONE ? SUB_GRAD : SUB_NONE), it, ncg);
       LAUNCHCHK(c);
       if (s->recording && s->smp_n < s->smp_cap && NK == 1 && (s->smp_seen++ % s->smp_stride) == 0) {
@@ -257,7 +258,7 @@ static int cg_run(salmon_ctx* c, int ncg, int precond) {
         }
         const Src kg{w[W_KG], (size_t)NL, nullptr, nullptr};
         if ((r = proj(ib, kg, PROJ_VEC, 1))) return r;
-        hipLaunchKernelGGL(k_cg_sub, gv, blk, 0, c->stream, S, (const cplx<double>*)zu, ib, kg, w[W_KG],
+        hipLaunchKernelGGL(k_cg_sub, gsub, blk, 0, c->stream, S, (const cplx<double>*)zu, ib, kg, w[W_KG],
                            (const cplx<double>*)w[W_G], (int)(precond == SALMON_PC_LEARNED ? SUB_LEARN : SUB_PCDIR),
                            it, ncg);
         LAUNCHCHK(c);
@@ -306,18 +307,18 @@ static int gram_schmidt_multi(salmon_ctx* c, cplx<double>* zu) {
   if ((r = gs_ensure(c))) return r;
   GsState* s = (GsState*)c->gs;
   State S = gs_view(c, s);
-  const dim3 gv(s->nchunk, s->NK), blk(256);
+  const dim3 gv(s->nchunk, s->NK), blk(256), gsub(S.nsub, s->NK);
   for (int ib = 0; ib < s->NB; ++ib) {
     const Src v{zu + (size_t)ib * s->NL, (size_t)s->NB * s->NL, nullptr, nullptr};
     for (int pass = 0; pass < 2 && ib > 0; ++pass) {
       hipLaunchKernelGGL(k_cg_proj, dim3(s->nchunk, (ib + kJG - 1) / kJG, s->NK), blk, 0, c->stream, S,
                          (const cplx<double>*)zu, ib, v, (int)PROJ_PLAIN, 0);
       LAUNCHCHK(c);
-      hipLaunchKernelGGL(k_gs_sub, gv, blk, 0, c->stream, S, zu, ib, pass == 1 ? 1 : 0);
+      hipLaunchKernelGGL(k_gs_sub, gsub, blk, 0, c->stream, S, zu, ib, pass == 1 ? 1 : 0);
       LAUNCHCHK(c);
     }
     if (ib == 0) {
-      hipLaunchKernelGGL(k_gs_sub, gv, blk, 0, c->stream, S, zu, 0, 1);
+      hipLaunchKernelGGL(k_gs_sub, gsub, blk, 0, c->stream, S, zu, 0, 1);
       LAUNCHCHK(c);
     }
     hipLaunchKernelGGL(k_gs_scale, gv, blk, 0, c->stream, S, zu, ib);

@@ -65,6 +65,7 @@ __global__ void k_pc_stats_fin(const double* __restrict__ part, int nchunk, int 
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n) return;
   double a[3] = {0.0, 0.0, 0.0};
+#pragma unroll 8
   for (int c = 0; c < nchunk; ++c)
     for (int q = 0; q < 3; ++q) a[q] += part[((size_t)s * nchunk + c) * 3 + q];
   const double phi = 0.5 * atan2(a[1], a[0]);
@@ -206,6 +207,7 @@ __global__ void __launch_bounds__(kBlk) k_conv_bwd_w_sum(const float* __restrict
   if (i >= 27 * cico) return;
   const int t = i / cico, k = i % cico;
   double s = 0.0;
+#pragma unroll 8
   for (int b = 0; b < nblk; ++b) s += part[((size_t)t * nblk + b) * cico + k];
   grad[i] = s;
 }
@@ -256,6 +258,7 @@ __global__ void k_pc_dots_fin(Batch B, const double* __restrict__ part, int nchu
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= B.n * (B.NB + 1)) return;
   double re = 0.0, im = 0.0;
+#pragma unroll 8
   for (int c = 0; c < nchunk; ++c) {
     re += part[((size_t)i * nchunk + c) * 2];
     im += part[((size_t)i * nchunk + c) * 2 + 1];
@@ -301,6 +304,7 @@ __global__ void __launch_bounds__(kBlk) k_pc_normalise(const cplx<double>* __res
                                                        double* __restrict__ nu) {
   const int s = blockIdx.y, i = blockIdx.x * kBlk + threadIdx.x;
   double a = 0.0;
+#pragma unroll 8
   for (int c = 0; c < nchunk; ++c) a += part[(size_t)s * nchunk + c];
   const double n = sqrt(h * a);
   if (blockIdx.x == 0 && threadIdx.x == 0) nu[s] = n;
@@ -335,6 +339,7 @@ __global__ void k_pc_objective(Batch B, const double* __restrict__ part, int nch
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= B.n) return;
   double a[3] = {0.0, 0.0, 0.0};
+#pragma unroll 8
   for (int c = 0; c < nchunk; ++c)
     for (int q = 0; q < 3; ++q) a[q] += part[((size_t)s * nchunk + c) * 3 + q];
   const double h = B.h, th = B.theta[s], lam = B.lam[s];

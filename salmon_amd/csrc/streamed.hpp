@@ -367,6 +367,7 @@ __global__ void __launch_bounds__(256) k_density_final(const double* __restrict_
   const int p = blockIdx.x * 256 + threadIdx.x;
   if (p >= NL) return;
   double r = 0;
+#pragma unroll 8
   for (int c = 0; c < nchunk; ++c) r += part[(size_t)c * NL + p];
   rho[p] = r;
 }
@@ -381,6 +382,7 @@ __global__ void __launch_bounds__(256) k_density_slices(const double* __restrict
   if (p >= NL) return;
   const int c0 = sl * per_slice, c1 = min(nchunk, c0 + per_slice);
   double r = 0;
+#pragma unroll 8
   for (int c = c0; c < c1; ++c) r += part[(size_t)c * NL + p];
   part2[(size_t)sl * NL + p] = r;
 }
