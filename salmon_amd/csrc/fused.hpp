@@ -717,6 +717,7 @@ __global__ void __launch_bounds__(NY * NZ * XS, WPE) k_fused_step(FusedArgs<T> a
       for (int w = 0; w < NW; ++w) s[i] += red[w * 10 + i];
     }
     double occsum = 0;
+#pragma unroll 8
     for (int ib = b0; ib < b1; ++ib) occsum += a.occ[(size_t)ik * a.NB + ib];
     double* r = a.wg_rows + (size_t)blockIdx.x * 8;
     const double f = g.Hxyz / g.aLxyz;
