@@ -171,8 +171,8 @@ def main():
                     help="also run one 1 GiB device copy (reference size for the HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU: the k-point sharding over several GPUs is not built "
-              "(every GPU box here has one GPU); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU: the k-point sharding over several GPUs is not built; "
+              "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     from salmon_amd import gpu
     ev = HipEvents()
