@@ -94,6 +94,16 @@ def workload(seed=0, nsteps=2000):
     return s, ac
 
 
+def ref_flop_per_orbital_step(s):
+    """The reference's own FLOP model of one RT step of one orbital
+    (ARTED/modules/performance_analyzer.f90:242-301): four H applications of
+    158 flop per grid point for the stencil and 4 + 21 sum_lma Mps for the
+    projectors, and 8 flop per point per Taylor term for the update."""
+    mps = np.asarray(s.Mps)
+    sum_mps = int(np.sum(mps[np.asarray(s.a_tbl) - 1]))
+    return 4 * 158 * s.NL + 4 * (4 + 21 * sum_mps) + 4 * 8 * s.NL
+
+
 def run_gpu(s, ac, prec, warmup, steps, ev, scf=False):
     from salmon_amd import gpu
     with gpu.Context(s, prec) as c:
@@ -221,6 +231,13 @@ def main():
             cpu = cpu_baseline(args.cpu_nk, args.cpu_steps)
         except Exception as e:  # reported, not fatal
             cpu = {"error": str(e)[:300]}
+    fps = ref_flop_per_orbital_step(s) * nkb
+    flop_model = {"gflop_per_step": fps / 1e9, "mixed_tflops": fps / (ms32 * 1e-3) / 1e12,
+                  "fp64_tflops": fps / (ms64 * 1e-3) / 1e12,
+                  "source": "ARTED/modules/performance_analyzer.f90:242-301 (stencil 158 flop/point, "
+                            "KB 4 + 21 sum Mps, update 8 flop/point/term; four H applications per step)"}
+    if isinstance(cpu, dict) and "value" in cpu:
+        flop_model["cpu_baseline_gflops"] = cpu["value"] * ref_flop_per_orbital_step(s) / 1e9
     line = {
         "metric": "Taylor steps*k*band/s (16^3, 512k x 32) FP64 vs mixed @ J(t) rel err<=1e-5",
         "value": v32, "unit": "k*band*steps/s", "n_gpus": 1, "steps": steps, "warmup": args.warmup,
@@ -234,6 +251,7 @@ def main():
         "J_rel_err": jerr, "E_rel_err": eerr,
         "roofline": roof,
         "scf": scf,
+        "flop_model": flop_model,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))
