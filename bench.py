@@ -104,9 +104,11 @@ def ref_flop_per_orbital_step(s):
     return 4 * 158 * s.NL + 4 * (4 + 21 * sum_mps) + 4 * 8 * s.NL
 
 
-def run_gpu(s, ac, prec, warmup, steps, ev, scf=False):
+def run_gpu(s, ac, prec, warmup, steps, ev, scf=False, etrs=False):
     from salmon_amd import gpu
     with gpu.Context(s, prec) as c:
+        if etrs:  # enforced time-reversal symmetry: two half steps per step
+            c.set_propagator("etrs")
         if scf:  # Vloc = V_H + Vpsl + V_xc (PZ81) refreshed after every step
             c.set_scf(s.vloc, gpu.XC_PZ)
         c.rt_run(ac[: warmup + 1], warmup)
@@ -225,6 +227,12 @@ def main():
     scf = {"steps": nscf, "ms_per_step": s32["ms"] / nscf, "value": nkb * nscf / (s32["ms"] * 1e-3),
            "fp64": {"ms_per_step": s64["ms"] / nscf, "value": nkb * nscf / (s64["ms"] * 1e-3)},
            "J_rel_err": float(np.max(np.abs(s32["J"] - s64["J"])) / np.max(np.abs(s64["J"])))}
+    # the same workload with the ETRS propagator (frozen potential)
+    e64 = run_gpu(s, ac, gpu.FP64, args.warmup, nscf, ev, etrs=True)
+    e32 = run_gpu(s, ac, gpu.MIXED, args.warmup, nscf, ev, etrs=True)
+    etrs = {"steps": nscf, "ms_per_step": e32["ms"] / nscf, "value": nkb * nscf / (e32["ms"] * 1e-3),
+            "fp64": {"ms_per_step": e64["ms"] / nscf, "value": nkb * nscf / (e64["ms"] * 1e-3)},
+            "J_rel_err": float(np.max(np.abs(e32["J"] - e64["J"])) / np.max(np.abs(e64["J"])))}
     cpu = None
     if not args.no_cpu:
         try:
@@ -251,6 +259,7 @@ def main():
         "J_rel_err": jerr, "E_rel_err": eerr,
         "roofline": roof,
         "scf": scf,
+        "etrs": etrs,
         "flop_model": flop_model,
         "cpu_baseline": cpu,
     }
