@@ -130,10 +130,14 @@ import sys, time, json, os
 sys.path.insert(0, sys.argv[1])
 import numpy as np
 from salmon_amd import synth
-nk, nsteps, kind = int(sys.argv[2]), int(sys.argv[3]), sys.argv[4]
-s, ac = synth.config2(nsteps=nsteps + 1, orbitals=False)
-s.kAc0 = np.ascontiguousarray(s.kAc0[:, :nk]); s.occ = np.ascontiguousarray(s.occ[:nk])
-s.zu = synth.random_orbitals(np.random.default_rng(7), nk, s.NB, s.NL, s.Hxyz)
+nk, nsteps, kind, which = int(sys.argv[2]), int(sys.argv[3]), sys.argv[4], sys.argv[5]
+if which == "cfg1":
+    s, ac = synth.config1()
+    nsteps = min(nsteps, ac.shape[0] - 2)
+else:
+    s, ac = synth.config2(nsteps=nsteps + 1, orbitals=False)
+    s.kAc0 = np.ascontiguousarray(s.kAc0[:, :nk]); s.occ = np.ascontiguousarray(s.occ[:nk])
+    s.zu = synth.random_orbitals(np.random.default_rng(7), nk, s.NB, s.NL, s.Hxyz)
 if kind == "reference":
     from oracle.ref import Ref
     R = Ref(s)
@@ -146,29 +150,64 @@ else:
     threads = lib().or_num_threads()
     O.rt_run(ac[:2], 1)
     t0 = time.perf_counter(); O.rt_run(ac[1:nsteps + 2], nsteps); t = time.perf_counter() - t0
-print(json.dumps(dict(seconds=t, threads=threads, nkb=nk * s.NB, steps=nsteps)))
+print(json.dumps(dict(seconds=t, threads=threads, nkb=s.NK * s.NB, steps=nsteps)))
 """
 
 
-def cpu_baseline(nk=64, nsteps=4):
-    """The reference's Fortran (oracle/_ref) on a bounded sample of the same
-    workload, on the host cores, in a subprocess (its module globals hold one
-    system per process)."""
-    from oracle import ref
-    kind = "reference" if ref.available() else "port"
-    env = dict(os.environ)
-    env.setdefault("OMP_NUM_THREADS", str(min(os.cpu_count() or 1, 16)))
-    env.setdefault("OMP_PROC_BIND", "close")
-    env.setdefault("OMP_PLACES", "cores")
-    out = subprocess.run([sys.executable, "-c", CPU_SAMPLE_SCRIPT, ROOT, str(nk), str(nsteps), kind],
+def host_cpu():
+    """The host's CPU model, logical CPU count and the CPUs this process may run on."""
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        allowed = len(os.sched_getaffinity(0))
+    except AttributeError:
+        allowed = os.cpu_count() or 1
+    return dict(model=model, nproc=os.cpu_count() or 1, allowed=allowed)
+
+
+def _cpu_run(kind, which, nk, nsteps, env):
+    out = subprocess.run([sys.executable, "-c", CPU_SAMPLE_SCRIPT, ROOT, str(nk), str(nsteps), kind, which],
                          capture_output=True, text=True, env=env, timeout=600)
     if out.returncode != 0:
         raise RuntimeError("cpu baseline failed: " + out.stderr[-2000:])
-    d = json.loads(out.stdout.strip().splitlines()[-1])
+    return json.loads(out.stdout.strip().splitlines()[-1])
+
+
+def cpu_baseline(nk=64, nsteps=4):
+    """The reference's Fortran (oracle/_ref, built with flang -O3 -fopenmp and
+    the reference's default stencil flags) on a bounded sample of the same
+    workload, and on the whole config-1 run (16^3, Gamma, 16 orbitals, 100
+    steps), on the host cores, in subprocesses (its module globals hold one
+    system per process).  Threads: OMP_NUM_THREADS where the environment sets
+    it (the GPU box gives each GPU's jobs a 16-CPU share and sets it to 16),
+    else every CPU this process may run on."""
+    from oracle import ref
+    kind = "reference" if ref.available() else "port"
+    host = host_cpu()
+    env = dict(os.environ)
+    env.setdefault("OMP_NUM_THREADS", str(host["allowed"]))
+    env.setdefault("OMP_PROC_BIND", "close")
+    env.setdefault("OMP_PLACES", "cores")
+    d = _cpu_run(kind, "cfg2", nk, nsteps, env)
     rate = d["nkb"] * d["steps"] / d["seconds"]
-    return dict(value=rate, unit="k*band*steps/s", cores=int(d["threads"]), kind=kind,
-                sample=f"{nk} of 512 k points x 32 bands on the 16^3 cell, {nsteps} steps after 1 warm-up "
-                       f"(Taylor-4 + density + current + energy per step), {d['seconds']:.2f} s")
+    res = dict(value=rate, unit="k*band*steps/s", cores=int(d["threads"]), kind=kind,
+               sample=f"{nk} of 512 k points x 32 bands on the 16^3 cell, {nsteps} steps after 1 warm-up "
+                      f"(Taylor-4 + density + current + energy per step), {d['seconds']:.2f} s",
+               host=dict(host, omp_num_threads=env["OMP_NUM_THREADS"]))
+    try:
+        d1 = _cpu_run(kind, "cfg1", 1, 99, env)
+        res["config1"] = dict(value=d1["nkb"] * d1["steps"] / d1["seconds"], unit="k*band*steps/s",
+                              steps=d1["steps"], seconds=d1["seconds"], threads=int(d1["threads"]))
+    except Exception as e:  # reported, not fatal
+        res["config1"] = {"error": str(e)[:300]}
+    return res
 
 
 def main():

@@ -64,6 +64,7 @@ struct GsState {
     hipGraphExec_t exec = nullptr;
     long key[4] = {-1, -1, -1, -1};
     long warm[4] = {-1, -1, -1, -1};
+    unsigned long gen = 0;  // the context's buffer generation of key and warm
   } g_cg, g_gs;
 };
 
@@ -329,21 +330,32 @@ static int gram_schmidt_multi(salmon_ctx* c, cplx<double>* zu) {
 
 // Runs `enqueue` through the graph cache: eagerly on the first call of a key
 // (buffers are allocated there), captured and instantiated on the second,
-// replayed afterwards.  A failed capture leaves the key to eager launches.
+// replayed afterwards.  A captured graph holds the device pointers of its
+// launches, so it is dropped (and the key starts over eagerly) once the
+// context has freed or replaced any buffer since (salmon_ctx::gen).  A failed
+// capture leaves the key to eager launches.
 template <typename F>
 static int graph_run(salmon_ctx* c, GsState::Graph& g, const long key[4], F enqueue) {
   if (!c->graphs) return enqueue();
+  if (g.gen != c->gen) {
+    graph_free(g);
+    g.gen = c->gen;
+  }
   const bool same = std::equal(key, key + 4, g.key);
   if (g.exec && same) {
     HIPCHK(c, hipGraphLaunch(g.exec, c->stream));
     return SALMON_OK;
   }
   if (!std::equal(key, key + 4, g.warm)) {
+    const int r = enqueue();
+    // buffers allocated by this eager call belong to the generation a capture will see
+    g.gen = c->gen;
     std::copy(key, key + 4, g.warm);
-    return enqueue();
+    return r;
   }
   graph_free(g);
   std::copy(key, key + 4, g.warm);
+  const unsigned long gen0 = c->gen;
   hipGraph_t graph = nullptr;
   HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
   const int r = enqueue();
@@ -353,6 +365,14 @@ static int graph_run(salmon_ctx* c, GsState::Graph& g, const long key[4], F enqu
     return r;
   }
   if (e != hipSuccess || !graph) return fail(c, SALMON_ERR_HIP, "stream capture: %s", hipGetErrorString(e));
+  if (c->gen != gen0) {
+    // a buffer was replaced while the launches were recorded: run them
+    // eagerly instead and capture on a later call
+    (void)hipGraphDestroy(graph);
+    g.gen = c->gen;
+    for (int i = 0; i < 4; ++i) g.warm[i] = -1;
+    return enqueue();
+  }
   hipGraphExec_t exec = nullptr;
   const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
   (void)hipGraphDestroy(graph);

@@ -153,6 +153,7 @@ static int pc_infer(salmon_ctx* c, GsState* s, salmon_pcnet* net, const cplx<dou
   const hipStream_t st = c->stream;
   if (I.NK != NK || I.NL != NL) {
     for (void* b : I.bufs) (void)hipFree(b);
+    if (!I.bufs.empty()) ++c->gen;
     I.bufs.clear();
     const size_t v = (size_t)NK * NL;
     int r = 0;

@@ -44,6 +44,10 @@ struct salmon_ctx {
   void* d_uvpsi = nullptr;
   int uvpsi_nlma = 0;         // projectors d_uvpsi is sized for
   bool graphs = true;         // CG / Gram-Schmidt through the graph cache
+  // bumped whenever a device buffer that a launch may reference is freed or
+  // replaced; a captured graph holds raw pointers and is dropped once the
+  // generation it was captured at is gone
+  unsigned long gen = 0;
   int propagator = SALMON_PROP_MIDPOINT;  // of rt_run
   // self-consistent potential (salmon_gpu_set_scf): Vloc = V_H[rho] + Vpsl + V_xc[rho]
   bool scf = false;
@@ -55,6 +59,8 @@ struct salmon_ctx {
   void* d_ph = nullptr;       // KB phases of the streamed Taylor step (NK, NPI)
   size_t ph_bytes = 0;
   double* d_rows = nullptr;   // (NKB*8)
+  double* d_res = nullptr;    // rt_run's per-step results (8 per step), grown on demand
+  size_t res_cap = 0;         // steps d_res holds
   double* d_sum = nullptr;    // (8)
   double* d_rho = nullptr;    // (NL)
   double* d_rho_part = nullptr;
@@ -104,6 +110,7 @@ int dalloc(salmon_ctx* c, X** p, size_t n) {
   if (*p) {
     (void)hipFree(*p);
     *p = nullptr;
+    if (c) ++c->gen;
   }
   if (n == 0) n = 1;
   hipError_t e = hipMalloc((void**)p, n * sizeof(X));
@@ -189,6 +196,7 @@ int taylor_streamed(salmon_ctx* c, const double* kbase, double ax, double ay, do
       if (c->d_ph) (void)hipFree(c->d_ph);
       c->d_ph = nullptr;
       c->ph_bytes = 0;
+      ++c->gen;
       if (hipMalloc(&c->d_ph, nph * sizeof(cplx<T>)) != hipSuccess) return fail(c, SALMON_ERR_NOMEM, "phase table");
       c->ph_bytes = nph * sizeof(cplx<T>);
     }
@@ -655,6 +663,9 @@ int salmon_gpu_set_projectors(salmon_ctx* c, int NI, int Nps, const int* Mps, co
     }
     tstart.push_back((int)ejp.size());
   }
+  // the old tables, projection and work buffers go: drop graphs that captured them
+  (void)hipStreamSynchronize(c->stream);
+  ++c->gen;
   for (void* p : c->kb_bufs) (void)hipFree(p);
   c->kb_bufs.clear();
   auto up = [&](const void* src, size_t bytes, void** dst) -> int {
@@ -693,6 +704,8 @@ int salmon_gpu_set_kpoints(salmon_ctx* c, int NK, int NB, const double* kAc0, co
   if (!c) return SALMON_ERR_ARG;
   if (NK < 1 || NB < 1 || !kAc0 || !occ) return fail(c, SALMON_ERR_ARG, "NK, NB must be >= 1 with kAc0, occ given");
   if ((size_t)NK * NB * c->g.NL > ((size_t)1 << 40)) return fail(c, SALMON_ERR_ARG, "orbital set too large");
+  (void)hipStreamSynchronize(c->stream);
+  ++c->gen;  // orbitals, k tables and work buffers are replaced
   c->NK = NK; c->NB = NB; c->NKB = NK * NB;
   int r;
   if ((r = dalloc(c, &c->d_kac0, 3 * (size_t)NK)) || (r = dalloc(c, &c->d_kac, 3 * (size_t)NK)) ||
@@ -1132,6 +1145,8 @@ int salmon_gpu_set_graphs(salmon_ctx* c, int enable) {
 
 int salmon_gpu_set_pcnet(salmon_ctx* c, salmon_pcnet* n) {
   if (!c) return SALMON_ERR_ARG;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  ++c->gen;  // a captured learned CG reads the previous network's weights
   c->pcnet = n;
   return SALMON_OK;
 }
@@ -1257,8 +1272,14 @@ int salmon_gpu_rt_run(salmon_ctx* c, const double* ac, int nsteps, double dt, in
   if (r) return r;
   if (!ac || nsteps < 0) return fail(c, SALMON_ERR_ARG, "ac is NULL or nsteps < 0");
   if (nsteps == 0) return SALMON_OK;
-  double* d_res = nullptr;
-  if ((r = dalloc(c, &d_res, 8 * (size_t)nsteps))) return r;
+  // the per-step results stay allocated between runs (no allocation or free
+  // inside a run of the usual length)
+  if ((size_t)nsteps > c->res_cap) {
+    (void)hipStreamSynchronize(c->stream);
+    if ((r = dalloc(c, &c->d_res, 8 * (size_t)nsteps))) return r;
+    c->res_cap = (size_t)nsteps;
+  }
+  double* d_res = c->d_res;
   if (c->scf) {
     r = c->precision == SALMON_FP64 ? rt_run_scf<double>(c, ac, nsteps, dt, reortho_every, d_res)
                                     : rt_run_scf<float>(c, ac, nsteps, dt, reortho_every, d_res);
@@ -1293,7 +1314,6 @@ int salmon_gpu_rt_run(salmon_ctx* c, const double* ac, int nsteps, double dt, in
     }
   }
   (void)hipStreamSynchronize(c->stream);
-  (void)hipFree(d_res);
   return r;
 }
 
@@ -1398,6 +1418,7 @@ int salmon_gpu_finalize(salmon_ctx* c) {
   dfree((void**)&c->d_vloc); dfree((void**)&c->d_vloc_f); dfree((void**)&c->d_rho); dfree((void**)&c->d_rho_part);
   dfree((void**)&c->d_rho_part2);
   dfree((void**)&c->d_kac0); dfree((void**)&c->d_kac); dfree((void**)&c->d_occ); dfree((void**)&c->d_rows);
+  dfree((void**)&c->d_res);
   dfree((void**)&c->d_sum); dfree((void**)&c->d_kone); dfree((void**)&c->d_oneocc); dfree(&c->d_zu);
   dfree((void**)&c->d_f64); dfree(&c->d_uvpsi); dfree(&c->d_ph);
   dfree((void**)&c->d_vpsl); dfree((void**)&c->d_vold); dfree((void**)&c->d_scf_tw); dfree(&c->d_scf_buf[0]); dfree(&c->d_scf_buf[1]);

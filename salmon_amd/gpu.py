@@ -161,11 +161,7 @@ class Context:
             raise SalmonGPUError(f"salmon_gpu_init: {lib().salmon_gpu_errstr(rc).decode()} "
                                  f"({lib().salmon_gpu_last_error(None).decode()})")
         self._check(lib().salmon_gpu_set_potential(self._h, _p(_f64(sys.vloc))), "set_potential")
-        keep = [_i32(sys.Mps), _i32(sys.Jxyz), _i32(sys.Jxx), _i32(sys.Jyy), _i32(sys.Jzz), _i32(sys.a_tbl),
-                _f64(sys.uV), _i32(sys.iuV)]
-        self._check(lib().salmon_gpu_set_projectors(self._h, sys.NI, sys.Nps, _p(keep[0]), _p(keep[1]), _p(keep[2]),
-                                                    _p(keep[3]), _p(keep[4]), sys.Nlma, _p(keep[5]), _p(keep[6]),
-                                                    _p(keep[7])), "set_projectors")
+        self.set_projectors(sys)
         self._check(lib().salmon_gpu_set_kpoints(self._h, sys.NK, sys.NB, _p(_f64(sys.kAc0)), _p(_f64(sys.occ))),
                     "set_kpoints")
         if orbitals:
@@ -175,6 +171,15 @@ class Context:
         if rc != 0:
             msg = lib().salmon_gpu_last_error(self._h).decode() if self._h else ""
             raise SalmonGPUError(f"{what}: {lib().salmon_gpu_errstr(rc).decode()}: {msg}")
+
+    def set_projectors(self, sys):
+        """Uploads the KB tables of sys (Mps, Jxyz, Jxx/Jyy/Jzz, a_tbl, uV, iuV);
+        called again when the atoms move (prep_ps_periodic)."""
+        keep = [_i32(sys.Mps), _i32(sys.Jxyz), _i32(sys.Jxx), _i32(sys.Jyy), _i32(sys.Jzz), _i32(sys.a_tbl),
+                _f64(sys.uV), _i32(sys.iuV)]
+        self._check(lib().salmon_gpu_set_projectors(self._h, sys.NI, sys.Nps, _p(keep[0]), _p(keep[1]), _p(keep[2]),
+                                                    _p(keep[3]), _p(keep[4]), sys.Nlma, _p(keep[5]), _p(keep[6]),
+                                                    _p(keep[7])), "set_projectors")
 
     def close(self):
         if self._h:

@@ -176,3 +176,58 @@ def test_cg_and_gram_schmidt_graph_replays_match_eager():
             zg = c.get_orbitals()
             ev, it = c.cg(s.kAc0, 3, gpu.PC_KINETIC)
             assert all(np.array_equal(a, b) for a, b in zip(outs[0], (zg, ev, it, c.get_orbitals())))
+
+
+def _moved_atoms(s, shift, seed):
+    """A copy of s whose atoms are moved by `shift` (bohr): new KB tables with
+    the same projector count and a different number of sphere points."""
+    t = s.copy()
+    aL = np.array([s.aLx, s.aLy, s.aLz])
+    t.Rion = np.mod(s.Rion + np.asarray(shift)[None, :], aL[None, :])
+    coeff = np.random.default_rng(seed).uniform(-2.0, 2.0, size=(s.NI, 4))
+    (t.Mps, t.Jxyz, t.Jxx, t.Jyy, t.Jzz, t.a_tbl, t.uV, t.iuV) = synth.projector_tables(
+        t.Rion, s.rc, (s.NLx, s.NLy, s.NLz), tuple(aL), 0.8, coeff)
+    return t
+
+
+def test_cg_graph_dropped_when_projectors_or_kpoints_change():
+    """A context whose CG runs from a captured graph gets new projector tables
+    with the same projector count (moved atoms: every table is reallocated and
+    the sphere-point count changes), then new k points of the same count: the
+    next CG calls (eager, captured, replayed) use the new tables and buffers
+    and equal an eager context set up with them, and the oracle."""
+    s = synth.make_system(N=(12, 12, 12), L=7.0, nk3=(2, 1, 1), NB=6, seed=31, atoms="random", natoms=5,
+                          name="graph12")
+    t = _moved_atoms(s, (0.21, 0.13, -0.4), 5)
+    assert t.Nlma == s.Nlma and int(np.sum(t.Mps)) != int(np.sum(s.Mps))
+    u = t.copy()
+    u.kAc0 = np.ascontiguousarray(s.kAc0 + np.array([[0.05], [-0.02], [0.01]]))
+    with _ctx(s) as c:
+        for rep in range(3):  # eager, captured, replayed
+            c.put_orbitals(s.zu)
+            c.cg(s.kAc0, 3, gpu.PC_KINETIC)
+        c.set_projectors(t)
+        runs_t = []
+        for rep in range(3):
+            c.put_orbitals(s.zu)
+            ev, it = c.cg(t.kAc0, 3, gpu.PC_KINETIC)
+            runs_t.append((ev, it, c.get_orbitals()))
+        c._check(gpu.lib().salmon_gpu_set_kpoints(c._h, u.NK, u.NB, gpu._p(gpu._f64(u.kAc0)),
+                                                 gpu._p(gpu._f64(u.occ))), "set_kpoints")
+        runs_u = []
+        for rep in range(3):
+            c.put_orbitals(s.zu)
+            ev, it = c.cg(u.kAc0, 3, gpu.PC_KINETIC)
+            runs_u.append((ev, it, c.get_orbitals()))
+    for ref_sys, runs in ((t, runs_t), (u, runs_u)):
+        with _ctx(ref_sys) as c:
+            c.set_graphs(False)
+            ev, it = c.cg(ref_sys.kAc0, 3, gpu.PC_KINETIC)
+            eager = (ev, it, c.get_orbitals())
+        for r in runs:
+            assert all(np.array_equal(a, b) for a, b in zip(eager, r))
+        zr, evr, itr = Oracle(ref_sys).cg(ref_sys.kAc0, s.zu, 3, precond=gpu.PC_KINETIC)
+        print(f"[gpu] cg after table change: zu {rel(runs[-1][2], zr):.2e} esp_var {rel(runs[-1][0], evr):.2e}",
+              flush=True)
+        assert rel(runs[-1][2], zr) < TOL_CG
+        assert rel(runs[-1][0], evr) < TOL_CG
