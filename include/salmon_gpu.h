@@ -166,9 +166,9 @@ int salmon_gpu_ground_state(salmon_ctx *ctx, const double *kAc, int max_sweeps, 
  * sqrt(NL), to a search direction (rotated back).  With SALMON_PC_LEARNED
  * each CG iteration is a steepest-descent step along that direction with the
  * 2x2 Rayleigh-Ritz of CG.f90:107-119 (no conjugation: the network is not a
- * fixed linear operator).  salmon_gpu_ground_state uses it while every sweep
- * lowers the band-energy sum by at least 0.3 of the previous sweep's drop and
- * then falls back to SALMON_PC_KINETIC for the rest of the solve.
+ * fixed linear operator).  salmon_gpu_ground_state uses it for the first
+ * sweeps and then falls back to SALMON_PC_KINETIC for the rest of the solve
+ * (salmon_gpu_set_pc_handover).
  *
  * Training (one k point per context): salmon_gpu_pc_record makes the next CG
  * calls store each iteration's orthogonalised residual, iterate, band and
@@ -193,6 +193,12 @@ int salmon_gpu_pcnet_finalize(salmon_pcnet *net);
 int salmon_gpu_pcnet_apply(salmon_pcnet *net, salmon_ctx *ctx, const double *g, double *d);
 /* network used by SALMON_PC_LEARNED on ctx (NULL detaches; not owned) */
 int salmon_gpu_set_pcnet(salmon_ctx *ctx, salmon_pcnet *net);
+/* No counterpart in the reference.  When salmon_gpu_ground_state with
+ * SALMON_PC_LEARNED hands over to SALMON_PC_KINETIC: after at most
+ * max_learned_sweeps sweeps (0: no limit), and earlier at the first sweep
+ * (from the third on) that lowers the band-energy sum by less than
+ * stall_ratio times the previous sweep's drop. */
+int salmon_gpu_set_pc_handover(salmon_ctx *ctx, int max_learned_sweeps, double stall_ratio);
 /* No counterpart in the reference.  CG and Gram-Schmidt calls run through a
  * graph cache (captured on the second call with the same arguments, replayed
  * after); enable = 0 runs them eagerly.  While a context's stream is being

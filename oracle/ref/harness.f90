@@ -24,6 +24,11 @@
 !                         frozen potential (no_update_func='y').
 module ref_harness
   use iso_c_binding
+#ifdef ARTED_GPU_DROPIN
+  ! host reads and writes of zu_t mark their transfers, as the reference's
+  ! OpenACC build does with "!$acc update self/device(zu_t)"
+  use arted_gpu_state, only: arted_gpu_update_self, arted_gpu_update_device
+#endif
   use Global_Variables
   use opt_variables
   use timer, only: timer_initialize
@@ -137,10 +142,16 @@ contains
   subroutine ref_put_orbitals(zu) bind(C, name="ref_put_orbitals")
     complex(c_double_complex), intent(in) :: zu(NL, NB, NK)
     zu_t = zu
+#ifdef ARTED_GPU_DROPIN
+    call arted_gpu_update_device(zu_t)
+#endif
   end subroutine
 
   subroutine ref_get_orbitals(zu) bind(C, name="ref_get_orbitals")
     complex(c_double_complex), intent(out) :: zu(NL, NB, NK)
+#ifdef ARTED_GPU_DROPIN
+    call arted_gpu_update_self(zu_t)
+#endif
     zu = zu_t
   end subroutine
 
@@ -202,15 +213,24 @@ contains
   end subroutine
 
   subroutine ref_gram_schmidt() bind(C, name="ref_gram_schmidt")
+#ifdef ARTED_GPU_DROPIN
+    call arted_gpu_update_self(zu_t)
+#endif
     zu_GS = zu_t
     call Gram_Schmidt
     zu_t = zu_GS
+#ifdef ARTED_GPU_DROPIN
+    call arted_gpu_update_device(zu_t)
+#endif
   end subroutine
 
   subroutine ref_sp_energy(kac_, esp_out) bind(C, name="ref_sp_energy")
     real(c_double), intent(in) :: kac_(NK, 3)
     real(c_double), intent(out) :: esp_out(NB, NK)
     kAc = kac_
+#ifdef ARTED_GPU_DROPIN
+    call arted_gpu_update_self(zu_t)
+#endif
     zu_GS = zu_t
     call sp_energy_omp
     esp_out = esp
@@ -221,9 +241,15 @@ contains
     integer(c_int), value :: ncg
     real(c_double), intent(out) :: esp_var_out(NB, NK)
     kAc = kac_
+#ifdef ARTED_GPU_DROPIN
+    call arted_gpu_update_self(zu_t)
+#endif
     zu_GS = zu_t
     call CG_omp(ncg)
     zu_t = zu_GS
+#ifdef ARTED_GPU_DROPIN
+    call arted_gpu_update_device(zu_t)
+#endif
     esp_var_out = esp_var
   end subroutine
 

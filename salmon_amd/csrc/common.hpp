@@ -31,6 +31,8 @@ struct KBTables {
   const int* pidx;        // (NPI) 0-based grid index of each packed point
   const double* pos;      // (NPI*3) unwrapped coordinates Lx*Hx - Jxx*aLx, ...
   const int* lma_atom;    // (Nlma) 0-based atom of each projector
+  const int* atom_lma;    // (NI+1) first projector of each atom (projectors grouped by atom)
+  int max_lpa;            // most projectors of one atom
   const double* lma_sign; // (Nlma) iuV as a real
   const double* uv;       // (Nlma*Nps) uV(j,ilma) at [ilma*Nps + j]
   const int* pt2t;        // (NL) touched-point number of a grid point, or -1

@@ -17,6 +17,7 @@
 
 using namespace salmon;
 
+
 struct salmon_ctx {
   int precision = SALMON_FP64;
   int device = 0;
@@ -77,6 +78,11 @@ struct salmon_ctx {
   void* fz = nullptr;  // state owned by the fused path
   void* gs = nullptr;  // state owned by the ground-state path
   void* pcnet = nullptr;  // network of the learned CG preconditioner (not owned)
+  // learned -> kinetic handover of salmon_gpu_ground_state: at most
+  // pc_max_sweeps learned sweeps (0: no limit), and earlier once a sweep lowers
+  // the band-energy sum by less than pc_stall times the previous sweep's drop
+  int pc_max_sweeps = 0;
+  double pc_stall = 0.3;
 };
 
 namespace {
@@ -162,14 +168,26 @@ const double* vloc_ptr<double>(const salmon_ctx* c) { return c->d_vloc; }
 template <>
 const float* vloc_ptr<float>(const salmon_ctx* c) { return c->d_vloc_f; }
 
+// bands per k point from which the KB projection runs on the matrix cores
+// (16-band tiles; fewer bands leave most of a tile's rows empty)
+constexpr int kMfmaMinBands = 8;
+
 // H applied to nOrb orbitals (rows of NL) from `in` to `out` (optionally tt).
 template <typename T>
 int apply_h(salmon_ctx* c, const cplx<T>* in, cplx<T>* out, cplx<T>* tt, int orb0, int nOrb, int NB, int NK,
             const double* kbase, double ax, double ay, double az, void* uvbuf = nullptr,
             const cplx<T>* ph = nullptr) {
   cplx<T>* uv = (cplx<T>*)(uvbuf ? uvbuf : c->d_uvpsi);
-  dim3 gp((c->kb.Nlma + 3) / 4, nOrb);
-  if (c->kb.Nlma > 0) {
+  if (c->kb.Nlma > 0 && NB >= kMfmaMinBands) {
+    // the bands of each k point as the rows of one matrix-core GEMM per atom
+    const int tpk = (NB + 15) / 16;
+    const int k0 = orb0 / NB, k1 = (orb0 + nOrb - 1) / NB;
+    const dim3 gm(c->kb.NI, (k1 - k0 + 1) * tpk, (c->kb.max_lpa + 7) / 8);
+    hipLaunchKernelGGL(k_kb_project_mfma<T>, gm, dim3(64), 0, c->stream, in, orb0, nOrb, NB, NK, k0, tpk, kbase, ax,
+                       ay, az, c->g, c->kb, uv, ph);
+    LAUNCHCHK(c);
+  } else if (c->kb.Nlma > 0) {
+    dim3 gp((c->kb.Nlma + 3) / 4, nOrb);
     hipLaunchKernelGGL(k_kb_project<T>, gp, dim3(256), 0, c->stream, in, orb0, NB, NK, kbase, ax, ay, az, c->g,
                        c->kb, uv, ph);
     LAUNCHCHK(c);
@@ -323,51 +341,70 @@ int gram_schmidt_multi_fwd(salmon_ctx* c, cplx<double>* zu) {
 
 namespace {
 
-// V_H and V_xc of the device density rho: Hartree by per-axis DFTs and the
-// Coulomb kernel, then Vloc = V_H + Vpsl + V_xc into the context's potential
-// (set_v) and/or the parts into vh, vxc (device, nullable).
-int scf_potential(salmon_ctx* c, const double* rho, bool set_v, int xc, double* vh, double* vxc) {
+// DFT twiddles (cos, sin)(2 pi j / N) of the three axes, kMaxN rows per axis.
+void scf_twiddles(const Grid& g, double* tw) {
+  const int N[3] = {g.NLx, g.NLy, g.NLz};
+  for (size_t i = 0; i < (size_t)3 * 2 * salmon::scf::kMaxN; ++i) tw[i] = 0.0;
+  for (int a = 0; a < 3; ++a)
+    for (int j = 0; j < N[a]; ++j) {
+      const double ang = 2.0 * M_PI * j / N[a];
+      tw[(size_t)a * 2 * salmon::scf::kMaxN + 2 * j] = std::cos(ang);
+      tw[(size_t)a * 2 * salmon::scf::kMaxN + 2 * j + 1] = std::sin(ang);
+    }
+}
+
+// V_H and V_xc of the device density rho with the twiddles tw and two complex
+// grids A, B (device): Hartree by per-axis DFTs and the Coulomb kernel, then
+// Vloc = V_H + Vpsl + V_xc into the context's potential (set_v) and/or the
+// parts into vh, vxc (device, nullable).  Launches only.
+int scf_potential_on(salmon_ctx* c, const double* rho, bool set_v, int xc, double* vh, double* vxc, const double* tw,
+                     cplx<double>* A, cplx<double>* B) {
   const Grid& g = c->g;
-  if (g.NLx > salmon::scf::kMaxN || g.NLy > salmon::scf::kMaxN || g.NLz > salmon::scf::kMaxN)
-    return fail(c, SALMON_ERR_UNSUPPORTED, "Hartree solve limited to %d points per axis", salmon::scf::kMaxN);
-  int r;
-  if (!c->d_scf_tw) {
-    std::vector<double> tw((size_t)3 * 2 * salmon::scf::kMaxN, 0.0);
-    const int N[3] = {g.NLx, g.NLy, g.NLz};
-    for (int a = 0; a < 3; ++a)
-      for (int j = 0; j < N[a]; ++j) {
-        const double ang = 2.0 * M_PI * j / N[a];
-        tw[(size_t)a * 2 * salmon::scf::kMaxN + 2 * j] = std::cos(ang);
-        tw[(size_t)a * 2 * salmon::scf::kMaxN + 2 * j + 1] = std::sin(ang);
-      }
-    if ((r = dalloc(c, &c->d_scf_tw, tw.size()))) return r;
-    HIPCHK(c, hipMemcpy(c->d_scf_tw, tw.data(), sizeof(double) * tw.size(), hipMemcpyHostToDevice));
-  }
-  for (int i = 0; i < 2; ++i)
-    if (!c->d_scf_buf[i] && (r = dalloc(c, (cplx<double>**)&c->d_scf_buf[i], (size_t)g.NL))) return r;
-  cplx<double>* A = (cplx<double>*)c->d_scf_buf[0];
-  cplx<double>* B = (cplx<double>*)c->d_scf_buf[1];
   const dim3 gr((g.NL + 255) / 256), bl(256);
   hipLaunchKernelGGL(salmon::scf::k_real_to_cplx, gr, bl, 0, c->stream, rho, A, g.NL);
-  hipLaunchKernelGGL(salmon::scf::k_dft_axis, gr, bl, 0, c->stream, (const cplx<double>*)A, B, g, 2, -1, c->d_scf_tw);
-  hipLaunchKernelGGL(salmon::scf::k_dft_axis, gr, bl, 0, c->stream, (const cplx<double>*)B, A, g, 1, -1, c->d_scf_tw);
-  hipLaunchKernelGGL(salmon::scf::k_dft_axis, gr, bl, 0, c->stream, (const cplx<double>*)A, B, g, 0, -1, c->d_scf_tw);
+  hipLaunchKernelGGL(salmon::scf::k_dft_axis, gr, bl, 0, c->stream, (const cplx<double>*)A, B, g, 2, -1, tw);
+  hipLaunchKernelGGL(salmon::scf::k_dft_axis, gr, bl, 0, c->stream, (const cplx<double>*)B, A, g, 1, -1, tw);
+  hipLaunchKernelGGL(salmon::scf::k_dft_axis, gr, bl, 0, c->stream, (const cplx<double>*)A, B, g, 0, -1, tw);
   hipLaunchKernelGGL(salmon::scf::k_coulomb, gr, bl, 0, c->stream, B, g);
-  hipLaunchKernelGGL(salmon::scf::k_dft_axis, gr, bl, 0, c->stream, (const cplx<double>*)B, A, g, 2, +1, c->d_scf_tw);
-  hipLaunchKernelGGL(salmon::scf::k_dft_axis, gr, bl, 0, c->stream, (const cplx<double>*)A, B, g, 1, +1, c->d_scf_tw);
-  hipLaunchKernelGGL(salmon::scf::k_dft_axis, gr, bl, 0, c->stream, (const cplx<double>*)B, A, g, 0, +1, c->d_scf_tw);
+  hipLaunchKernelGGL(salmon::scf::k_dft_axis, gr, bl, 0, c->stream, (const cplx<double>*)B, A, g, 2, +1, tw);
+  hipLaunchKernelGGL(salmon::scf::k_dft_axis, gr, bl, 0, c->stream, (const cplx<double>*)A, B, g, 1, +1, tw);
+  hipLaunchKernelGGL(salmon::scf::k_dft_axis, gr, bl, 0, c->stream, (const cplx<double>*)B, A, g, 0, +1, tw);
   LAUNCHCHK(c);
   if (set_v) {
     hipLaunchKernelGGL(salmon::scf::k_vloc, gr, bl, 0, c->stream, (const cplx<double>*)A, rho, (const double*)c->d_vpsl,
                        xc, g.NL, c->d_vloc, c->d_vloc_f, vh, vxc);
   } else {
     // parts only: Vpsl = 0 into a scratch potential (the second grid)
-    double* scratch = (double*)B;
+    double* pot = (double*)B;
     hipLaunchKernelGGL(salmon::scf::k_vloc, gr, bl, 0, c->stream, (const cplx<double>*)A, rho, (const double*)nullptr,
-                       xc, g.NL, scratch, (float*)nullptr, vh, vxc);
+                       xc, g.NL, pot, (float*)nullptr, vh, vxc);
   }
   LAUNCHCHK(c);
   return SALMON_OK;
+}
+
+int scf_check_grid(salmon_ctx* c) {
+  const Grid& g = c->g;
+  if (g.NLx > salmon::scf::kMaxN || g.NLy > salmon::scf::kMaxN || g.NLz > salmon::scf::kMaxN)
+    return fail(c, SALMON_ERR_UNSUPPORTED, "Hartree solve limited to %d points per axis", salmon::scf::kMaxN);
+  return SALMON_OK;
+}
+
+// scf_potential_on with the context's own twiddles and grids (allocated on first use).
+int scf_potential(salmon_ctx* c, const double* rho, bool set_v, int xc, double* vh, double* vxc) {
+  const Grid& g = c->g;
+  int r;
+  if ((r = scf_check_grid(c))) return r;
+  if (!c->d_scf_tw) {
+    std::vector<double> tw((size_t)3 * 2 * salmon::scf::kMaxN);
+    scf_twiddles(g, tw.data());
+    if ((r = dalloc(c, &c->d_scf_tw, tw.size()))) return r;
+    HIPCHK(c, hipMemcpy(c->d_scf_tw, tw.data(), sizeof(double) * tw.size(), hipMemcpyHostToDevice));
+  }
+  for (int i = 0; i < 2; ++i)
+    if (!c->d_scf_buf[i] && (r = dalloc(c, (cplx<double>**)&c->d_scf_buf[i], (size_t)g.NL))) return r;
+  return scf_potential_on(c, rho, set_v, xc, vh, vxc, c->d_scf_tw, (cplx<double>*)c->d_scf_buf[0],
+                          (cplx<double>*)c->d_scf_buf[1]);
 }
 
 // One RT step n on resident orbitals (streamed path), A(t_n) = a0,
@@ -548,7 +585,9 @@ int salmon_gpu_init(const salmon_grid_t* grid, int precision, salmon_ctx** out) 
   if (!grid || !out) return SALMON_ERR_ARG;
   *out = nullptr;
   tl_init_err.clear();
-  if (grid->NLx < 9 || grid->NLy < 9 || grid->NLz < 9) return SALMON_ERR_ARG;  // Nd=4 stencil needs > 2*Nd points
+  // the stencil wraps each neighbour index once, as the reference's periodic tables
+  // modx/mody/modz do (ARTED/modules/opt_variables.f90:155-167): Nd = 4 points per axis or more
+  if (grid->NLx < 4 || grid->NLy < 4 || grid->NLz < 4) return SALMON_ERR_ARG;
   if (precision != SALMON_FP64 && precision != SALMON_MIXED) return SALMON_ERR_ARG;
   if (!(grid->aLx > 0 && grid->aLy > 0 && grid->aLz > 0)) return SALMON_ERR_ARG;
   salmon_ctx* c = new salmon_ctx();
@@ -645,6 +684,12 @@ int salmon_gpu_set_projectors(salmon_ctx* c, int NI, int Nps, const int* Mps, co
     lma_atom[l] = a;
     sign[l] = (double)iuV[l];
   }
+  // first projector of each atom (atoms without projectors: an empty range)
+  std::vector<int> atom_lma(NI + 1, Nlma);
+  for (int l = Nlma - 1; l >= 0; --l) atom_lma[lma_atom[l]] = l;
+  for (int a = NI - 1; a >= 0; --a) atom_lma[a] = std::min(atom_lma[a], atom_lma[a + 1]);
+  int max_lpa = 0;
+  for (int a = 0; a < NI; ++a) max_lpa = std::max(max_lpa, atom_lma[a + 1] - atom_lma[a]);
   // transposed table: per touched grid point, its (projector, packed point) entries
   std::vector<std::vector<std::pair<int, int>>> per(g.NL);
   for (int l = 0; l < Nlma; ++l) {
@@ -679,11 +724,13 @@ int salmon_gpu_set_projectors(salmon_ctx* c, int NI, int Nps, const int* Mps, co
   };
   KBTables kb{};
   kb.NI = NI; kb.Nps = Nps; kb.Nlma = Nlma; kb.NPI = NPI; kb.NT = NT; kb.NE = (int)ejp.size();
+  kb.max_lpa = max_lpa;
   int r;
   if ((r = up(start.data(), sizeof(int) * start.size(), (void**)&kb.atom_start)) ||
       (r = up(pidx.data(), sizeof(int) * pidx.size(), (void**)&kb.pidx)) ||
       (r = up(pos.data(), sizeof(double) * pos.size(), (void**)&kb.pos)) ||
       (r = up(lma_atom.data(), sizeof(int) * lma_atom.size(), (void**)&kb.lma_atom)) ||
+      (r = up(atom_lma.data(), sizeof(int) * atom_lma.size(), (void**)&kb.atom_lma)) ||
       (r = up(sign.data(), sizeof(double) * sign.size(), (void**)&kb.lma_sign)) ||
       (r = up(uV, sizeof(double) * (size_t)Nps * Nlma, (void**)&kb.uv)) ||
       (r = up(pt2t.data(), sizeof(int) * pt2t.size(), (void**)&kb.pt2t)) ||
@@ -933,10 +980,6 @@ static int gs_check(salmon_ctx* c, const double* kAc) {
   return gs_ensure(c);
 }
 
-// fraction of the previous sweep's energy drop below which the learned
-// preconditioner hands over to the kinetic one
-constexpr double kPcStall = 0.3;
-
 static int check_precond(salmon_ctx* c, int precond) {
   if (precond != SALMON_PC_NONE && precond != SALMON_PC_KINETIC && precond != SALMON_PC_LEARNED)
     return fail(c, SALMON_ERR_ARG, "unknown preconditioner %d", precond);
@@ -992,12 +1035,14 @@ int salmon_gpu_ground_state(salmon_ctx* c, const double* kAc, int max_sweeps, in
       change = std::max(change, std::fabs(e1[o] - e0[o]));
       sum += e1[o];
     }
-    // the learned direction leads while each sweep lowers the band-energy sum
-    // by at least kPcStall of the previous sweep's drop; then the kinetic
-    // preconditioner takes over for the rest of the solve
-    if (pc == SALMON_PC_LEARNED && n > 1) {
+    // the learned direction leads for at most pc_max_sweeps sweeps, and only
+    // while each sweep lowers the band-energy sum by at least pc_stall of the
+    // previous sweep's drop; then the kinetic preconditioner takes over for
+    // the rest of the solve
+    if (pc == SALMON_PC_LEARNED) {
       const double drop = prev_sum - sum;
-      if (n > 2 && !(drop >= kPcStall * prev_drop)) pc = SALMON_PC_KINETIC;
+      if (n > 2 && !(drop >= c->pc_stall * prev_drop)) pc = SALMON_PC_KINETIC;
+      if (c->pc_max_sweeps > 0 && n >= c->pc_max_sweeps) pc = SALMON_PC_KINETIC;
       prev_drop = drop;
     }
     prev_sum = sum;
@@ -1148,6 +1193,15 @@ int salmon_gpu_set_pcnet(salmon_ctx* c, salmon_pcnet* n) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   ++c->gen;  // a captured learned CG reads the previous network's weights
   c->pcnet = n;
+  return SALMON_OK;
+}
+
+int salmon_gpu_set_pc_handover(salmon_ctx* c, int max_learned_sweeps, double stall_ratio) {
+  if (!c) return SALMON_ERR_ARG;
+  if (max_learned_sweeps < 0 || !(stall_ratio >= 0.0))
+    return fail(c, SALMON_ERR_ARG, "max_learned_sweeps < 0 or stall_ratio not >= 0");
+  c->pc_max_sweeps = max_learned_sweeps;
+  c->pc_stall = stall_ratio;
   return SALMON_OK;
 }
 

@@ -69,6 +69,103 @@ __global__ void __launch_bounds__(256) k_kb_project(const cplx<T>* __restrict__ 
                      ph ? ph + (size_t)(ikb / NB) * kb.NPI : nullptr);
 }
 
+// ---------------------------------------------------------------------------
+// The same projection for the bands of one k point at once, on the matrix
+// cores (the OpenACC build batches phase 1 of pseudo_pt over (ikb, ilma),
+// ARTED/common/acc/hpsi.f90:72-93).  Per (atom, k point, tile of 16 bands,
+// block of 8 projectors) one wave forms the real GEMM
+//   C(16 bands x 16) = [psi_re | psi_im](16 x 2M) * [[w_re, w_im], [-w_im, w_re]](2M x 16)
+// over the atom's M sphere points, w(j,l) = uV(j,l) e^{i k.(r_j - T_j)}, with
+// columns 0..7 the real and 8..15 the imaginary parts of the block's
+// projections: two v_mfma_*_16x16x4 per four sphere points.  Operand layout
+// of the 16x16x4 MFMA: lane i holds A[i%16][i/16] and B[i/16][i%16], and the
+// accumulator rows 4*(i/16)+r (r = 0..3) of column i%16.  Every index a lane
+// reads is bounded by its own validity test, whatever the layout.
+// ---------------------------------------------------------------------------
+template <typename T>
+struct MfmaAcc;
+template <>
+struct MfmaAcc<double> {
+  typedef double v4 __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ v4 fma(double a, double b, v4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+};
+template <>
+struct MfmaAcc<float> {
+  typedef float v4 __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ v4 fma(float a, float b, v4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+};
+
+// grid: (NI, band tiles of the chunk, projector blocks of 8); block: one wave.
+// The chunk holds orbitals orb0 .. orb0+norb-1 (ikb = ik*NB + band); tile t
+// covers k point k0 + t / tpk and bands 16*(t % tpk) .. +15.
+template <typename T>
+__global__ void __launch_bounds__(64) k_kb_project_mfma(const cplx<T>* __restrict__ psi, int orb0, int norb, int NB,
+                                                        int NK, int k0, int tpk, const double* __restrict__ kac0,
+                                                        double ax, double ay, double az, Grid g, KBTables kb,
+                                                        cplx<T>* __restrict__ uvpsi, const cplx<T>* __restrict__ ph) {
+  typedef typename MfmaAcc<T>::v4 v4;
+  const int lane = threadIdx.x;
+  const int a = blockIdx.x;
+  const int ik = k0 + (int)blockIdx.y / tpk;
+  const int b0 = ((int)blockIdx.y % tpk) * 16;
+  const int l0 = kb.atom_lma[a] + 8 * (int)blockIdx.z, l1 = kb.atom_lma[a + 1];
+  if (ik >= NK || l0 >= l1) return;
+  const int j0 = kb.atom_start[a], m = kb.atom_start[a + 1] - j0;
+  // this lane's A row (a band) and B column (a projector, real or imaginary part)
+  const int row = lane & 15, kq = lane >> 4;
+  const int band = b0 + row;
+  const int o = ik * NB + band - orb0;  // chunk-local orbital
+  const bool rvalid = band < NB && o >= 0 && o < norb;
+  const cplx<T>* prow = psi + (size_t)(rvalid ? o : 0) * g.NL;
+  const int col = lane & 15;
+  const int lc = l0 + (col & 7);
+  const bool cvalid = lc < l1;
+  const double* ucol = kb.uv + (size_t)(cvalid ? lc : 0) * kb.Nps;
+  const cplx<T>* phk = ph ? ph + (size_t)ik * kb.NPI : nullptr;
+  double kx = 0, ky = 0, kz = 0;
+  if (!ph) orbital_k(kac0, NK, ik, ax, ay, az, kx, ky, kz);
+  v4 acc = {0, 0, 0, 0};
+  for (int jj = 0; jj < m; jj += 4) {
+    const int j = jj + kq;
+    const bool jvalid = j < m;
+    cplx<T> e{0, 0};
+    if (rvalid && jvalid) e = prow[kb.pidx[j0 + j]];
+    T wr = 0, wi = 0;
+    if (cvalid && jvalid) {
+      T sn, cs;
+      if (phk) {
+        cs = phk[j0 + j].re;
+        sn = phk[j0 + j].im;
+      } else {
+        const int jp = j0 + j;
+        const double kr = kx * kb.pos[3 * jp] + ky * kb.pos[3 * jp + 1] + kz * kb.pos[3 * jp + 2];
+        sincos_t((T)kr, &sn, &cs);
+      }
+      const T u = (T)ucol[j];
+      wr = u * cs;
+      wi = u * sn;
+    }
+    // A = psi_re against [w_re | w_im], then A = psi_im against [-w_im | w_re]
+    acc = MfmaAcc<T>::fma(e.re, col < 8 ? wr : wi, acc);
+    acc = MfmaAcc<T>::fma(e.im, col < 8 ? -wi : wr, acc);
+  }
+  if (!cvalid) return;
+  const T f = (T)(g.Hxyz * kb.lma_sign[lc]);
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int br = b0 + 4 * kq + r;
+    const int orow = ik * NB + br - orb0;
+    if (br < NB && orow >= 0 && orow < norb) {
+      T* dst = (T*)&uvpsi[(size_t)orow * kb.Nlma + lc];
+      dst[col < 8 ? 0 : 1] = acc[r] * f;
+    }
+  }
+}
+
 // e^{i k.(r_j - T_j)} of every sphere point for every k point at kAc0 + a
 // (NK, NPI): the phases of the KB projection and apply, shared by the four H
 // applications of a Taylor step and by all bands of a k point.  FP64 sincos,

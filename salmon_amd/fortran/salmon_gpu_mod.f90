@@ -217,6 +217,14 @@ module salmon_gpu
       type(c_ptr), value :: ctx, net
     end function
 
+    integer(c_int) function salmon_gpu_set_pc_handover(ctx, max_learned_sweeps, stall_ratio) &
+        bind(C, name="salmon_gpu_set_pc_handover")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: max_learned_sweeps
+      real(c_double), value :: stall_ratio
+    end function
+
     integer(c_int) function salmon_gpu_pc_record(ctx, max_samples, stride) bind(C, name="salmon_gpu_pc_record")
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx

@@ -85,6 +85,7 @@ SIGNATURES = {
     "salmon_gpu_set_scf": [_vp, _vp, _ci],
     "salmon_gpu_hartree_xc": [_vp, _vp, _ci, _vp, _vp],
     "salmon_gpu_set_pcnet": [_vp, _vp],
+    "salmon_gpu_set_pc_handover": [_vp, _ci, _cd],
     "salmon_gpu_pc_record": [_vp, _ci, _ci],
     "salmon_gpu_pc_samples": [_vp, _vp, _vp],
     "salmon_gpu_pc_get_sample": [_vp, _ci, _vp, _vp, _vp, _vp, _vp],
@@ -334,6 +335,13 @@ class Context:
         vh, vx = np.zeros(self.sys.NL), np.zeros(self.sys.NL)
         self._check(lib().salmon_gpu_hartree_xc(self._h, _p(r), int(xc), _p(vh), _p(vx)), "hartree_xc")
         return vh, vx
+
+    def set_pc_handover(self, max_learned_sweeps, stall_ratio=0.3):
+        """Learned -> kinetic handover of ground_state: after at most
+        max_learned_sweeps sweeps (0: no limit), earlier when a sweep's drop of
+        the band-energy sum falls below stall_ratio times the previous one."""
+        self._check(lib().salmon_gpu_set_pc_handover(self._h, int(max_learned_sweeps), float(stall_ratio)),
+                    "set_pc_handover")
 
     def set_graphs(self, enable):
         """CG / Gram-Schmidt through the graph cache (default) or eagerly; a

@@ -115,32 +115,47 @@ def sweeps_to(history, ref, acc):
     return int(hit[0]) + 1 if hit.size else None
 
 
-def evaluate(net, n_eval=50, acc=1e-8, max_sweeps=120, nb=NB, log=_log):
+def evaluate(net, n_eval=50, acc=1e-8, max_sweeps=120, nb=NB, policies=((0, 0.3),), log=_log):
+    """Sweeps to acc for the kinetic solve and for the learned solve under each
+    handover policy (max learned sweeps, stall ratio), from the same orbitals."""
     gpu.lib()
 
     def one(i):
         s = synth.config5(seed=EVAL_SEED0 + i, NB=nb)
+        row = dict(seed=EVAL_SEED0 + i)
         with gpu.Context(s, gpu.FP64) as c:
             c.set_graphs(False)  # contexts are driven from several threads at once
             rk = c.ground_state(s.kAc0, max_sweeps, iter_cg_max=NCG, precond=gpu.PC_KINETIC, tol=1e-12)
             ref = rk["esp"]
-            c.put_orbitals(s.zu)
+            row["kinetic"] = sweeps_to(rk["history"], ref, acc)
             c.set_pcnet(net)
-            rl = c.ground_state(s.kAc0, max_sweeps, iter_cg_max=NCG, precond=gpu.PC_LEARNED, tol=1e-12)
-        nk, nl = sweeps_to(rk["history"], ref, acc), sweeps_to(rl["history"], ref, acc)
-        row = dict(seed=EVAL_SEED0 + i, kinetic=nk, learned=nl,
-                   learned_final_err=float(np.max(np.abs(rl["esp"] - ref))))
-        log(f"[pctrain] eval seed {row['seed']}: sweeps to {acc:g} Ha kinetic {nk} learned {nl} "
-            f"(final err {row['learned_final_err']:.1e})")
+            for p in policies:
+                c.put_orbitals(s.zu)
+                c.set_pc_handover(*p)
+                rl = c.ground_state(s.kAc0, max_sweeps, iter_cg_max=NCG, precond=gpu.PC_LEARNED, tol=1e-12)
+                key = "learned" if len(policies) == 1 else f"learned_{p[0]}_{p[1]:g}"
+                row[key] = sweeps_to(rl["history"], ref, acc)
+                row[key + "_final_err"] = float(np.max(np.abs(rl["esp"] - ref)))
+        log(f"[pctrain] eval seed {row['seed']}: sweeps to {acc:g} Ha " +
+            " ".join(f"{k} {v}" for k, v in row.items() if k != "seed" and not k.endswith("_err")))
         return row
 
     with ThreadPoolExecutor(max_workers=WORKERS) as ex:
         rows = list(ex.map(one, range(n_eval)))
-    ok = [r for r in rows if r["kinetic"] and r["learned"]]
-    ratio = float(np.sum([r["kinetic"] for r in ok]) / max(1, np.sum([r["learned"] for r in ok])))
-    return dict(rows=rows, acc=acc, ncg=NCG, nb=nb, sweep_ratio=ratio,
-                converged_kinetic=sum(1 for r in rows if r["kinetic"]),
-                converged_learned=sum(1 for r in rows if r["learned"]))
+    keys = [k for k in rows[0] if k.startswith("learned") and not k.endswith("_err")]
+    out = dict(rows=rows, acc=acc, ncg=NCG, nb=nb, policies=[list(p) for p in policies],
+               converged_kinetic=sum(1 for r in rows if r["kinetic"]))
+    for k in keys:
+        ok = [r for r in rows if r["kinetic"] and r[k]]
+        ratios = np.array([r["kinetic"] / r[k] for r in ok]) if ok else np.array([np.nan])
+        out[k] = dict(sweep_ratio=float(np.sum([r["kinetic"] for r in ok]) / max(1, np.sum([r[k] for r in ok]))),
+                      per_seed_ratio_median=float(np.median(ratios)),
+                      per_seed_ratio_p10_p90=[float(np.percentile(ratios, 10)), float(np.percentile(ratios, 90))],
+                      converged=sum(1 for r in rows if r[k]), better=sum(1 for r in ok if r[k] < r["kinetic"]),
+                      worse=sum(1 for r in ok if r[k] > r["kinetic"]))
+    if len(keys) == 1:
+        out.update(sweep_ratio=out[keys[0]]["sweep_ratio"], converged_learned=out[keys[0]]["converged"])
+    return out
 
 
 def main(argv=None):
@@ -158,6 +173,8 @@ def main(argv=None):
     ap.add_argument("--weights", default="salmon_amd/data/pcnet_cfg5.npy")
     ap.add_argument("--out", default=None)
     ap.add_argument("--no-graphs", action="store_true", help="solve: run CG eagerly (no captured graphs)")
+    ap.add_argument("--policies", default="0:0.3",
+                    help="eval: learned->kinetic handover policies 'max_learned_sweeps:stall_ratio,...'")
     a = ap.parse_args(argv)
     if a.mode == "solve":
         pc = {"none": gpu.PC_NONE, "kinetic": gpu.PC_KINETIC, "learned": gpu.PC_LEARNED}[a.precond]
@@ -184,7 +201,8 @@ def main(argv=None):
         _log(f"[pctrain] weights written to {a.out or a.weights}")
     else:
         net = gpu.PcNet(weights=np.load(a.weights))
-        res = evaluate(net, a.eval, nb=a.nb)
+        pols = tuple((int(p.split(":")[0]), float(p.split(":")[1])) for p in a.policies.split(","))
+        res = evaluate(net, a.eval, nb=a.nb, policies=pols)
         _log(json.dumps({k: v for k, v in res.items() if k != "rows"}))
         if a.out:
             with open(a.out, "w") as f:

@@ -1,11 +1,21 @@
-"""Runs the reference's RT step loop (oracle/_ref harness) on a golden case,
-with either the reference's own hamiltonian or the GPU drop-in linked in.
-Invoked in a subprocess by test_fortran_dropin.py (module globals: one
-system per process).  Prints one JSON line."""
+"""Runs the reference's own Fortran (oracle/_ref harness) on a golden case,
+either all-CPU (libsalmon_ref.so) or with the GPU drop-ins of
+salmon_amd/fortran linked in place of the files they replace
+(libsalmon_ref_gpu.so: hamiltonian, current_acc_impl, psi_rho_RT/GS,
+Total_Energy_omp, module hpsi, CG_omp, Gram_Schmidt).  Invoked in a
+subprocess by test_fortran_dropin.py (module globals: one system per
+process).  Prints one JSON line.
+
+  python dropin_run.py CASE cpu|gpu STEPS          the RT step loop
+  python dropin_run.py CASE cpu-gs|gpu-gs NCG      Gram_Schmidt then CG_omp
+  python dropin_run.py CASE cpu-obs|gpu-obs 0      hpsi, sp_energy, current, density, energy
+  python dropin_run.py cfg2 gpu-time|cpu-time STEPS  config-2 step time of the loop
+"""
 import ctypes
 import json
 import os
 import sys
+import time
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 sys.path.insert(0, ROOT)
@@ -17,34 +27,75 @@ from helpers import load_case  # noqa: E402
 from oracle import ref  # noqa: E402
 
 
+def _lib_path(which):
+    return ref.LIB_GPU if which.startswith("gpu") else ref.LIB
+
+
+def _counter(name, which):
+    return ctypes.c_int.in_dll(ref.lib(), name).value if which.startswith("gpu") else -1
+
+
+def _sample(z):
+    return [z.real.ravel()[::97].tolist(), z.imag.ravel()[::97].tolist()]
+
+
 def ground_state(case, which, ncg):
     """Gram_Schmidt then CG_omp(ncg) through the reference's harness."""
     s, d = load_case(case)
-    path = ref.LIB_GPU if which == "gpu" else ref.LIB
-    R = ref.Ref(s, lib_path=path)
+    R = ref.Ref(s, lib_path=_lib_path(which))
     R.gram_schmidt()
     zg = R.get()
     ev = R.cg(s.kAc0, ncg)
     z = R.get()
-    calls = ctypes.c_int.in_dll(ref.lib(), "arted_gpu_gs_calls").value if which == "gpu" else -1
-    print(json.dumps(dict(zg=[zg.real.ravel()[::97].tolist(), zg.imag.ravel()[::97].tolist()],
-                          z=[z.real.ravel()[::97].tolist(), z.imag.ravel()[::97].tolist()],
-                          esp_var=ev.ravel().tolist(), calls=calls)))
+    print(json.dumps(dict(zg=_sample(zg), z=_sample(z), esp_var=ev.ravel().tolist(),
+                          calls=_counter("arted_gpu_gs_calls", which))))
 
 
-def main(case, which, nsteps):
-    if which.endswith("-gs"):
-        return ground_state(case, which[:-3], nsteps)
+def observables(case, which):
+    """One call of each observable and operator on the golden orbitals."""
+    from oracle.cpu import kac_at
     s, d = load_case(case)
-    path = ref.LIB_GPU if which == "gpu" else ref.LIB
-    R = ref.Ref(s, lib_path=path)
-    r = R.rt_run(d["ac"], nsteps)
+    R = ref.Ref(s, lib_path=_lib_path(which))
+    ik = s.NK - 1
+    h, t = R.hpsi(ik, d["hpsi_k"], s.zu[ik, s.NB - 1])
+    esp = R.sp_energy(s.kAc0)
+    k1 = kac_at(s, d["ac"][1])
+    j = R.current(k1)
+    rho = R.density()
+    ek, en = R.energy(k1)
+    print(json.dumps(dict(hpsi=_sample(h), tt=_sample(t), esp=esp.ravel().tolist(), J=j.tolist(),
+                          rho=rho.tolist(), ekin=ek, enl=en, transfers=_counter("arted_gpu_orbital_transfers", which))))
+
+
+def timing(which, nsteps):
+    """The reference's RT step loop at the config-2 size (16^3, 512 k x 32
+    bands): ms per step after one warm-up step."""
+    from salmon_amd import synth
+    s, ac = synth.config2(nsteps=nsteps + 1, orbitals=False)
+    s.zu = synth.random_orbitals(np.random.default_rng(7), s.NK, s.NB, s.NL, s.Hxyz, orthonormalise=False)
+    R = ref.Ref(s, lib_path=_lib_path(which))
+    R.rt_run(ac[:2], 1)
+    t0 = time.perf_counter()
+    r = R.rt_run(ac[1:nsteps + 2], nsteps)
+    t = time.perf_counter() - t0
+    print(json.dumps(dict(ms_per_step=t / nsteps * 1e3, steps=nsteps, J=r["J"].tolist(),
+                          threads=R.num_threads(), transfers=_counter("arted_gpu_orbital_transfers", which))))
+
+
+def main(case, which, n):
+    if which.endswith("-gs"):
+        return ground_state(case, which[:-3], n)
+    if which.endswith("-obs"):
+        return observables(case, which)
+    if which.endswith("-time"):
+        return timing(which, n)
+    s, d = load_case(case)
+    R = ref.Ref(s, lib_path=_lib_path(which))
+    r = R.rt_run(d["ac"], n)
     z = R.get()
-    calls = -1
-    if which == "gpu":
-        calls = ctypes.c_int.in_dll(ref.lib(), "arted_gpu_hamiltonian_calls").value
-    print(json.dumps(dict(J=r["J"].tolist(), E=r["E"].tolist(), zabs=float(np.abs(z).sum()),
-                          z0=[float(z.real.ravel()[7]), float(z.imag.ravel()[7])], calls=calls)))
+    print(json.dumps(dict(J=r["J"].tolist(), E=r["E"].tolist(), zabs=float(np.abs(z).sum()), z=_sample(z),
+                          calls=_counter("arted_gpu_hamiltonian_calls", which),
+                          transfers=_counter("arted_gpu_orbital_transfers", which))))
 
 
 if __name__ == "__main__":

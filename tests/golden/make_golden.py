@@ -3,7 +3,7 @@ Fortran (oracle/_ref/libsalmon_ref.so, compiled from the SALMON sources by
 oracle/ref/Makefile).  Run in the build container, where the reference tree
 exists:
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py [case ...]
 
 Each case runs in a subprocess because the reference keeps its state in
 module globals (one system per process).  A fixture holds the inputs (the
@@ -31,6 +31,10 @@ CASES = {
                      natoms=5, pulse=dict(nsteps=5, amplitude_A=0.1, omega=0.5, tpulse=2.0)),
     # the 16^3 Gamma-point workload with 16 orbitals, 100 steps at A=(0.01,0,0)
     "cfg1": dict(kind="cfg1", seed=0),
+    # 8^3 grid on the 10.26 bohr cell (the grid of the reference's ARTED RT
+    # testsuite, testsuites/002_bulk_rt), 4 k points x 3 bands, diamond atoms
+    "grid8": dict(kind="make", N=(8, 8, 8), L=10.26, nk3=(2, 1, 2), NB=3, seed=8, atoms="diamond",
+                  pulse=dict(nsteps=6, amplitude_A=0.05, omega=0.3, tpulse=2.0)),
 }
 
 
@@ -107,7 +111,8 @@ def main():
     from oracle import ref
     if not ref.build():
         raise SystemExit("reference build not available")
-    for name in CASES:
+    names = sys.argv[1:] or list(CASES)  # python make_golden.py [case ...]
+    for name in names:
         print("case", name, flush=True)
         subprocess.check_call([sys.executable, os.path.abspath(__file__), "--case", name])
 

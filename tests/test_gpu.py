@@ -14,7 +14,7 @@ from salmon_amd import gpu, synth
 
 pytestmark = pytest.mark.gpu
 
-CASES = ["small_k", "noncubic"]
+CASES = ["small_k", "noncubic", "grid8"]
 TOL64 = 1e-12
 
 
@@ -271,3 +271,23 @@ def test_fused_overlapping_spheres_against_oracle():
     assert rel(r["J"], o["J"]) < 1e-11
     assert rel(r["E"], o["E"]) < TOL64
     assert rel(r["rho"], o["rho"]) < TOL64  # density partials of the fused kernel
+
+
+def test_streamed_many_bands_mfma_projection_against_oracle():
+    """With 8 or more bands per k point the streamed step forms the KB
+    projections of a k point's bands on the matrix cores (16-band tiles; 20
+    bands leave the second tile partly empty); non-cubic grid, random atoms
+    with overlapping spheres, two k points.  FP64 against the oracle to
+    1e-12; complex64 against the FP64 oracle to 1e-5 (J) and 1e-4 (orbitals)."""
+    s = synth.make_system(N=(12, 14, 16), L=(7.0, 8.0, 9.0), nk3=(2, 1, 1), NB=20, seed=23, atoms="random",
+                          natoms=5, name="mfma20")
+    ac = synth.acos2_pulse(3, s.dt, 0.05, 0.3, 2.0)
+    o = Oracle(s).rt_run(ac, 3)
+    for prec, tol, tolz in ((gpu.FP64, 1e-12, 1e-12), (gpu.MIXED, 1e-5, 1e-4)):
+        with gpu.Context(s, prec) as c:
+            r = c.rt_run(ac, 3)
+            z = c.get_orbitals()
+        ej = np.max(np.abs(r["J"] - o["J"])) / np.max(np.abs(o["J"]))
+        print(f"[gpu] mfma projection prec={prec}: J {ej:.2e} E {rel(r['E'], o['E']):.2e} zu {rel(z, o['zu']):.2e}",
+              flush=True)
+        assert ej < tol and rel(r["E"], o["E"]) < tol and rel(z, o["zu"]) < tolz

@@ -17,7 +17,7 @@ from salmon_amd import gpu, synth
 
 pytestmark = pytest.mark.gpu
 
-CASES = ["small_k", "noncubic"]
+CASES = ["small_k", "noncubic", "grid8"]
 TOL_CG = 1e-12
 
 

@@ -9,7 +9,7 @@ from helpers import load_case, rel
 from oracle.cpu import Oracle, kac_at, midpoint_kac
 from salmon_amd import synth
 
-CASES = ["small_k", "noncubic"]
+CASES = ["small_k", "noncubic", "grid8"]
 
 
 # ---------------------------------------------------------------- reference
