@@ -78,15 +78,18 @@ __global__ void __launch_bounds__(256) k_kb_project(const cplx<T>* __restrict__ 
 // over the atom's M sphere points, w(j,l) = uV(j,l) e^{i k.(r_j - T_j)}, with
 // columns 0..7 the real and 8..15 the imaginary parts of the block's
 // projections: two v_mfma_*_16x16x4 per four sphere points.  Operand layout
-// of the 16x16x4 MFMA: lane i holds A[i%16][i/16] and B[i/16][i%16], and the
-// accumulator rows 4*(i/16)+r (r = 0..3) of column i%16.  Every index a lane
-// reads is bounded by its own validity test, whatever the layout.
+// of the 16x16x4 MFMA: lane i holds A[i%16][i/16] and B[i/16][i%16]; its four
+// accumulator values are rows row(i/16, r), r = 0..3, of column i%16, with
+// row(g, r) = 4g + r for the f32 instruction and g + 4r for the f64 one (read
+// back on gfx950 with one-hot operands).  Every index a lane reads is bounded
+// by its own validity test, whatever the layout.
 // ---------------------------------------------------------------------------
 template <typename T>
 struct MfmaAcc;
 template <>
 struct MfmaAcc<double> {
   typedef double v4 __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ int row(int g, int r) { return g + 4 * r; }
   static __device__ __forceinline__ v4 fma(double a, double b, v4 c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
   }
@@ -94,6 +97,7 @@ struct MfmaAcc<double> {
 template <>
 struct MfmaAcc<float> {
   typedef float v4 __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ int row(int g, int r) { return 4 * g + r; }
   static __device__ __forceinline__ v4 fma(float a, float b, v4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
   }
@@ -157,7 +161,7 @@ __global__ void __launch_bounds__(64) k_kb_project_mfma(const cplx<T>* __restric
   const T f = (T)(g.Hxyz * kb.lma_sign[lc]);
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
-    const int br = b0 + 4 * kq + r;
+    const int br = b0 + MfmaAcc<T>::row(kq, r);
     const int orow = ik * NB + br - orb0;
     if (br < NB && orow >= 0 && orow < norb) {
       T* dst = (T*)&uvpsi[(size_t)orow * kb.Nlma + lc];

@@ -47,7 +47,10 @@ contains
     if (present(ttpsi)) ttpsi(0:NLz-1, 0:NLy-1, 0:NLx-1) = t
   end subroutine
 
-  ! H psi on the GPU (the RT drop-in's context holds the grid and tables)
+  ! H psi on the GPU (the RT drop-in's context holds the grid and tables).
+  ! The reference calls hpsi_omp_KB_GS from inside OpenMP loops over k points
+  ! (GS/sp_energy.f90:39-49); the context, its stream and staging buffers
+  ! serve one caller at a time, so the calls are serialised.
   subroutine gpu_hpsi(ik, psi, hp, tt, with_tt)
     use Global_Variables, only: NL, kAc, Vloc
     use salmon_global, only: alocal_laser
@@ -62,11 +65,13 @@ contains
     real(8) :: k(3)
     type(c_ptr) :: ptt
     if (alocal_laser == 'y') call err_finalize('GPU hpsi: alocal_laser is not built')
-    call arted_gpu_setup()
     k = kAc(ik, :)
-    call arted_gpu_check(salmon_gpu_set_potential(ctx, Vloc), 'set_potential')
     ptt = c_null_ptr
     if (with_tt) ptt = c_loc(tt)
+!$omp critical (arted_gpu_context)
+    call arted_gpu_setup()
+    call arted_gpu_check(salmon_gpu_set_potential(ctx, Vloc), 'set_potential')
     call arted_gpu_check(salmon_gpu_hpsi(ctx, k, c_loc(psi), c_loc(hp), ptt), 'hpsi')
+!$omp end critical (arted_gpu_context)
   end subroutine
 end module hpsi
