@@ -303,6 +303,19 @@ enum { SALMON_XC_NONE = 0, SALMON_XC_PZ = 1 };
 int salmon_gpu_set_scf(salmon_ctx *ctx, const double *vpsl, int xc);
 int salmon_gpu_hartree_xc(salmon_ctx *ctx, const double *rho, int xc, double *vh, double *vxc);
 
+/* salmon_gpu_hartree_xc without any allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight loops, hosts that forbid hipMalloc).  rho, vh and
+ * vxc are DEVICE pointers of NL doubles (vh, vxc nullable); the work is
+ * enqueued on the context's stream and the call returns without waiting.
+ * Its DFT grids and twiddles live in one fixed device array per GPU (8 MiB,
+ * held until the process ends, set up by salmon_gpu_init), so grids of at
+ * most SALMON_NOALLOC_MAX_NL points are accepted (larger: SALMON_ERR_UNSUPPORTED).
+ * Every context on a GPU shares that array: calls that overlap on one GPU
+ * (several streams or threads) race and must be ordered by the caller. */
+enum { SALMON_NOALLOC_MAX_NL = 64 * 64 * 64 };
+int salmon_gpu_hartree_xc_noalloc(salmon_ctx *ctx, const double *rho_dev, int xc, double *vh_dev, double *vxc_dev);
+
 /* Waits for all work of the context. */
 int salmon_gpu_synchronize(salmon_ctx *ctx);
 

@@ -8,6 +8,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -557,6 +558,52 @@ int rt_run_scf(salmon_ctx* c, const double* ac, int nsteps, double dt, int reort
   return r;
 }
 
+// Fixed device scratch of salmon_gpu_hartree_xc_noalloc: the DFT twiddles
+// (3 axes x kMaxN x (cos, sin)) followed by two complex grids of at most
+// SALMON_NOALLOC_MAX_NL points.  In host code `scratch` only names the
+// symbol; its device address on each GPU comes from hipGetSymbolAddress
+// (noalloc_setup) and only that address is handed to kernels.
+constexpr size_t kNoAllocTw = (size_t)3 * 2 * salmon::scf::kMaxN;
+constexpr size_t kNoAllocDoubles = kNoAllocTw + (size_t)4 * SALMON_NOALLOC_MAX_NL;
+static __device__ __align__(256) double scratch[kNoAllocDoubles];
+
+constexpr int kNoAllocMaxDevices = 64;
+std::mutex g_noalloc_mu;
+double* g_noalloc_ptr[kNoAllocMaxDevices] = {};
+hipError_t g_noalloc_err[kNoAllocMaxDevices] = {};
+
+// Once per device, with that device current (salmon_gpu_init): the device
+// address of this GPU's scratch copy.
+void noalloc_setup(int device) {
+  if (device < 0 || device >= kNoAllocMaxDevices) return;
+  std::lock_guard<std::mutex> lk(g_noalloc_mu);
+  if (g_noalloc_ptr[device]) return;
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  g_noalloc_err[device] = e;
+  if (e == hipSuccess) g_noalloc_ptr[device] = (double*)p;
+}
+
+double* noalloc_ptr(int device, hipError_t* err) {
+  if (device < 0 || device >= kNoAllocMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lk(g_noalloc_mu);
+  *err = g_noalloc_err[device];
+  return g_noalloc_ptr[device];
+}
+
+// (cos, sin)(2 pi j / N_a) for j < N_a, zero rows beyond, as scf_twiddles
+// lays them out
+__global__ void __launch_bounds__(256) k_noalloc_twiddles(double* tw, int nx, int ny, int nz) {
+  const int q = blockIdx.x * 256 + threadIdx.x;
+  if (q >= 3 * salmon::scf::kMaxN) return;
+  const int a = q / salmon::scf::kMaxN, j = q - a * salmon::scf::kMaxN;
+  const int n = a == 0 ? nx : (a == 1 ? ny : nz);
+  double s = 0.0, c = 0.0;
+  if (j < n) sincospi(2.0 * j / n, &s, &c);
+  tw[2 * q] = c;
+  tw[2 * q + 1] = s;
+}
+
 }  // namespace
 
 extern "C" {
@@ -632,6 +679,7 @@ int salmon_gpu_init(const salmon_grid_t* grid, int precision, salmon_ctx** out) 
     return SALMON_ERR_HIP;
   }
   c->fused = fused_config(c);
+  noalloc_setup(c->device);
   *out = c;
   return SALMON_OK;
 }
@@ -1171,6 +1219,28 @@ int salmon_gpu_hartree_xc(salmon_ctx* c, const double* rho, int xc, double* vh, 
   }
   (void)hipFree(d_in);
   return r;
+}
+
+int salmon_gpu_hartree_xc_noalloc(salmon_ctx* c, const double* rho_dev, int xc, double* vh_dev, double* vxc_dev) {
+  if (!c || !rho_dev) return c ? fail(c, SALMON_ERR_ARG, "rho_dev is NULL") : SALMON_ERR_ARG;
+  if (xc != SALMON_XC_NONE && xc != SALMON_XC_PZ) return fail(c, SALMON_ERR_ARG, "xc %d is neither NONE nor PZ", xc);
+  const Grid& g = c->g;
+  if ((size_t)g.NL > (size_t)SALMON_NOALLOC_MAX_NL)
+    return fail(c, SALMON_ERR_UNSUPPORTED, "grid of %d points exceeds the allocation-free limit of %d", g.NL,
+                (int)SALMON_NOALLOC_MAX_NL);
+  int r;
+  if ((r = scf_check_grid(c))) return r;
+  hipError_t e = hipSuccess;
+  double* base = noalloc_ptr(c->device, &e);
+  if (!base)
+    return fail(c, SALMON_ERR_HIP, "device scratch not set up on device %d: %s", c->device, hipGetErrorString(e));
+  double* tw = base;
+  cplx<double>* A = (cplx<double>*)(base + kNoAllocTw);
+  cplx<double>* B = A + g.NL;
+  hipLaunchKernelGGL(k_noalloc_twiddles, dim3((3 * salmon::scf::kMaxN + 255) / 256), dim3(256), 0, c->stream, tw,
+                     g.NLx, g.NLy, g.NLz);
+  LAUNCHCHK(c);
+  return scf_potential_on(c, rho_dev, false, xc, vh_dev, vxc_dev, tw, A, B);
 }
 
 int salmon_gpu_set_propagator(salmon_ctx* c, int propagator) {

@@ -84,6 +84,7 @@ SIGNATURES = {
     "salmon_gpu_set_propagator": [_vp, _ci],
     "salmon_gpu_set_scf": [_vp, _vp, _ci],
     "salmon_gpu_hartree_xc": [_vp, _vp, _ci, _vp, _vp],
+    "salmon_gpu_hartree_xc_noalloc": [_vp, _vp, _ci, _vp, _vp],
     "salmon_gpu_set_pcnet": [_vp, _vp],
     "salmon_gpu_set_pc_handover": [_vp, _ci, _cd],
     "salmon_gpu_pc_record": [_vp, _ci, _ci],
@@ -335,6 +336,13 @@ class Context:
         vh, vx = np.zeros(self.sys.NL), np.zeros(self.sys.NL)
         self._check(lib().salmon_gpu_hartree_xc(self._h, _p(r), int(xc), _p(vh), _p(vx)), "hartree_xc")
         return vh, vx
+
+    def hartree_xc_noalloc(self, rho_dev, xc, vh_dev, vxc_dev):
+        """hartree_xc on device buffers (addresses as ints, vh/vxc may be 0),
+        enqueued on the context's stream without allocating; call
+        synchronize() before reading the results."""
+        self._check(lib().salmon_gpu_hartree_xc_noalloc(self._h, _vp(rho_dev), int(xc), _vp(vh_dev or None),
+                                                        _vp(vxc_dev or None)), "hartree_xc_noalloc")
 
     def set_pc_handover(self, max_learned_sweeps, stall_ratio=0.3):
         """Learned -> kinetic handover of ground_state: after at most
