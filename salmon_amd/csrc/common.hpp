@@ -190,6 +190,15 @@ __device__ __forceinline__ void block_sum256(double (&v)[NV], double* smem) {
   __syncthreads();
 }
 
+// Workgroup barrier for exchanges through LDS only: it waits for this wave's
+// LDS operations but not for its outstanding global loads and stores, which
+// __syncthreads also waits for.
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 __device__ __forceinline__ void sincos_t(double x, double* s, double* c) { sincos(x, s, c); }
 __device__ __forceinline__ void sincos_t(float x, float* s, float* c) { sincosf(x, s, c); }
 

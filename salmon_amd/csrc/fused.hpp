@@ -23,6 +23,8 @@
 namespace salmon {
 
 constexpr int kFusedMaxNPI = 1152;      // sphere points per cell (LDS correction array)
+constexpr int kFusedMaxAtoms = 64;      // atoms per cell (LDS atom table)
+constexpr int kFusedSpinMax = 1 << 14;  // bound of one KB handover wait (s_sleep 1 each, ~0.5 ms)
 constexpr int kFusedMaxLmaPerAtom = 4;  // projectors per atom
 constexpr int kLA = kFusedMaxLmaPerAtom;
 
@@ -228,9 +230,23 @@ __global__ void __launch_bounds__(NY * NZ * XS, WPE) k_fused_step(FusedArgs<T> a
   int* gix = reinterpret_cast<int*>(ug + kLA * MT);         // grid index of each sphere point
   double* rhol = reinterpret_cast<double*>(gix + MT);       // density of the workgroup's bands
   double* red = rhol + (LT ? NL : 0);
+  // per atom: its first sphere point, and per projector Hxyz iuV (kfac, in
+  // the propagation precision) and iuV (ksgn), so the KB loops read no global
+  // memory between two barriers
+  T* kfac = reinterpret_cast<T*>(red + 10 * NW);
+  float* ksgn = reinterpret_cast<float*>(kfac + kFusedMaxAtoms * kLA);
+  int* katom = reinterpret_cast<int*>(ksgn + kFusedMaxAtoms * kLA);
+  // image offsets of each sphere point, (jx+2) + 5 (jy+2) + 25 (jz+2), for
+  // its unwrapped position l H - j L in the nonlocal observables
+  // KB handover: the KB waves' count of published applications, and a flag
+  // set when a wait ran out
+  int* kbcnt = katom + kFusedMaxAtoms + 1;
+  int* kbfail = kbcnt + 1;
+  unsigned char* gimg = reinterpret_cast<unsigned char*>(kbfail + 1);
 
   const int t = threadIdx.x;
   const int lane = t & (kWave - 1), wv = t / kWave;
+  const int wvu = __builtin_amdgcn_readfirstlane(wv);  // the same, known to be wave-uniform
   const int xs = t / NYZ, yz = t - xs * NYZ;
   const int iy = yz / NZ, iz = yz - iy * NZ;
   const int x0 = xs * PX;
@@ -283,6 +299,16 @@ __global__ void __launch_bounds__(NY * NZ * XS, WPE) k_fused_step(FusedArgs<T> a
   // per-wave FP64 accumulators of the workgroup's observables, in LDS:
   // |psi|^2, V|psi|^2, kinetic, current stencil x/y/z, nonlocal current x/y/z, Enl
   if (lane < 10) red[wv * 10 + lane] = 0.0;
+  if (t == 0) {
+    *kbcnt = 0;
+    *kbfail = 0;
+  }
+  for (int q = t; q <= a.NI; q += NT) katom[q] = a.atom_start[q];
+  for (int q = t; q < a.NI * kLA; q += NT) {
+    const double sg = a.sgnp[q];
+    kfac[q] = (T)(g.Hxyz * sg);
+    ksgn[q] = (float)sg;
+  }
   // density partial of this workgroup: each thread owns its points, so the
   // slice is updated without atomics
   double* rho = a.rho_part + (size_t)blockIdx.x * NL;
@@ -296,6 +322,7 @@ __global__ void __launch_bounds__(NY * NZ * XS, WPE) k_fused_step(FusedArgs<T> a
       em[q] = phase_at<T, NX, NY, NZ>(phm, code);
       en[q] = phase_at<T, NX, NY, NZ>(phn, code);
       gix[q] = a.jp_grid[q];
+      gimg[q] = (unsigned char)((code & 1023) / NX + 5 * (((code >> 10) & 1023) / NY) + 25 * ((code >> 20) / NZ));
       const T* ur = a.uvp + (size_t)a.jp_atom[q] * kLA * a.Nps + a.jp_j[q];
 #pragma unroll
       for (int l = 0; l < kLA; ++l) ug[l * kFusedMaxNPI + q] = ur[(size_t)l * a.Nps];
@@ -303,6 +330,27 @@ __global__ void __launch_bounds__(NY * NZ * XS, WPE) k_fused_step(FusedArgs<T> a
     __syncthreads();
   }
 
+  // Wave roles of one H application.
+  // SEQ (complex64, and FP64 with overlapping spheres): every wave computes the
+  // KB terms of an atom, a barrier (and the fold of the terms of points shared
+  // by several spheres), then every wave runs the stencil of its points and
+  // adds the KB term, the Taylor sum and the x window update at once.
+  // AB (FP64 without overlapping spheres, whose KB loops read their tables
+  // from global memory): one wave of each pair that shares a SIMD (waves w and
+  // w+4, or 2j and 2j+1, whichever way the workgroup is placed: waves 0, 2,
+  // 5, 7) computes the KB terms of two atoms and publishes them through an LDS
+  // counter before its stencil, while its partner starts the stencil at once;
+  // both add the KB terms once the counter shows every KB wave's terms, so the
+  // latency-bound KB loops overlap stencil arithmetic.  Measured on the 16^3
+  // step: AB is faster for FP64 and slower for complex64, whose stencil is
+  // bound by LDS bandwidth as much as by VALU issue and pays for AB's extra
+  // LDS round trip of the stencil part.
+  constexpr bool AB = !OV && sizeof(T) == 8;
+  static_assert(!AB || NW == 8, "KB wave pairing assumes 8 waves");
+  const int order = ((wvu >> 2) ^ wvu) & 1;
+  const bool kbw = !AB || order == 0;
+  const int kat0 = AB ? (wvu >> 1) : wv, kats = AB ? NW / 2 : NW;
+  int kseq = 0;  // KB publications expected so far
   for (int ib = b0; ib < b1; ++ib) {
     const size_t ikb = (size_t)ik * a.NB + ib;
     const double occ = a.occ[ikb];
@@ -316,7 +364,7 @@ __global__ void __launch_bounds__(NY * NZ * XS, WPE) k_fused_step(FusedArgs<T> a
       win[H + i] = v;
       bufA[p] = v;
     }
-    __syncthreads();
+    lds_barrier();
     cplx<T>* in = bufA;
     cplx<T>* out = bufB;
 #pragma unroll 1
@@ -327,27 +375,61 @@ __global__ void __launch_bounds__(NY * NZ * XS, WPE) k_fused_step(FusedArgs<T> a
         if (!a.prop) break;
       }
       const bool obs = DEFER && n == 0 && a.obs;
-      if constexpr (XS > 1) {
-#pragma unroll
-        for (int i = 0; i < H; ++i) {
-          win[i] = in[((x0 - H + i + NX) % NX) * NYZ + yz];
-          win[H + PX + i] = in[((x0 + PX + i) % NX) * NYZ + yz];
-        }
-      }
       const int* jp_phase = opaque(a.jp_phase);
       const int* jp_grid = opaque(a.jp_grid);
       const T* uvt = opaque(a.uvp);
+      auto kb_phase = [&]() __attribute__((always_inline)) {
       // KB term of pseudo_pt (hpsi.f90:176-206) per atom in one wave:
       //   uVpsi(l) = Hxyz iuV(l) sum_j uV(j,l) e^{ik.r_j} psi(J_j)       (projection)
       //   dps(j)   = e^{-ik.r_j} sum_l uV(j,l) uVpsi(l)                  (back-projection)
-      // the wave sums leave uVpsi in every lane, so no barrier separates the two
-      for (int at = wv; at < a.NI; at += NW) {
-        const int j0 = a.atom_start[at], m = a.atom_start[at + 1] - j0;
+      // the wave sums leave uVpsi in every lane, so no barrier separates the
+      // two.  The first two passes of 64 sphere points (every point when the
+      // atom has at most 128) load together and keep their phase, orbital
+      // value and projector values in registers for the back-projection and
+      // the nonlocal observables; further points take a plain loop.
+      for (int at = kat0; at < a.NI; at += kats) {
+        const int j0 = katom[at], m = katom[at + 1] - j0;
+        if (m == 0) continue;
         const T* ua = uvt + (size_t)at * kLA * a.Nps;
+        T f[kLA];
+#pragma unroll
+        for (int l = 0; l < kLA; ++l) f[l] = kfac[at * kLA + l];
+        cplx<T> e2[2], v2[2];
+        T u2[2][kLA];
+        int jp2[2];
+        bool ok2[2];
+#pragma unroll
+        for (int s2 = 0; s2 < 2; ++s2) {
+          const int j = lane + s2 * kWave;
+          ok2[s2] = j < m;
+          const int jc = ok2[s2] ? j : 0;  // an unused lane reads the atom's first point
+          const int jp = j0 + jc;
+          jp2[s2] = jp;
+          if constexpr (LT) {
+            e2[s2] = em[jp];
+            v2[s2] = in[gix[jp]];
+          } else {
+            e2[s2] = phase_at<T, NX, NY, NZ>(phm, jp_phase[jp]);
+            v2[s2] = in[jp_grid[jp]];
+          }
+#pragma unroll
+          for (int l = 0; l < kLA; ++l) u2[s2][l] = LT ? ug[l * kFusedMaxNPI + jp] : ua[l * a.Nps + jc];
+        }
         T q[2 * kLA];
 #pragma unroll
         for (int l = 0; l < 2 * kLA; ++l) q[l] = 0;
-        for (int j = lane; j < m; j += kWave) {
+#pragma unroll
+        for (int s2 = 0; s2 < 2; ++s2) {
+          if (ok2[s2]) {
+            const cplx<T> w = cmul(e2[s2], v2[s2]);
+#pragma unroll
+            for (int l = 0; l < kLA; ++l) {
+              q[2 * l] += u2[s2][l] * w.re;
+              q[2 * l + 1] += u2[s2][l] * w.im;
+            }
+          }
+        }
+        for (int j = lane + 2 * kWave; j < m; j += kWave) {
           const int jp = j0 + j;
           cplx<T> w;
           if constexpr (LT) w = cmul(em[jp], in[gix[jp]]);
@@ -363,11 +445,21 @@ __global__ void __launch_bounds__(NY * NZ * XS, WPE) k_fused_step(FusedArgs<T> a
         T sr[kLA], si[kLA];
 #pragma unroll
         for (int l = 0; l < kLA; ++l) {
-          const T f = (T)(g.Hxyz * a.sgnp[at * kLA + l]);
-          sr[l] = lane_read(q[0], wave_sums_lane(2 * l)) * f;
-          si[l] = lane_read(q[0], wave_sums_lane(2 * l + 1)) * f;
+          sr[l] = lane_read(q[0], wave_sums_lane(2 * l)) * f[l];
+          si[l] = lane_read(q[0], wave_sums_lane(2 * l + 1)) * f[l];
         }
-        for (int j = lane; j < m; j += kWave) {
+#pragma unroll
+        for (int s2 = 0; s2 < 2; ++s2) {
+          T tr = 0, ti = 0;
+#pragma unroll
+          for (int l = 0; l < kLA; ++l) {
+            tr += u2[s2][l] * sr[l];
+            ti += u2[s2][l] * si[l];
+          }
+          const cplx<T> e = e2[s2];
+          if (ok2[s2]) dps[jp2[s2]] = cplx<T>{e.re * tr + e.im * ti, e.re * ti - e.im * tr};  // conj(e) * t
+        }
+        for (int j = lane + 2 * kWave; j < m; j += kWave) {
           const int jp = j0 + j;
           T tr = 0, ti = 0;
 #pragma unroll
@@ -379,95 +471,94 @@ __global__ void __launch_bounds__(NY * NZ * XS, WPE) k_fused_step(FusedArgs<T> a
           const cplx<T> e = LT ? em[jp] : phase_at<T, NX, NY, NZ>(phm, jp_phase[jp]);
           dps[jp] = cplx<T>{e.re * tr + e.im * ti, e.re * ti - e.im * tr};  // conj(e) * t
         }
-      }
-      if (DEFER && obs) {
-        // nonlocal current and energy of this orbital (current.f90:247-269,
-        // total_energy.f90:191-199) at k_obs: one wave per atom; per projector
-        // l the projection U = sum_j uV e psi and its first moments
-        // X_d = sum_j uV r_d e psi, all in FP64
-        const double* uv64 = opaque(a.uvp64);
-        const double* jp_pos = opaque(a.jp_pos);
-        // projectors per pass: all four in complex64; one at a time in FP64,
-        // where the x window and the Taylor sum hold more registers
-        constexpr int LB = sizeof(T) == 4 ? kLA : 1;
-        for (int at = wv; at < a.NI; at += NW) {
-          const int j0 = a.atom_start[at], m = a.atom_start[at + 1] - j0;
-          const double hx = g.Hxyz, f = occ / g.aLxyz * 2.0 * hx;
-#pragma unroll 1
-          for (int lb = 0; lb < kLA; lb += LB) {
-            const double* ua = uv64 + ((size_t)at * kLA + lb) * a.Nps;
-            double q[8 * LB];  // per l: Ur, Ui, Xr_x, Xi_x, Xr_y, Xi_y, Xr_z, Xi_z
+        if constexpr (DEFER && LT) {
+          if (obs) {
+            // nonlocal current and energy of this orbital (current.f90:247-269,
+            // total_energy.f90:191-199) at k_obs: per projector l the
+            // projection U = sum_j uV e psi and its first moments
+            // X_d = sum_j uV r_d e psi, products and sums in FP64; uV and the
+            // orbital values from the registers above, the position l H - j L
+            // from the grid point and its image offsets
+            const double hx = g.Hxyz, fo = occ / g.aLxyz * 2.0 * hx;
+            double qo[8 * kLA];  // per l: Ur, Ui, Xr_x, Xi_x, Xr_y, Xi_y, Xr_z, Xi_z
 #pragma unroll
-            for (int l = 0; l < 8 * LB; ++l) q[l] = 0;
-            for (int j = lane; j < m; j += kWave) {
-              const int jp = j0 + j;
-              cplx<T> w;
-              if constexpr (LT) w = cmul(en[jp], in[gix[jp]]);
-              else w = cmul(phase_at<T, NX, NY, NZ>(phn, jp_phase[jp]), in[jp_grid[jp]]);
-              const double px = jp_pos[3 * jp], py = jp_pos[3 * jp + 1], pz = jp_pos[3 * jp + 2];
+            for (int l = 0; l < 8 * kLA; ++l) qo[l] = 0;
+            auto moments = [&](int jp, const cplx<T>& v, const T (&u)[kLA]) __attribute__((always_inline)) {
+              const int pg = gix[jp], c = gimg[jp];
+              const cplx<T> w = cmul(en[jp], v);
+              const double px = (pg / NYZ) * g.Hx - (c % 5 - 2) * g.aLx;
+              const double py = ((pg / NZ) % NY) * g.Hy - ((c / 5) % 5 - 2) * g.aLy;
+              const double pz = (pg % NZ) * g.Hz - (c / 25 - 2) * g.aLz;
 #pragma unroll
-              for (int l = 0; l < LB; ++l) {
-                const double u = ua[l * a.Nps + j];
-                const double tr = u * (double)w.re, ti = u * (double)w.im;
-                q[8 * l + 0] += tr;
-                q[8 * l + 1] += ti;
-                q[8 * l + 2] += px * tr;
-                q[8 * l + 3] += px * ti;
-                q[8 * l + 4] += py * tr;
-                q[8 * l + 5] += py * ti;
-                q[8 * l + 6] += pz * tr;
-                q[8 * l + 7] += pz * ti;
+              for (int l = 0; l < kLA; ++l) {
+                const double uu = (double)u[l];
+                const double tr = uu * (double)w.re, ti = uu * (double)w.im;
+                qo[8 * l + 0] += tr;
+                qo[8 * l + 1] += ti;
+                qo[8 * l + 2] += px * tr;
+                qo[8 * l + 3] += px * ti;
+                qo[8 * l + 4] += py * tr;
+                qo[8 * l + 5] += py * ti;
+                qo[8 * l + 6] += pz * tr;
+                qo[8 * l + 7] += pz * ti;
               }
-            }
-            wave_sums(q);
+            };
 #pragma unroll
-            for (int l = 0; l < LB; ++l) {
+            for (int s2 = 0; s2 < 2; ++s2)
+              if (ok2[s2]) moments(jp2[s2], v2[s2], u2[s2]);
+            for (int j = lane + 2 * kWave; j < m; j += kWave) {
+              const int jp = j0 + j;
+              T u[kLA];
+#pragma unroll
+              for (int l = 0; l < kLA; ++l) u[l] = ug[l * kFusedMaxNPI + jp];
+              moments(jp, in[gix[jp]], u);
+            }
+            wave_sums(qo);
+#pragma unroll
+            for (int l = 0; l < kLA; ++l) {
               double r[8];
 #pragma unroll
-              for (int c = 0; c < 8; ++c) r[c] = lane_read(q[0], wave_sums_lane(8 * l + c));
+              for (int cc = 0; cc < 8; ++cc) r[cc] = lane_read(qo[0], wave_sums_lane(8 * l + cc));
               if (lane == 0) {
-                const double sg = a.sgnp[at * kLA + lb + l];
+                const double sg = ksgn[at * kLA + l];
                 const double Ur = r[0] * hx * sg, Ui = r[1] * hx * sg;
-                red[wv * 10 + 6] += f * (r[2] * Ui - r[3] * Ur);
-                red[wv * 10 + 7] += f * (r[4] * Ui - r[5] * Ur);
-                red[wv * 10 + 8] += f * (r[6] * Ui - r[7] * Ur);
+                red[wv * 10 + 6] += fo * (r[2] * Ui - r[3] * Ur);
+                red[wv * 10 + 7] += fo * (r[4] * Ui - r[5] * Ur);
+                red[wv * 10 + 8] += fo * (r[6] * Ui - r[7] * Ur);
                 red[wv * 10 + 9] += occ * (r[0] * r[0] + r[1] * r[1]) * hx * hx * sg;
               }
             }
           }
         }
       }
-      __syncthreads();  // KB terms complete
-      if constexpr (OV) {
-        // points inside several spheres (overlapping atoms): fold the other
-        // sphere points' terms into the first, in table order
-        for (int o = t; o < a.NOV; o += NT) {
-          const int q0 = a.ov_start[o], q1 = a.ov_start[o + 1];
-          cplx<T> s0 = dps[a.ov_jp[q0]];
-          for (int q = q0 + 1; q < q1; ++q) {
-            const cplx<T> d = dps[a.ov_jp[q]];
-            s0.re += d.re;
-            s0.im += d.im;
-          }
-          dps[a.ov_jp[q0]] = s0;
-        }
-        __syncthreads();
-      }
-      // stencil: h = (V+A) psi - 1/2 sum_a sum_n lap_n (psi+ + psi-) - i sum_a k_a sum_n nab_n (psi+ - psi-)
-      // One LDS base address per neighbour direction; the x offset of each own
-      // point is a compile-time constant that folds into the ds_read offset.
-      const cplx<T>* zb[8];
-      const cplx<T>* yb[8];
-      nb_bases<T, NY, NZ>(in, x0, iy, iz, zb, yb);
-      const T* vrow = opaque(a.vloc) + x0 * NYZ + yz;
+      };  // kb_phase
       const T zr = (n == 1) ? a.z2r : ((n == 3) ? a.z4r : (T)0);
       const T zi = (n == 0) ? a.z1i : ((n == 2) ? a.z3i : (T)0);
-      // Each result goes to LDS and into the Taylor sum at once; it replaces
-      // its input in the x window only when no later own point needs that
-      // input (a delay of 4 points; the first 4 of a periodic column wait to
-      // the end), which keeps few results live in registers.
-      auto sweep = [&](auto obs_c) {
+      // stencil: h = (V+A) psi - 1/2 sum_a sum_n lap_n (psi+ + psi-) - i sum_a k_a sum_n nab_n (psi+ - psi-)
+      // The stencil part of each result goes to LDS; the KB term and the
+      // Taylor sum follow after the barrier that completes the KB terms
+      // (finalize below).  The sweep loads its x window (own points and the
+      // halo of 4) and forms its LDS bases itself, so nothing of it stays
+      // live across the KB phase.
+      auto sweep = [&](auto obs_c) __attribute__((always_inline)) {
         constexpr bool OBS = decltype(obs_c)::value;
+        if constexpr (XS > 1) {
+          // SEQ keeps its own points' inputs in the window from the previous
+          // sweep (or the orbital's load) and reloads the halo only
+#pragma unroll
+          for (int i = 0; i < W; ++i)
+            if (AB || i < H || i >= H + PX) win[i] = in[((x0 - H + i + NX) % NX) * NYZ + yz];
+        } else {
+#pragma unroll
+          for (int i = 0; i < NX; ++i) win[i] = in[i * NYZ + yz];
+        }
+        // One LDS base address per neighbour direction; the x offset of each
+        // own point is a compile-time constant that folds into the ds_read offset.
+        const cplx<T>* zb[8];
+        const cplx<T>* yb[8];
+        nb_bases<T, NY, NZ>(in, x0, iy, iz, zb, yb);
+        const T* vrow = opaque(a.vloc) + x0 * NYZ + yz;
+        cplx<T> h[AB ? 1 : PX];
         // observables of the input orbital at k_obs, FP64 sums.  The
         // stencil's neighbour sums L = sum_a sum_n (-lap_n/2)(psi+ + psi-) and
         // G_a = sum_n nab_n (psi+ - psi-) give per point the kinetic term
@@ -481,7 +572,6 @@ __global__ void __launch_bounds__(NY * NZ * XS, WPE) k_fused_step(FusedArgs<T> a
         // precision, products and sums in FP64.
         double n2 = 0, vs = 0, te = 0, js[3] = {0, 0, 0};
         const double* v64 = opaque(a.vloc64) + x0 * NYZ + yz;
-        cplx<T> h[PX];
 #pragma unroll
         for (int i = 0; i < PX; ++i) {
           const cplx<T> e0 = win[H + i];
@@ -514,26 +604,35 @@ __global__ void __launch_bounds__(NY * NZ * XS, WPE) k_fused_step(FusedArgs<T> a
             lr = vAi * e0.re + Lr + (kmx * gi[0] + kmy * gi[1] + kmz * gi[2]);
             li = vAi * e0.im + Li - (kmx * gr[0] + kmy * gr[1] + kmz * gr[2]);
           }
-          // KB term of the sphere point covering this point (branch-free:
-          // uncovered points read the zero entry dps[NPI])
-          const cplx<T> d = dps[(prim[i >> 1] >> (16 * (i & 1))) & 0xffff];
-          h[i].re = lr + d.re;
-          h[i].im = li + d.im;
-          out[(x0 + i) * NYZ + yz] = h[i];
-          acc[i].re += zr * h[i].re - zi * h[i].im;
-          acc[i].im += zr * h[i].im + zi * h[i].re;
-          if constexpr (XS > 1) {
-            if (i >= 4) win[H + i - 4] = h[i - 4];
+          if constexpr (AB) {
+            out[(x0 + i) * NYZ + yz] = cplx<T>{lr, li};  // the KB term follows in finalize
           } else {
-            if (i >= 8 && i - 4 < NX - 4) win[i - 4] = h[i - 4];
+            // KB term of the sphere point covering this point (branch-free:
+            // uncovered points read the zero entry dps[NPI]); the result goes
+            // to LDS and into the Taylor sum, and replaces its input in the x
+            // window once no later own point needs that input (a delay of 4
+            // points; the first 4 of a periodic column wait to the end)
+            const cplx<T> d = dps[(prim[i >> 1] >> (16 * (i & 1))) & 0xffff];
+            h[i].re = lr + d.re;
+            h[i].im = li + d.im;
+            if (n < 3) out[(x0 + i) * NYZ + yz] = h[i];  // the fourth result only enters the Taylor sum
+            acc[i].re += zr * h[i].re - zi * h[i].im;
+            acc[i].im += zr * h[i].im + zi * h[i].re;
+            if constexpr (XS > 1) {
+              if (i >= 4) win[H + i - 4] = h[i - 4];
+            } else {
+              if (i >= 8 && i - 4 < NX - 4) win[i - 4] = h[i - 4];
+            }
           }
         }
+        if constexpr (!AB) {
 #pragma unroll
-        for (int i = 0; i < PX; ++i) {
-          if constexpr (XS > 1) {
-            if (i >= PX - 4) win[H + i] = h[i];
-          } else {
-            if (i < 4 || i >= NX - 4) win[i] = h[i];
+          for (int i = 0; i < PX; ++i) {
+            if constexpr (XS > 1) {
+              if (i >= PX - 4) win[H + i] = h[i];
+            } else {
+              if (i < 4 || i >= NX - 4) win[i] = h[i];
+            }
           }
         }
         if constexpr (OBS) {
@@ -552,13 +651,73 @@ __global__ void __launch_bounds__(NY * NZ * XS, WPE) k_fused_step(FusedArgs<T> a
           }
         }
       };
-      if constexpr (DEFER) {
-        if (obs) sweep(BoolC<true>{});
-        else sweep(BoolC<false>{});
+      auto stencil_phase = [&]() __attribute__((always_inline)) {
+        if constexpr (DEFER) {
+          if (obs) sweep(BoolC<true>{});
+          else sweep(BoolC<false>{});
+        } else {
+          sweep(BoolC<false>{});
+        }
+      };
+      if constexpr (AB) {
+        if (kbw) {
+          kb_phase();
+          // this wave's KB terms of this application are in LDS
+          if (lane == 0) __hip_atomic_fetch_add(kbcnt, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
+        stencil_phase();
+        // wait for every KB wave's terms; the wait is bounded so that a wave
+        // always reaches the barrier below (a wait that runs out poisons the
+        // workgroup's sums instead of hanging)
+        kseq += NW / 2;
+        for (int spins = 0;; ++spins) {
+          const int v = __builtin_amdgcn_readfirstlane(
+              __hip_atomic_load(kbcnt, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP));
+          if (v >= kseq) break;
+          if (spins >= kFusedSpinMax) {
+            if (lane == 0) *kbfail = 1;
+            break;
+          }
+          __builtin_amdgcn_s_sleep(1);
+        }
       } else {
-        sweep(BoolC<false>{});
+        kb_phase();
+        lds_barrier();  // KB terms complete
       }
-      __syncthreads();
+      if constexpr (OV) {
+        // points inside several spheres (overlapping atoms): fold the other
+        // sphere points' terms into the first, in table order
+        for (int o = t; o < a.NOV; o += NT) {
+          const int q0 = a.ov_start[o], q1 = a.ov_start[o + 1];
+          cplx<T> s0 = dps[a.ov_jp[q0]];
+          for (int q = q0 + 1; q < q1; ++q) {
+            const cplx<T> d = dps[a.ov_jp[q]];
+            s0.re += d.re;
+            s0.im += d.im;
+          }
+          dps[a.ov_jp[q0]] = s0;
+        }
+        lds_barrier();
+      }
+      if constexpr (AB) {
+        // finalize: result = stencil part + KB term of the sphere point
+        // covering the point (branch-free: uncovered points read the zero
+        // entry dps[NPI]); into LDS and the Taylor sum
+#pragma unroll
+        for (int i = 0; i < PX; ++i) {
+          const int p = (x0 + i) * NYZ + yz;
+          cplx<T> h = out[p];
+          const cplx<T> d = dps[(prim[i >> 1] >> (16 * (i & 1))) & 0xffff];
+          h.re += d.re;
+          h.im += d.im;
+          if (n < 3) out[p] = h;  // the fourth result only enters the Taylor sum
+          acc[i].re += zr * h.re - zi * h.im;
+          acc[i].im += zr * h.im + zi * h.re;
+        }
+      } else {
+        stencil_phase();
+      }
+      lds_barrier();
       cplx<T>* tmp = in;
       in = out;
       out = tmp;
@@ -596,7 +755,7 @@ __global__ void __launch_bounds__(NY * NZ * XS, WPE) k_fused_step(FusedArgs<T> a
     // its observables unless the launch is the first half of an ETRS step
     // (a uniform branch: the barriers inside are taken by all threads or none)
     if (a.obs) {
-    __syncthreads();
+    lds_barrier();
     if constexpr (XS > 1) {
 #pragma unroll
       for (int i = 0; i < H; ++i) {
@@ -659,7 +818,7 @@ __global__ void __launch_bounds__(NY * NZ * XS, WPE) k_fused_step(FusedArgs<T> a
     // one wave per atom; per projector l the projection U = sum_j uV e psi and
     // its first moments X_d = sum_j uV r_d e psi, all in FP64
     for (int at = wv; at < a.NI; at += NW) {
-      const int j0 = a.atom_start[at], m = a.atom_start[at + 1] - j0;
+      const int j0 = katom[at], m = katom[at + 1] - j0;
       const double* ua = uv64 + (size_t)at * kLA * a.Nps;
       double q[8 * kLA];  // per l: Ur, Ui, Xr_x, Xi_x, Xr_y, Xi_y, Xr_z, Xi_z
 #pragma unroll
@@ -691,7 +850,7 @@ __global__ void __launch_bounds__(NY * NZ * XS, WPE) k_fused_step(FusedArgs<T> a
 #pragma unroll
         for (int c = 0; c < 8; ++c) r[c] = lane_read(q[0], wave_sums_lane(8 * l + c));
         if (lane == 0) {
-          const double sg = a.sgnp[at * kLA + l];
+          const double sg = ksgn[at * kLA + l];
           const double Ur = r[0] * hx * sg, Ui = r[1] * hx * sg;
           red[wv * 10 + 6] += f * (r[2] * Ui - r[3] * Ur);
           red[wv * 10 + 7] += f * (r[4] * Ui - r[5] * Ur);
@@ -728,6 +887,9 @@ __global__ void __launch_bounds__(NY * NZ * XS, WPE) k_fused_step(FusedArgs<T> a
     r[5] = s[1] * g.Hxyz;
     r[6] = s[0] * g.Hxyz;
     r[7] = 0.0;
+    if (*kbfail) {
+      for (int i = 0; i < 8; ++i) r[i] = __builtin_nan("");
+    }
   }
 }
 
@@ -815,7 +977,8 @@ static int fused_set_tables(salmon_ctx* c) {
   fused_free_tables(f);
   const salmon::KBTables& kb = c->kb;
   const salmon::Grid& g = c->g;
-  if (kb.NPI + 1 > salmon::kFusedMaxNPI) return SALMON_OK;  // the streamed path serves this case
+  if (kb.NPI + 1 > salmon::kFusedMaxNPI || kb.NI > salmon::kFusedMaxAtoms)
+    return SALMON_OK;  // the streamed path serves this case
   std::vector<int> start(kb.NI + 1), pidx(kb.NPI), lma_atom(kb.Nlma);
   std::vector<double> pos(3 * (size_t)kb.NPI), uv((size_t)kb.Nlma * kb.Nps), sign(kb.Nlma);
   if (hipMemcpy(start.data(), kb.atom_start, sizeof(int) * start.size(), hipMemcpyDeviceToHost) != hipSuccess ||
@@ -901,7 +1064,9 @@ static size_t fused_lds_bytes() {
   constexpr size_t MT = LT ? salmon::kFusedMaxNPI : 0;
   return sizeof(salmon::cplx<T>) * (2 * NX * NY * NZ + 2 * 5 * (NX + NY + NZ) + salmon::kFusedMaxNPI + 2 * MT) +
          sizeof(T) * salmon::kLA * MT + sizeof(int) * MT + sizeof(double) * (LT ? NX * NY * NZ : 0) +
-         sizeof(double) * 10 * ((NY * NZ * XS + 63) / 64);
+         sizeof(double) * 10 * ((NY * NZ * XS + 63) / 64) +
+         (sizeof(T) + sizeof(float)) * salmon::kFusedMaxAtoms * salmon::kLA + sizeof(int) * (salmon::kFusedMaxAtoms + 3) +
+         MT;  // gimg, one byte per sphere point
 }
 
 template <typename T, int NX, int NY, int NZ, int XS, int WPE, bool OV, bool RP>
