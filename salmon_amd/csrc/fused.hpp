@@ -351,6 +351,14 @@ __global__ void __launch_bounds__(NY * NZ * XS, WPE) k_fused_step(FusedArgs<T> a
   const bool kbw = !AB || order == 0;
   const int kat0 = AB ? (wvu >> 1) : wv, kats = AB ? NW / 2 : NW;
   int kseq = 0;  // KB publications expected so far
+  // complex64 (SEQ): the potential of the thread's own points stays in
+  // registers for the whole launch, so the sweeps issue no global loads
+  constexpr bool VREG = sizeof(T) == 4 && !AB;
+  T vf[VREG ? PX : 1];
+  if constexpr (VREG) {
+#pragma unroll
+    for (int i = 0; i < PX; ++i) vf[i] = a.vloc[(x0 + i) * NYZ + yz];
+  }
   for (int ib = b0; ib < b1; ++ib) {
     const size_t ikb = (size_t)ik * a.NB + ib;
     const double occ = a.occ[ikb];
@@ -480,52 +488,57 @@ __global__ void __launch_bounds__(NY * NZ * XS, WPE) k_fused_step(FusedArgs<T> a
             // orbital values from the registers above, the position l H - j L
             // from the grid point and its image offsets
             const double hx = g.Hxyz, fo = occ / g.aLxyz * 2.0 * hx;
-            double qo[8 * kLA];  // per l: Ur, Ui, Xr_x, Xi_x, Xr_y, Xi_y, Xr_z, Xi_z
+            // two projectors per pass, which halves the live FP64 sums
+            constexpr int LH = kLA / 2;
 #pragma unroll
-            for (int l = 0; l < 8 * kLA; ++l) qo[l] = 0;
-            auto moments = [&](int jp, const cplx<T>& v, const T (&u)[kLA]) __attribute__((always_inline)) {
-              const int pg = gix[jp], c = gimg[jp];
-              const cplx<T> w = cmul(en[jp], v);
-              const double px = (pg / NYZ) * g.Hx - (c % 5 - 2) * g.aLx;
-              const double py = ((pg / NZ) % NY) * g.Hy - ((c / 5) % 5 - 2) * g.aLy;
-              const double pz = (pg % NZ) * g.Hz - (c / 25 - 2) * g.aLz;
+            for (int lb = 0; lb < kLA; lb += LH) {
+              double qo[8 * LH];  // per l: Ur, Ui, Xr_x, Xi_x, Xr_y, Xi_y, Xr_z, Xi_z
 #pragma unroll
-              for (int l = 0; l < kLA; ++l) {
-                const double uu = (double)u[l];
-                const double tr = uu * (double)w.re, ti = uu * (double)w.im;
-                qo[8 * l + 0] += tr;
-                qo[8 * l + 1] += ti;
-                qo[8 * l + 2] += px * tr;
-                qo[8 * l + 3] += px * ti;
-                qo[8 * l + 4] += py * tr;
-                qo[8 * l + 5] += py * ti;
-                qo[8 * l + 6] += pz * tr;
-                qo[8 * l + 7] += pz * ti;
+              for (int l = 0; l < 8 * LH; ++l) qo[l] = 0;
+              auto moments = [&](int jp, const cplx<T>& v, const T (&u)[kLA]) __attribute__((always_inline)) {
+                const int pg = gix[jp], c = gimg[jp];
+                const cplx<T> w = cmul(en[jp], v);
+                const double px = (pg / NYZ) * g.Hx - (c % 5 - 2) * g.aLx;
+                const double py = ((pg / NZ) % NY) * g.Hy - ((c / 5) % 5 - 2) * g.aLy;
+                const double pz = (pg % NZ) * g.Hz - (c / 25 - 2) * g.aLz;
+#pragma unroll
+                for (int l = 0; l < LH; ++l) {
+                  const double uu = (double)u[lb + l];
+                  const double tr = uu * (double)w.re, ti = uu * (double)w.im;
+                  qo[8 * l + 0] += tr;
+                  qo[8 * l + 1] += ti;
+                  qo[8 * l + 2] += px * tr;
+                  qo[8 * l + 3] += px * ti;
+                  qo[8 * l + 4] += py * tr;
+                  qo[8 * l + 5] += py * ti;
+                  qo[8 * l + 6] += pz * tr;
+                  qo[8 * l + 7] += pz * ti;
+                }
+              };
+#pragma unroll
+              for (int s2 = 0; s2 < 2; ++s2)
+                if (ok2[s2]) moments(jp2[s2], v2[s2], u2[s2]);
+              for (int j = lane + 2 * kWave; j < m; j += kWave) {
+                const int jp = j0 + j;
+                T u[kLA];
+#pragma unroll
+                for (int l = 0; l < kLA; ++l) u[l] = ug[l * kFusedMaxNPI + jp];
+                moments(jp, in[gix[jp]], u);
               }
-            };
+              wave_sums(qo);
 #pragma unroll
-            for (int s2 = 0; s2 < 2; ++s2)
-              if (ok2[s2]) moments(jp2[s2], v2[s2], u2[s2]);
-            for (int j = lane + 2 * kWave; j < m; j += kWave) {
-              const int jp = j0 + j;
-              T u[kLA];
+              for (int l = 0; l < LH; ++l) {
+                double r[8];
 #pragma unroll
-              for (int l = 0; l < kLA; ++l) u[l] = ug[l * kFusedMaxNPI + jp];
-              moments(jp, in[gix[jp]], u);
-            }
-            wave_sums(qo);
-#pragma unroll
-            for (int l = 0; l < kLA; ++l) {
-              double r[8];
-#pragma unroll
-              for (int cc = 0; cc < 8; ++cc) r[cc] = lane_read(qo[0], wave_sums_lane(8 * l + cc));
-              if (lane == 0) {
-                const double sg = ksgn[at * kLA + l];
-                const double Ur = r[0] * hx * sg, Ui = r[1] * hx * sg;
-                red[wv * 10 + 6] += fo * (r[2] * Ui - r[3] * Ur);
-                red[wv * 10 + 7] += fo * (r[4] * Ui - r[5] * Ur);
-                red[wv * 10 + 8] += fo * (r[6] * Ui - r[7] * Ur);
-                red[wv * 10 + 9] += occ * (r[0] * r[0] + r[1] * r[1]) * hx * hx * sg;
+                for (int cc = 0; cc < 8; ++cc) r[cc] = lane_read(qo[0], wave_sums_lane(8 * l + cc));
+                if (lane == 0) {
+                  const double sg = ksgn[at * kLA + lb + l];
+                  const double Ur = r[0] * hx * sg, Ui = r[1] * hx * sg;
+                  red[wv * 10 + 6] += fo * (r[2] * Ui - r[3] * Ur);
+                  red[wv * 10 + 7] += fo * (r[4] * Ui - r[5] * Ur);
+                  red[wv * 10 + 8] += fo * (r[6] * Ui - r[7] * Ur);
+                  red[wv * 10 + 9] += occ * (r[0] * r[0] + r[1] * r[1]) * hx * hx * sg;
+                }
               }
             }
           }
@@ -575,7 +588,9 @@ __global__ void __launch_bounds__(NY * NZ * XS, WPE) k_fused_step(FusedArgs<T> a
 #pragma unroll
         for (int i = 0; i < PX; ++i) {
           const cplx<T> e0 = win[H + i];
-          const T vAi = vrow[i * NYZ] + A;
+          T vAi;
+          if constexpr (VREG) vAi = vf[i] + A;
+          else vAi = vrow[i * NYZ] + A;
           T lr, li;
           if constexpr (OBS) {
             T Lr = 0, Li = 0;
@@ -584,7 +599,9 @@ __global__ void __launch_bounds__(NY * NZ * XS, WPE) k_fused_step(FusedArgs<T> a
             const double er = (double)e0.re, ei = (double)e0.im;
             const double a2 = er * er + ei * ei;
             n2 += a2;
-            vs += v64[i * NYZ] * a2;
+            // complex64: the FP32 potential the propagation uses, promoted
+            if constexpr (VREG) vs += (double)vf[i] * a2;
+            else vs += v64[i * NYZ] * a2;
             if constexpr (!RP) rho_acc[(x0 + i) * NYZ + yz] += occ * a2;
             te += er * (double)Lr + ei * (double)Li;
 #pragma unroll
