@@ -1396,12 +1396,13 @@ int salmon_gpu_rt_run(salmon_ctx* c, const double* ac, int nsteps, double dt, in
   if (r) return r;
   if (!ac || nsteps < 0) return fail(c, SALMON_ERR_ARG, "ac is NULL or nsteps < 0");
   if (nsteps == 0) return SALMON_OK;
-  // the per-step results stay allocated between runs (no allocation or free
-  // inside a run of the usual length)
+  // the per-step results stay allocated between runs, sized for at least
+  // 4096 steps (256 KiB), so runs of the usual length allocate nothing
   if ((size_t)nsteps > c->res_cap) {
+    const size_t cap = std::max<size_t>((size_t)nsteps, std::max<size_t>(4096, 2 * c->res_cap));
     (void)hipStreamSynchronize(c->stream);
-    if ((r = dalloc(c, &c->d_res, 8 * (size_t)nsteps))) return r;
-    c->res_cap = (size_t)nsteps;
+    if ((r = dalloc(c, &c->d_res, 8 * cap))) return r;
+    c->res_cap = cap;
   }
   double* d_res = c->d_res;
   if (c->scf) {
