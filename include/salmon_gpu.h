@@ -239,14 +239,26 @@ int salmon_gpu_rt_run(salmon_ctx *ctx, const double *ac, int nsteps, double dt, 
  * step replayed from a captured HIP graph.  Contexts are referenced, not
  * copied: their resident orbitals advance; they must not be used elsewhere
  * while the batch runs and must outlive it.  All contexts share the batch's
- * precision, A(t) and dt.  salmon_gpu_batch_tune times the stencil's z
- * blocking (1, 2, 4 or 8 points per thread along z) per grid shape and keeps
- * the fastest (rt_run tunes on first use); salmon_gpu_batch_zblock reports
- * the choice.  Outputs: jt(nsys, nsteps, 3), et(nsys, nsteps) =
- * Ekin+Enl+Eloc, norm(nsys, nsteps), row-major, each nullable. */
+ * precision, A(t) and dt.
+ * Two step kernels: SALMON_BATCH_SWEEP, one persistent launch per step whose
+ * workgroups take (system, orbitals) items from a device work counter and
+ * run each orbital's whole step (grids of at least 8 points per axis whose
+ * x plane fits a 512-thread workgroup's columns), and SALMON_BATCH_PHASES,
+ * one launch per phase over all systems (any grid).  SALMON_BATCH_AUTO (the
+ * default) takes the sweep when every system fits it; salmon_gpu_batch_kernel
+ * reports the kernel in use after tune (-1 before).  For the phase kernel
+ * salmon_gpu_batch_tune times the stencil's z blocking (1, 2, 4 or 8 points
+ * per thread along z) per grid shape and keeps the fastest (rt_run tunes on
+ * first use); salmon_gpu_batch_zblock reports the choice.  Outputs:
+ * jt(nsys, nsteps, 3), et(nsys, nsteps) = Ekin+Enl+Eloc, norm(nsys, nsteps),
+ * row-major, each nullable. */
 typedef struct salmon_batch salmon_batch;
+enum { SALMON_BATCH_AUTO = 0, SALMON_BATCH_PHASES = 1, SALMON_BATCH_SWEEP = 2 };
 int salmon_gpu_batch_create(int precision, salmon_batch **batch);
 int salmon_gpu_batch_add(salmon_batch *batch, salmon_ctx *ctx);
+/* SALMON_ERR_ARG from tune/rt_run if SALMON_BATCH_SWEEP is asked for and a system does not fit it */
+int salmon_gpu_batch_set_kernel(salmon_batch *batch, int kernel);
+int salmon_gpu_batch_kernel(const salmon_batch *batch);
 int salmon_gpu_batch_tune(salmon_batch *batch);
 int salmon_gpu_batch_zblock(salmon_batch *batch, int sys);
 int salmon_gpu_batch_rt_run(salmon_batch *batch, const double *ac, int nsteps, double dt, double *jt, double *et,

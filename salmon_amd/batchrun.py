@@ -1,11 +1,12 @@
 """The 64-system batched workload: independent seeded systems with grids from
 {12^3, 16^3, 20x20x28, 24^3, 32^3}, bands U{8..64}, k points U{1..64}, atoms
 U{1..16} (synth.config4_system), propagated together with the batched step
-(one launch per phase over all systems, the step replayed from a captured HIP
-graph, z blocking tuned per grid shape), FP64 and complex64.  The same systems
-propagated one context after another give the unbatched rate beside it.
+(the persistent sweep kernel, one launch per step, or with --kernel phases one
+launch per phase over all systems with z blocking tuned per grid shape; the
+step replayed from a captured HIP graph), FP64 and complex64.  The same
+systems propagated one context after another give the unbatched rate beside it.
 
-  python -m salmon_amd.batchrun --steps 10 [--scale 1.0] [--out result.json]
+  python -m salmon_amd.batchrun --steps 10 [--scale 1.0] [--kernel auto|sweep|phases] [--out result.json]
 """
 from __future__ import annotations
 
@@ -26,7 +27,10 @@ def _log(msg):
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 
 
-def run(nsys=64, steps=10, scale=1.0, precisions=(gpu.FP64, gpu.MIXED), sequential=True, log=_log):
+KERNELS = {"auto": gpu.BATCH_AUTO, "sweep": gpu.BATCH_SWEEP, "phases": gpu.BATCH_PHASES}
+
+
+def run(nsys=64, steps=10, scale=1.0, precisions=(gpu.FP64, gpu.MIXED), sequential=True, log=_log, kernel="auto"):
     t0 = time.time()
     systems = [synth.config4_system(i, scale=scale) for i in range(nsys)]
     orbs = sum(s.NK * s.NB for s in systems)
@@ -39,14 +43,16 @@ def run(nsys=64, steps=10, scale=1.0, precisions=(gpu.FP64, gpu.MIXED), sequenti
         name = "fp64" if prec == gpu.FP64 else "mixed"
         ctxs = [gpu.Context(s, prec) for s in systems]
         with gpu.Batch(ctxs, prec) as b:
+            b.set_kernel(KERNELS[kernel])
             tt = time.time()
             zb = b.tune()
             tune_s = time.time() - tt
+            kern = "sweep" if b.kernel() == gpu.BATCH_SWEEP else "phases"
             b.rt_run(ac[:2], 1, 0.08)  # warm-up step (captures the graph)
             tt = time.time()
             r = b.rt_run(ac[1:], steps, 0.08)
             dt = time.time() - tt
-        res = dict(ms_per_step=dt / steps * 1e3, orbital_steps_per_s=orbs * steps / dt,
+        res = dict(kernel=kern, ms_per_step=dt / steps * 1e3, orbital_steps_per_s=orbs * steps / dt,
                    point_orbital_steps_per_s=pts * steps / dt, tune_s=tune_s,
                    zblock={f"{s.NLx}x{s.NLy}x{s.NLz}": int(z) for s, z in zip(systems, zb)},
                    finite=bool(np.all(np.isfinite(r["J"])) and np.all(np.isfinite(r["E"]))))
@@ -56,8 +62,9 @@ def run(nsys=64, steps=10, scale=1.0, precisions=(gpu.FP64, gpu.MIXED), sequenti
         res["algorithmic_bytes_per_step"] = 12 * S * pts
         res["roofline_gbs"] = res["algorithmic_bytes_per_step"] / (res["ms_per_step"] * 1e-3) / 1e9
         res["roofline_frac"] = res["roofline_gbs"] / HBM_PEAK_GBS
+        zdesc = f"z blocking {res['zblock']}" if kern == "phases" else "persistent sweep"
         log(f"[batch] {name}: {res['ms_per_step']:.2f} ms/step, {res['orbital_steps_per_s']:.3e} orbital-steps/s "
-            f"(tune {tune_s:.1f} s, z blocking {res['zblock']}); {res['roofline_gbs']:.0f} GB/s of algorithmic "
+            f"(tune {tune_s:.1f} s, {zdesc}); {res['roofline_gbs']:.0f} GB/s of algorithmic "
             f"bytes = {res['roofline_frac']:.3f} of {HBM_PEAK_GBS:.0f} GB/s")
         if sequential:
             # the same kernels one context at a time (each context frees its
@@ -84,9 +91,10 @@ def main(argv=None):
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--scale", type=float, default=1.0)
     ap.add_argument("--no-sequential", action="store_true")
+    ap.add_argument("--kernel", choices=sorted(KERNELS), default="auto")
     ap.add_argument("--out", default=None)
     a = ap.parse_args(argv)
-    res = run(a.systems, a.steps, a.scale, sequential=not a.no_sequential)
+    res = run(a.systems, a.steps, a.scale, sequential=not a.no_sequential, kernel=a.kernel)
     _log(json.dumps(res))
     if a.out:
         with open(a.out, "w") as f:

@@ -1,12 +1,15 @@
 // Batched real-time step over independent systems of different grids, band
-// counts, k points and atoms.  Each phase of the step (KB projection and
-// stencil for each of the four Taylor terms, the Taylor fold, the
-// observables, their per-system sums) is one launch covering every system:
-// blocks are laid out system after system and a block finds its system in a
-// per-phase block -> system table (one load).  One step is captured
-// in a HIP graph and replayed; A(t) comes from a device table indexed by a
-// device step counter.  The stencil's z blocking (points per thread along z)
-// is chosen per grid shape by timing the candidates on the system's own data.
+// counts, k points and atoms: the system descriptors, the KB phases and the
+// one-launch-per-phase kernels.  In that kernel set each phase of the step
+// (KB projection and stencil for each of the four Taylor terms, the Taylor
+// fold, the observables, their per-system sums) is one launch covering every
+// system: blocks are laid out system after system and a block finds its
+// system in a per-phase block -> system table (one load).  The stencil's z
+// blocking (points per thread along z) is chosen per grid shape by timing the
+// candidates on the system's own data.  The persistent one-launch-per-step
+// kernel is in batch_sweep.hpp.  Either way one step is captured in a HIP
+// graph and replayed; A(t) comes from a device table indexed by a device step
+// counter.
 #pragma once
 #include "streamed.hpp"
 
@@ -23,6 +26,7 @@ struct Sys {
   cplx<T>* h[4];
   cplx<T>* uv;  // (NKB, Nlma)
   cplx<T>* ph;  // (NK, NPI) KB phases at this step's midpoint k
+  cplx<T>* phn; // (NK, NPI) KB phases at k + A(t+dt), for the observables
   cplx<T>* kbv; // (NKB, NT) KB term of every touched point of the current H application
   const T* vloc;
   const double* vloc64;
@@ -30,6 +34,10 @@ struct Sys {
   const double* occ;
   double* rows;  // (NKB, 8)
   int NK, NB, NKB, zb;
+  int sw_zb, sw_tw;          // persistent sweep: column length along z, waves per team
+  const int* kb_order;       // persistent sweep: atoms grouped by colour (no shared grid point within a colour)
+  const int* kb_cstart;      // (ncolor+1) first entry of each colour in kb_order
+  int ncolor;
   int pblk, hblk, kblk;      // projection, stencil and KB-apply blocks per orbital
   long first[kPhases + 1];   // first block of the system in each phase
 };
@@ -78,7 +86,7 @@ __global__ void __launch_bounds__(256) k_b_project(const Sys<T>* __restrict__ S,
 
 // The KB phases of every system's k points (one grid row per system) at this
 // step's midpoint k, shared by the step's four projections and stencils, or
-// (at_new) at k + A(t+dt) for the observables.
+// (at_new, into phn) at k + A(t+dt) for the observables.
 template <typename T>
 __global__ void __launch_bounds__(256) k_b_phase(const Sys<T>* __restrict__ S, Step st, int at_new) {
   const Sys<T>& y = S[blockIdx.y];
@@ -92,7 +100,7 @@ __global__ void __launch_bounds__(256) k_b_phase(const Sys<T>* __restrict__ S, S
   orbital_k(y.kac0, y.NK, ik, a[0], a[1], a[2], kx, ky, kz);
   double s, c;
   sincos(kx * y.kb.pos[3 * jp] + ky * y.kb.pos[3 * jp + 1] + kz * y.kb.pos[3 * jp + 2], &s, &c);
-  y.ph[i] = cplx<T>{(T)c, (T)s};
+  (at_new ? y.phn : y.ph)[i] = cplx<T>{(T)c, (T)s};
 }
 
 // KB term of every touched grid point of every orbital (pseudo_pt phase 2,
@@ -289,7 +297,7 @@ __global__ void __launch_bounds__(256) k_b_observe(const Sys<T>* __restrict__ S,
   double kx, ky, kz;
   orbital_k(y.kac0, y.NK, o / y.NB, an[0], an[1], an[2], kx, ky, kz);
   observe_orbital<T>(y.zu + (size_t)o * y.g.NL, kx, ky, kz, y.occ[o], y.vloc64, y.g, y.kb, y.rows + (size_t)o * 8,
-                     y.ph + (size_t)(o / y.NB) * y.kb.NPI);
+                     y.phn + (size_t)(o / y.NB) * y.kb.NPI);
 }
 
 // Per-system sums of the rows (one block per system) into res(step, s, 0..7).

@@ -5,7 +5,9 @@
 #pragma once
 #include <array>
 
-#include "batch.hpp"
+#include <algorithm>
+
+#include "batch_sweep.hpp"
 
 using namespace salmon::batch;
 
@@ -27,6 +29,14 @@ struct salmon_batch {
   size_t cap_steps = 0;
   size_t cap_ac = 0;              // doubles in d_ac
   long max_ph = 0;                // largest NK * NPI over the systems
+  // persistent sweep (one launch per step) or one launch per phase
+  int kernel_req = SALMON_BATCH_AUTO;
+  bool sweep = false;
+  int2* d_items = nullptr;        // (system, first orbital), costliest first (in bufs)
+  int nitems = 0;
+  int* d_next = nullptr;          // work counter of the sweep launch
+  unsigned sweep_grid = 0;
+  size_t sweep_lds = 0;
   hipGraphExec_t exec = nullptr;
   double graph_dt = 0.0;
   long graph_stride = 0;          // the A(t) table stride the graph was captured with
@@ -55,7 +65,141 @@ static void batch_release(salmon_batch* b) {
   b->bufs.clear();
   b->d_sys = nullptr;
   for (int ph = 0; ph < kPhases; ++ph) b->d_owner[ph] = nullptr;
+  b->d_items = nullptr;
+  b->nitems = 0;
+  b->sweep = false;
   b->ready = false;
+}
+
+// Column length (points along z per thread) and team size (waves) of the
+// persistent sweep for one grid: the candidate that leaves the fewest team
+// threads without a column, the longer column on a tie; {0, 0} if no
+// candidate fits a workgroup or the grid is too small for single periodic wraps.
+template <typename T>
+static std::array<int, 2> sweep_shape(const Grid& g) {
+  constexpr int zbmax = sizeof(T) == 4 ? 4 : 2;
+  std::array<int, 2> best{0, 0};
+  if (g.NLx < 8 || g.NLy < 8 || g.NLz < 8) return best;
+  const int P = g.NLy * g.NLz;
+  double bu = -1.0;
+  for (int zb = 1; zb <= zbmax; ++zb) {
+    if (g.NLz % zb) continue;
+    const int ncol = P / zb;
+    int tw = 1;
+    while (tw < kSweepWaves && tw * kWave < ncol) tw *= 2;
+    if (tw * kWave < ncol) continue;
+    const double u = (double)ncol / (tw * kWave);
+    if (u > bu + 1e-12 || (u > bu - 1e-12 && zb > best[0])) {
+      bu = u;
+      best = {zb, tw};
+    }
+  }
+  return best;
+}
+
+// The persistent sweep's work list, LDS size and grid; b->sweep stays false
+// (one launch per phase) if the phase kernel is asked for or, under
+// SALMON_BATCH_AUTO, a system does not fit the sweep.
+template <typename T>
+static int sweep_plan(salmon_batch* b, std::vector<Sys<T>>& S) {
+  b->sweep = false;
+  if (b->kernel_req == SALMON_BATCH_PHASES) return SALMON_OK;
+  const bool must = b->kernel_req == SALMON_BATCH_SWEEP;
+  std::vector<std::array<long, 3>> items;  // (cost, system, first orbital)
+  size_t lds = 0;
+  for (size_t i = 0; i < S.size(); ++i) {
+    S[i].kb_order = S[i].kb_cstart = nullptr;
+    S[i].ncolor = 0;
+  }
+  for (size_t i = 0; i < S.size(); ++i) {
+    const auto sh = sweep_shape<T>(S[i].g);
+    if (!sh[0]) {
+      if (must) return bfail(b, SALMON_ERR_ARG, "system %zu (%dx%dx%d) does not fit the persistent sweep", i,
+                             S[i].g.NLx, S[i].g.NLy, S[i].g.NLz);
+      return SALMON_OK;
+    }
+    S[i].sw_zb = sh[0];
+    S[i].sw_tw = sh[1];
+    const int teams = kSweepWaves / sh[1];
+    const int zb = sh[0], cs = zb % 2 == 0 ? zb + 1 : zb;  // col_stride<zb>()
+    const size_t PS = (size_t)S[i].g.NLy * S[i].g.NLz / zb * cs;
+    lds = std::max(lds, (size_t)kSweepHead + (size_t)teams * (2 * PS + S[i].kb.Nlma) * sizeof(cplx<T>));
+    for (int o = 0; o < S[i].NKB; o += teams)
+      items.push_back({(long)S[i].g.NL * std::min(teams, S[i].NKB - o), (long)i, (long)o});
+  }
+  // colours of the atoms of every system: greedy, an atom takes the first
+  // colour none of whose atoms shares a grid point with it
+  for (size_t i = 0; i < S.size(); ++i) {
+    const KBTables& kb = S[i].kb;
+    if (kb.NI <= 0 || kb.Nlma <= 0) continue;
+    std::vector<int> ast(kb.NI + 1), pidx(std::max(1, kb.NPI));
+    BHIP(b, hipMemcpy(ast.data(), kb.atom_start, sizeof(int) * (kb.NI + 1), hipMemcpyDeviceToHost));
+    if (kb.NPI > 0) BHIP(b, hipMemcpy(pidx.data(), kb.pidx, sizeof(int) * kb.NPI, hipMemcpyDeviceToHost));
+    std::vector<std::vector<char>> used;
+    std::vector<int> colour(kb.NI);
+    for (int a = 0; a < kb.NI; ++a) {
+      int c = 0;
+      for (;; ++c) {
+        if (c == (int)used.size()) used.emplace_back(S[i].g.NL, 0);
+        bool clash = false;
+        for (int jp = ast[a]; jp < ast[a + 1] && !clash; ++jp) clash = used[c][pidx[jp]] != 0;
+        if (!clash) break;
+      }
+      for (int jp = ast[a]; jp < ast[a + 1]; ++jp) used[c][pidx[jp]] = 1;
+      colour[a] = c;
+    }
+    const int nc = (int)used.size();
+    std::vector<int> tab;  // kb_order (NI), then kb_cstart (nc + 1)
+    std::vector<int> cst(nc + 1, 0);
+    for (int c = 0; c < nc; ++c) {
+      cst[c] = (int)tab.size();
+      for (int a = 0; a < kb.NI; ++a)
+        if (colour[a] == c) tab.push_back(a);
+    }
+    cst[nc] = (int)tab.size();
+    tab.insert(tab.end(), cst.begin(), cst.end());
+    int* d = nullptr;
+    if (hipMalloc((void**)&d, sizeof(int) * tab.size()) != hipSuccess)
+      return bfail(b, SALMON_ERR_NOMEM, "batch atom colours");
+    b->bufs.push_back(d);
+    BHIP(b, hipMemcpy(d, tab.data(), sizeof(int) * tab.size(), hipMemcpyHostToDevice));
+    S[i].kb_order = d;
+    S[i].kb_cstart = d + kb.NI;
+    S[i].ncolor = nc;
+  }
+  if (lds > 65536 || items.empty()) {
+    if (must) return bfail(b, SALMON_ERR_ARG, "the persistent sweep needs %zu bytes of LDS", lds);
+    return SALMON_OK;
+  }
+  std::stable_sort(items.begin(), items.end(), [](const std::array<long, 3>& a, const std::array<long, 3>& c) {
+    return a[0] > c[0];
+  });
+  std::vector<int2> it(items.size());
+  for (size_t k = 0; k < items.size(); ++k) it[k] = int2{(int)items[k][1], (int)items[k][2]};
+  if (hipMalloc((void**)&b->d_items, sizeof(int2) * it.size()) != hipSuccess)
+    return bfail(b, SALMON_ERR_NOMEM, "batch work list");
+  b->bufs.push_back(b->d_items);
+  BHIP(b, hipMemcpy(b->d_items, it.data(), sizeof(int2) * it.size(), hipMemcpyHostToDevice));
+  if (!b->d_next) BHIP(b, hipMalloc((void**)&b->d_next, sizeof(int)));
+  BHIP(b, hipMemset(b->d_next, 0, sizeof(int)));
+  int dev = 0, ncu = 0, per = 0;
+  BHIP(b, hipGetDevice(&dev));
+  BHIP(b, hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+  const void* kern = nullptr;
+  if constexpr (sizeof(T) == 4)
+    kern = (const void*)k_b_sweep<T, 4>;
+  else
+    kern = (const void*)k_b_sweep<T, 2>;
+  BHIP(b, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kern, kSweepThreads, lds));
+  if (per < 1) {
+    if (must) return bfail(b, SALMON_ERR_ARG, "the persistent sweep does not fit a compute unit");
+    return SALMON_OK;
+  }
+  b->nitems = (int)it.size();
+  b->sweep_lds = lds;
+  b->sweep_grid = (unsigned)std::min<long>((long)ncu * per, (long)it.size());
+  b->sweep = true;
+  return SALMON_OK;
 }
 
 template <typename T>
@@ -78,6 +222,7 @@ static int batch_build(salmon_batch* b) {
     y.NB = c->NB;
     y.NKB = c->NKB;
     y.zb = b->zb[i];
+    y.sw_zb = y.sw_tw = 0;
     const size_t nv = (size_t)c->NKB * c->g.NL;
     for (int t = 0; t < 4; ++t) {
       if (hipMalloc((void**)&y.h[t], nv * sizeof(cplx<T>)) != hipSuccess)
@@ -86,11 +231,13 @@ static int batch_build(salmon_batch* b) {
     }
     if (hipMalloc((void**)&y.uv, (size_t)c->NKB * std::max(1, c->kb.Nlma) * sizeof(cplx<T>)) != hipSuccess ||
         hipMalloc((void**)&y.ph, (size_t)c->NK * std::max(1, c->kb.NPI) * sizeof(cplx<T>)) != hipSuccess ||
+        hipMalloc((void**)&y.phn, (size_t)c->NK * std::max(1, c->kb.NPI) * sizeof(cplx<T>)) != hipSuccess ||
         hipMalloc((void**)&y.kbv, (size_t)c->NKB * std::max(1, c->kb.NT) * sizeof(cplx<T>)) != hipSuccess ||
         hipMalloc((void**)&y.rows, (size_t)c->NKB * 8 * sizeof(double)) != hipSuccess)
       return bfail(b, SALMON_ERR_NOMEM, "batch buffers (system %d)", i);
     b->bufs.push_back(y.uv);
     b->bufs.push_back(y.ph);
+    b->bufs.push_back(y.phn);
     b->bufs.push_back(y.kbv);
     b->bufs.push_back(y.rows);
     b->max_ph = std::max(b->max_ph, (long)c->NK * c->kb.NPI);
@@ -133,6 +280,8 @@ static int batch_build(salmon_batch* b) {
     b->bufs.push_back(b->d_owner[ph]);
     BHIP(b, hipMemcpy(b->d_owner[ph], own.data(), sizeof(unsigned short) * own.size(), hipMemcpyHostToDevice));
   }
+  int r = sweep_plan<T>(b, S);
+  if (r) return r;
   if (hipMalloc(&b->d_sys, sizeof(Sys<T>) * n) != hipSuccess) return bfail(b, SALMON_ERR_NOMEM, "batch descriptors");
   b->bufs.push_back(b->d_sys);
   BHIP(b, hipMemcpy(b->d_sys, S.data(), sizeof(Sys<T>) * n, hipMemcpyHostToDevice));
@@ -147,6 +296,23 @@ static int batch_enqueue_step(salmon_batch* b, double dt, long ac_stride) {
   const Sys<T>* S = (const Sys<T>*)b->d_sys;
   const Step st{b->d_ac, b->d_step, dt, {b->d_owner[0], b->d_owner[1], b->d_owner[2], b->d_owner[3], b->d_owner[4]},
                 ac_stride};
+  if (b->sweep) {
+    if (b->max_ph > 0) {
+      hipLaunchKernelGGL(k_b_phase<T>, dim3((unsigned)((b->max_ph + 255) / 256), n), dim3(256), 0, b->stream, S, st, 0);
+      hipLaunchKernelGGL(k_b_phase<T>, dim3((unsigned)((b->max_ph + 255) / 256), n), dim3(256), 0, b->stream, S, st, 1);
+    }
+    hipLaunchKernelGGL(k_b_qreset, dim3(1), dim3(64), 0, b->stream, b->d_next);
+    if constexpr (sizeof(T) == 4)
+      hipLaunchKernelGGL((k_b_sweep<T, 4>), dim3(b->sweep_grid), dim3(kSweepThreads), b->sweep_lds, b->stream, S, st,
+                         (const int2*)b->d_items, b->nitems, b->d_next);
+    else
+      hipLaunchKernelGGL((k_b_sweep<T, 2>), dim3(b->sweep_grid), dim3(kSweepThreads), b->sweep_lds, b->stream, S, st,
+                         (const int2*)b->d_items, b->nitems, b->d_next);
+    hipLaunchKernelGGL(k_b_sum<T>, dim3(n), dim3(256), 0, b->stream, S, n, b->d_step, b->d_res);
+    hipLaunchKernelGGL(k_b_tick, dim3(1), dim3(64), 0, b->stream, b->d_step);
+    BHIP(b, hipGetLastError());
+    return SALMON_OK;
+  }
   if (b->max_ph > 0)
     hipLaunchKernelGGL(k_b_phase<T>, dim3((unsigned)((b->max_ph + 255) / 256), n), dim3(256), 0, b->stream, S, st, 0);
   for (int t = 0; t < 4; ++t) {
@@ -243,6 +409,10 @@ static int batch_tune(salmon_batch* b) {
     b->cap_ac = 6;
   }
   BHIP(b, hipMemset(b->d_step, 0, sizeof(int)));
+  if (b->sweep) {  // the persistent sweep chooses its column shapes without timing
+    b->tuned = true;
+    return SALMON_OK;
+  }
   std::vector<std::array<int, 4>> shapes;  // (NLx, NLy, NLz, zb)
   std::vector<int> zb(b->ctx.size(), 1);
   for (size_t i = 0; i < b->ctx.size(); ++i) {

@@ -1,0 +1,496 @@
+// Persistent batched step: one launch per RT step covers every system.
+//
+// A 512-thread workgroup pulls work items (system, first orbital) from a
+// device counter, costliest first, and runs the whole step of its orbitals:
+// for each of the four Taylor terms the KB projection, one sweep over the
+// grid for the stencil and the local potential, and the KB term added to the
+// touched points; the Taylor fold is part of the fourth sweep, and a fifth
+// sweep with its own projections evaluates the observables at k + A(t+dt).
+//
+// Sweep layout: the workgroup's waves form teams of TW waves, one orbital per
+// team.  Thread i of a team owns the column of ZB consecutive z points at
+// in-plane offset i*ZB of every x plane and walks x = 0 .. NLx-1 with the
+// column's planes x-4 .. x+5 in registers (one global load per value and
+// sweep, the x neighbours never leave the thread).  The y and z neighbours
+// come from the current plane, which the team stores in one of two LDS plane
+// buffers (x parity) before a barrier; a column takes col_stride<ZB> slots
+// there, one more than ZB for even ZB, so that the lanes' 8- and 16-byte
+// reads fall in distinct banks.  The per-point operands of plane x+1 (the
+// potential, and for the fold the orbital and its first two H applications)
+// are loaded during step x, before the window load of plane x+5.
+#pragma once
+#include "batch.hpp"
+
+namespace salmon {
+namespace batch {
+
+constexpr int kSweepThreads = 512;
+constexpr int kSweepWaves = kSweepThreads / kWave;
+constexpr int kColWin = 10;      // register window of a column: planes x-4 .. x+5
+constexpr int kRedStride = 10;   // doubles per wave in the reduction area
+constexpr int kSweepHead = 16 + kSweepWaves * kRedStride * 8;  // bytes before the team areas
+
+// Per-point operands of a sweep besides its input.
+template <typename T>
+struct SweepIO {
+  const cplx<T>* in;   // orbital the stencil applies to
+  cplx<T>* out;        // H in (term < 3), or the folded orbital (term 3)
+  cplx<T>* zu;         // term 3: the orbital at t (read, then overwritten by out == zu)
+  const cplx<T>* h0;   // term 3: H zu
+  const cplx<T>* h1;   // term 3: H^2 zu
+};
+
+__device__ __forceinline__ int wrap_once(int v, int n) { return v < 0 ? v + n : (v >= n ? v - n : v); }
+
+// LDS slots per column of ZB points
+template <int ZB>
+__host__ __device__ constexpr int col_stride() {
+  return ZB % 2 == 0 ? ZB + 1 : ZB;
+}
+
+// One sweep of a team over its orbital.  MODE 0: out = (T + V) in (stencil
+// and local potential, no KB term); MODE 1: zu += z1 h0 + z2 h1 + z3 in +
+// z4 (T + V) in; MODE 2: observables of in, the per-thread sums
+// acc = {|psi|^2, V|psi|^2, Re conj(psi) L, Im conj(psi) G_x, _y, _z}
+// (stencil sums L and G as in observe_orbital, streamed.hpp).
+template <typename T, int ZB, int MODE>
+__device__ __forceinline__ void col_sweep(const Grid& g, bool act, int tl, cplx<T>* __restrict__ pl,
+                                          const SweepIO<T>& io, const T* __restrict__ vl,
+                                          const double* __restrict__ vl64, double kx, double ky, double kz,
+                                          double dt, double (&acc)[6]) {
+  const int NLx = g.NLx, NLy = g.NLy, NLz = g.NLz, P = NLy * NLz;
+  const bool col = act && tl * ZB < P;
+  const int w0 = col ? tl * ZB : 0;
+  const int iy = w0 / NLz, iz0 = w0 - iy * NLz;
+  const int line = iy * NLz;
+  constexpr int CS = col_stride<ZB>();
+  const int PS = (P / ZB) * CS;  // LDS slots of one plane
+  // LDS slot of in-plane offset w: column w / ZB, point w % ZB
+  auto slot = [](int w) { return (w / ZB) * CS + w % ZB; };
+  // LDS slots of the z window outside the column (z = iz0-4 .. iz0-1 and
+  // iz0+ZB .. iz0+ZB+3) and of the y neighbour runs (iy+n, then iy-n)
+  int zo[8], yo[8];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    zo[j] = slot(line + wrap_once(iz0 - 4 + j, NLz));
+    zo[4 + j] = slot(line + wrap_once(iz0 + ZB + j, NLz));
+  }
+#pragma unroll
+  for (int n = 1; n <= 4; ++n) {
+    yo[n - 1] = slot(wrap_once(iy + n, NLy) * NLz + iz0);
+    yo[3 + n] = slot(wrap_once(iy - n, NLy) * NLz + iz0);
+  }
+  const int own = (w0 / ZB) * CS;
+  // coefficients per axis (x, y, z) and n: C = lap, D = k nab (MODE 2: nab)
+  const double kk[3] = {kx, ky, kz};
+  T C[3][5], D[3][5];
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int n = 1; n <= 4; ++n) {
+      C[a][n] = (T)g.lap[a][n];
+      D[a][n] = MODE == 2 ? (T)g.nab[a][n] : (T)(kk[a] * g.nab[a][n]);
+    }
+  const T A = (T)((kx * kx + ky * ky + kz * kz - (g.lap[0][0] + g.lap[1][0] + g.lap[2][0])) * 0.5);
+  const T z1i = (T)(-dt), z2r = (T)(-dt * dt / 2.0), z3i = (T)(dt * dt * dt / 6.0),
+          z4r = (T)(dt * dt * dt * dt / 24.0);
+  const cplx<T>* __restrict__ in = io.in;
+
+  cplx<T> win[kColWin][ZB];
+  auto ld = [&](int X, cplx<T>(&d)[ZB]) {
+    const int Xs = wrap_once(X, NLx);
+    load_run<T, ZB>(in + (size_t)Xs * P + w0, d);
+  };
+  // per-point operands, double-buffered by plane parity
+  T V[2][ZB];
+  double V64[2][ZB];
+  cplx<T> zz[2][ZB], a0[2][ZB], a1[2][ZB];
+  auto ld_ops = [&](int X, int b) {
+    const size_t pb = (size_t)X * P + w0;
+    if constexpr (MODE == 2) {
+#pragma unroll
+      for (int q = 0; q < ZB; ++q) V64[b][q] = vl64[pb + q];
+    } else {
+#pragma unroll
+      for (int q = 0; q < ZB; ++q) V[b][q] = vl[pb + q];
+    }
+    if constexpr (MODE == 1) {
+      load_run<T, ZB>(io.zu + pb, zz[b]);
+      load_run<T, ZB>(io.h0 + pb, a0[b]);
+      load_run<T, ZB>(io.h1 + pb, a1[b]);
+    }
+  };
+  if (col) {
+    ld_ops(0, 0);
+#pragma unroll
+    for (int X = -4; X <= 4; ++X) ld(X, win[(X + kColWin) % kColWin]);
+  }
+  for (int x0 = 0; x0 < NLx; x0 += kColWin) {
+#pragma unroll
+    for (int j = 0; j < kColWin; ++j) {
+      const int x = x0 + j;
+      if (x >= NLx) break;
+      const size_t pb = (size_t)x * P + w0;
+      const int cb = j & 1;  // operand buffer of plane x (x0 is even)
+      if (col) {
+        ld_ops(x + 1 < NLx ? x + 1 : 0, cb ^ 1);
+        ld(x + 5, win[(j + 5) % kColWin]);
+      }
+      cplx<T>* pln = pl + cb * PS;
+      if (col) {
+#pragma unroll
+        for (int q = 0; q < ZB; ++q) pln[own + q] = win[j][q];
+      }
+      lds_barrier();
+      if (!col) continue;
+      // z window: 4 values either side from LDS, the column itself from registers
+      cplx<T> zw[ZB + 8];
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        zw[t] = pln[zo[t]];
+        zw[ZB + 4 + t] = pln[zo[4 + t]];
+      }
+#pragma unroll
+      for (int q = 0; q < ZB; ++q) zw[4 + q] = win[j][q];
+      if constexpr (MODE < 2) {
+        T vr[ZB], vi[ZB], wr[ZB], wi[ZB];
+#pragma unroll
+        for (int q = 0; q < ZB; ++q) {
+          vr[q] = vi[q] = wr[q] = wi[q] = 0;
+#pragma unroll
+          for (int n = 1; n <= 4; ++n) {
+            const cplx<T> ep = zw[4 + q + n], em = zw[4 + q - n];
+            vr[q] += C[2][n] * (ep.re + em.re);
+            vi[q] += C[2][n] * (ep.im + em.im);
+            wr[q] += D[2][n] * (ep.re - em.re);
+            wi[q] += D[2][n] * (ep.im - em.im);
+          }
+        }
+#pragma unroll
+        for (int n = 1; n <= 4; ++n) {
+#pragma unroll
+          for (int q = 0; q < ZB; ++q) {
+            const cplx<T> ep = pln[yo[n - 1] + q], em = pln[yo[3 + n] + q];
+            vr[q] += C[1][n] * (ep.re + em.re);
+            vi[q] += C[1][n] * (ep.im + em.im);
+            wr[q] += D[1][n] * (ep.re - em.re);
+            wi[q] += D[1][n] * (ep.im - em.im);
+          }
+        }
+#pragma unroll
+        for (int n = 1; n <= 4; ++n) {
+#pragma unroll
+          for (int q = 0; q < ZB; ++q) {
+            const cplx<T> ep = win[(j + n) % kColWin][q], em = win[(j + kColWin - n) % kColWin][q];
+            vr[q] += C[0][n] * (ep.re + em.re);
+            vi[q] += C[0][n] * (ep.im + em.im);
+            wr[q] += D[0][n] * (ep.re - em.re);
+            wi[q] += D[0][n] * (ep.im - em.im);
+          }
+        }
+        cplx<T> f[ZB];
+#pragma unroll
+        for (int q = 0; q < ZB; ++q) {
+          const cplx<T> e = win[j][q];
+          f[q].re = (V[cb][q] + A) * e.re - (T)0.5 * vr[q] + wi[q];
+          f[q].im = (V[cb][q] + A) * e.im - (T)0.5 * vi[q] - wr[q];
+        }
+        if constexpr (MODE == 0) {
+          store_run<T, ZB>(io.out + pb, f);
+        } else {
+          // fold in the order of k_b_fold: z1 h0, z2 h1, z3 h2, z4 h3
+#pragma unroll
+          for (int q = 0; q < ZB; ++q) {
+            cplx<T> z = zz[cb][q];
+            const cplx<T> e = win[j][q];
+            z.re += -z1i * a0[cb][q].im;
+            z.im += z1i * a0[cb][q].re;
+            z.re += z2r * a1[cb][q].re;
+            z.im += z2r * a1[cb][q].im;
+            z.re += -z3i * e.im;
+            z.im += z3i * e.re;
+            z.re += z4r * f[q].re;
+            z.im += z4r * f[q].im;
+            f[q] = z;
+          }
+          store_run<T, ZB>(io.out + pb, f);
+        }
+      } else {
+#pragma unroll
+        for (int q = 0; q < ZB; ++q) {
+          T Lr = 0, Li = 0, Gr[3] = {0, 0, 0}, Gi[3] = {0, 0, 0};
+#pragma unroll
+          for (int n = 1; n <= 4; ++n) {
+            const cplx<T> ep = zw[4 + q + n], em = zw[4 + q - n];
+            Lr += C[2][n] * (ep.re + em.re);
+            Li += C[2][n] * (ep.im + em.im);
+            Gr[2] += D[2][n] * (ep.re - em.re);
+            Gi[2] += D[2][n] * (ep.im - em.im);
+          }
+#pragma unroll
+          for (int n = 1; n <= 4; ++n) {
+            const cplx<T> ep = pln[yo[n - 1] + q], em = pln[yo[3 + n] + q];
+            Lr += C[1][n] * (ep.re + em.re);
+            Li += C[1][n] * (ep.im + em.im);
+            Gr[1] += D[1][n] * (ep.re - em.re);
+            Gi[1] += D[1][n] * (ep.im - em.im);
+          }
+#pragma unroll
+          for (int n = 1; n <= 4; ++n) {
+            const cplx<T> ep = win[(j + n) % kColWin][q], em = win[(j + kColWin - n) % kColWin][q];
+            Lr += C[0][n] * (ep.re + em.re);
+            Li += C[0][n] * (ep.im + em.im);
+            Gr[0] += D[0][n] * (ep.re - em.re);
+            Gi[0] += D[0][n] * (ep.im - em.im);
+          }
+          const double er = (double)win[j][q].re, ei = (double)win[j][q].im;
+          const double a2 = er * er + ei * ei;
+          acc[0] += a2;
+          acc[1] += V64[cb][q] * a2;
+          acc[2] += er * (double)Lr + ei * (double)Li;
+#pragma unroll
+          for (int d = 0; d < 3; ++d) acc[3 + d] += er * (double)Gi[d] - ei * (double)Gr[d];
+        }
+      }
+    }
+  }
+}
+
+// KB parts of one orbital, one wave per atom (lanes over the atom's sphere
+// points, every orbital value gathered once per atom); kKbPass projectors of
+// the atom per pass over its sphere.
+constexpr int kKbPass = 4;
+
+// Projections of atom a (pseudo_pt phase 1, ARTED/common/hpsi.f90:183-193):
+// U(il) = Hxyz iuV(il) sum_j uV(j,il) e^{i k.r_j} E(Jxyz(j)), phases ph.
+template <typename T>
+__device__ __forceinline__ void kb_project_atom(const KBTables& kb, double hxyz, int a, const cplx<T>* __restrict__ E,
+                                                const cplx<T>* __restrict__ ph, cplx<T>* U, int lane) {
+  const int j0 = kb.atom_start[a], m = kb.atom_start[a + 1] - j0;
+  const int l1 = kb.atom_lma[a + 1];
+  for (int lc = kb.atom_lma[a]; lc < l1; lc += kKbPass) {
+    T v[2 * kKbPass];
+#pragma unroll
+    for (int k = 0; k < 2 * kKbPass; ++k) v[k] = 0;
+    for (int j = lane; j < m; j += kWave) {
+      const int jp = j0 + j;
+      const cplx<T> e = E[kb.pidx[jp]];
+      const cplx<T> p = ph[jp];
+      const T wr = p.re * e.re - p.im * e.im, wi = p.re * e.im + p.im * e.re;
+#pragma unroll
+      for (int l = 0; l < kKbPass; ++l) {
+        if (lc + l < l1) {
+          const T u = (T)kb.uv[(size_t)(lc + l) * kb.Nps + j];
+          v[2 * l] += u * wr;
+          v[2 * l + 1] += u * wi;
+        }
+      }
+    }
+    wave_sums<2 * kKbPass>(v);
+#pragma unroll
+    for (int k = 0; k < 2 * kKbPass; ++k) {
+      const int il = lc + k / 2;
+      if (lane == wave_sums_lane(k) && il < l1) ((T*)U)[2 * il + (k & 1)] = v[0] * (T)(hxyz * kb.lma_sign[il]);
+    }
+  }
+}
+
+// dst(Jxyz(j)) += fac sum_il uV(j,il) e^{-i k.r_j} U(il) over atom a's sphere
+// (pseudo_pt phase 2, ARTED/common/hpsi.f90:195-206).  Atoms whose spheres
+// share a grid point must not run at the same time.
+template <typename T>
+__device__ __forceinline__ void kb_scatter_atom(const KBTables& kb, int a, const cplx<T>* __restrict__ ph,
+                                                const cplx<T>* U, cplx<T>* __restrict__ dst, T fac, int lane) {
+  const int j0 = kb.atom_start[a], m = kb.atom_start[a + 1] - j0;
+  const int l0 = kb.atom_lma[a], l1 = kb.atom_lma[a + 1];
+  for (int j = lane; j < m; j += kWave) {
+    const int jp = j0 + j;
+    const cplx<T> p = ph[jp];
+    T fr = 0, fi = 0;
+    for (int il = l0; il < l1; ++il) {
+      const T u = (T)kb.uv[(size_t)il * kb.Nps + j];
+      const cplx<T> w = U[il];
+      fr += u * (p.re * w.re + p.im * w.im);
+      fi += u * (p.re * w.im - p.im * w.re);
+    }
+    cplx<T>* d = dst + kb.pidx[jp];
+    cplx<T> v = *d;
+    v.re += fac * fr;
+    v.im += fac * fi;
+    *d = v;
+  }
+}
+
+// Nonlocal observables of atom a at the phases ph (current.f90:172-273 and
+// total_energy.f90:191-199, as in observe_orbital): adds Jx, Jy, Jz, Enl.
+template <typename T>
+__device__ __forceinline__ void kb_observe_atom(const KBTables& kb, const Grid& g, double occ, int a,
+                                                const cplx<T>* __restrict__ E, const cplx<T>* __restrict__ ph,
+                                                double (&nl4)[4], int lane) {
+  const int j0 = kb.atom_start[a], m = kb.atom_start[a + 1] - j0;
+  const int l1 = kb.atom_lma[a + 1];
+  for (int lc = kb.atom_lma[a]; lc < l1; lc += kKbPass) {
+    // per projector: sum u t and sum r u t (t = e^{ik.r} psi), real and imaginary parts
+    double v[8 * kKbPass];
+#pragma unroll
+    for (int k = 0; k < 8 * kKbPass; ++k) v[k] = 0;
+    for (int j = lane; j < m; j += kWave) {
+      const int jp = j0 + j;
+      const double px = kb.pos[3 * jp], py = kb.pos[3 * jp + 1], pz = kb.pos[3 * jp + 2];
+      const double c = (double)ph[jp].re, sn = (double)ph[jp].im;
+      const cplx<T> e = E[kb.pidx[jp]];
+      const double tr0 = c * (double)e.re - sn * (double)e.im, ti0 = c * (double)e.im + sn * (double)e.re;
+#pragma unroll
+      for (int l = 0; l < kKbPass; ++l) {
+        if (lc + l < l1) {
+          const double u = kb.uv[(size_t)(lc + l) * kb.Nps + j];
+          const double tr = u * tr0, ti = u * ti0;
+          double* q = v + 8 * l;
+          q[0] += tr;
+          q[1] += ti;
+          q[2] += px * tr;
+          q[3] += px * ti;
+          q[4] += py * tr;
+          q[5] += py * ti;
+          q[6] += pz * tr;
+          q[7] += pz * ti;
+        }
+      }
+    }
+    wave_sums<8 * kKbPass>(v);
+    for (int l = 0; l < kKbPass && lc + l < l1; ++l) {
+      const int il = lc + l;
+      double q[8];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) q[k] = lane_read(v[0], wave_sums_lane(8 * l + k));
+      const double sg = kb.lma_sign[il];
+      const double Ur = q[0] * g.Hxyz * sg, Ui = q[1] * g.Hxyz * sg;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        const double Xr = q[2 + 2 * d] * g.Hxyz, Xi = q[3 + 2 * d] * g.Hxyz;
+        nl4[d] += occ / g.aLxyz * 2.0 * (Xr * Ui - Xi * Ur);
+      }
+      const double hr = q[0] * g.Hxyz, hi = q[1] * g.Hxyz;
+      nl4[3] += occ * (hr * hr + hi * hi) * sg;
+    }
+  }
+}
+
+// Whole RT step of the orbitals o0 .. o0 + teams - 1 of system y.
+template <typename T, int ZB>
+__device__ __forceinline__ void sweep_item(const Sys<T>& y, const Step& st, int s, int o0, unsigned char* lds) {
+  const Grid& g = y.g;
+  const KBTables& kb = y.kb;
+  const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / kWave));
+  const int tw = y.sw_tw, team = wv / tw, wt = wv - team * tw, nt = tw * kWave;
+  const int tl = (int)threadIdx.x - team * nt;
+  const int lane = (int)threadIdx.x & (kWave - 1);
+  const int o = o0 + team;
+  const bool act = o < y.NKB;
+  const int PS = (g.NLy * g.NLz / ZB) * col_stride<ZB>();  // LDS slots of one plane
+  const int nlma = kb.Nlma;
+  double* red = (double*)(lds + 16);
+  cplx<T>* pl = (cplx<T>*)(lds + kSweepHead) + (size_t)team * (2 * PS + nlma);
+  cplx<T>* U = pl + 2 * PS;
+  const size_t NL = (size_t)g.NL;
+  const size_t ob = (size_t)(act ? o : 0) * NL;
+  const int ik = (act ? o : 0) / y.NB;
+  double am[3], an[3];
+  step_fields(st, s, am, an);
+  double kx, ky, kz;
+  orbital_k(y.kac0, y.NK, ik, am[0], am[1], am[2], kx, ky, kz);
+  const cplx<T>* phk = y.ph + (size_t)ik * kb.NPI;
+  double acc[6] = {0, 0, 0, 0, 0, 0};
+  for (int term = 0; term < 4; ++term) {
+    const cplx<T>* in = (term == 0 ? y.zu : y.h[term - 1]) + ob;
+    if (act)
+      for (int a = wt; a < kb.NI; a += tw) kb_project_atom<T>(kb, g.Hxyz, a, in, phk, U, lane);
+    __syncthreads();
+    SweepIO<T> io{in, (term < 3 ? y.h[term] : y.zu) + ob, y.zu + ob, y.h[0] + ob, y.h[1] + ob};
+    if (term < 3)
+      col_sweep<T, ZB, 0>(g, act, tl, pl, io, y.vloc, y.vloc64, kx, ky, kz, st.dt, acc);
+    else
+      col_sweep<T, ZB, 1>(g, act, tl, pl, io, y.vloc, y.vloc64, kx, ky, kz, st.dt, acc);
+    __syncthreads();
+    // KB term: the atoms colour by colour (no two atoms of a colour share a
+    // grid point), the atoms of a colour over the team's waves
+    const T fac = term < 3 ? (T)1 : (T)(st.dt * st.dt * st.dt * st.dt / 24.0);
+    for (int c = 0; c < y.ncolor; ++c) {
+      if (act)
+        for (int k = y.kb_cstart[c] + wt; k < y.kb_cstart[c + 1]; k += tw)
+          kb_scatter_atom<T>(kb, y.kb_order[k], phk, U, io.out, fac, lane);
+      __syncthreads();
+    }
+  }
+  // observables of the new orbital at k + A(t+dt)
+  orbital_k(y.kac0, y.NK, ik, an[0], an[1], an[2], kx, ky, kz);
+  const cplx<T>* phn = y.phn + (size_t)ik * kb.NPI;
+  const SweepIO<T> io{y.zu + ob, nullptr, nullptr, nullptr, nullptr};
+  col_sweep<T, ZB, 2>(g, act, tl, pl, io, y.vloc, y.vloc64, kx, ky, kz, st.dt, acc);
+  const double occ = act ? y.occ[o] : 0.0;
+  double nl4[4] = {0, 0, 0, 0};  // nonlocal Jx, Jy, Jz, Enl of this wave's atoms
+  if (act)
+    for (int a = wt; a < kb.NI; a += tw) kb_observe_atom<T>(kb, g, occ, a, y.zu + ob, phn, nl4, lane);
+#pragma unroll
+  for (int i = 0; i < 6; ++i) acc[i] = wave_sum(acc[i]);
+  if (lane == 0) {
+#pragma unroll
+    for (int i = 0; i < 6; ++i) red[wv * kRedStride + i] = acc[i];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) red[wv * kRedStride + 6 + i] = nl4[i];
+  }
+  __syncthreads();
+  if (act && tl == 0) {
+    double v[kRedStride];
+#pragma unroll
+    for (int i = 0; i < kRedStride; ++i) v[i] = red[(team * tw) * kRedStride + i];
+    for (int w = 1; w < tw; ++w)
+#pragma unroll
+      for (int i = 0; i < kRedStride; ++i) v[i] += red[(team * tw + w) * kRedStride + i];
+    const double kk[3] = {kx, ky, kz};
+    const double A0 = -(g.lap[0][0] + g.lap[1][0] + g.lap[2][0]) * 0.5;
+    const double te = -0.5 * v[2];
+    const double sc = g.Hxyz / g.aLxyz;
+    double* r = y.rows + (size_t)o * 8;
+    for (int d = 0; d < 3; ++d) r[d] = (occ * kk[d] * v[0] + v[3 + d] * occ) * sc + v[6 + d];
+    r[3] = occ * (A0 * v[0] + te + kx * v[3] + ky * v[4] + kz * v[5]) * g.Hxyz +
+           occ * (kx * kx + ky * ky + kz * kz) / 2.0;
+    r[4] = v[9];
+    r[5] = occ * v[1] * g.Hxyz;
+    r[6] = occ * v[0] * g.Hxyz;
+    r[7] = 0.0;
+  }
+  __syncthreads();
+}
+
+// Work items: (system, first orbital), costliest first; *next is zero at launch.
+template <typename T, int ZBMAX>
+__global__ void __launch_bounds__(kSweepThreads) k_b_sweep(const Sys<T>* __restrict__ S, Step st,
+                                                           const int2* __restrict__ items, int nitems,
+                                                           int* __restrict__ next) {
+  extern __shared__ __align__(16) unsigned char lds[];
+  int* slot = (int*)lds;
+  for (;;) {
+    if (threadIdx.x == 0) *slot = atomicAdd(next, 1);
+    __syncthreads();
+    const int it = __builtin_amdgcn_readfirstlane(*slot);
+    __syncthreads();
+    if (it >= nitems) break;
+    const int2 wk = items[it];
+    const int s = __builtin_amdgcn_readfirstlane(wk.x), o0 = __builtin_amdgcn_readfirstlane(wk.y);
+    const Sys<T>& y = S[s];
+    switch (y.sw_zb) {
+      case 1: sweep_item<T, 1>(y, st, s, o0, lds); break;
+      case 2: sweep_item<T, 2>(y, st, s, o0, lds); break;
+      case 3: if constexpr (ZBMAX >= 3) sweep_item<T, 3>(y, st, s, o0, lds); break;
+      default: if constexpr (ZBMAX >= 4) sweep_item<T, 4>(y, st, s, o0, lds); break;
+    }
+  }
+}
+
+__global__ void k_b_qreset(int* next) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *next = 0;
+}
+
+}  // namespace batch
+}  // namespace salmon

@@ -1470,6 +1470,22 @@ int salmon_gpu_batch_add(salmon_batch* b, salmon_ctx* c) {
   return SALMON_OK;
 }
 
+int salmon_gpu_batch_set_kernel(salmon_batch* b, int kernel) {
+  if (!b) return SALMON_ERR_ARG;
+  if (kernel != SALMON_BATCH_AUTO && kernel != SALMON_BATCH_PHASES && kernel != SALMON_BATCH_SWEEP)
+    return bfail(b, SALMON_ERR_ARG, "unknown batch kernel %d", kernel);
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  batch_release(b);
+  b->tuned = false;
+  b->kernel_req = kernel;
+  return SALMON_OK;
+}
+
+int salmon_gpu_batch_kernel(const salmon_batch* b) {
+  if (!b || !b->ready || !b->tuned) return -1;
+  return b->sweep ? SALMON_BATCH_SWEEP : SALMON_BATCH_PHASES;
+}
+
 int salmon_gpu_batch_tune(salmon_batch* b) {
   if (!b) return SALMON_ERR_ARG;
   if (b->ctx.empty()) return bfail(b, SALMON_ERR_STATE, "empty batch");
@@ -1508,6 +1524,7 @@ int salmon_gpu_batch_finalize(salmon_batch* b) {
   if (b->stream) (void)hipStreamSynchronize(b->stream);
   batch_release(b);
   if (b->d_step) (void)hipFree(b->d_step);
+  if (b->d_next) (void)hipFree(b->d_next);
   if (b->d_ac) (void)hipFree(b->d_ac);
   if (b->d_res) (void)hipFree(b->d_res);
   if (b->stream) (void)hipStreamDestroy(b->stream);

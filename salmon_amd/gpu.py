@@ -38,6 +38,7 @@ LIB_PATH = os.path.join(HERE, "lib", "libsalmon_gpu.so")
 FP64 = 0
 MIXED = 1
 PATH_AUTO, PATH_STREAMED, PATH_FUSED = 0, 1, 2
+BATCH_AUTO, BATCH_PHASES, BATCH_SWEEP = 0, 1, 2
 PROP_MIDPOINT, PROP_ETRS = 0, 1
 XC_NONE, XC_PZ = 0, 1
 PC_NONE, PC_KINETIC, PC_LEARNED = 0, 1, 2
@@ -96,6 +97,8 @@ SIGNATURES = {
     "salmon_gpu_rt_run": [_vp, _vp, _ci, _cd, _ci, _vp, _vp, _vp, _vp],
     "salmon_gpu_batch_create": [_ci, ctypes.POINTER(_vp)],
     "salmon_gpu_batch_add": [_vp, _vp],
+    "salmon_gpu_batch_set_kernel": [_vp, _ci],
+    "salmon_gpu_batch_kernel": [_vp],
     "salmon_gpu_batch_tune": [_vp],
     "salmon_gpu_batch_zblock": [_vp, _ci],
     "salmon_gpu_batch_rt_run": [_vp, _vp, _ci, _cd, _vp, _vp, _vp],
@@ -457,6 +460,15 @@ class Batch:
         if rc != 0:
             msg = lib().salmon_gpu_batch_last_error(self._h).decode() if self._h else ""
             raise SalmonGPUError(f"{what}: {lib().salmon_gpu_errstr(rc).decode()}: {msg}")
+
+    def set_kernel(self, kernel):
+        """BATCH_AUTO, BATCH_PHASES (one launch per phase) or BATCH_SWEEP (one
+        persistent launch per step)."""
+        self._check(lib().salmon_gpu_batch_set_kernel(self._h, int(kernel)), "batch_set_kernel")
+
+    def kernel(self):
+        """The step kernel in use (BATCH_PHASES or BATCH_SWEEP); -1 before tune."""
+        return lib().salmon_gpu_batch_kernel(self._h)
 
     def tune(self):
         self._check(lib().salmon_gpu_batch_tune(self._h), "batch_tune")

@@ -1,7 +1,8 @@
 """Batched independent systems on the GPU: every system of a batch (different
-grids, bands, k points and atoms; one launch per phase over all systems, the
-step replayed from a captured graph, autotuned z blocking) against the same
-system run alone through the single-context streamed path.
+grids, bands, k points and atoms; the persistent sweep kernel and the
+one-launch-per-phase kernel with autotuned z blocking, the step replayed from
+a captured graph) against the same system run alone through the
+single-context streamed path.
 
 Tolerance: FP64 1e-12 relative on J(t), E(t) and the norms (the batched
 stencil keeps the single-context summation order; only FMA contraction may
@@ -26,13 +27,14 @@ def _ac(n):
     return synth.acos2_pulse(n, 0.08, 0.05, 0.057, 826.8)
 
 
+@pytest.mark.parametrize("kernel", [gpu.BATCH_SWEEP, gpu.BATCH_PHASES])
 @pytest.mark.parametrize("prec", [gpu.FP64, gpu.MIXED])
-def test_batch_matches_single_contexts(prec):
+def test_batch_matches_single_contexts(prec, kernel):
     nsteps = 6
     ac = _ac(nsteps)
     systems = _systems()
-    print(f"[gpu] batch prec={prec}: " + ", ".join(f"{s.NLx}x{s.NLy}x{s.NLz}/{s.NK}k/{s.NB}b" for s in systems),
-          flush=True)
+    print(f"[gpu] batch prec={prec} kernel={kernel}: "
+          + ", ".join(f"{s.NLx}x{s.NLy}x{s.NLz}/{s.NK}k/{s.NB}b" for s in systems), flush=True)
     singles = []
     for s in systems:
         with gpu.Context(s, prec) as c:
@@ -40,7 +42,9 @@ def test_batch_matches_single_contexts(prec):
             singles.append(c.rt_run(ac, nsteps, dt=0.08))
     ctxs = [gpu.Context(s, prec) for s in systems]
     with gpu.Batch(ctxs, prec) as b:
+        b.set_kernel(kernel)
         zb = b.tune()
+        assert b.kernel() == kernel
         print(f"[gpu] batch z blocking {zb}", flush=True)
         assert all(s.NLz % z == 0 for s, z in zip(systems, zb))
         r1 = b.rt_run(ac[: 3 + 1], 3, 0.08)
@@ -75,6 +79,25 @@ def test_batch_orbitals_match_single_context():
             b.rt_run(ac, nsteps, 0.08)
         z2 = c.get_orbitals()
     assert rel(z2, z1) < 1e-13
+
+
+def test_batch_sweep_needs_eight_points_per_axis():
+    """A 6^3 grid is outside the persistent sweep: asking for the sweep fails
+    with a message, the automatic choice runs it with the phase kernel."""
+    s = synth.make_system(N=(6, 6, 6), L=6.0, nk3=(1, 1, 1), NB=2, seed=3)
+    ac = _ac(2)
+    with gpu.Context(s, gpu.FP64) as c:
+        c.set_path(gpu.PATH_STREAMED)
+        ref = c.rt_run(ac, 2, dt=0.08)
+    with gpu.Context(s, gpu.FP64) as c:
+        with gpu.Batch([c], gpu.FP64) as b:
+            b.set_kernel(gpu.BATCH_SWEEP)
+            with pytest.raises(gpu.SalmonGPUError, match="persistent sweep"):
+                b.tune()
+            b.set_kernel(gpu.BATCH_AUTO)
+            r = b.rt_run(ac, 2, 0.08)
+            assert b.kernel() == gpu.BATCH_PHASES
+    assert rel(r["J"][0], ref["J"]) < 1e-12 and rel(r["E"][0], ref["E"]) < 1e-12
 
 
 def test_batch_rejects_mixed_precisions():

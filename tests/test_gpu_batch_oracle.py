@@ -2,8 +2,8 @@
 
 One system of each grid shape of the 64-system workload (12^3, 20x20x28,
 32^3, 16^3, 24^3; bands and k points scaled by 0.12) runs through the batched
-step -- one launch per phase over all systems, the step replayed from a
-captured graph, tuned z blocking -- and each system's J(t), E(t), norm and
+step -- the persistent sweep kernel and the one-launch-per-phase kernel, the
+step replayed from a captured graph -- and each system's J(t), E(t), norm and
 final orbitals are compared with the oracle's RT loop (oracle/salmon_oracle.c
 or_rt_run: dt_evolve.f90:106-216 then control_sc.f90:203-222).  The
 single-context streamed path is held to the oracle on the same shapes.
@@ -48,17 +48,20 @@ def _jerr(j, jr):
     return float(np.max(np.abs(j - jr)) / np.max(np.abs(jr)))
 
 
+@pytest.mark.parametrize("kernel", [gpu.BATCH_SWEEP, gpu.BATCH_PHASES])
 @pytest.mark.parametrize("prec", [gpu.FP64, gpu.MIXED])
-def test_batch_every_shape_against_oracle(prec):
+def test_batch_every_shape_against_oracle(prec, kernel):
     nsteps = 3
     ac = _ac(nsteps)
     systems = [synth.config4_system(i, scale=0.12) for i in SHAPE_SYSTEMS]
-    print(f"[gpu] batch vs oracle prec={prec}: " + ", ".join(f"{_shape(s)}/{s.NK}k/{s.NB}b/{s.NI}at" for s in systems),
-          flush=True)
+    print(f"[gpu] batch vs oracle prec={prec} kernel={kernel}: "
+          + ", ".join(f"{_shape(s)}/{s.NK}k/{s.NB}b/{s.NI}at" for s in systems), flush=True)
     ctxs = [gpu.Context(s, prec) for s in systems]
     try:
         with gpu.Batch(ctxs, prec) as b:
+            b.set_kernel(kernel)
             r = b.rt_run(ac, nsteps, DT)
+            assert b.kernel() == kernel
         zs = [c.get_orbitals() for c in ctxs]
     finally:
         for c in ctxs:
