@@ -242,8 +242,9 @@ int salmon_gpu_rt_run(salmon_ctx *ctx, const double *ac, int nsteps, double dt, 
  * precision, A(t) and dt.
  * Two step kernels: SALMON_BATCH_SWEEP, one persistent launch per step whose
  * workgroups take (system, orbitals) items from a device work counter and
- * run each orbital's whole step (grids of at least 8 points per axis whose
- * x plane fits a 512-thread workgroup's columns), and SALMON_BATCH_PHASES,
+ * run each orbital's whole step (grids of at least 8 points per axis whose x
+ * plane splits into at most 256 columns of up to 4 z points in complex64, 512
+ * columns of up to 2 z points in FP64), and SALMON_BATCH_PHASES,
  * one launch per phase over all systems (any grid).  SALMON_BATCH_AUTO (the
  * default) takes the sweep when every system fits it; salmon_gpu_batch_kernel
  * reports the kernel in use after tune (-1 before).  For the phase kernel

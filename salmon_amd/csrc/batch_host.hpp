@@ -86,7 +86,7 @@ static std::array<int, 2> sweep_shape(const Grid& g) {
     if (g.NLz % zb) continue;
     const int ncol = P / zb;
     int tw = 1;
-    while (tw < kSweepWaves && tw * kWave < ncol) tw *= 2;
+    while (tw < sweep_waves<T>() && tw * kWave < ncol) tw *= 2;
     if (tw * kWave < ncol) continue;
     const double u = (double)ncol / (tw * kWave);
     if (u > bu + 1e-12 || (u > bu - 1e-12 && zb > best[0])) {
@@ -120,9 +120,9 @@ static int sweep_plan(salmon_batch* b, std::vector<Sys<T>>& S) {
     }
     S[i].sw_zb = sh[0];
     S[i].sw_tw = sh[1];
-    const int teams = kSweepWaves / sh[1];
+    const int teams = sweep_waves<T>() / sh[1];
     const int zb = sh[0], cs = zb % 2 == 0 ? zb + 1 : zb;  // col_stride<zb>()
-    const size_t PS = (size_t)S[i].g.NLy * S[i].g.NLz / zb * cs;
+    const size_t PS = ((size_t)S[i].g.NLy * S[i].g.NLz / zb + 1) * cs;  // plane_slots<zb>(P)
     lds = std::max(lds, (size_t)kSweepHead + (size_t)teams * (2 * PS + S[i].kb.Nlma) * sizeof(cplx<T>));
     for (int o = 0; o < S[i].NKB; o += teams)
       items.push_back({(long)S[i].g.NL * std::min(teams, S[i].NKB - o), (long)i, (long)o});
@@ -190,7 +190,7 @@ static int sweep_plan(salmon_batch* b, std::vector<Sys<T>>& S) {
     kern = (const void*)k_b_sweep<T, 4>;
   else
     kern = (const void*)k_b_sweep<T, 2>;
-  BHIP(b, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kern, kSweepThreads, lds));
+  BHIP(b, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kern, sweep_waves<T>() * kWave, lds));
   if (per < 1) {
     if (must) return bfail(b, SALMON_ERR_ARG, "the persistent sweep does not fit a compute unit");
     return SALMON_OK;
@@ -303,10 +303,10 @@ static int batch_enqueue_step(salmon_batch* b, double dt, long ac_stride) {
     }
     hipLaunchKernelGGL(k_b_qreset, dim3(1), dim3(64), 0, b->stream, b->d_next);
     if constexpr (sizeof(T) == 4)
-      hipLaunchKernelGGL((k_b_sweep<T, 4>), dim3(b->sweep_grid), dim3(kSweepThreads), b->sweep_lds, b->stream, S, st,
+      hipLaunchKernelGGL((k_b_sweep<T, 4>), dim3(b->sweep_grid), dim3(sweep_waves<T>() * kWave), b->sweep_lds, b->stream, S, st,
                          (const int2*)b->d_items, b->nitems, b->d_next);
     else
-      hipLaunchKernelGGL((k_b_sweep<T, 2>), dim3(b->sweep_grid), dim3(kSweepThreads), b->sweep_lds, b->stream, S, st,
+      hipLaunchKernelGGL((k_b_sweep<T, 2>), dim3(b->sweep_grid), dim3(sweep_waves<T>() * kWave), b->sweep_lds, b->stream, S, st,
                          (const int2*)b->d_items, b->nitems, b->d_next);
     hipLaunchKernelGGL(k_b_sum<T>, dim3(n), dim3(256), 0, b->stream, S, n, b->d_step, b->d_res);
     hipLaunchKernelGGL(k_b_tick, dim3(1), dim3(64), 0, b->stream, b->d_step);

@@ -1,6 +1,6 @@
 // Persistent batched step: one launch per RT step covers every system.
 //
-// A 512-thread workgroup pulls work items (system, first orbital) from a
+// A workgroup (256 threads for complex64, 512 for FP64) pulls work items (system, first orbital) from a
 // device counter, costliest first, and runs the whole step of its orbitals:
 // for each of the four Taylor terms the KB projection, one sweep over the
 // grid for the stencil and the local potential, and the KB term added to the
@@ -24,11 +24,17 @@
 namespace salmon {
 namespace batch {
 
-constexpr int kSweepThreads = 512;
-constexpr int kSweepWaves = kSweepThreads / kWave;
+// waves per workgroup: complex64 teams need at most 4 waves (two 256-thread
+// workgroups share a compute unit, each with its own barriers); FP64 columns
+// of at most 2 points need 8 for a 32^2 plane
+template <typename T>
+constexpr int sweep_waves() {
+  return sizeof(T) == 4 ? 4 : 8;
+}
+constexpr int kSweepMaxWaves = 8;
 constexpr int kColWin = 10;      // register window of a column: planes x-4 .. x+5
 constexpr int kRedStride = 10;   // doubles per wave in the reduction area
-constexpr int kSweepHead = 16 + kSweepWaves * kRedStride * 8;  // bytes before the team areas
+constexpr int kSweepHead = 16 + kSweepMaxWaves * kRedStride * 8;  // bytes before the team areas
 
 // Per-point operands of a sweep besides its input.
 template <typename T>
@@ -48,6 +54,52 @@ __host__ __device__ constexpr int col_stride() {
   return ZB % 2 == 0 ? ZB + 1 : ZB;
 }
 
+// LDS slots of one plane buffer: the columns, then one column that threads
+// without a column of their own write to
+template <int ZB>
+__host__ __device__ constexpr int plane_slots(int P) {
+  return (P / ZB + 1) * col_stride<ZB>();
+}
+
+typedef unsigned int u32x2_t __attribute__((ext_vector_type(2)));
+typedef unsigned int u32x4_t __attribute__((ext_vector_type(4)));
+
+// Buffer resource over nbytes at base (wave-uniform base): stores at offsets
+// at or past nbytes are dropped by the hardware, which lets threads without
+// work issue the same stores as the others.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t orbital_rsrc(const void* base, unsigned nbytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)nbytes, 0x00020000);
+}
+constexpr int kDropOffset = 0x7ffffff0;
+
+// ZB consecutive values at byte offset off of the resource (16-byte stores for
+// complex64 pairs and FP64 values, 8-byte stores otherwise)
+template <typename T, int ZB>
+__device__ __forceinline__ void store_run_rsrc(__amdgpu_buffer_rsrc_t rs, int off, const cplx<T> (&v)[ZB]) {
+  if constexpr (sizeof(T) == 4 && ZB % 2 == 0) {
+#pragma unroll
+    for (int h = 0; h < ZB / 2; ++h) {
+      const u32x4_t d = {__float_as_uint(v[2 * h].re), __float_as_uint(v[2 * h].im), __float_as_uint(v[2 * h + 1].re),
+                         __float_as_uint(v[2 * h + 1].im)};
+      __builtin_amdgcn_raw_buffer_store_b128(d, rs, off + 16 * h, 0, 0);
+    }
+  } else if constexpr (sizeof(T) == 4) {
+#pragma unroll
+    for (int q = 0; q < ZB; ++q) {
+      const u32x2_t d = {__float_as_uint(v[q].re), __float_as_uint(v[q].im)};
+      __builtin_amdgcn_raw_buffer_store_b64(d, rs, off + 8 * q, 0, 0);
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < ZB; ++q) {
+      const unsigned long long r = (unsigned long long)__double_as_longlong((double)v[q].re);
+      const unsigned long long i = (unsigned long long)__double_as_longlong((double)v[q].im);
+      const u32x4_t d = {(unsigned)r, (unsigned)(r >> 32), (unsigned)i, (unsigned)(i >> 32)};
+      __builtin_amdgcn_raw_buffer_store_b128(d, rs, off + 16 * q, 0, 0);
+    }
+  }
+}
+
 // One sweep of a team over its orbital.  MODE 0: out = (T + V) in (stencil
 // and local potential, no KB term); MODE 1: zu += z1 h0 + z2 h1 + z3 in +
 // z4 (T + V) in; MODE 2: observables of in, the per-thread sums
@@ -64,7 +116,7 @@ __device__ __forceinline__ void col_sweep(const Grid& g, bool act, int tl, cplx<
   const int iy = w0 / NLz, iz0 = w0 - iy * NLz;
   const int line = iy * NLz;
   constexpr int CS = col_stride<ZB>();
-  const int PS = (P / ZB) * CS;  // LDS slots of one plane
+  const int PS = plane_slots<ZB>(P);
   // LDS slot of in-plane offset w: column w / ZB, point w % ZB
   auto slot = [](int w) { return (w / ZB) * CS + w % ZB; };
   // LDS slots of the z window outside the column (z = iz0-4 .. iz0-1 and
@@ -80,7 +132,10 @@ __device__ __forceinline__ void col_sweep(const Grid& g, bool act, int tl, cplx<
     yo[n - 1] = slot(wrap_once(iy + n, NLy) * NLz + iz0);
     yo[3 + n] = slot(wrap_once(iy - n, NLy) * NLz + iz0);
   }
-  const int own = (w0 / ZB) * CS;
+  const int own = col ? (w0 / ZB) * CS : (P / ZB) * CS;  // threads without a column: the spare column
+  // the output: stores of threads without a column go past its end
+  const __amdgpu_buffer_rsrc_t ors =
+      orbital_rsrc(MODE < 2 ? (const void*)io.out : (const void*)io.in, (unsigned)(g.NL * sizeof(cplx<T>)));
   // coefficients per axis (x, y, z) and n: C = lap, D = k nab (MODE 2: nab)
   const double kk[3] = {kx, ky, kz};
   T C[3][5], D[3][5];
@@ -120,29 +175,25 @@ __device__ __forceinline__ void col_sweep(const Grid& g, bool act, int tl, cplx<
       load_run<T, ZB>(io.h1 + pb, a1[b]);
     }
   };
-  if (col) {
-    ld_ops(0, 0);
+  // every thread loads (threads without a column from a valid point of the
+  // orbital) so that no branch separates the loads from their uses
+  ld_ops(0, 0);
 #pragma unroll
-    for (int X = -4; X <= 4; ++X) ld(X, win[(X + kColWin) % kColWin]);
-  }
+  for (int X = -4; X <= 4; ++X) ld(X, win[(X + kColWin) % kColWin]);
   for (int x0 = 0; x0 < NLx; x0 += kColWin) {
 #pragma unroll
     for (int j = 0; j < kColWin; ++j) {
       const int x = x0 + j;
       if (x >= NLx) break;
       const size_t pb = (size_t)x * P + w0;
+      const int soff = col ? (int)(pb * sizeof(cplx<T>)) : kDropOffset;
       const int cb = j & 1;  // operand buffer of plane x (x0 is even)
-      if (col) {
-        ld_ops(x + 1 < NLx ? x + 1 : 0, cb ^ 1);
-        ld(x + 5, win[(j + 5) % kColWin]);
-      }
+      ld_ops(x + 1 < NLx ? x + 1 : 0, cb ^ 1);
+      ld(x + 5, win[(j + 5) % kColWin]);
       cplx<T>* pln = pl + cb * PS;
-      if (col) {
 #pragma unroll
-        for (int q = 0; q < ZB; ++q) pln[own + q] = win[j][q];
-      }
+      for (int q = 0; q < ZB; ++q) pln[own + q] = win[j][q];
       lds_barrier();
-      if (!col) continue;
       // z window: 4 values either side from LDS, the column itself from registers
       cplx<T> zw[ZB + 8];
 #pragma unroll
@@ -196,7 +247,7 @@ __device__ __forceinline__ void col_sweep(const Grid& g, bool act, int tl, cplx<
           f[q].im = (V[cb][q] + A) * e.im - (T)0.5 * vi[q] - wr[q];
         }
         if constexpr (MODE == 0) {
-          store_run<T, ZB>(io.out + pb, f);
+          store_run_rsrc<T, ZB>(ors, soff, f);
         } else {
           // fold in the order of k_b_fold: z1 h0, z2 h1, z3 h2, z4 h3
 #pragma unroll
@@ -213,7 +264,7 @@ __device__ __forceinline__ void col_sweep(const Grid& g, bool act, int tl, cplx<
             z.im += z4r * f[q].im;
             f[q] = z;
           }
-          store_run<T, ZB>(io.out + pb, f);
+          store_run_rsrc<T, ZB>(ors, soff, f);
         }
       } else {
 #pragma unroll
@@ -243,7 +294,8 @@ __device__ __forceinline__ void col_sweep(const Grid& g, bool act, int tl, cplx<
             Gr[0] += D[0][n] * (ep.re - em.re);
             Gi[0] += D[0][n] * (ep.im - em.im);
           }
-          const double er = (double)win[j][q].re, ei = (double)win[j][q].im;
+          // threads without a column add zeros
+          const double er = col ? (double)win[j][q].re : 0.0, ei = col ? (double)win[j][q].im : 0.0;
           const double a2 = er * er + ei * ei;
           acc[0] += a2;
           acc[1] += V64[cb][q] * a2;
@@ -387,7 +439,7 @@ __device__ __forceinline__ void sweep_item(const Sys<T>& y, const Step& st, int 
   const int lane = (int)threadIdx.x & (kWave - 1);
   const int o = o0 + team;
   const bool act = o < y.NKB;
-  const int PS = (g.NLy * g.NLz / ZB) * col_stride<ZB>();  // LDS slots of one plane
+  const int PS = plane_slots<ZB>(g.NLy * g.NLz);
   const int nlma = kb.Nlma;
   double* red = (double*)(lds + 16);
   cplx<T>* pl = (cplx<T>*)(lds + kSweepHead) + (size_t)team * (2 * PS + nlma);
@@ -465,7 +517,7 @@ __device__ __forceinline__ void sweep_item(const Sys<T>& y, const Step& st, int 
 
 // Work items: (system, first orbital), costliest first; *next is zero at launch.
 template <typename T, int ZBMAX>
-__global__ void __launch_bounds__(kSweepThreads) k_b_sweep(const Sys<T>* __restrict__ S, Step st,
+__global__ void __launch_bounds__(sweep_waves<T>() * kWave) __attribute__((amdgpu_waves_per_eu(2))) k_b_sweep(const Sys<T>* __restrict__ S, Step st,
                                                            const int2* __restrict__ items, int nitems,
                                                            int* __restrict__ next) {
   extern __shared__ __align__(16) unsigned char lds[];
