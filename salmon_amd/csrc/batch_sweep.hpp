@@ -310,8 +310,11 @@ __device__ __forceinline__ void col_sweep(const Grid& g, bool act, int tl, cplx<
 
 // KB parts of one orbital, one wave per atom (lanes over the atom's sphere
 // points, every orbital value gathered once per atom); kKbPass projectors of
-// the atom per pass over its sphere.
+// the atom per pass over its sphere, kKbChunks 64-point chunks of the sphere
+// with their loads in flight together (indices past the sphere clamped to its
+// first point and weighted by zero, so no branch separates loads from uses).
 constexpr int kKbPass = 4;
+constexpr int kKbChunks = 3;
 
 // Projections of atom a (pseudo_pt phase 1, ARTED/common/hpsi.f90:183-193):
 // U(il) = Hxyz iuV(il) sum_j uV(j,il) e^{i k.r_j} E(Jxyz(j)), phases ph.
@@ -324,15 +327,25 @@ __device__ __forceinline__ void kb_project_atom(const KBTables& kb, double hxyz,
     T v[2 * kKbPass];
 #pragma unroll
     for (int k = 0; k < 2 * kKbPass; ++k) v[k] = 0;
-    for (int j = lane; j < m; j += kWave) {
-      const int jp = j0 + j;
-      const cplx<T> e = E[kb.pidx[jp]];
-      const cplx<T> p = ph[jp];
-      const T wr = p.re * e.re - p.im * e.im, wi = p.re * e.im + p.im * e.re;
+    for (int jb = 0; jb < m; jb += kKbChunks * kWave) {
+      int jj[kKbChunks];
+      bool ok[kKbChunks];
+      cplx<T> e[kKbChunks], p[kKbChunks];
 #pragma unroll
-      for (int l = 0; l < kKbPass; ++l) {
-        if (lc + l < l1) {
-          const T u = (T)kb.uv[(size_t)(lc + l) * kb.Nps + j];
+      for (int h = 0; h < kKbChunks; ++h) {
+        const int j = jb + h * kWave + lane;
+        ok[h] = j < m;
+        jj[h] = ok[h] ? j : 0;
+        e[h] = E[kb.pidx[j0 + jj[h]]];
+        p[h] = ph[j0 + jj[h]];
+      }
+#pragma unroll
+      for (int h = 0; h < kKbChunks; ++h) {
+        const T wr = p[h].re * e[h].re - p[h].im * e[h].im, wi = p[h].re * e[h].im + p[h].im * e[h].re;
+#pragma unroll
+        for (int l = 0; l < kKbPass; ++l) {
+          const T u0 = (T)kb.uv[(size_t)min(lc + l, kb.Nlma - 1) * kb.Nps + jj[h]];
+          const T u = (ok[h] && lc + l < l1) ? u0 : (T)0;
           v[2 * l] += u * wr;
           v[2 * l + 1] += u * wi;
         }
@@ -348,28 +361,42 @@ __device__ __forceinline__ void kb_project_atom(const KBTables& kb, double hxyz,
 }
 
 // dst(Jxyz(j)) += fac sum_il uV(j,il) e^{-i k.r_j} U(il) over atom a's sphere
-// (pseudo_pt phase 2, ARTED/common/hpsi.f90:195-206).  Atoms whose spheres
-// share a grid point must not run at the same time.
+// (pseudo_pt phase 2, ARTED/common/hpsi.f90:195-206); rs is dst's buffer
+// resource (stores past the sphere are dropped).  Atoms whose spheres share
+// a grid point must not run at the same time.
 template <typename T>
 __device__ __forceinline__ void kb_scatter_atom(const KBTables& kb, int a, const cplx<T>* __restrict__ ph,
-                                                const cplx<T>* U, cplx<T>* __restrict__ dst, T fac, int lane) {
+                                                const cplx<T>* U, const cplx<T>* dst, __amdgpu_buffer_rsrc_t rs, T fac,
+                                                int lane) {
   const int j0 = kb.atom_start[a], m = kb.atom_start[a + 1] - j0;
   const int l0 = kb.atom_lma[a], l1 = kb.atom_lma[a + 1];
-  for (int j = lane; j < m; j += kWave) {
-    const int jp = j0 + j;
-    const cplx<T> p = ph[jp];
-    T fr = 0, fi = 0;
-    for (int il = l0; il < l1; ++il) {
-      const T u = (T)kb.uv[(size_t)il * kb.Nps + j];
-      const cplx<T> w = U[il];
-      fr += u * (p.re * w.re + p.im * w.im);
-      fi += u * (p.re * w.im - p.im * w.re);
+  for (int jb = 0; jb < m; jb += kKbChunks * kWave) {
+    int jj[kKbChunks], pt[kKbChunks];
+    bool ok[kKbChunks];
+    cplx<T> p[kKbChunks], d[kKbChunks];
+#pragma unroll
+    for (int h = 0; h < kKbChunks; ++h) {
+      const int j = jb + h * kWave + lane;
+      ok[h] = j < m;
+      jj[h] = ok[h] ? j : 0;
+      p[h] = ph[j0 + jj[h]];
+      pt[h] = kb.pidx[j0 + jj[h]];
+      d[h] = dst[pt[h]];
     }
-    cplx<T>* d = dst + kb.pidx[jp];
-    cplx<T> v = *d;
-    v.re += fac * fr;
-    v.im += fac * fi;
-    *d = v;
+#pragma unroll
+    for (int h = 0; h < kKbChunks; ++h) {
+      T fr = 0, fi = 0;
+      for (int il = l0; il < l1; ++il) {
+        const T u = (T)kb.uv[(size_t)il * kb.Nps + jj[h]];
+        const cplx<T> w = U[il];
+        fr += u * (p[h].re * w.re + p[h].im * w.im);
+        fi += u * (p[h].re * w.im - p[h].im * w.re);
+      }
+      cplx<T> r[1] = {d[h]};
+      r[0].re += fac * fr;
+      r[0].im += fac * fi;
+      store_run_rsrc<T, 1>(rs, ok[h] ? pt[h] * (int)sizeof(cplx<T>) : kDropOffset, r);
+    }
   }
 }
 
@@ -386,16 +413,28 @@ __device__ __forceinline__ void kb_observe_atom(const KBTables& kb, const Grid& 
     double v[8 * kKbPass];
 #pragma unroll
     for (int k = 0; k < 8 * kKbPass; ++k) v[k] = 0;
-    for (int j = lane; j < m; j += kWave) {
-      const int jp = j0 + j;
-      const double px = kb.pos[3 * jp], py = kb.pos[3 * jp + 1], pz = kb.pos[3 * jp + 2];
-      const double c = (double)ph[jp].re, sn = (double)ph[jp].im;
-      const cplx<T> e = E[kb.pidx[jp]];
-      const double tr0 = c * (double)e.re - sn * (double)e.im, ti0 = c * (double)e.im + sn * (double)e.re;
+    for (int jb = 0; jb < m; jb += kKbChunks * kWave) {
+      int jj[kKbChunks];
+      bool ok[kKbChunks];
+      cplx<T> e[kKbChunks];
 #pragma unroll
-      for (int l = 0; l < kKbPass; ++l) {
-        if (lc + l < l1) {
-          const double u = kb.uv[(size_t)(lc + l) * kb.Nps + j];
+      for (int h = 0; h < kKbChunks; ++h) {
+        const int j = jb + h * kWave + lane;
+        ok[h] = j < m;
+        jj[h] = ok[h] ? j : 0;
+        e[h] = E[kb.pidx[j0 + jj[h]]];
+      }
+#pragma unroll
+      for (int h = 0; h < kKbChunks; ++h) {
+        const int jp = j0 + jj[h];
+        const double px = kb.pos[3 * jp], py = kb.pos[3 * jp + 1], pz = kb.pos[3 * jp + 2];
+        const double c = (double)ph[jp].re, sn = (double)ph[jp].im;
+        const double tr0 = c * (double)e[h].re - sn * (double)e[h].im;
+        const double ti0 = c * (double)e[h].im + sn * (double)e[h].re;
+#pragma unroll
+        for (int l = 0; l < kKbPass; ++l) {
+          const double u0 = kb.uv[(size_t)min(lc + l, kb.Nlma - 1) * kb.Nps + jj[h]];
+          const double u = (ok[h] && lc + l < l1) ? u0 : 0.0;
           const double tr = u * tr0, ti = u * ti0;
           double* q = v + 8 * l;
           q[0] += tr;
@@ -467,10 +506,11 @@ __device__ __forceinline__ void sweep_item(const Sys<T>& y, const Step& st, int 
     // KB term: the atoms colour by colour (no two atoms of a colour share a
     // grid point), the atoms of a colour over the team's waves
     const T fac = term < 3 ? (T)1 : (T)(st.dt * st.dt * st.dt * st.dt / 24.0);
+    const __amdgpu_buffer_rsrc_t rs = orbital_rsrc(io.out, (unsigned)(NL * sizeof(cplx<T>)));
     for (int c = 0; c < y.ncolor; ++c) {
       if (act)
         for (int k = y.kb_cstart[c] + wt; k < y.kb_cstart[c + 1]; k += tw)
-          kb_scatter_atom<T>(kb, y.kb_order[k], phk, U, io.out, fac, lane);
+          kb_scatter_atom<T>(kb, y.kb_order[k], phk, U, io.out, rs, fac, lane);
       __syncthreads();
     }
   }
