@@ -40,6 +40,8 @@ struct FusedArgs {
   int obs;               // 1: observables of the input orbitals at k0 + an
   int rho_post;          // 1: the density of psi(t+dt), summed where it is stored
                          // (the observables sweep then leaves the density alone)
+  int rho;               // 1: the observables sweep sums the density of its orbitals
+                         // (a frozen-potential run needs only its last step's)
   double am[3], an[3];
   T z1i, z2r, z3i, z4r;
   T lapc[3][5];          // -lap(n)/2 per axis (propagation) -- uniform, kept in SGPRs
@@ -313,8 +315,10 @@ __global__ void __launch_bounds__(NY * NZ * XS, WPE) k_fused_step(FusedArgs<T> a
   // slice is updated without atomics
   double* rho = a.rho_part + (size_t)blockIdx.x * NL;
   double* rho_acc = LT ? rhol : rho;
+  if (LT || RP || a.rho) {
 #pragma unroll
-  for (int i = 0; i < PX; ++i) rho_acc[(x0 + i) * NYZ + yz] = 0.0;
+    for (int i = 0; i < PX; ++i) rho_acc[(x0 + i) * NYZ + yz] = 0.0;
+  }
   __syncthreads();
   if constexpr (LT) {
     for (int q = t; q < a.NPI; q += NT) {
@@ -796,7 +800,7 @@ __global__ void __launch_bounds__(NY * NZ * XS, WPE) k_fused_step(FusedArgs<T> a
       const cplx<T>* yb[8];
       nb_bases<T, NY, NZ>(out, x0, iy, iz, zb, yb);
       const double* vrow = opaque(a.vloc64) + x0 * NYZ + yz;
-      double* rrow = rho + x0 * NYZ + yz;
+      const __amdgpu_buffer_rsrc_t rrs = __builtin_amdgcn_make_buffer_rsrc(rho, (short)0, NL * (int)sizeof(double), 0x00020000);
 #pragma unroll
       for (int i = 0; i < PX; ++i) {
         T lr = 0, li = 0;
@@ -806,7 +810,16 @@ __global__ void __launch_bounds__(NY * NZ * XS, WPE) k_fused_step(FusedArgs<T> a
         const double a2 = er * er + ei * ei;
         n2 += a2;
         vs += vrow[i * NYZ] * a2;
-        if constexpr (!RP) rrow[i * NYZ] += occ * a2;
+        if constexpr (!RP) {
+          // through the buffer resource: with a.rho = 0 the offset is past the
+          // slice, so the load returns 0 and the store is dropped, no memory access
+          const int off = a.rho ? (int)(((x0 + i) * NYZ + yz) * sizeof(double)) : 0x7ffffff0;
+          const auto ld = __builtin_amdgcn_raw_buffer_load_b64(rrs, off, 0, 0);
+          const double v = __longlong_as_double(((long long)ld[1] << 32) | ld[0]) + occ * a2;
+          const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+          typedef unsigned int u2 __attribute__((ext_vector_type(2)));
+          __builtin_amdgcn_raw_buffer_store_b64(u2{(unsigned)b, (unsigned)(b >> 32)}, rrs, off, 0, 0);
+        }
         te += er * (double)lr + ei * (double)li;
 #pragma unroll
         for (int d = 0; d < 3; ++d) js[d] += er * (double)gi[d] - ei * (double)gr[d];
@@ -880,8 +893,10 @@ __global__ void __launch_bounds__(NY * NZ * XS, WPE) k_fused_step(FusedArgs<T> a
     }
   }
   if constexpr (LT) {
+    if (RP || a.rho) {
 #pragma unroll
-    for (int i = 0; i < PX; ++i) rho[(x0 + i) * NYZ + yz] = rhol[(x0 + i) * NYZ + yz];
+      for (int i = 0; i < PX; ++i) rho[(x0 + i) * NYZ + yz] = rhol[(x0 + i) * NYZ + yz];
+    }
   }
   // per-workgroup sums of the per-wave accumulators, in wave order
   __syncthreads();
@@ -1115,10 +1130,11 @@ static int fused_launch(salmon_ctx* c, const salmon::FusedArgs<T>& args, int nwg
 // the entry orbitals for an FP64 launch without amid (no density then).
 // Either may be null (not both).
 // rho_post (with amid): the density of psi(t+dt) into d_rho instead of that
-// of the entry orbitals.
+// of the entry orbitals.  want_rho false: the observables leave d_rho as it is
+// (a frozen-potential run sums the density of its last step only).
 template <typename T>
 static int fused_step(salmon_ctx* c, const double* amid, const double* aobs, double dt, double* res_dev,
-                      bool rho_post = false) {
+                      bool rho_post = false, bool want_rho = true) {
   FusedState* f = (FusedState*)c->fz;
   if (!f || !f->tables) return fail(c, SALMON_ERR_UNSUPPORTED, "fused tables not built");
   int ncu = 256;
@@ -1158,6 +1174,7 @@ static int fused_step(salmon_ctx* c, const double* amid, const double* aobs, dou
   a.prop = amid != nullptr;
   a.obs = aobs != nullptr;
   a.rho_post = rho_post && amid != nullptr;
+  a.rho = want_rho;
   for (int d = 0; d < 3; ++d) {
     a.am[d] = amid ? amid[d] : 0.0;
     a.an[d] = aobs ? aobs[d] : 0.0;
@@ -1195,5 +1212,6 @@ static int fused_step(salmon_ctx* c, const double* amid, const double* aobs, dou
   }
   // the FP64 observables-only launch sums no density (d_rho keeps rho(t+dt))
   if (sizeof(T) == 8 && !a.prop) return SALMON_OK;
+  if (!a.rho && !a.rho_post) return SALMON_OK;
   return density_reduce(c, f->rho_part, nwg);
 }

@@ -437,7 +437,9 @@ int rt_step(salmon_ctx* c, const double a0[3], const double a1[3], double dt, do
 // n+1 during its first H application, which reads psi(t_{n+1}) anyway; an
 // observables-only launch closes the run, and one runs before each
 // re-orthonormalisation, so every result is of the same orbitals as in the
-// step-by-step order (step, observables, re-orthonormalisation).
+// step-by-step order (step, observables, re-orthonormalisation).  With the
+// potential frozen only the last step's density is kept (the run's rho
+// output), so only the launch with the last step's observables sums it.
 template <typename T>
 int rt_run_fused(salmon_ctx* c, const double* ac, int nsteps, double dt, int reortho_every, double* d_res) {
   int r = SALMON_OK;
@@ -456,10 +458,10 @@ int rt_run_fused(salmon_ctx* c, const double* ac, int nsteps, double dt, int reo
       for (int d = 0; d < 3; ++d) anew[d] = ac[3 * (n + 1) + d];
       if (etrs) {
         kpoint(n, 0, a);
-        if ((r = fused_step<T>(c, a, nullptr, dts, nullptr))) break;
+        if ((r = fused_step<T>(c, a, nullptr, dts, nullptr, false, false))) break;
       }
       kpoint(n, 1, a);
-      r = fused_step<T>(c, a, anew, dts, d_res + 8 * (size_t)n);
+      r = fused_step<T>(c, a, anew, dts, d_res + 8 * (size_t)n, false, n == nsteps - 1);
       if (r == SALMON_OK && reortho_every > 0 && (n + 1) % reortho_every == 0) r = gram_schmidt_dev(c);
     }
     return r;
@@ -469,20 +471,21 @@ int rt_run_fused(salmon_ctx* c, const double* ac, int nsteps, double dt, int reo
   for (int n = 0; n < nsteps && r == SALMON_OK; ++n) {
     double a[3];
     kpoint(n, 0, a);
-    r = fused_step<T>(c, a, pending ? aobs : nullptr, dts, pending ? d_res + 8 * (size_t)(n - 1) : nullptr);
+    r = fused_step<T>(c, a, pending ? aobs : nullptr, dts, pending ? d_res + 8 * (size_t)(n - 1) : nullptr, false,
+                      false);
     if (r == SALMON_OK && etrs) {
       kpoint(n, 1, a);
-      r = fused_step<T>(c, a, nullptr, dts, nullptr);
+      r = fused_step<T>(c, a, nullptr, dts, nullptr, false, false);
     }
     for (int d = 0; d < 3; ++d) aobs[d] = ac[3 * (n + 1) + d];
     pending = true;
     if (r == SALMON_OK && reortho_every > 0 && (n + 1) % reortho_every == 0) {
-      r = fused_step<T>(c, nullptr, aobs, dt, d_res + 8 * (size_t)n);
+      r = fused_step<T>(c, nullptr, aobs, dt, d_res + 8 * (size_t)n, false, n == nsteps - 1);
       pending = false;
       if (r == SALMON_OK) r = gram_schmidt_dev(c);
     }
   }
-  if (r == SALMON_OK && pending) r = fused_step<T>(c, nullptr, aobs, dt, d_res + 8 * (size_t)(nsteps - 1));
+  if (r == SALMON_OK && pending) r = fused_step<T>(c, nullptr, aobs, dt, d_res + 8 * (size_t)(nsteps - 1), false, true);
   return r;
 }
 
