@@ -292,6 +292,17 @@ module salmon_gpu
       type(c_ptr), value :: batch, ctx
     end function
 
+    integer(c_int) function salmon_gpu_batch_set_kernel(batch, kernel) bind(C, name="salmon_gpu_batch_set_kernel")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: batch
+      integer(c_int), value :: kernel
+    end function
+
+    integer(c_int) function salmon_gpu_batch_kernel(batch) bind(C, name="salmon_gpu_batch_kernel")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: batch
+    end function
+
     integer(c_int) function salmon_gpu_batch_tune(batch) bind(C, name="salmon_gpu_batch_tune")
       import :: c_int, c_ptr
       type(c_ptr), value :: batch
