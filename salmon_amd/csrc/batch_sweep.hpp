@@ -400,6 +400,67 @@ __device__ __forceinline__ void kb_scatter_atom(const KBTables& kb, int a, const
   }
 }
 
+// kb_scatter_atom for an atom whose sphere no other atom touches and which has
+// at most kKbPass projectors: the updated values are then final, so the
+// projections of the next Taylor term (whose input dst is) are formed from
+// them in the same pass, as kb_project_atom forms them, and replace the
+// atom's U(il).
+template <typename T>
+__device__ __forceinline__ void kb_scatter_project_atom(const KBTables& kb, double hxyz, int a,
+                                                        const cplx<T>* __restrict__ ph, cplx<T>* U, const cplx<T>* dst,
+                                                        __amdgpu_buffer_rsrc_t rs, T fac, int lane) {
+  const int j0 = kb.atom_start[a], m = kb.atom_start[a + 1] - j0;
+  const int l0 = kb.atom_lma[a], l1 = kb.atom_lma[a + 1];
+  T v[2 * kKbPass];
+#pragma unroll
+  for (int k = 0; k < 2 * kKbPass; ++k) v[k] = 0;
+  for (int jb = 0; jb < m; jb += kKbChunks * kWave) {
+    int jj[kKbChunks], pt[kKbChunks];
+    bool ok[kKbChunks];
+    cplx<T> p[kKbChunks], d[kKbChunks];
+#pragma unroll
+    for (int h = 0; h < kKbChunks; ++h) {
+      const int j = jb + h * kWave + lane;
+      ok[h] = j < m;
+      jj[h] = ok[h] ? j : 0;
+      p[h] = ph[j0 + jj[h]];
+      pt[h] = kb.pidx[j0 + jj[h]];
+      d[h] = dst[pt[h]];
+    }
+#pragma unroll
+    for (int h = 0; h < kKbChunks; ++h) {
+      T u[kKbPass];
+      T fr = 0, fi = 0;
+#pragma unroll
+      for (int l = 0; l < kKbPass; ++l) {
+        const int il = min(l0 + l, kb.Nlma - 1);
+        const T u0 = (T)kb.uv[(size_t)il * kb.Nps + jj[h]];
+        u[l] = l0 + l < l1 ? u0 : (T)0;
+        const cplx<T> w = U[il];
+        fr += u[l] * (p[h].re * w.re + p[h].im * w.im);
+        fi += u[l] * (p[h].re * w.im - p[h].im * w.re);
+      }
+      cplx<T> r[1] = {d[h]};
+      r[0].re += fac * fr;
+      r[0].im += fac * fi;
+      store_run_rsrc<T, 1>(rs, ok[h] ? pt[h] * (int)sizeof(cplx<T>) : kDropOffset, r);
+      const T wr = p[h].re * r[0].re - p[h].im * r[0].im, wi = p[h].re * r[0].im + p[h].im * r[0].re;
+#pragma unroll
+      for (int l = 0; l < kKbPass; ++l) {
+        const T um = ok[h] ? u[l] : (T)0;
+        v[2 * l] += um * wr;
+        v[2 * l + 1] += um * wi;
+      }
+    }
+  }
+  wave_sums<2 * kKbPass>(v);
+#pragma unroll
+  for (int k = 0; k < 2 * kKbPass; ++k) {
+    const int il = l0 + k / 2;
+    if (lane == wave_sums_lane(k) && il < l1) ((T*)U)[2 * il + (k & 1)] = v[0] * (T)(hxyz * kb.lma_sign[il]);
+  }
+}
+
 // Nonlocal observables of atom a at the phases ph (current.f90:172-273 and
 // total_energy.f90:191-199, as in observe_orbital): adds Jx, Jy, Jz, Enl.
 template <typename T>
@@ -492,9 +553,13 @@ __device__ __forceinline__ void sweep_item(const Sys<T>& y, const Step& st, int 
   orbital_k(y.kac0, y.NK, ik, am[0], am[1], am[2], kx, ky, kz);
   const cplx<T>* phk = y.ph + (size_t)ik * kb.NPI;
   double acc[6] = {0, 0, 0, 0, 0, 0};
+  // no two atom spheres share a grid point and no atom has more than kKbPass
+  // projectors: the back-projection of term n forms the projections of term
+  // n+1 (kb_scatter_project_atom), so only term 0 projects on its own
+  const bool fuse = y.ncolor == 1 && kb.max_lpa <= kKbPass;
   for (int term = 0; term < 4; ++term) {
     const cplx<T>* in = (term == 0 ? y.zu : y.h[term - 1]) + ob;
-    if (act)
+    if (act && (term == 0 || !fuse))
       for (int a = wt; a < kb.NI; a += tw) kb_project_atom<T>(kb, g.Hxyz, a, in, phk, U, lane);
     __syncthreads();
     SweepIO<T> io{in, (term < 3 ? y.h[term] : y.zu) + ob, y.zu + ob, y.h[0] + ob, y.h[1] + ob};
@@ -508,9 +573,15 @@ __device__ __forceinline__ void sweep_item(const Sys<T>& y, const Step& st, int 
     const T fac = term < 3 ? (T)1 : (T)(st.dt * st.dt * st.dt * st.dt / 24.0);
     const __amdgpu_buffer_rsrc_t rs = orbital_rsrc(io.out, (unsigned)(NL * sizeof(cplx<T>)));
     for (int c = 0; c < y.ncolor; ++c) {
-      if (act)
-        for (int k = y.kb_cstart[c] + wt; k < y.kb_cstart[c + 1]; k += tw)
-          kb_scatter_atom<T>(kb, y.kb_order[k], phk, U, io.out, rs, fac, lane);
+      if (act) {
+        if (fuse && term < 3) {
+          for (int k = y.kb_cstart[c] + wt; k < y.kb_cstart[c + 1]; k += tw)
+            kb_scatter_project_atom<T>(kb, g.Hxyz, y.kb_order[k], phk, U, io.out, rs, fac, lane);
+        } else {
+          for (int k = y.kb_cstart[c] + wt; k < y.kb_cstart[c + 1]; k += tw)
+            kb_scatter_atom<T>(kb, y.kb_order[k], phk, U, io.out, rs, fac, lane);
+        }
+      }
       __syncthreads();
     }
   }
