@@ -48,6 +48,7 @@ struct Step {
   double dt;
   const unsigned short* owner[kPhases];  // system of every block, per phase
   long ac_stride;    // doubles between the A(t) tables of consecutive systems (0: one shared table)
+  int obs_only;      // persistent sweep: the launch that closes a run (observables only)
 };
 
 __device__ __forceinline__ int find_sys(const Step& st, int phase, long b) { return st.owner[phase][b]; }
@@ -84,23 +85,30 @@ __global__ void __launch_bounds__(256) k_b_project(const Sys<T>* __restrict__ S,
                      y.ph + (size_t)(o / y.NB) * y.kb.NPI);
 }
 
-// The KB phases of every system's k points (one grid row per system) at this
-// step's midpoint k, shared by the step's four projections and stencils, or
-// (at_new, into phn) at k + A(t+dt) for the observables.
+// The KB phases of every system's k points (one grid row per system) at k +
+// A of the step n the device counter holds: which = 0, the midpoint
+// (A(t_n) + A(t_{n+1}))/2, into ph, shared by the step's four projections and
+// stencils; 1, A(t_{n+1}), into phn, for the observables after the step (one
+// launch per phase); 2, A(t_n), into phn, for the observables of the
+// orbitals entering step n (persistent sweep; at the end of a run n = nsteps,
+// the last row of the A table).
 template <typename T>
-__global__ void __launch_bounds__(256) k_b_phase(const Sys<T>* __restrict__ S, Step st, int at_new) {
+__global__ void __launch_bounds__(256) k_b_phase(const Sys<T>* __restrict__ S, Step st, int which) {
   const Sys<T>& y = S[blockIdx.y];
   const long i = (long)blockIdx.x * 256 + threadIdx.x;
   if (i >= (long)y.NK * y.kb.NPI) return;
   const int ik = (int)(i / y.kb.NPI), jp = (int)(i - (long)ik * y.kb.NPI);
-  double am[3], an[3];
-  step_fields(st, blockIdx.y, am, an);
+  const int n = *st.step;
+  const double* ac = st.ac + (size_t)blockIdx.y * st.ac_stride;
+  double a[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d)
+    a[d] = which == 0 ? 0.5 * (ac[3 * n + d] + ac[3 * (n + 1) + d]) : ac[3 * (n + (which == 1 ? 1 : 0)) + d];
   double kx, ky, kz;
-  const double* a = at_new ? an : am;
   orbital_k(y.kac0, y.NK, ik, a[0], a[1], a[2], kx, ky, kz);
   double s, c;
   sincos(kx * y.kb.pos[3 * jp] + ky * y.kb.pos[3 * jp + 1] + kz * y.kb.pos[3 * jp + 2], &s, &c);
-  (at_new ? y.phn : y.ph)[i] = cplx<T>{(T)c, (T)s};
+  (which ? y.phn : y.ph)[i] = cplx<T>{(T)c, (T)s};
 }
 
 // KB term of every touched grid point of every orbital (pseudo_pt phase 2,
@@ -300,10 +308,14 @@ __global__ void __launch_bounds__(256) k_b_observe(const Sys<T>* __restrict__ S,
                      y.phn + (size_t)(o / y.NB) * y.kb.NPI);
 }
 
-// Per-system sums of the rows (one block per system) into res(step, s, 0..7).
+// Per-system sums of the rows (one block per system) into res(n - lag, s, 0..7),
+// n the device step counter (lag 1: the rows hold the previous step's
+// observables; nothing before the first step).
 template <typename T>
-__global__ void __launch_bounds__(256) k_b_sum(const Sys<T>* __restrict__ S, int nsys, int* step, double* res) {
+__global__ void __launch_bounds__(256) k_b_sum(const Sys<T>* __restrict__ S, int nsys, int* step, double* res,
+                                               int lag) {
   __shared__ double smem[4 * 8];
+  if (*step - lag < 0) return;
   const Sys<T>& y = S[blockIdx.x];
   double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   for (int r = threadIdx.x; r < y.NKB; r += 256) {
@@ -312,7 +324,7 @@ __global__ void __launch_bounds__(256) k_b_sum(const Sys<T>* __restrict__ S, int
   }
   block_sum256<8>(v, smem);
   if (threadIdx.x == 0) {
-    const int n = *step;
+    const int n = *step - lag;
 #pragma unroll
     for (int i = 0; i < 8; ++i) res[((size_t)n * nsys + blockIdx.x) * 8 + i] = v[i];
   }

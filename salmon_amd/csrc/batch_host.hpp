@@ -297,9 +297,11 @@ static int batch_enqueue_step(salmon_batch* b, double dt, long ac_stride) {
   const Step st{b->d_ac, b->d_step, dt, {b->d_owner[0], b->d_owner[1], b->d_owner[2], b->d_owner[3], b->d_owner[4]},
                 ac_stride};
   if (b->sweep) {
+    // the step's launch also forms the previous step's observables (at
+    // A(t_n)), whose rows the sum then takes
     if (b->max_ph > 0) {
       hipLaunchKernelGGL(k_b_phase<T>, dim3((unsigned)((b->max_ph + 255) / 256), n), dim3(256), 0, b->stream, S, st, 0);
-      hipLaunchKernelGGL(k_b_phase<T>, dim3((unsigned)((b->max_ph + 255) / 256), n), dim3(256), 0, b->stream, S, st, 1);
+      hipLaunchKernelGGL(k_b_phase<T>, dim3((unsigned)((b->max_ph + 255) / 256), n), dim3(256), 0, b->stream, S, st, 2);
     }
     hipLaunchKernelGGL(k_b_qreset, dim3(1), dim3(64), 0, b->stream, b->d_next);
     if constexpr (sizeof(T) == 4)
@@ -308,7 +310,7 @@ static int batch_enqueue_step(salmon_batch* b, double dt, long ac_stride) {
     else
       hipLaunchKernelGGL((k_b_sweep<T, 2>), dim3(b->sweep_grid), dim3(sweep_waves<T>() * kWave), b->sweep_lds, b->stream, S, st,
                          (const int2*)b->d_items, b->nitems, b->d_next);
-    hipLaunchKernelGGL(k_b_sum<T>, dim3(n), dim3(256), 0, b->stream, S, n, b->d_step, b->d_res);
+    hipLaunchKernelGGL(k_b_sum<T>, dim3(n), dim3(256), 0, b->stream, S, n, b->d_step, b->d_res, 1);
     hipLaunchKernelGGL(k_b_tick, dim3(1), dim3(64), 0, b->stream, b->d_step);
     BHIP(b, hipGetLastError());
     return SALMON_OK;
@@ -329,8 +331,31 @@ static int batch_enqueue_step(salmon_batch* b, double dt, long ac_stride) {
   if (b->max_ph > 0)
     hipLaunchKernelGGL(k_b_phase<T>, dim3((unsigned)((b->max_ph + 255) / 256), n), dim3(256), 0, b->stream, S, st, 1);
   hipLaunchKernelGGL(k_b_observe<T>, dim3((unsigned)b->first[3][0]), dim3(256), 0, b->stream, S, n, st);
-  hipLaunchKernelGGL(k_b_sum<T>, dim3(n), dim3(256), 0, b->stream, S, n, b->d_step, b->d_res);
+  hipLaunchKernelGGL(k_b_sum<T>, dim3(n), dim3(256), 0, b->stream, S, n, b->d_step, b->d_res, 0);
   hipLaunchKernelGGL(k_b_tick, dim3(1), dim3(64), 0, b->stream, b->d_step);
+  BHIP(b, hipGetLastError());
+  return SALMON_OK;
+}
+
+// After the last step of a run on the persistent sweep: the observables of
+// the final orbitals (at A(t_nsteps), the device counter then holding nsteps)
+// into the run's last result row.
+template <typename T>
+static int batch_close_sweep(salmon_batch* b, double dt, long ac_stride) {
+  const int n = (int)b->ctx.size();
+  const Sys<T>* S = (const Sys<T>*)b->d_sys;
+  const Step st{b->d_ac, b->d_step, dt, {b->d_owner[0], b->d_owner[1], b->d_owner[2], b->d_owner[3], b->d_owner[4]},
+                ac_stride, 1};
+  if (b->max_ph > 0)
+    hipLaunchKernelGGL(k_b_phase<T>, dim3((unsigned)((b->max_ph + 255) / 256), n), dim3(256), 0, b->stream, S, st, 2);
+  hipLaunchKernelGGL(k_b_qreset, dim3(1), dim3(64), 0, b->stream, b->d_next);
+  if constexpr (sizeof(T) == 4)
+    hipLaunchKernelGGL((k_b_sweep<T, 4>), dim3(b->sweep_grid), dim3(sweep_waves<T>() * kWave), b->sweep_lds, b->stream,
+                       S, st, (const int2*)b->d_items, b->nitems, b->d_next);
+  else
+    hipLaunchKernelGGL((k_b_sweep<T, 2>), dim3(b->sweep_grid), dim3(sweep_waves<T>() * kWave), b->sweep_lds, b->stream,
+                       S, st, (const int2*)b->d_items, b->nitems, b->d_next);
+  hipLaunchKernelGGL(k_b_sum<T>, dim3(n), dim3(256), 0, b->stream, S, n, b->d_step, b->d_res, 1);
   BHIP(b, hipGetLastError());
   return SALMON_OK;
 }
@@ -483,6 +508,7 @@ static int batch_run(salmon_batch* b, const double* ac, int nsteps, double dt, d
     b->graph_stride = stride;
   }
   for (int s = 0; s < nsteps; ++s) BHIP(b, hipGraphLaunch(b->exec, b->stream));
+  if (b->sweep && (r = batch_close_sweep<T>(b, dt, stride))) return r;
   std::vector<double> res(8 * (size_t)n * nsteps);
   BHIP(b, hipMemcpyAsync(res.data(), b->d_res, sizeof(double) * res.size(), hipMemcpyDeviceToHost, b->stream));
   BHIP(b, hipStreamSynchronize(b->stream));

@@ -104,12 +104,16 @@ __device__ __forceinline__ void store_run_rsrc(__amdgpu_buffer_rsrc_t rs, int of
 // and local potential, no KB term); MODE 1: zu += z1 h0 + z2 h1 + z3 in +
 // z4 (T + V) in; MODE 2: observables of in, the per-thread sums
 // acc = {|psi|^2, V|psi|^2, Re conj(psi) L, Im conj(psi) G_x, _y, _z}
-// (stencil sums L and G as in observe_orbital, streamed.hpp).
+// (stencil sums L and G as in observe_orbital, streamed.hpp; V the potential
+// the propagation uses); MODE 3: MODE 0 and MODE 2 in one sweep.  Per point
+// the neighbour sums L = sum_a sum_n lap_n (psi+ + psi-) and, per axis,
+// G_a = sum_n nab_n (psi+ - psi-); the H applications form the k term as
+// sum_a k_a G_a in all four Taylor terms (the same arithmetic in each keeps
+// the complex64 Taylor step norm-consistent).
 template <typename T, int ZB, int MODE>
 __device__ __forceinline__ void col_sweep(const Grid& g, bool act, int tl, cplx<T>* __restrict__ pl,
-                                          const SweepIO<T>& io, const T* __restrict__ vl,
-                                          const double* __restrict__ vl64, double kx, double ky, double kz,
-                                          double dt, double (&acc)[6]) {
+                                          const SweepIO<T>& io, const T* __restrict__ vl, double kx, double ky,
+                                          double kz, double dt, double (&acc)[6]) {
   const int NLx = g.NLx, NLy = g.NLy, NLz = g.NLz, P = NLy * NLz;
   const bool col = act && tl * ZB < P;
   const int w0 = col ? tl * ZB : 0;
@@ -135,16 +139,16 @@ __device__ __forceinline__ void col_sweep(const Grid& g, bool act, int tl, cplx<
   const int own = col ? (w0 / ZB) * CS : (P / ZB) * CS;  // threads without a column: the spare column
   // the output: stores of threads without a column go past its end
   const __amdgpu_buffer_rsrc_t ors =
-      orbital_rsrc(MODE < 2 ? (const void*)io.out : (const void*)io.in, (unsigned)(g.NL * sizeof(cplx<T>)));
-  // coefficients per axis (x, y, z) and n: C = lap, D = k nab (MODE 2: nab)
-  const double kk[3] = {kx, ky, kz};
+      orbital_rsrc(MODE != 2 ? (const void*)io.out : (const void*)io.in, (unsigned)(g.NL * sizeof(cplx<T>)));
+  // coefficients per axis (x, y, z) and n: C = lap, D = nab
+  const T kt[3] = {(T)kx, (T)ky, (T)kz};
   T C[3][5], D[3][5];
 #pragma unroll
   for (int a = 0; a < 3; ++a)
 #pragma unroll
     for (int n = 1; n <= 4; ++n) {
       C[a][n] = (T)g.lap[a][n];
-      D[a][n] = MODE == 2 ? (T)g.nab[a][n] : (T)(kk[a] * g.nab[a][n]);
+      D[a][n] = (T)g.nab[a][n];
     }
   const T A = (T)((kx * kx + ky * ky + kz * kz - (g.lap[0][0] + g.lap[1][0] + g.lap[2][0])) * 0.5);
   const T z1i = (T)(-dt), z2r = (T)(-dt * dt / 2.0), z3i = (T)(dt * dt * dt / 6.0),
@@ -158,17 +162,11 @@ __device__ __forceinline__ void col_sweep(const Grid& g, bool act, int tl, cplx<
   };
   // per-point operands, double-buffered by plane parity
   T V[2][ZB];
-  double V64[2][ZB];
   cplx<T> zz[2][ZB], a0[2][ZB], a1[2][ZB];
   auto ld_ops = [&](int X, int b) {
     const size_t pb = (size_t)X * P + w0;
-    if constexpr (MODE == 2) {
 #pragma unroll
-      for (int q = 0; q < ZB; ++q) V64[b][q] = vl64[pb + q];
-    } else {
-#pragma unroll
-      for (int q = 0; q < ZB; ++q) V[b][q] = vl[pb + q];
-    }
+    for (int q = 0; q < ZB; ++q) V[b][q] = vl[pb + q];
     if constexpr (MODE == 1) {
       load_run<T, ZB>(io.zu + pb, zz[b]);
       load_run<T, ZB>(io.h0 + pb, a0[b]);
@@ -203,20 +201,33 @@ __device__ __forceinline__ void col_sweep(const Grid& g, bool act, int tl, cplx<
       }
 #pragma unroll
       for (int q = 0; q < ZB; ++q) zw[4 + q] = win[j][q];
-      if constexpr (MODE < 2) {
-        T vr[ZB], vi[ZB], wr[ZB], wi[ZB];
+      // per point: L (vr, vi), the k term sum_a k_a G_a (wr, wi), the
+      // observables' sums from L and each G_a as its axis completes
+      T vr[ZB], vi[ZB], wr[ZB], wi[ZB];
+      double er[ZB], ei[ZB];
 #pragma unroll
-        for (int q = 0; q < ZB; ++q) {
-          vr[q] = vi[q] = wr[q] = wi[q] = 0;
+      for (int q = 0; q < ZB; ++q) {
+        // threads without a column add zeros to the observables
+        er[q] = col ? (double)win[j][q].re : 0.0;
+        ei[q] = col ? (double)win[j][q].im : 0.0;
+        T gr = 0, gi = 0;
+        vr[q] = vi[q] = 0;
 #pragma unroll
-          for (int n = 1; n <= 4; ++n) {
-            const cplx<T> ep = zw[4 + q + n], em = zw[4 + q - n];
-            vr[q] += C[2][n] * (ep.re + em.re);
-            vi[q] += C[2][n] * (ep.im + em.im);
-            wr[q] += D[2][n] * (ep.re - em.re);
-            wi[q] += D[2][n] * (ep.im - em.im);
-          }
+        for (int n = 1; n <= 4; ++n) {
+          const cplx<T> ep = zw[4 + q + n], em = zw[4 + q - n];
+          vr[q] += C[2][n] * (ep.re + em.re);
+          vi[q] += C[2][n] * (ep.im + em.im);
+          gr += D[2][n] * (ep.re - em.re);
+          gi += D[2][n] * (ep.im - em.im);
         }
+        wr[q] = kt[2] * gr;
+        wi[q] = kt[2] * gi;
+        if constexpr (MODE >= 2) acc[5] += er[q] * (double)gi - ei[q] * (double)gr;
+      }
+      {
+        T gr[ZB], gi[ZB];
+#pragma unroll
+        for (int q = 0; q < ZB; ++q) gr[q] = gi[q] = 0;
 #pragma unroll
         for (int n = 1; n <= 4; ++n) {
 #pragma unroll
@@ -224,10 +235,21 @@ __device__ __forceinline__ void col_sweep(const Grid& g, bool act, int tl, cplx<
             const cplx<T> ep = pln[yo[n - 1] + q], em = pln[yo[3 + n] + q];
             vr[q] += C[1][n] * (ep.re + em.re);
             vi[q] += C[1][n] * (ep.im + em.im);
-            wr[q] += D[1][n] * (ep.re - em.re);
-            wi[q] += D[1][n] * (ep.im - em.im);
+            gr[q] += D[1][n] * (ep.re - em.re);
+            gi[q] += D[1][n] * (ep.im - em.im);
           }
         }
+#pragma unroll
+        for (int q = 0; q < ZB; ++q) {
+          wr[q] += kt[1] * gr[q];
+          wi[q] += kt[1] * gi[q];
+          if constexpr (MODE >= 2) acc[4] += er[q] * (double)gi[q] - ei[q] * (double)gr[q];
+        }
+      }
+      {
+        T gr[ZB], gi[ZB];
+#pragma unroll
+        for (int q = 0; q < ZB; ++q) gr[q] = gi[q] = 0;
 #pragma unroll
         for (int n = 1; n <= 4; ++n) {
 #pragma unroll
@@ -235,10 +257,27 @@ __device__ __forceinline__ void col_sweep(const Grid& g, bool act, int tl, cplx<
             const cplx<T> ep = win[(j + n) % kColWin][q], em = win[(j + kColWin - n) % kColWin][q];
             vr[q] += C[0][n] * (ep.re + em.re);
             vi[q] += C[0][n] * (ep.im + em.im);
-            wr[q] += D[0][n] * (ep.re - em.re);
-            wi[q] += D[0][n] * (ep.im - em.im);
+            gr[q] += D[0][n] * (ep.re - em.re);
+            gi[q] += D[0][n] * (ep.im - em.im);
           }
         }
+#pragma unroll
+        for (int q = 0; q < ZB; ++q) {
+          wr[q] += kt[0] * gr[q];
+          wi[q] += kt[0] * gi[q];
+          if constexpr (MODE >= 2) acc[3] += er[q] * (double)gi[q] - ei[q] * (double)gr[q];
+        }
+      }
+      if constexpr (MODE >= 2) {
+#pragma unroll
+        for (int q = 0; q < ZB; ++q) {
+          const double a2 = er[q] * er[q] + ei[q] * ei[q];
+          acc[0] += a2;
+          acc[1] += (double)V[cb][q] * a2;
+          acc[2] += er[q] * (double)vr[q] + ei[q] * (double)vi[q];
+        }
+      }
+      if constexpr (MODE != 2) {
         cplx<T> f[ZB];
 #pragma unroll
         for (int q = 0; q < ZB; ++q) {
@@ -246,9 +285,7 @@ __device__ __forceinline__ void col_sweep(const Grid& g, bool act, int tl, cplx<
           f[q].re = (V[cb][q] + A) * e.re - (T)0.5 * vr[q] + wi[q];
           f[q].im = (V[cb][q] + A) * e.im - (T)0.5 * vi[q] - wr[q];
         }
-        if constexpr (MODE == 0) {
-          store_run_rsrc<T, ZB>(ors, soff, f);
-        } else {
+        if constexpr (MODE == 1) {
           // fold in the order of k_b_fold: z1 h0, z2 h1, z3 h2, z4 h3
 #pragma unroll
           for (int q = 0; q < ZB; ++q) {
@@ -264,45 +301,8 @@ __device__ __forceinline__ void col_sweep(const Grid& g, bool act, int tl, cplx<
             z.im += z4r * f[q].im;
             f[q] = z;
           }
-          store_run_rsrc<T, ZB>(ors, soff, f);
         }
-      } else {
-#pragma unroll
-        for (int q = 0; q < ZB; ++q) {
-          T Lr = 0, Li = 0, Gr[3] = {0, 0, 0}, Gi[3] = {0, 0, 0};
-#pragma unroll
-          for (int n = 1; n <= 4; ++n) {
-            const cplx<T> ep = zw[4 + q + n], em = zw[4 + q - n];
-            Lr += C[2][n] * (ep.re + em.re);
-            Li += C[2][n] * (ep.im + em.im);
-            Gr[2] += D[2][n] * (ep.re - em.re);
-            Gi[2] += D[2][n] * (ep.im - em.im);
-          }
-#pragma unroll
-          for (int n = 1; n <= 4; ++n) {
-            const cplx<T> ep = pln[yo[n - 1] + q], em = pln[yo[3 + n] + q];
-            Lr += C[1][n] * (ep.re + em.re);
-            Li += C[1][n] * (ep.im + em.im);
-            Gr[1] += D[1][n] * (ep.re - em.re);
-            Gi[1] += D[1][n] * (ep.im - em.im);
-          }
-#pragma unroll
-          for (int n = 1; n <= 4; ++n) {
-            const cplx<T> ep = win[(j + n) % kColWin][q], em = win[(j + kColWin - n) % kColWin][q];
-            Lr += C[0][n] * (ep.re + em.re);
-            Li += C[0][n] * (ep.im + em.im);
-            Gr[0] += D[0][n] * (ep.re - em.re);
-            Gi[0] += D[0][n] * (ep.im - em.im);
-          }
-          // threads without a column add zeros
-          const double er = col ? (double)win[j][q].re : 0.0, ei = col ? (double)win[j][q].im : 0.0;
-          const double a2 = er * er + ei * ei;
-          acc[0] += a2;
-          acc[1] += V64[cb][q] * a2;
-          acc[2] += er * (double)Lr + ei * (double)Li;
-#pragma unroll
-          for (int d = 0; d < 3; ++d) acc[3 + d] += er * (double)Gi[d] - ei * (double)Gr[d];
-        }
+        store_run_rsrc<T, ZB>(ors, soff, f);
       }
     }
   }
@@ -528,14 +528,19 @@ __device__ __forceinline__ void kb_observe_atom(const KBTables& kb, const Grid& 
   }
 }
 
-// Whole RT step of the orbitals o0 .. o0 + teams - 1 of system y.
+// Whole RT step of the orbitals o0 .. o0 + teams - 1 of system y.  The
+// observables of the orbitals as they enter (those of the previous step, at
+// k + A(t_n) for the launch of step n) are formed in the step's first sweep,
+// which reads those orbitals anyway, and written to the rows; the first step
+// of a run has none.  st.obs_only: only those observables (the launch that
+// closes a run, after its last step).
 template <typename T, int ZB>
 __device__ __forceinline__ void sweep_item(const Sys<T>& y, const Step& st, int s, int o0, unsigned char* lds) {
   const Grid& g = y.g;
   const KBTables& kb = y.kb;
   const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / kWave));
-  const int tw = y.sw_tw, team = wv / tw, wt = wv - team * tw, nt = tw * kWave;
-  const int tl = (int)threadIdx.x - team * nt;
+  const int tw = y.sw_tw, team = wv / tw, wt = wv - team * tw;
+  const int tl = (int)threadIdx.x - team * tw * kWave;
   const int lane = (int)threadIdx.x & (kWave - 1);
   const int o = o0 + team;
   const bool act = o < y.NKB;
@@ -547,61 +552,84 @@ __device__ __forceinline__ void sweep_item(const Sys<T>& y, const Step& st, int 
   const size_t NL = (size_t)g.NL;
   const size_t ob = (size_t)(act ? o : 0) * NL;
   const int ik = (act ? o : 0) / y.NB;
-  double am[3], an[3];
-  step_fields(st, s, am, an);
-  double kx, ky, kz;
-  orbital_k(y.kac0, y.NK, ik, am[0], am[1], am[2], kx, ky, kz);
-  const cplx<T>* phk = y.ph + (size_t)ik * kb.NPI;
-  double acc[6] = {0, 0, 0, 0, 0, 0};
-  // no two atom spheres share a grid point and no atom has more than kKbPass
-  // projectors: the back-projection of term n forms the projections of term
-  // n+1 (kb_scatter_project_atom), so only term 0 projects on its own
-  const bool fuse = y.ncolor == 1 && kb.max_lpa <= kKbPass;
-  for (int term = 0; term < 4; ++term) {
-    const cplx<T>* in = (term == 0 ? y.zu : y.h[term - 1]) + ob;
-    if (act && (term == 0 || !fuse))
-      for (int a = wt; a < kb.NI; a += tw) kb_project_atom<T>(kb, g.Hxyz, a, in, phk, U, lane);
-    __syncthreads();
-    SweepIO<T> io{in, (term < 3 ? y.h[term] : y.zu) + ob, y.zu + ob, y.h[0] + ob, y.h[1] + ob};
-    if (term < 3)
-      col_sweep<T, ZB, 0>(g, act, tl, pl, io, y.vloc, y.vloc64, kx, ky, kz, st.dt, acc);
-    else
-      col_sweep<T, ZB, 1>(g, act, tl, pl, io, y.vloc, y.vloc64, kx, ky, kz, st.dt, acc);
-    __syncthreads();
-    // KB term: the atoms colour by colour (no two atoms of a colour share a
-    // grid point), the atoms of a colour over the team's waves
-    const T fac = term < 3 ? (T)1 : (T)(st.dt * st.dt * st.dt * st.dt / 24.0);
-    const __amdgpu_buffer_rsrc_t rs = orbital_rsrc(io.out, (unsigned)(NL * sizeof(cplx<T>)));
-    for (int c = 0; c < y.ncolor; ++c) {
-      if (act) {
-        if (fuse && term < 3) {
-          for (int k = y.kb_cstart[c] + wt; k < y.kb_cstart[c + 1]; k += tw)
-            kb_scatter_project_atom<T>(kb, g.Hxyz, y.kb_order[k], phk, U, io.out, rs, fac, lane);
-        } else {
-          for (int k = y.kb_cstart[c] + wt; k < y.kb_cstart[c + 1]; k += tw)
-            kb_scatter_atom<T>(kb, y.kb_order[k], phk, U, io.out, rs, fac, lane);
-        }
-      }
-      __syncthreads();
+  const int n = *st.step;
+  const bool obs = st.obs_only || n > 0;
+  // the observables' k: kAc0 + A(t_n)
+  const double* acs = st.ac + (size_t)s * st.ac_stride;
+  double kox, koy, koz;
+  orbital_k(y.kac0, y.NK, ik, acs[3 * n], acs[3 * n + 1], acs[3 * n + 2], kox, koy, koz);
+  const cplx<T>* phn = y.phn + (size_t)ik * kb.NPI;
+  const double occ = act ? y.occ[o] : 0.0;
+  // the observables' per-wave sums go to the reduction area as soon as they
+  // are formed (not kept in registers through the step)
+  auto flush_acc = [&](double (&acc)[6]) {
+#pragma unroll
+    for (int i = 0; i < 6; ++i) acc[i] = wave_sum(acc[i]);
+    if (lane == 0) {
+#pragma unroll
+      for (int i = 0; i < 6; ++i) red[wv * kRedStride + i] = acc[i];
+    }
+  };
+  if (obs) {
+    // nonlocal observables of the entering orbitals (the fourth sweep changes them)
+    double nl4[4] = {0, 0, 0, 0};  // nonlocal Jx, Jy, Jz, Enl of this wave's atoms
+    if (act)
+      for (int a = wt; a < kb.NI; a += tw) kb_observe_atom<T>(kb, g, occ, a, y.zu + ob, phn, nl4, lane);
+    if (lane == 0) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) red[wv * kRedStride + 6 + i] = nl4[i];
     }
   }
-  // observables of the new orbital at k + A(t+dt)
-  orbital_k(y.kac0, y.NK, ik, an[0], an[1], an[2], kx, ky, kz);
-  const cplx<T>* phn = y.phn + (size_t)ik * kb.NPI;
-  const SweepIO<T> io{y.zu + ob, nullptr, nullptr, nullptr, nullptr};
-  col_sweep<T, ZB, 2>(g, act, tl, pl, io, y.vloc, y.vloc64, kx, ky, kz, st.dt, acc);
-  const double occ = act ? y.occ[o] : 0.0;
-  double nl4[4] = {0, 0, 0, 0};  // nonlocal Jx, Jy, Jz, Enl of this wave's atoms
-  if (act)
-    for (int a = wt; a < kb.NI; a += tw) kb_observe_atom<T>(kb, g, occ, a, y.zu + ob, phn, nl4, lane);
-#pragma unroll
-  for (int i = 0; i < 6; ++i) acc[i] = wave_sum(acc[i]);
-  if (lane == 0) {
-#pragma unroll
-    for (int i = 0; i < 6; ++i) red[wv * kRedStride + i] = acc[i];
-#pragma unroll
-    for (int i = 0; i < 4; ++i) red[wv * kRedStride + 6 + i] = nl4[i];
+  if (st.obs_only) {
+    const SweepIO<T> io{y.zu + ob, nullptr, nullptr, nullptr, nullptr};
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    col_sweep<T, ZB, 2>(g, act, tl, pl, io, y.vloc, kox, koy, koz, st.dt, acc);
+    flush_acc(acc);
+  } else {
+    double am[3], an[3];
+    step_fields(st, s, am, an);
+    double kx, ky, kz;
+    orbital_k(y.kac0, y.NK, ik, am[0], am[1], am[2], kx, ky, kz);
+    const cplx<T>* phk = y.ph + (size_t)ik * kb.NPI;
+    // no two atom spheres share a grid point and no atom has more than kKbPass
+    // projectors: the back-projection of term n forms the projections of term
+    // n+1 (kb_scatter_project_atom), so only term 0 projects on its own
+    const bool fuse = y.ncolor == 1 && kb.max_lpa <= kKbPass;
+    for (int term = 0; term < 4; ++term) {
+      const cplx<T>* in = (term == 0 ? y.zu : y.h[term - 1]) + ob;
+      if (act && (term == 0 || !fuse))
+        for (int a = wt; a < kb.NI; a += tw) kb_project_atom<T>(kb, g.Hxyz, a, in, phk, U, lane);
+      __syncthreads();
+      SweepIO<T> io{in, (term < 3 ? y.h[term] : y.zu) + ob, y.zu + ob, y.h[0] + ob, y.h[1] + ob};
+      double acc[6] = {0, 0, 0, 0, 0, 0};
+      if (term == 0 && obs) {
+        col_sweep<T, ZB, 3>(g, act, tl, pl, io, y.vloc, kx, ky, kz, st.dt, acc);
+        flush_acc(acc);
+      } else if (term < 3) {
+        col_sweep<T, ZB, 0>(g, act, tl, pl, io, y.vloc, kx, ky, kz, st.dt, acc);
+      } else {
+        col_sweep<T, ZB, 1>(g, act, tl, pl, io, y.vloc, kx, ky, kz, st.dt, acc);
+      }
+      __syncthreads();
+      // KB term: the atoms colour by colour (no two atoms of a colour share a
+      // grid point), the atoms of a colour over the team's waves
+      const T fac = term < 3 ? (T)1 : (T)(st.dt * st.dt * st.dt * st.dt / 24.0);
+      const __amdgpu_buffer_rsrc_t rs = orbital_rsrc(io.out, (unsigned)(NL * sizeof(cplx<T>)));
+      for (int c = 0; c < y.ncolor; ++c) {
+        if (act) {
+          if (fuse && term < 3) {
+            for (int k = y.kb_cstart[c] + wt; k < y.kb_cstart[c + 1]; k += tw)
+              kb_scatter_project_atom<T>(kb, g.Hxyz, y.kb_order[k], phk, U, io.out, rs, fac, lane);
+          } else {
+            for (int k = y.kb_cstart[c] + wt; k < y.kb_cstart[c + 1]; k += tw)
+              kb_scatter_atom<T>(kb, y.kb_order[k], phk, U, io.out, rs, fac, lane);
+          }
+        }
+        __syncthreads();
+      }
+    }
   }
+  if (!obs) return;  // (the step's last barrier is behind)
   __syncthreads();
   if (act && tl == 0) {
     double v[kRedStride];
@@ -610,14 +638,14 @@ __device__ __forceinline__ void sweep_item(const Sys<T>& y, const Step& st, int 
     for (int w = 1; w < tw; ++w)
 #pragma unroll
       for (int i = 0; i < kRedStride; ++i) v[i] += red[(team * tw + w) * kRedStride + i];
-    const double kk[3] = {kx, ky, kz};
+    const double kk[3] = {kox, koy, koz};
     const double A0 = -(g.lap[0][0] + g.lap[1][0] + g.lap[2][0]) * 0.5;
     const double te = -0.5 * v[2];
     const double sc = g.Hxyz / g.aLxyz;
     double* r = y.rows + (size_t)o * 8;
     for (int d = 0; d < 3; ++d) r[d] = (occ * kk[d] * v[0] + v[3 + d] * occ) * sc + v[6 + d];
-    r[3] = occ * (A0 * v[0] + te + kx * v[3] + ky * v[4] + kz * v[5]) * g.Hxyz +
-           occ * (kx * kx + ky * ky + kz * kz) / 2.0;
+    r[3] = occ * (A0 * v[0] + te + kox * v[3] + koy * v[4] + koz * v[5]) * g.Hxyz +
+           occ * (kox * kox + koy * koy + koz * koz) / 2.0;
     r[4] = v[9];
     r[5] = occ * v[1] * g.Hxyz;
     r[6] = occ * v[0] * g.Hxyz;
