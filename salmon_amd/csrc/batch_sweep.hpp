@@ -463,17 +463,23 @@ __device__ __forceinline__ void kb_scatter_project_atom(const KBTables& kb, doub
 
 // Nonlocal observables of atom a at the phases ph (current.f90:172-273 and
 // total_energy.f90:191-199, as in observe_orbital): adds Jx, Jy, Jz, Enl.
-template <typename T>
+// PROJ: from the same gathered values also the projections at the phases
+// pp into U, as kb_project_atom forms them.
+template <typename T, bool PROJ = false>
 __device__ __forceinline__ void kb_observe_atom(const KBTables& kb, const Grid& g, double occ, int a,
                                                 const cplx<T>* __restrict__ E, const cplx<T>* __restrict__ ph,
-                                                double (&nl4)[4], int lane) {
+                                                double (&nl4)[4], int lane, const cplx<T>* __restrict__ pp = nullptr,
+                                                cplx<T>* U = nullptr) {
   const int j0 = kb.atom_start[a], m = kb.atom_start[a + 1] - j0;
   const int l1 = kb.atom_lma[a + 1];
   for (int lc = kb.atom_lma[a]; lc < l1; lc += kKbPass) {
     // per projector: sum u t and sum r u t (t = e^{ik.r} psi), real and imaginary parts
     double v[8 * kKbPass];
+    T vp[2 * kKbPass];
 #pragma unroll
     for (int k = 0; k < 8 * kKbPass; ++k) v[k] = 0;
+#pragma unroll
+    for (int k = 0; k < 2 * kKbPass; ++k) vp[k] = 0;
     for (int jb = 0; jb < m; jb += kKbChunks * kWave) {
       int jj[kKbChunks];
       bool ok[kKbChunks];
@@ -492,10 +498,21 @@ __device__ __forceinline__ void kb_observe_atom(const KBTables& kb, const Grid& 
         const double c = (double)ph[jp].re, sn = (double)ph[jp].im;
         const double tr0 = c * (double)e[h].re - sn * (double)e[h].im;
         const double ti0 = c * (double)e[h].im + sn * (double)e[h].re;
+        T pwr = 0, pwi = 0;
+        if constexpr (PROJ) {
+          const cplx<T> q = pp[jp];
+          pwr = q.re * e[h].re - q.im * e[h].im;
+          pwi = q.re * e[h].im + q.im * e[h].re;
+        }
 #pragma unroll
         for (int l = 0; l < kKbPass; ++l) {
           const double u0 = kb.uv[(size_t)min(lc + l, kb.Nlma - 1) * kb.Nps + jj[h]];
           const double u = (ok[h] && lc + l < l1) ? u0 : 0.0;
+          if constexpr (PROJ) {
+            const T ut = (ok[h] && lc + l < l1) ? (T)u0 : (T)0;
+            vp[2 * l] += ut * pwr;
+            vp[2 * l + 1] += ut * pwi;
+          }
           const double tr = u * tr0, ti = u * ti0;
           double* q = v + 8 * l;
           q[0] += tr;
@@ -507,6 +524,14 @@ __device__ __forceinline__ void kb_observe_atom(const KBTables& kb, const Grid& 
           q[6] += pz * tr;
           q[7] += pz * ti;
         }
+      }
+    }
+    if constexpr (PROJ) {
+      wave_sums<2 * kKbPass>(vp);
+#pragma unroll
+      for (int k = 0; k < 2 * kKbPass; ++k) {
+        const int il = lc + k / 2;
+        if (lane == wave_sums_lane(k) && il < l1) ((T*)U)[2 * il + (k & 1)] = vp[0] * (T)(g.Hxyz * kb.lma_sign[il]);
       }
     }
     wave_sums<8 * kKbPass>(v);
@@ -570,11 +595,20 @@ __device__ __forceinline__ void sweep_item(const Sys<T>& y, const Step& st, int 
       for (int i = 0; i < 6; ++i) red[wv * kRedStride + i] = acc[i];
     }
   };
+  const cplx<T>* phk = y.ph + (size_t)ik * kb.NPI;
   if (obs) {
-    // nonlocal observables of the entering orbitals (the fourth sweep changes them)
+    // nonlocal observables of the entering orbitals (the fourth sweep changes
+    // them); in a step launch the same gathers give the first Taylor term's
+    // KB projections
     double nl4[4] = {0, 0, 0, 0};  // nonlocal Jx, Jy, Jz, Enl of this wave's atoms
-    if (act)
-      for (int a = wt; a < kb.NI; a += tw) kb_observe_atom<T>(kb, g, occ, a, y.zu + ob, phn, nl4, lane);
+    if (act) {
+      if (st.obs_only) {
+        for (int a = wt; a < kb.NI; a += tw) kb_observe_atom<T>(kb, g, occ, a, y.zu + ob, phn, nl4, lane);
+      } else {
+        for (int a = wt; a < kb.NI; a += tw)
+          kb_observe_atom<T, true>(kb, g, occ, a, y.zu + ob, phn, nl4, lane, phk, U);
+      }
+    }
     if (lane == 0) {
 #pragma unroll
       for (int i = 0; i < 4; ++i) red[wv * kRedStride + 6 + i] = nl4[i];
@@ -590,14 +624,14 @@ __device__ __forceinline__ void sweep_item(const Sys<T>& y, const Step& st, int 
     step_fields(st, s, am, an);
     double kx, ky, kz;
     orbital_k(y.kac0, y.NK, ik, am[0], am[1], am[2], kx, ky, kz);
-    const cplx<T>* phk = y.ph + (size_t)ik * kb.NPI;
     // no two atom spheres share a grid point and no atom has more than kKbPass
     // projectors: the back-projection of term n forms the projections of term
     // n+1 (kb_scatter_project_atom), so only term 0 projects on its own
     const bool fuse = y.ncolor == 1 && kb.max_lpa <= kKbPass;
     for (int term = 0; term < 4; ++term) {
       const cplx<T>* in = (term == 0 ? y.zu : y.h[term - 1]) + ob;
-      if (act && (term == 0 || !fuse))
+      // term 0: projected with the observables above when there are any
+      if (act && ((term == 0 && !obs) || (term > 0 && !fuse)))
         for (int a = wt; a < kb.NI; a += tw) kb_project_atom<T>(kb, g.Hxyz, a, in, phk, U, lane);
       __syncthreads();
       SweepIO<T> io{in, (term < 3 ? y.h[term] : y.zu) + ob, y.zu + ob, y.h[0] + ob, y.h[1] + ob};
