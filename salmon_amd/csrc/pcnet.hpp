@@ -463,25 +463,37 @@ __global__ void __launch_bounds__(kBlk) k_conv_mfma(const unsigned short* __rest
   const int m = lane & 15, kq = lane >> 4;
   const int p = p0 + m;
   const bool ok = p < g.NL;
-  const int iz = p % g.NLz, iy = (p / g.NLz) % g.NLy, ix = p / (g.NLz * g.NLy);
+  const int pc = ok ? p : 0;  // points past the grid read point 0, weighted by zero
+  const int iz = pc % g.NLz, iy = (pc / g.NLz) % g.NLy, ix = pc / (g.NLz * g.NLy);
   const unsigned short* a = in + (size_t)s * g.NL * CI;
-  f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
-  for (int k0 = 0; k0 < K; k0 += 32) {
-    const int kb = k0 + 8 * kq;
-    union { bf16x8 v; unsigned short h[8]; uint4 q; } A, B;
+  // every reduction block's neighbour values are gathered before the first
+  // MFMA (taps past 27 and points past the grid load valid data and are
+  // zeroed by a select), so the gathers are in flight together
+  constexpr int NKB = K / 32;
+  union Frag { bf16x8 v; unsigned short h[8]; uint4 q; };
+  Frag A[NKB];
+#pragma unroll
+  for (int i = 0; i < NKB; ++i) {
+    const int kb = 32 * i + 8 * kq;
     if constexpr (CI % 8 == 0) {
       const int t = kb / CI, c0 = kb % CI;
-      if (ok && t < 27) A.q = *(const uint4*)(a + (size_t)nbr(ix, iy, iz, t, g) * CI + c0);
-      else A.q = make_uint4(0u, 0u, 0u, 0u);
+      const uint4 q = *(const uint4*)(a + (size_t)nbr(ix, iy, iz, t < 27 ? t : 26, g) * CI + c0);
+      A[i].q = (ok && t < 27) ? q : make_uint4(0u, 0u, 0u, 0u);
     } else {
 #pragma unroll
       for (int j = 0; j < 8; ++j) {
         const int kk = kb + j, t = kk / CI, ci = kk % CI;
-        A.h[j] = (ok && t < 27) ? a[(size_t)nbr(ix, iy, iz, t, g) * CI + ci] : (unsigned short)0;
+        const unsigned short x = a[(size_t)nbr(ix, iy, iz, t < 27 ? t : 26, g) * CI + ci];
+        A[i].h[j] = (ok && t < 27) ? x : (unsigned short)0;
       }
     }
-    B.q = *(const uint4*)(sb + m * K + kb);
-    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A.v, B.v, acc, 0, 0, 0);
+  }
+  f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+  for (int i = 0; i < NKB; ++i) {
+    Frag B;
+    B.q = *(const uint4*)(sb + m * K + 32 * i + 8 * kq);
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[i].v, B.v, acc, 0, 0, 0);
   }
   const int n = m;
   if (n < CO) {
