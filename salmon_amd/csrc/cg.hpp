@@ -138,19 +138,37 @@ __global__ void __launch_bounds__(256) k_cg_proj(State S, const cplx<double>* __
   double acc[2 * kJG];
 #pragma unroll
   for (int q = 0; q < 2 * kJG; ++q) acc[q] = 0.0;
+  // every load of the block is unconditional (points past the grid read
+  // point 0, bands past ib the last lower band; their terms are zeroed by a
+  // select), so no branch separates the loads from their uses and they are
+  // in flight together; each band's sum takes its points in the same order
+  cplx<double> e[kPPT];
+  int pc[kPPT];
+  bool okp[kPPT];
 #pragma unroll
   for (int i = 0; i < kPPT; ++i) {
     const int p = c * kChunk + i * 256 + threadIdx.x;
-    if (p < S.NL) {
-      const cplx<double> e = src.at(k, S.NL, p, xhx);
+    okp[i] = p < S.NL;
+    pc[i] = okp[i] ? p : 0;
+  }
+  // (Src::at with its source choice outside the point loop)
+  if (src.v) {
 #pragma unroll
-      for (int j = 0; j < kJG; ++j) {
-        if (j < nj) {
-          const cplx<double> b = B[(size_t)(g0 + j) * S.NL + p];
-          acc[2 * j] += b.re * e.re + b.im * e.im;
-          acc[2 * j + 1] += b.re * e.im - b.im * e.re;
-        }
-      }
+    for (int i = 0; i < kPPT; ++i) e[i] = src.v[(size_t)k * src.vstride + pc[i]];
+  } else {
+#pragma unroll
+    for (int i = 0; i < kPPT; ++i) e[i] = src.at(k, S.NL, pc[i], xhx);
+  }
+#pragma unroll
+  for (int j = 0; j < kJG; ++j) {
+    const int jb = g0 + (j < nj ? j : nj - 1);
+#pragma unroll
+    for (int i = 0; i < kPPT; ++i) {
+      const cplx<double> b0 = B[(size_t)jb * S.NL + pc[i]];
+      const bool w = okp[i] && j < nj;
+      const double br = w ? b0.re : 0.0, bi = w ? b0.im : 0.0;
+      acc[2 * j] += br * e[i].re + bi * e[i].im;
+      acc[2 * j + 1] += br * e[i].im - bi * e[i].re;
     }
   }
   __shared__ double smem[4 * 2 * kJG];
