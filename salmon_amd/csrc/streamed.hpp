@@ -29,22 +29,45 @@ __device__ __forceinline__ void kb_project_wave(const cplx<T>* __restrict__ E, i
   const int lane = threadIdx.x & (kWave - 1);
   const int a = kb.lma_atom[ilma];
   const int j0 = kb.atom_start[a], m = kb.atom_start[a + 1] - j0;
+  // three 64-point chunks of the sphere with their loads in flight together:
+  // indices past the sphere are clamped to its first point and weighted by
+  // zero, so no branch separates the gathers from their uses
+  constexpr int NC = 3;
   T sr = 0, si = 0;
-  for (int j = lane; j < m; j += kWave) {
-    const int jp = j0 + j;
-    T s, c;
-    if (ph) {
-      c = ph[jp].re;
-      s = ph[jp].im;
-    } else {
-      const double kr = kx * kb.pos[3 * jp] + ky * kb.pos[3 * jp + 1] + kz * kb.pos[3 * jp + 2];
-      sincos_t((T)kr, &s, &c);
+  for (int jb = 0; jb < m; jb += NC * kWave) {
+    int jj[NC];
+    bool ok[NC];
+    cplx<T> e[NC];
+    T u[NC], c[NC], s[NC];
+#pragma unroll
+    for (int h = 0; h < NC; ++h) {
+      const int j = jb + h * kWave + lane;
+      ok[h] = j < m;
+      jj[h] = ok[h] ? j : 0;
+      e[h] = E[kb.pidx[j0 + jj[h]]];
+      const T u0 = (T)kb.uv[(size_t)ilma * kb.Nps + jj[h]];
+      u[h] = ok[h] ? u0 : (T)0;
     }
-    const T u = (T)kb.uv[(size_t)ilma * kb.Nps + j];
-    const cplx<T> e = E[kb.pidx[jp]];
-    // (u c + i u s) * (e.re + i e.im)
-    sr += u * (c * e.re - s * e.im);
-    si += u * (c * e.im + s * e.re);
+    if (ph) {
+#pragma unroll
+      for (int h = 0; h < NC; ++h) {
+        c[h] = ph[j0 + jj[h]].re;
+        s[h] = ph[j0 + jj[h]].im;
+      }
+    } else {
+#pragma unroll
+      for (int h = 0; h < NC; ++h) {
+        const int jp = j0 + jj[h];
+        const double kr = kx * kb.pos[3 * jp] + ky * kb.pos[3 * jp + 1] + kz * kb.pos[3 * jp + 2];
+        sincos_t((T)kr, &s[h], &c[h]);
+      }
+    }
+#pragma unroll
+    for (int h = 0; h < NC; ++h) {
+      // (u c + i u s) * (e.re + i e.im)
+      sr += u[h] * (c[h] * e[h].re - s[h] * e[h].im);
+      si += u[h] * (c[h] * e[h].im + s[h] * e[h].re);
+    }
   }
   sr = wave_sum(sr);
   si = wave_sum(si);
