@@ -100,6 +100,47 @@ def test_batch_sweep_needs_eight_points_per_axis():
     assert rel(r["J"][0], ref["J"]) < 1e-12 and rel(r["E"][0], ref["E"]) < 1e-12
 
 
+@pytest.mark.parametrize("prec", [gpu.FP64, gpu.MIXED])
+def test_batch_sweep_odd_grids_against_single_contexts(prec):
+    """The persistent sweep on grids outside the 64-system workload: odd and
+    unequal axes (columns of 1, 2 and 3 z points, teams of 1 and 2 waves,
+    idle threads in a team), random atoms in non-cubic cells; two runs in a
+    row (each closed by its observables-only launch, the second continuing
+    from the first's orbitals) against the single-context streamed path."""
+    systems = [synth.make_system(N=(9, 10, 11), L=(6.0, 6.5, 7.0), nk3=(2, 1, 1), NB=3, seed=21, atoms="random",
+                                 natoms=3),
+               synth.make_system(N=(12, 8, 10), L=(7.5, 5.0, 6.3), nk3=(1, 2, 1), NB=2, seed=22, atoms="random",
+                                 natoms=2),
+               synth.make_system(N=(8, 8, 8), L=5.2, nk3=(1, 1, 2), NB=4, seed=23, atoms="random", natoms=1)]
+    nsteps = 4
+    ac = _ac(nsteps)
+    singles = []
+    for s in systems:
+        with gpu.Context(s, prec) as c:
+            c.set_path(gpu.PATH_STREAMED)
+            singles.append(c.rt_run(ac, nsteps, dt=0.08))
+    ctxs = [gpu.Context(s, prec) for s in systems]
+    try:
+        with gpu.Batch(ctxs, prec) as b:
+            b.set_kernel(gpu.BATCH_SWEEP)
+            r1 = b.rt_run(ac[:3], 2, 0.08)
+            assert b.kernel() == gpu.BATCH_SWEEP
+            r2 = b.rt_run(ac[2:], 2, 0.08)
+    finally:
+        for c in ctxs:
+            c.close()
+    tol = 1e-12 if prec == gpu.FP64 else 1e-5
+    for i, r in enumerate(singles):
+        J = np.concatenate([r1["J"][i], r2["J"][i]])
+        E = np.concatenate([r1["E"][i], r2["E"][i]])
+        N = np.concatenate([r1["norm"][i], r2["norm"][i]])
+        s = systems[i]
+        ej = np.max(np.abs(J - r["J"])) / np.max(np.abs(r["J"]))
+        print(f"[gpu] sweep {s.NLx}x{s.NLy}x{s.NLz} prec={prec}: J {ej:.2e} E {rel(E, r['E']):.2e} "
+              f"norm {rel(N, r['norm']):.2e}", flush=True)
+        assert ej < tol and rel(E, r["E"]) < tol and rel(N, r["norm"]) < tol
+
+
 def test_batch_rejects_mixed_precisions():
     s = synth.config4_system(0, scale=0.1)
     with gpu.Context(s, gpu.FP64) as c64, gpu.Context(s, gpu.MIXED) as c32:
