@@ -65,6 +65,7 @@ def lib(path=None):
         L.ref_sp_energy.argtypes = [vp, vp]
         L.ref_cg.argtypes = [vp, ci, vp]
         L.ref_rt_run.argtypes = [vp, ci, vp, vp, ci]
+        L.ref_move_atoms.argtypes = [ci, vp, vp, vp, vp, vp, vp, vp, vp]
         L.ref_num_threads.restype = ci
         _lib = L
     return _lib
@@ -159,6 +160,15 @@ class Ref:
         et = np.zeros(nsteps)
         lib().ref_rt_run(_p(ac), int(nsteps), _p(jt), _p(et), int(energy_every))
         return dict(J=jt, E=et)
+
+    def move_atoms(self, Rion, Mps, Jxyz, Jxx, Jyy, Jzz, uV, vloc):
+        """New atom positions with their sphere tables, uV and local potential
+        (same atoms and projectors), as an Ehrenfest step leaves them."""
+        i32 = lambda x: np.ascontiguousarray(x, dtype=np.int32)
+        f64 = lambda x: np.ascontiguousarray(x, dtype=np.float64)
+        a = [i32(Mps), i32(Jxyz), i32(Jxx), i32(Jyy), i32(Jzz), f64(uV), f64(Rion), f64(vloc)]
+        assert a[1].shape[0] == self.sys.NI and a[5].shape[0] == self.sys.Nlma and a[7].size == self.sys.NL
+        lib().ref_move_atoms(int(a[1].shape[1]), *[_p(x) for x in a])
 
     def num_threads(self):
         return lib().ref_num_threads()

@@ -22,6 +22,9 @@
 !   the RT step order ... ARTED/control/control_sc.f90:190-222 with
 !                         ARTED/RT/dt_evolve.f90:43-51 (middlepoint) and a
 !                         frozen potential (no_update_func='y').
+!   moved atoms ......... ARTED/preparation/prep_ps.f90 ('update_all'), the
+!                         state control_sc.f90:152-178 leaves in an
+!                         Ehrenfest step (ref_move_atoms).
 module ref_harness
   use iso_c_binding
 #ifdef ARTED_GPU_DROPIN
@@ -102,6 +105,7 @@ contains
     allocate(zproj(Nps,Nlma,NK_s:NK_e), ekr_omp(Nps,NI,NK_s:NK_e))
 
     dt = dt_
+    use_ehrenfest_md = 'n'
     functional = 'PZ'
     Sym = 1
     alocal_laser = 'n'
@@ -131,6 +135,40 @@ contains
     ! G-space parts of Total_Energy_omp are outside this path: empty G range
     NG = 0; NG_s = 1; NG_e = 0; nGzero = 0
     allocate(Gx(0), Gy(0), Gz(0), rhoe_G(0), rhoion_G(0), dVloc_G(0,1))
+  end subroutine
+
+  ! Moved atoms in an Ehrenfest run: the module state prep_ps_periodic
+  ! ('update_all') leaves for the next step (ARTED/preparation/prep_ps.f90:
+  ! 54-66 projector index freed, 95-116 Nps-sized arrays reallocated when Nps
+  ! changes, 118-144 Jxyz/Jxx/Jyy/Jzz and the padded index zKxyz, 253-256
+  ! init_projector), with the sphere tables, uV and the local potential of
+  ! the new positions passed in (calc_jxyz / calc_uv / calc_vpsl are outside
+  ! this path).  Same atoms and projectors (NI, Nlma, a_tbl, iuV unchanged).
+  subroutine ref_move_atoms(nps_, mps_, jxyz_, jxx_, jyy_, jzz_, uv_, rion_, vloc_) bind(C, name="ref_move_atoms")
+    integer(c_int), value :: nps_
+    integer(c_int), intent(in) :: mps_(NI), jxyz_(nps_, NI), jxx_(nps_, NI), jyy_(nps_, NI), jzz_(nps_, NI)
+    real(c_double), intent(in) :: uv_(nps_, Nlma), rion_(3, NI), vloc_(NL)
+    integer :: a, j, i
+
+    use_ehrenfest_md = 'y'
+    if (nps_ /= Nps) then
+      deallocate(Jxyz, Jxx, Jyy, Jzz, zJxyz, zKxyz, ekr_omp, uV, zproj)
+      Nps = nps_
+      allocate(Jxyz(Nps,NI), Jxx(Nps,NI), Jyy(Nps,NI), Jzz(Nps,NI), zJxyz(Nps,NI), zKxyz(Nps,NI))
+      allocate(ekr_omp(Nps,NI,NK_s:NK_e), uV(Nps,Nlma), zproj(Nps,Nlma,NK_s:NK_e))
+    end if
+    Mps = mps_; Jxyz = jxyz_; Jxx = jxx_; Jyy = jyy_; Jzz = jzz_; uV = uv_
+    Rion = rion_; Vloc = vloc_; Vpsl = vloc_
+    zJxyz = Jxyz - 1
+    zKxyz = 0
+    do a = 1, NI
+      do j = 1, Mps(a)
+        i = Jxyz(j,a)
+        zKxyz(j,a) = Lx(i)*PNLy*PNLz + Ly(i)*PNLz + Lz(i)
+      end do
+    end do
+    deallocate(nprojector, idx_proj, idx_lma, pseudo_start_idx)
+    call init_projector(zKxyz)
   end subroutine
 
   subroutine ref_get_fd(out) bind(C, name="ref_get_fd")

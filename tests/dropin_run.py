@@ -9,6 +9,7 @@ process).  Prints one JSON line.
   python dropin_run.py CASE cpu|gpu STEPS          the RT step loop
   python dropin_run.py CASE cpu-gs|gpu-gs NCG      Gram_Schmidt then CG_omp
   python dropin_run.py CASE cpu-obs|gpu-obs 0      hpsi, sp_energy, current, density, energy
+  python dropin_run.py CASE cpu-md|gpu-md STEPS    the RT step loop, atom 1 moving from step 3 on
   python dropin_run.py cfg2 gpu-time|cpu-time STEPS  config-2 step time of the loop
 """
 import ctypes
@@ -67,6 +68,34 @@ def observables(case, which):
                           rho=rho.tolist(), ekin=ek, enl=en, transfers=_counter("arted_gpu_orbital_transfers", which))))
 
 
+def moving_atom(case, which, nsteps):
+    """The RT step loop with the first atom moving every step from the third
+    on, as in an Ehrenfest run: before each such step the harness takes the
+    new position's sphere tables, uV and local potential (the state
+    prep_ps_periodic('update_all') leaves), and the drop-in re-uploads them."""
+    from salmon_amd import synth
+    s, d = load_case(case)
+    R = ref.Ref(s, lib_path=_lib_path(which))
+    ac = d["ac"]
+    coeff = np.where(s.iuV.reshape(s.NI, -1) > 0, 1.0, -1.0)
+    J, E, mps = [], [], []
+    for it in range(nsteps):
+        if it >= 2:
+            Rion = s.Rion.copy()
+            Rion[0] += (it - 1) * np.array([0.19, -0.11, 0.07])
+            t = synth.projector_tables(Rion, s.rc, (s.NLx, s.NLy, s.NLz), (s.aLx, s.aLy, s.aLz), 0.8, coeff)
+            assert np.array_equal(t[5], s.a_tbl) and np.array_equal(t[7], s.iuV)
+            R.move_atoms(Rion, t[0], t[1], t[2], t[3], t[4], t[6], s.vloc * (1.0 + 0.01 * it))
+            mps.append(t[0].tolist())
+        r = R.rt_run(ac[it:it + 2], 1)
+        J.append(r["J"][0].tolist())
+        E.append(float(r["E"][0]))
+    z = R.get()
+    print(json.dumps(dict(J=J, E=E, z=_sample(z), zabs=float(np.abs(z).sum()), mps=mps,
+                          calls=_counter("arted_gpu_hamiltonian_calls", which),
+                          transfers=_counter("arted_gpu_orbital_transfers", which))))
+
+
 def timing(which, nsteps):
     """The reference's RT step loop at the config-2 size (16^3, 512 k x 32
     bands): ms per step after one warm-up step."""
@@ -89,6 +118,8 @@ def main(case, which, n):
         return observables(case, which)
     if which.endswith("-time"):
         return timing(which, n)
+    if which.endswith("-md"):
+        return moving_atom(case, which, n)
     s, d = load_case(case)
     R = ref.Ref(s, lib_path=_lib_path(which))
     r = R.rt_run(d["ac"], n)

@@ -58,6 +58,82 @@ def test_reference_loop_with_gpu_dropins(case, nsteps):
     assert abs(gpu["zabs"] - cpu["zabs"]) / cpu["zabs"] < 1e-12
 
 
+def _moved_systems(case, nsteps):
+    """The systems of the moving-atom run of dropin_run.py: atom 1 displaced
+    before every step from the third on, with its tables and local potential."""
+    import dataclasses
+    from helpers import load_case
+    from salmon_amd import synth
+    s, d = load_case(case)
+    coeff = np.where(s.iuV.reshape(s.NI, -1) > 0, 1.0, -1.0)
+    out = []
+    for it in range(nsteps):
+        if it < 2:
+            out.append(s)
+            continue
+        Rion = s.Rion.copy()
+        Rion[0] += (it - 1) * np.array([0.19, -0.11, 0.07])
+        t = synth.projector_tables(Rion, s.rc, (s.NLx, s.NLy, s.NLz), (s.aLx, s.aLy, s.aLz), 0.8, coeff)
+        out.append(dataclasses.replace(s, Rion=Rion, Mps=t[0], Jxyz=t[1], Jxx=t[2], Jyy=t[3], Jzz=t[4], uV=t[6],
+                                       vloc=s.vloc * (1.0 + 0.01 * it)))
+    return out, d
+
+
+@pytest.mark.parametrize("case", ["small_k", "noncubic"])
+def test_reference_moving_atom_matches_oracle(case):
+    """Pins the harness's moved-atom state (ref_move_atoms: the tables
+    prep_ps_periodic('update_all') leaves): the reference's step loop with atom
+    1 moving every step from the third on equals the oracle run step by step
+    on each step's system.  Atom 1's sphere grows past Nps in small_k (the
+    arrays are reallocated) and shrinks in noncubic.  CPU only."""
+    _need_ref()
+    from oracle.cpu import Oracle
+    nsteps = 6
+    r = _run(case, "cpu-md", nsteps)
+    systems, d = _moved_systems(case, nsteps)
+    assert [list(x.Mps) for x in systems[2:]] == r["mps"]
+    if case == "small_k":
+        assert max(systems[-1].Mps) > systems[0].Nps
+    zu = systems[0].zu
+    J, E = [], []
+    for it, s in enumerate(systems):
+        o = Oracle(s)
+        x = o.rt_run(d["ac"][it:it + 2], 1, zu=zu)
+        zu = x["zu"]
+        J.append(x["J"][0])
+        E.append(x["E"][0])
+    z = [zu.real.ravel()[::97], zu.imag.ravel()[::97]]
+    print(f"moving atom {case}: J {_rel(r['J'], J):.2e} E {_rel(r['E'], E):.2e} z {_rel(r['z'], z):.2e}")
+    assert _rel(r["J"], J) < 1e-11
+    assert _rel(r["E"], E) < 1e-12
+    assert _rel(r["z"], z) < 1e-12
+    # the moved atom changes the trajectory
+    plain = Oracle(systems[0]).rt_run(d["ac"], nsteps)
+    assert _rel(r["J"], plain["J"]) > 1e-4
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", ["small_k", "noncubic"])
+def test_reference_loop_with_gpu_dropins_moving_atom(case):
+    """An Ehrenfest-style run: the drop-in hamiltonian re-uploads the
+    projector tables every step (use_ehrenfest_md='y') while atom 1 moves, the
+    orbitals staying on the GPU; against the all-CPU reference loop."""
+    _need_ref()
+    nsteps = 6
+    print(f"[gpu] drop-in moving atom {case} {nsteps} steps", flush=True)
+    cpu = _run(case, "cpu-md", nsteps)
+    gpu = _run(case, "gpu-md", nsteps)
+    assert gpu["calls"] == nsteps
+    assert gpu["transfers"] == 2
+    assert gpu["mps"] == cpu["mps"]
+    print(f"[gpu] drop-in moving atom {case}: J {_rel(gpu['J'], cpu['J']):.2e} E {_rel(gpu['E'], cpu['E']):.2e} "
+          f"z {_rel(gpu['z'], cpu['z']):.2e}", flush=True)
+    assert _rel(gpu["J"], cpu["J"]) < 1e-11
+    assert _rel(gpu["E"], cpu["E"]) < 1e-12
+    assert _rel(gpu["z"], cpu["z"]) < 1e-12
+    assert abs(gpu["zabs"] - cpu["zabs"]) / cpu["zabs"] < 1e-12
+
+
 @pytest.mark.gpu
 @pytest.mark.parametrize("case", ["small_k", "noncubic", "grid8"])
 def test_reference_observables_with_gpu_dropins(case):
