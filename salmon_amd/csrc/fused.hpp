@@ -486,64 +486,85 @@ __global__ void __launch_bounds__(NY * NZ * XS, WPE) k_fused_step(FusedArgs<T> a
         if constexpr (DEFER && LT) {
           if (obs) {
             // nonlocal current and energy of this orbital (current.f90:247-269,
-            // total_energy.f90:191-199) at k_obs: per projector l the
-            // projection U = sum_j uV e psi and its first moments
-            // X_d = sum_j uV r_d e psi, products and sums in FP64; uV and the
-            // orbital values from the registers above, the position l H - j L
-            // from the grid point and its image offsets
+            // total_energy.f90:191-199) at k_obs, products and sums in FP64.
+            // First the projections U_l = sum_j uV(j,l) e_j psi_j.  The current
+            // sum_l sg_l Im(conj(U_l) X_l) of the first moments
+            // X_l = sum_j r_j uV(j,l) e_j psi_j is linear in the point terms once
+            // U is known: with w_j = e_j psi_j each point adds r_j c_j,
+            //   c_j = Re w_j sum_l sg_l Im U_l uV(j,l) - Im w_j sum_l sg_l Re U_l uV(j,l),
+            // so the wave sums 8 + 4 values instead of the 32 of (U, X).  uV and
+            // the orbital values come from the registers above, the position
+            // l H - j L from the grid point and its image offsets.
             const double hx = g.Hxyz, fo = occ / g.aLxyz * 2.0 * hx;
-            // two projectors per pass, which halves the live FP64 sums
-            constexpr int LH = kLA / 2;
+            double qu[2 * kLA];
 #pragma unroll
-            for (int lb = 0; lb < kLA; lb += LH) {
-              double qo[8 * LH];  // per l: Ur, Ui, Xr_x, Xi_x, Xr_y, Xi_y, Xr_z, Xi_z
+            for (int l = 0; l < 2 * kLA; ++l) qu[l] = 0;
+            cplx<T> w2[2];
 #pragma unroll
-              for (int l = 0; l < 8 * LH; ++l) qo[l] = 0;
-              auto moments = [&](int jp, const cplx<T>& v, const T (&u)[kLA]) __attribute__((always_inline)) {
-                const int pg = gix[jp], c = gimg[jp];
-                const cplx<T> w = cmul(en[jp], v);
-                const double px = (pg / NYZ) * g.Hx - (c % 5 - 2) * g.aLx;
-                const double py = ((pg / NZ) % NY) * g.Hy - ((c / 5) % 5 - 2) * g.aLy;
-                const double pz = (pg % NZ) * g.Hz - (c / 25 - 2) * g.aLz;
+            for (int s2 = 0; s2 < 2; ++s2) {
+              w2[s2] = cmul(en[jp2[s2]], v2[s2]);
+              if (ok2[s2]) {
 #pragma unroll
-                for (int l = 0; l < LH; ++l) {
-                  const double uu = (double)u[lb + l];
-                  const double tr = uu * (double)w.re, ti = uu * (double)w.im;
-                  qo[8 * l + 0] += tr;
-                  qo[8 * l + 1] += ti;
-                  qo[8 * l + 2] += px * tr;
-                  qo[8 * l + 3] += px * ti;
-                  qo[8 * l + 4] += py * tr;
-                  qo[8 * l + 5] += py * ti;
-                  qo[8 * l + 6] += pz * tr;
-                  qo[8 * l + 7] += pz * ti;
-                }
-              };
-#pragma unroll
-              for (int s2 = 0; s2 < 2; ++s2)
-                if (ok2[s2]) moments(jp2[s2], v2[s2], u2[s2]);
-              for (int j = lane + 2 * kWave; j < m; j += kWave) {
-                const int jp = j0 + j;
-                T u[kLA];
-#pragma unroll
-                for (int l = 0; l < kLA; ++l) u[l] = ug[l * kFusedMaxNPI + jp];
-                moments(jp, in[gix[jp]], u);
-              }
-              wave_sums(qo);
-#pragma unroll
-              for (int l = 0; l < LH; ++l) {
-                double r[8];
-#pragma unroll
-                for (int cc = 0; cc < 8; ++cc) r[cc] = lane_read(qo[0], wave_sums_lane(8 * l + cc));
-                if (lane == 0) {
-                  const double sg = ksgn[at * kLA + lb + l];
-                  const double Ur = r[0] * hx * sg, Ui = r[1] * hx * sg;
-                  red[wv * 10 + 6] += fo * (r[2] * Ui - r[3] * Ur);
-                  red[wv * 10 + 7] += fo * (r[4] * Ui - r[5] * Ur);
-                  red[wv * 10 + 8] += fo * (r[6] * Ui - r[7] * Ur);
-                  red[wv * 10 + 9] += occ * (r[0] * r[0] + r[1] * r[1]) * hx * hx * sg;
+                for (int l = 0; l < kLA; ++l) {
+                  qu[2 * l] += (double)u2[s2][l] * (double)w2[s2].re;
+                  qu[2 * l + 1] += (double)u2[s2][l] * (double)w2[s2].im;
                 }
               }
+            }
+            for (int j = lane + 2 * kWave; j < m; j += kWave) {
+              const int jp = j0 + j;
+              const cplx<T> w = cmul(en[jp], in[gix[jp]]);
+#pragma unroll
+              for (int l = 0; l < kLA; ++l) {
+                const double u = (double)ug[l * kFusedMaxNPI + jp];
+                qu[2 * l] += u * (double)w.re;
+                qu[2 * l + 1] += u * (double)w.im;
+              }
+            }
+            wave_sums(qu);
+            double P[kLA], Q[kLA], e9 = 0;
+#pragma unroll
+            for (int l = 0; l < kLA; ++l) {
+              const double ur = lane_read(qu[0], wave_sums_lane(2 * l));
+              const double ui = lane_read(qu[0], wave_sums_lane(2 * l + 1));
+              const double sg = ksgn[at * kLA + l];
+              P[l] = sg * ui;
+              Q[l] = sg * ur;
+              e9 += sg * (ur * ur + ui * ui);
+            }
+            double jm[4] = {0, 0, 0, 0};
+            auto moment = [&](int jp, const cplx<T>& w, const T (&u)[kLA]) __attribute__((always_inline)) {
+              const int pg = gix[jp], c = gimg[jp];
+              double sa = 0, sb = 0;
+#pragma unroll
+              for (int l = 0; l < kLA; ++l) {
+                sa += P[l] * (double)u[l];
+                sb += Q[l] * (double)u[l];
+              }
+              const double cj = (double)w.re * sa - (double)w.im * sb;
+              jm[0] += ((pg / NYZ) * g.Hx - (c % 5 - 2) * g.aLx) * cj;
+              jm[1] += (((pg / NZ) % NY) * g.Hy - ((c / 5) % 5 - 2) * g.aLy) * cj;
+              jm[2] += ((pg % NZ) * g.Hz - (c / 25 - 2) * g.aLz) * cj;
+            };
+#pragma unroll
+            for (int s2 = 0; s2 < 2; ++s2)
+              if (ok2[s2]) moment(jp2[s2], w2[s2], u2[s2]);
+            for (int j = lane + 2 * kWave; j < m; j += kWave) {
+              const int jp = j0 + j;
+              T u[kLA];
+#pragma unroll
+              for (int l = 0; l < kLA; ++l) u[l] = ug[l * kFusedMaxNPI + jp];
+              moment(jp, cmul(en[jp], in[gix[jp]]), u);
+            }
+            wave_sums(jm);
+            const double jx = lane_read(jm[0], wave_sums_lane(0));
+            const double jy = lane_read(jm[0], wave_sums_lane(1));
+            const double jz = lane_read(jm[0], wave_sums_lane(2));
+            if (lane == 0) {
+              red[wv * 10 + 6] += fo * hx * jx;
+              red[wv * 10 + 7] += fo * hx * jy;
+              red[wv * 10 + 8] += fo * hx * jz;
+              red[wv * 10 + 9] += occ * hx * hx * e9;
             }
           }
         }
@@ -657,9 +678,13 @@ __global__ void __launch_bounds__(NY * NZ * XS, WPE) k_fused_step(FusedArgs<T> a
           }
         }
         if constexpr (OBS) {
-          // this orbital's sums into the wave's accumulators
-          const double n2s = wave_sum(n2), vss = wave_sum(vs), tes = wave_sum(te);
-          const double j0 = wave_sum(js[0]), j1 = wave_sum(js[1]), j2 = wave_sum(js[2]);
+          // this orbital's sums into the wave's accumulators (one transposing
+          // wave reduction of the six sums)
+          double sv[8] = {n2, vs, te, js[0], js[1], js[2], 0.0, 0.0};
+          wave_sums(sv);
+          const double n2s = lane_read(sv[0], wave_sums_lane(0)), vss = lane_read(sv[0], wave_sums_lane(1));
+          const double tes = lane_read(sv[0], wave_sums_lane(2)), j0 = lane_read(sv[0], wave_sums_lane(3));
+          const double j1 = lane_read(sv[0], wave_sums_lane(4)), j2 = lane_read(sv[0], wave_sums_lane(5));
           if (lane == 0) {
             red[wv * 10 + 0] += occ * n2s;
             red[wv * 10 + 1] += occ * vss;
