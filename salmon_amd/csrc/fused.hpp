@@ -534,7 +534,7 @@ __global__ void __launch_bounds__(NY * NZ * XS, WPE) k_fused_step(FusedArgs<T> a
             }
             double jm[4] = {0, 0, 0, 0};
             auto moment = [&](int jp, const cplx<T>& w, const T (&u)[kLA]) __attribute__((always_inline)) {
-              const int pg = gix[jp], c = gimg[jp];
+              const unsigned pg = (unsigned)gix[jp], c = gimg[jp];
               double sa = 0, sb = 0;
 #pragma unroll
               for (int l = 0; l < kLA; ++l) {
@@ -542,9 +542,9 @@ __global__ void __launch_bounds__(NY * NZ * XS, WPE) k_fused_step(FusedArgs<T> a
                 sb += Q[l] * (double)u[l];
               }
               const double cj = (double)w.re * sa - (double)w.im * sb;
-              jm[0] += ((pg / NYZ) * g.Hx - (c % 5 - 2) * g.aLx) * cj;
-              jm[1] += (((pg / NZ) % NY) * g.Hy - ((c / 5) % 5 - 2) * g.aLy) * cj;
-              jm[2] += ((pg % NZ) * g.Hz - (c / 25 - 2) * g.aLz) * cj;
+              jm[0] += ((int)(pg / NYZ) * g.Hx - ((int)(c % 5) - 2) * g.aLx) * cj;
+              jm[1] += ((int)((pg / NZ) % NY) * g.Hy - ((int)((c / 5) % 5) - 2) * g.aLy) * cj;
+              jm[2] += ((int)(pg % NZ) * g.Hz - ((int)(c / 25) - 2) * g.aLz) * cj;
             };
 #pragma unroll
             for (int s2 = 0; s2 < 2; ++s2)
