@@ -329,6 +329,36 @@ int salmon_gpu_hartree_xc(salmon_ctx *ctx, const double *rho, int xc, double *vh
 enum { SALMON_NOALLOC_MAX_NL = 64 * 64 * 64 };
 int salmon_gpu_hartree_xc_noalloc(salmon_ctx *ctx, const double *rho_dev, int xc, double *vh_dev, double *vxc_dev);
 
+/* Region timers with the reference's region numbers (misc/timer.f90:21-54;
+ * timer_begin/timer_end :127-139, timer_get, timer_reset).  With the timers
+ * on, each entry point times its GPU work per region: the stream time between
+ * two events recorded around the region's launches, summed over calls.
+ * Regions nest as in the reference (SALMON_LOG_DYNAMICS holds the
+ * SALMON_LOG_DT_EVOLVE of its steps, which holds their SALMON_LOG_HPSI on the
+ * streamed path; the fused 16^3 step is one launch and counts as
+ * SALMON_LOG_DT_EVOLVE only).  The current and energy rows come from one
+ * kernel and count as SALMON_LOG_CURRENT; the Hartree and exchange-correlation
+ * potential from one chain and count as SALMON_LOG_HARTREE.  A region entered
+ * while the context's stream is captured into a graph is not timed (the region
+ * around the graph's replay is).  Every timed region is also a roctx range
+ * named after the reference's constant ("LOG_HPSI", ...), for
+ * rocprofv3 --marker-trace.  Timing adds a few microseconds per region. */
+enum {
+  SALMON_LOG_ALL = 0, SALMON_LOG_GROUND_STATE = 2, SALMON_LOG_DYNAMICS = 3,
+  SALMON_LOG_CG = 10, SALMON_LOG_DIAG = 11, SALMON_LOG_SP_ENERGY = 12, SALMON_LOG_GRAM_SCHMIDT = 13,
+  SALMON_LOG_DT_EVOLVE = 20, SALMON_LOG_HPSI = 21, SALMON_LOG_PSI_RHO = 22, SALMON_LOG_HARTREE = 23,
+  SALMON_LOG_CURRENT = 25, SALMON_LOG_TOTAL_ENERGY = 26,
+  SALMON_LOG_MAX = 41
+};
+/* on = 1 starts timing (the sums kept), 0 stops it. */
+int salmon_gpu_timer_enable(salmon_ctx *ctx, int on);
+/* timer_reset: all sums to zero (waits for the context's timed work). */
+int salmon_gpu_timer_reset(salmon_ctx *ctx);
+/* timer_get(id): seconds of GPU time and the number of timed calls of region
+ * id (0 <= id < SALMON_LOG_MAX; waits for the context's timed work).
+ * SALMON_LOG_ALL is the sum of the outermost regions' times. */
+int salmon_gpu_timer_get(salmon_ctx *ctx, int id, double *seconds, int *calls);
+
 /* Waits for all work of the context. */
 int salmon_gpu_synchronize(salmon_ctx *ctx);
 

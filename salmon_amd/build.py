@@ -18,6 +18,8 @@ HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-shared", f"--offload-arch={ARCH}",
              # keep hipcc from forming packed-FP32 instructions
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-Wall", "-Wno-unused-function"]
+# roctx ranges of the region timers (rocprofv3 --marker-trace)
+HIP_LIBS = ["-L/opt/rocm/lib", "-lrocprofiler-sdk-roctx", "-Wl,-rpath,/opt/rocm/lib"]
 
 
 def lib_path():
@@ -29,7 +31,7 @@ def build_hip(verbose=False):
     out = lib_path()
     tmp = out + ".tmp"
     cmd = [HIPCC] + HIP_FLAGS + ["-I", os.path.join(ROOT, "include"), "-o", tmp,
-                                 os.path.join(HERE, "csrc", HIP_SOURCES[0])]
+                                 os.path.join(HERE, "csrc", HIP_SOURCES[0])] + HIP_LIBS
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError("hipcc failed:\n" + res.stdout + res.stderr)

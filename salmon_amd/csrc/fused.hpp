@@ -1160,6 +1160,7 @@ static int fused_launch(salmon_ctx* c, const salmon::FusedArgs<T>& args, int nwg
 template <typename T>
 static int fused_step(salmon_ctx* c, const double* amid, const double* aobs, double dt, double* res_dev,
                       bool rho_post = false, bool want_rho = true) {
+  Region reg(c, SALMON_LOG_DT_EVOLVE);
   FusedState* f = (FusedState*)c->fz;
   if (!f || !f->tables) return fail(c, SALMON_ERR_UNSUPPORTED, "fused tables not built");
   int ncu = 256;

@@ -12,12 +12,30 @@
 #include <string>
 #include <vector>
 
+#include <rocprofiler-sdk-roctx/roctx.h>
+
 #include "../../include/salmon_gpu.h"
 #include "common.hpp"
 #include "streamed.hpp"
 
 using namespace salmon;
 
+// Region timers of a context (salmon_gpu_timer_*): per region the summed
+// stream time and the number of timed calls; event pairs of finished regions
+// wait in `pending` until the host asks for the sums.
+struct RegionTimers {
+  bool on = false;
+  int depth = 0;  // regions open on the host
+  double sec[SALMON_LOG_MAX] = {};
+  int calls[SALMON_LOG_MAX] = {};
+  struct Open {
+    int id;
+    bool outer;
+    hipEvent_t b, e;
+  };
+  std::vector<Open> pending;
+  std::vector<hipEvent_t> pool;
+};
 
 struct salmon_ctx {
   int precision = SALMON_FP64;
@@ -84,9 +102,105 @@ struct salmon_ctx {
   // the band-energy sum by less than pc_stall times the previous sweep's drop
   int pc_max_sweeps = 0;
   double pc_stall = 0.3;
+  RegionTimers* tm = nullptr;  // salmon_gpu_timer_enable
 };
 
 namespace {
+
+// the reference's names of the timed regions (misc/timer.f90:21-54)
+const char* log_name(int id) {
+  switch (id) {
+    case SALMON_LOG_GROUND_STATE: return "LOG_GROUND_STATE";
+    case SALMON_LOG_DYNAMICS: return "LOG_DYNAMICS";
+    case SALMON_LOG_CG: return "LOG_CG";
+    case SALMON_LOG_DIAG: return "LOG_DIAG";
+    case SALMON_LOG_SP_ENERGY: return "LOG_SP_ENERGY";
+    case SALMON_LOG_GRAM_SCHMIDT: return "LOG_GRAM_SCHMIDT";
+    case SALMON_LOG_DT_EVOLVE: return "LOG_DT_EVOLVE";
+    case SALMON_LOG_HPSI: return "LOG_HPSI";
+    case SALMON_LOG_PSI_RHO: return "LOG_PSI_RHO";
+    case SALMON_LOG_HARTREE: return "LOG_HARTREE";
+    case SALMON_LOG_CURRENT: return "LOG_CURRENT";
+    case SALMON_LOG_TOTAL_ENERGY: return "LOG_TOTAL_ENERGY";
+    default: return "LOG_OTHER";
+  }
+}
+
+// Sums the finished regions' event pairs (waits for their end events).
+void timers_resolve(salmon_ctx* c) {
+  RegionTimers* t = c->tm;
+  if (!t) return;
+  for (const auto& o : t->pending) {
+    float ms = 0.0f;
+    if (hipEventSynchronize(o.e) == hipSuccess && hipEventElapsedTime(&ms, o.b, o.e) == hipSuccess) {
+      t->sec[o.id] += 1e-3 * ms;
+      t->calls[o.id] += 1;
+      if (o.outer) {
+        t->sec[SALMON_LOG_ALL] += 1e-3 * ms;
+        t->calls[SALMON_LOG_ALL] += 1;
+      }
+    }
+    t->pool.push_back(o.b);
+    t->pool.push_back(o.e);
+  }
+  t->pending.clear();
+}
+
+// One timed region (timer_begin at construction, timer_end at scope exit):
+// events on the context's stream around the launches enqueued in between, and
+// a roctx range of the region's name.  Nothing is recorded with the timers
+// off or while the stream is being captured.
+class Region {
+ public:
+  Region(salmon_ctx* c, int id) : c_(c), id_(id) {
+    RegionTimers* t = c->tm;
+    if (!t || !t->on || id < 0 || id >= SALMON_LOG_MAX) return;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(c->stream, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return;
+    b_ = take(t);
+    e_ = take(t);
+    if (!b_ || !e_ || hipEventRecord(b_, c->stream) != hipSuccess) {
+      if (b_) t->pool.push_back(b_);
+      if (e_) t->pool.push_back(e_);
+      return;
+    }
+    outer_ = t->depth == 0;
+    ++t->depth;
+    roctxRangePushA(log_name(id));
+    live_ = true;
+  }
+  ~Region() {
+    if (!live_) return;
+    RegionTimers* t = c_->tm;
+    roctxRangePop();
+    --t->depth;
+    if (hipEventRecord(e_, c_->stream) == hipSuccess) {
+      t->pending.push_back({id_, outer_, b_, e_});
+    } else {
+      t->pool.push_back(b_);
+      t->pool.push_back(e_);
+    }
+    // bound the events held: a long timed run sums as it goes
+    if (t->pending.size() >= 4096) timers_resolve(c_);
+  }
+  Region(const Region&) = delete;
+  Region& operator=(const Region&) = delete;
+
+ private:
+  static hipEvent_t take(RegionTimers* t) {
+    if (!t->pool.empty()) {
+      hipEvent_t e = t->pool.back();
+      t->pool.pop_back();
+      return e;
+    }
+    hipEvent_t e = nullptr;
+    return hipEventCreate(&e) == hipSuccess ? e : nullptr;
+  }
+  salmon_ctx* c_;
+  int id_;
+  bool outer_ = false, live_ = false;
+  hipEvent_t b_ = nullptr, e_ = nullptr;
+};
 
 int fail(salmon_ctx* c, int code, const char* fmt, ...) {
   if (c) {
@@ -178,6 +292,7 @@ template <typename T>
 int apply_h(salmon_ctx* c, const cplx<T>* in, cplx<T>* out, cplx<T>* tt, int orb0, int nOrb, int NB, int NK,
             const double* kbase, double ax, double ay, double az, void* uvbuf = nullptr,
             const cplx<T>* ph = nullptr) {
+  Region reg(c, SALMON_LOG_HPSI);
   cplx<T>* uv = (cplx<T>*)(uvbuf ? uvbuf : c->d_uvpsi);
   if (c->kb.Nlma > 0 && NB >= kMfmaMinBands) {
     // the bands of each k point as the rows of one matrix-core GEMM per atom
@@ -202,6 +317,7 @@ int apply_h(salmon_ctx* c, const cplx<T>* in, cplx<T>* out, cplx<T>* tt, int orb
 
 template <typename T>
 int taylor_streamed(salmon_ctx* c, const double* kbase, double ax, double ay, double az, double dt) {
+  Region reg(c, SALMON_LOG_DT_EVOLVE);
   int r = ensure_work(c);
   if (r) return r;
   // zfac(n) = (-i dt)^n / n!  (hamiltonian.f90:46-49)
@@ -271,6 +387,7 @@ int density_reduce(salmon_ctx* c, const double* part, int nchunk) {
 
 template <typename T>
 int density_dev(salmon_ctx* c) {
+  Region reg(c, SALMON_LOG_PSI_RHO);
   const int NL = c->g.NL;
   if (!c->d_rho_part) {
     c->rho_chunks = std::max(1, std::min(c->NKB, std::max(1, 4096 * 1024 / std::max(1, NL))));
@@ -290,6 +407,7 @@ int gram_schmidt_multi_fwd(salmon_ctx* c, cplx<double>* zu);
 // FP64 Gram-Schmidt of the resident orbitals (for complex64 orbitals on an
 // FP64 copy)
 int gram_schmidt_dev(salmon_ctx* c) {
+  Region reg(c, SALMON_LOG_GRAM_SCHMIDT);
   const size_t n = (size_t)c->NKB * c->g.NL;
   if (c->precision == SALMON_FP64) return gram_schmidt_multi_fwd(c, (cplx<double>*)c->d_zu);
   if (!c->d_f64) {
@@ -393,6 +511,7 @@ int scf_check_grid(salmon_ctx* c) {
 
 // scf_potential_on with the context's own twiddles and grids (allocated on first use).
 int scf_potential(salmon_ctx* c, const double* rho, bool set_v, int xc, double* vh, double* vxc) {
+  Region reg(c, SALMON_LOG_HARTREE);
   const Grid& g = c->g;
   int r;
   if ((r = scf_check_grid(c))) return r;
@@ -426,6 +545,7 @@ int rt_step(salmon_ctx* c, const double a0[3], const double a1[3], double dt, do
     if ((r = taylor_streamed<T>(c, c->d_kac0, amid[0], amid[1], amid[2], dt))) return r;
   }
   if ((r = density_dev<T>(c))) return r;
+  Region reg(c, SALMON_LOG_CURRENT);
   if ((r = observe_rows<T>(c, c->d_kac0, anew[0], anew[1], anew[2]))) return r;
   hipLaunchKernelGGL(k_sum_rows, dim3(1), dim3(256), 0, c->stream, c->d_rows, c->NKB, res_dev);
   LAUNCHCHK(c);
@@ -551,6 +671,7 @@ int rt_run_scf(salmon_ctx* c, const double* ac, int nsteps, double dt, int reort
       // FP64: an observables-only launch with Vloc(t+dt) in place
       if ((r = fused_step<T>(c, nullptr, anew, dt, d_res + 8 * (size_t)n))) break;
     } else {
+      Region reg(c, SALMON_LOG_CURRENT);
       if ((r = observe_rows<T>(c, c->d_kac0, anew[0], anew[1], anew[2]))) break;
       hipLaunchKernelGGL(k_sum_rows, dim3(1), dim3(256), 0, c->stream, c->d_rows, c->NKB, d_res + 8 * (size_t)n);
       LAUNCHCHK(c);
@@ -930,6 +1051,7 @@ int salmon_gpu_hamiltonian(salmon_ctx* c, const double* kAc, double dt) {
 int salmon_gpu_current(salmon_ctx* c, const double* kAc, double jav[3]) {
   int r = check_ready(c, true);
   if (r) return r;
+  Region reg(c, SALMON_LOG_CURRENT);
   if (!jav) return fail(c, SALMON_ERR_ARG, "jav is NULL");
   if ((r = upload_kac(c, kAc))) return r;
   r = c->precision == SALMON_FP64 ? observe_rows<double>(c, c->d_kac, 0, 0, 0) : observe_rows<float>(c, c->d_kac, 0, 0, 0);
@@ -943,6 +1065,7 @@ int salmon_gpu_current(salmon_ctx* c, const double* kAc, double jav[3]) {
 int salmon_gpu_energy(salmon_ctx* c, const double* kAc, double* ekin, double* enl, double* eloc) {
   int r = check_ready(c, true);
   if (r) return r;
+  Region reg(c, SALMON_LOG_TOTAL_ENERGY);
   if ((r = upload_kac(c, kAc))) return r;
   r = c->precision == SALMON_FP64 ? observe_rows<double>(c, c->d_kac, 0, 0, 0) : observe_rows<float>(c, c->d_kac, 0, 0, 0);
   if (r) return r;
@@ -989,6 +1112,7 @@ int salmon_gpu_gram_schmidt(salmon_ctx* c) {
 int salmon_gpu_sp_energy(salmon_ctx* c, const double* kAc, double* esp) {
   int r = check_ready(c, true);
   if (r) return r;
+  Region reg(c, SALMON_LOG_SP_ENERGY);
   if (!esp) return fail(c, SALMON_ERR_ARG, "esp is NULL");
   if ((r = upload_kac(c, kAc))) return r;
   if ((r = ensure_work(c))) return r;
@@ -1051,6 +1175,7 @@ static int cg_results(salmon_ctx* c, double* esp_var, int* iters) {
 int salmon_gpu_cg(salmon_ctx* c, const double* kAc, int iter_cg_max, int precond, double* esp_var, int* iters) {
   int r = gs_check(c, kAc);
   if (r) return r;
+  Region reg(c, SALMON_LOG_CG);
   if (iter_cg_max < 0) return fail(c, SALMON_ERR_ARG, "iter_cg_max < 0");
   if ((r = check_precond(c, precond))) return r;
   if ((r = cg_call(c, iter_cg_max, precond))) return r;
@@ -1060,6 +1185,7 @@ int salmon_gpu_cg(salmon_ctx* c, const double* kAc, int iter_cg_max, int precond
 int salmon_gpu_subspace_diag(salmon_ctx* c, const double* kAc, double* esp) {
   int r = gs_check(c, kAc);
   if (r) return r;
+  Region reg(c, SALMON_LOG_DIAG);
   return subspace_diag(c, esp);
 }
 
@@ -1067,6 +1193,7 @@ int salmon_gpu_ground_state(salmon_ctx* c, const double* kAc, int max_sweeps, in
                             double tol, double* esp, double* esp_var, int* sweeps, double* history) {
   int r = gs_check(c, kAc);
   if (r) return r;
+  Region reg(c, SALMON_LOG_GROUND_STATE);
   if (max_sweeps < 1 || iter_cg_max < 1) return fail(c, SALMON_ERR_ARG, "max_sweeps and iter_cg_max must be >= 1");
   if ((r = check_precond(c, precond))) return r;
   std::vector<double> e0(c->NKB, 0.0), e1(c->NKB), var(c->NKB);
@@ -1398,6 +1525,7 @@ int salmon_gpu_rt_run(salmon_ctx* c, const double* ac, int nsteps, double dt, in
   int r = check_ready(c, true);
   if (r) return r;
   if (!ac || nsteps < 0) return fail(c, SALMON_ERR_ARG, "ac is NULL or nsteps < 0");
+  Region reg(c, SALMON_LOG_DYNAMICS);
   if (nsteps == 0) return SALMON_OK;
   // the per-step results stay allocated between runs, sized for at least
   // 4096 steps (256 KiB), so runs of the usual length allocate nothing
@@ -1541,6 +1669,34 @@ int salmon_gpu_synchronize(salmon_ctx* c) {
   return SALMON_OK;
 }
 
+int salmon_gpu_timer_enable(salmon_ctx* c, int on) {
+  if (!c) return SALMON_ERR_ARG;
+  if (on && !c->tm) c->tm = new RegionTimers();
+  if (c->tm) c->tm->on = on != 0;
+  return SALMON_OK;
+}
+
+int salmon_gpu_timer_reset(salmon_ctx* c) {
+  if (!c) return SALMON_ERR_ARG;
+  if (!c->tm) return SALMON_OK;
+  timers_resolve(c);
+  for (int i = 0; i < SALMON_LOG_MAX; ++i) {
+    c->tm->sec[i] = 0.0;
+    c->tm->calls[i] = 0;
+  }
+  return SALMON_OK;
+}
+
+int salmon_gpu_timer_get(salmon_ctx* c, int id, double* seconds, int* calls) {
+  if (!c) return SALMON_ERR_ARG;
+  if (id < 0 || id >= SALMON_LOG_MAX) return fail(c, SALMON_ERR_ARG, "timer region %d outside 0..%d", id, SALMON_LOG_MAX - 1);
+  if (!seconds && !calls) return fail(c, SALMON_ERR_ARG, "seconds and calls are NULL");
+  timers_resolve(c);
+  if (seconds) *seconds = c->tm ? c->tm->sec[id] : 0.0;
+  if (calls) *calls = c->tm ? c->tm->calls[id] : 0;
+  return SALMON_OK;
+}
+
 int salmon_gpu_set_path(salmon_ctx* c, int path) {
   if (!c) return SALMON_ERR_ARG;
   if (path != SALMON_PATH_AUTO && path != SALMON_PATH_STREAMED) return fail(c, SALMON_ERR_ARG, "unknown path %d", path);
@@ -1570,6 +1726,11 @@ int salmon_gpu_finalize(salmon_ctx* c) {
   for (int i = 0; i < 4; ++i) dfree(&c->d_work[i]);
   for (int i = 0; i < 3; ++i) dfree(&c->d_stage[i]);
   for (void* p : c->kb_bufs) (void)hipFree(p);
+  if (c->tm) {
+    timers_resolve(c);
+    for (hipEvent_t e : c->tm->pool) (void)hipEventDestroy(e);
+    delete c->tm;
+  }
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return SALMON_OK;

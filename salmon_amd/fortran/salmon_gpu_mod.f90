@@ -361,6 +361,27 @@ module salmon_gpu
       type(c_ptr), value :: ctx
     end function
 
+    ! region timers; region ids are the LOG_* numbers of the reference's
+    ! timer module (misc/timer.f90:21-54), e.g. LOG_DYNAMICS, LOG_HPSI
+    integer(c_int) function salmon_gpu_timer_enable(ctx, on) bind(C, name="salmon_gpu_timer_enable")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: on
+    end function
+
+    integer(c_int) function salmon_gpu_timer_reset(ctx) bind(C, name="salmon_gpu_timer_reset")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+
+    integer(c_int) function salmon_gpu_timer_get(ctx, id, seconds, calls) bind(C, name="salmon_gpu_timer_get")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: id
+      real(c_double), intent(out) :: seconds
+      integer(c_int), intent(out) :: calls
+    end function
+
     type(c_ptr) function salmon_gpu_stream(ctx) bind(C, name="salmon_gpu_stream")
       import :: c_ptr
       type(c_ptr), value :: ctx

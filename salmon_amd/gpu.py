@@ -43,6 +43,9 @@ PROP_MIDPOINT, PROP_ETRS = 0, 1
 XC_NONE, XC_PZ = 0, 1
 PC_NONE, PC_KINETIC, PC_LEARNED = 0, 1, 2
 PCNET_WEIGHTS = 27 * 2 * 16 + 27 * 16 * 16 + 27 * 16 * 2
+# region timers: the reference's region numbers (misc/timer.f90:21-54)
+LOG = dict(ALL=0, GROUND_STATE=2, DYNAMICS=3, CG=10, DIAG=11, SP_ENERGY=12, GRAM_SCHMIDT=13, DT_EVOLVE=20, HPSI=21,
+           PSI_RHO=22, HARTREE=23, CURRENT=25, TOTAL_ENERGY=26)
 
 
 class SalmonGPUError(RuntimeError):
@@ -108,6 +111,9 @@ SIGNATURES = {
     "salmon_gpu_set_path": [_vp, _ci],
     "salmon_gpu_path": [_vp],
     "salmon_gpu_synchronize": [_vp],
+    "salmon_gpu_timer_enable": [_vp, _ci],
+    "salmon_gpu_timer_reset": [_vp],
+    "salmon_gpu_timer_get": [_vp, _ci, _vp, _vp],
     "salmon_gpu_stream": [_vp],
     "salmon_gpu_errstr": [_ci],
     "salmon_gpu_last_error": [_vp],
@@ -383,6 +389,29 @@ class Context:
 
     def synchronize(self):
         self._check(lib().salmon_gpu_synchronize(self._h), "synchronize")
+
+    def timer_enable(self, on=True):
+        """Per-region GPU timers (timer_begin/timer_end of misc/timer.f90)."""
+        self._check(lib().salmon_gpu_timer_enable(self._h, int(bool(on))), "timer_enable")
+
+    def timer_reset(self):
+        self._check(lib().salmon_gpu_timer_reset(self._h), "timer_reset")
+
+    def timer_get(self, region):
+        """(seconds, calls) of a region, by number or by the name in LOG."""
+        rid = LOG[region] if isinstance(region, str) else int(region)
+        sec, calls = ctypes.c_double(), ctypes.c_int()
+        self._check(lib().salmon_gpu_timer_get(self._h, rid, ctypes.byref(sec), ctypes.byref(calls)), "timer_get")
+        return sec.value, calls.value
+
+    def timers(self):
+        """{name: (seconds, calls)} of every region timed so far."""
+        out = {}
+        for name in LOG:
+            t = self.timer_get(name)
+            if t[1]:
+                out[name] = t
+        return out
 
 
 class PcNet:

@@ -35,3 +35,23 @@ def test_errstr_without_device():
     L = gpu.lib()
     assert L.salmon_gpu_errstr(0) == b"ok"
     assert L.salmon_gpu_errstr(1) == b"invalid argument"
+
+
+def test_timer_entry_points_reject_null_context():
+    """The region timers' argument checks, without a device."""
+    L = gpu.lib()
+    s, n = ctypes.c_double(), ctypes.c_int()
+    assert L.salmon_gpu_timer_enable(None, 1) == 1
+    assert L.salmon_gpu_timer_reset(None) == 1
+    assert L.salmon_gpu_timer_get(None, gpu.LOG["DYNAMICS"], ctypes.byref(s), ctypes.byref(n)) == 1
+
+
+def test_timer_region_numbers_are_the_reference_ones():
+    """misc/timer.f90:21-54 numbers the regions; the header's enum and the
+    Python table carry the same numbers."""
+    src = open(os.path.join(ROOT, "include", "salmon_gpu.h")).read()
+    hdr = {m.group(1): int(m.group(2)) for m in re.finditer(r"SALMON_LOG_(\w+) = (\d+)", src)}
+    ref = dict(ALL=0, GROUND_STATE=2, DYNAMICS=3, CG=10, DIAG=11, SP_ENERGY=12, GRAM_SCHMIDT=13, DT_EVOLVE=20,
+               HPSI=21, PSI_RHO=22, HARTREE=23, CURRENT=25, TOTAL_ENERGY=26)
+    assert {k: v for k, v in hdr.items() if k != "MAX"} == ref
+    assert gpu.LOG == ref

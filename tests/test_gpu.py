@@ -98,6 +98,22 @@ def test_cfg1_trajectory_fp64_matches_reference():
     assert rel(z[:, :2], d["rt_zu_head"]) < 1e-10
 
 
+def test_cfg1_trajectory_mixed_against_reference():
+    """The complex64 step (fused 16^3 kernel, FP64 sums) on the same 100 steps,
+    held directly to the reference Fortran's J(t), E(t) and band norms.
+    Tolerance: 1e-5 relative (complex64 stencil, KB and Taylor)."""
+    s, d = load_case("cfg1")
+    with _ctx(s, gpu.MIXED) as c:
+        r = c.rt_run(d["ac"], 100)
+        z = c.get_orbitals()
+    bn = np.sum(np.abs(z) ** 2, axis=2) * s.Hxyz
+    print(f"[gpu] cfg1 mixed vs reference: J {rel(r['J'], d['rt_J']):.2e} E {rel(r['E'], d['rt_E']):.2e} "
+          f"band norms {rel(bn, d['rt_band_norm']):.2e}", flush=True)
+    assert rel(r["J"], d["rt_J"]) < 1e-5
+    assert rel(r["E"], d["rt_E"]) < 1e-5
+    assert rel(bn, d["rt_band_norm"]) < 1e-5
+
+
 def test_cfg3_mixed_precision_error_against_fp64():
     """complex64 stencil/KB/Taylor with FP64 sums and re-orthonormalisation
     every 50 steps, against the FP64 run with the same schedule."""
