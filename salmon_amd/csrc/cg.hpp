@@ -188,15 +188,27 @@ __global__ void __launch_bounds__(256) k_cg_proj(State S, const cplx<double>* __
   __syncthreads();
   if (!is_last) return;
   __shared__ cplx<double> zsh[kCgMaxNB];
-  // one thread per lower band sums its chunks in chunk order
+  // one thread per lower band sums its chunks in chunk order; the partials of
+  // kTailBatch chunks are loaded (clamped chunk index) before any is added,
+  // so one cache round trip serves the batch
+  constexpr int kTailBatch = 32;
   for (int j = threadIdx.x; j < ib; j += 256) {
     const int grp = j / kJG, jj = j % kJG;
     double sr = 0.0, si = 0.0;
-#pragma unroll 8
-    for (int cc = 0; cc < S.nchunk; ++cc) {
-      const size_t b = (size_t)(grp * S.nchunk + cc) * kPartQ;
-      sr += __hip_atomic_load(&part[b + 2 * jj], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      si += __hip_atomic_load(&part[b + 2 * jj + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    for (int c0 = 0; c0 < S.nchunk; c0 += kTailBatch) {
+      double vr[kTailBatch], vi[kTailBatch];
+#pragma unroll
+      for (int u = 0; u < kTailBatch; ++u) {
+        const size_t b = (size_t)(grp * S.nchunk + min(c0 + u, S.nchunk - 1)) * kPartQ;
+        vr[u] = __hip_atomic_load(&part[b + 2 * jj], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        vi[u] = __hip_atomic_load(&part[b + 2 * jj + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+#pragma unroll
+      for (int u = 0; u < kTailBatch; ++u) {
+        const bool ok = c0 + u < S.nchunk;
+        sr = ok ? sr + vr[u] : sr;
+        si = ok ? si + vi[u] : si;
+      }
     }
     zsh[j] = cplx<double>{sr * S.Hxyz, si * S.Hxyz};
   }
@@ -227,15 +239,34 @@ __global__ void __launch_bounds__(256) k_cg_proj(State S, const cplx<double>* __
     for (int j = threadIdx.x; j < ib; j += 256) S.zs[(size_t)k * S.NB + j] = zsh[j];
     return;
   }
+  // c = M b, one thread per row in column order; the row's entries of
+  // kTailBatch columns are loaded (clamped column index) before any is used
   for (int j = threadIdx.x; j < ib; j += 256) {
     double cr = zsh[j].re, ci = mode == PROJ_START ? 0.0 : zsh[j].im;
     if (mode == PROJ_START) {
-      for (int l = 0; l < j; ++l) cr += MR[(size_t)j * S.NB + l] * zsh[l].re;
+      for (int l0 = 0; l0 < j; l0 += kTailBatch) {
+        double mr[kTailBatch];
+#pragma unroll
+        for (int u = 0; u < kTailBatch; ++u) mr[u] = MR[(size_t)j * S.NB + min(l0 + u, j - 1)];
+#pragma unroll
+        for (int u = 0; u < kTailBatch; ++u) {
+          const int l = l0 + u;
+          cr = l < j ? cr + mr[u] * zsh[min(l, j - 1)].re : cr;
+        }
+      }
     } else {
-      for (int l = 0; l < j; ++l) {
-        const cplx<double> m = M[(size_t)j * S.NB + l], z = zsh[l];
-        cr += m.re * z.re - m.im * z.im;
-        ci += m.re * z.im + m.im * z.re;
+      for (int l0 = 0; l0 < j; l0 += kTailBatch) {
+        cplx<double> mm[kTailBatch];
+#pragma unroll
+        for (int u = 0; u < kTailBatch; ++u) mm[u] = M[(size_t)j * S.NB + min(l0 + u, j - 1)];
+#pragma unroll
+        for (int u = 0; u < kTailBatch; ++u) {
+          const int l = l0 + u;
+          const cplx<double> m = mm[u], z = zsh[min(l, j - 1)];
+          const double tr = cr + (m.re * z.re - m.im * z.im), ti = ci + (m.re * z.im + m.im * z.re);
+          cr = l < j ? tr : cr;
+          ci = l < j ? ti : ci;
+        }
       }
     }
     S.zs[(size_t)k * S.NB + j] = cplx<double>{cr, ci};
