@@ -276,13 +276,20 @@ const char *salmon_gpu_batch_last_error(const salmon_batch *batch);
 int salmon_gpu_batch_finalize(salmon_batch *batch);
 
 /* Kernel path of the RT step.  SALMON_PATH_AUTO uses the LDS-resident fused
- * step where the grid has a compiled shape (16^3) and the streamed kernels
- * otherwise; SALMON_PATH_STREAMED forces the streamed kernels.  Like the
- * reference's build-time stencil selection (ARTED/stencil/CMakeLists.txt:1-52)
- * the choice changes the code path, not the results beyond rounding.
- * salmon_gpu_path returns the path rt_run takes (SALMON_PATH_FUSED or
- * SALMON_PATH_STREAMED). */
-enum { SALMON_PATH_AUTO = 0, SALMON_PATH_STREAMED = 1, SALMON_PATH_FUSED = 2 };
+ * step where the grid has a compiled shape (16^3); on other grids of at least
+ * 8 points per axis, with the midpoint propagator and the frozen potential,
+ * the persistent sweep kernel of the batches (one launch per step, the
+ * context as a batch of one on its own stream, its work buffers four times
+ * the orbitals' size; a set-up that fails falls back); the streamed kernels
+ * otherwise.  SALMON_PATH_STREAMED forces the streamed kernels,
+ * SALMON_PATH_SWEEP the persistent sweep (rt_run then fails with
+ * SALMON_ERR_UNSUPPORTED where it does not apply).  Like the reference's
+ * build-time stencil selection (ARTED/stencil/CMakeLists.txt:1-52) the choice
+ * changes the code path, not the results beyond rounding.  salmon_gpu_path
+ * returns the path rt_run takes (SALMON_PATH_FUSED, SALMON_PATH_SWEEP or
+ * SALMON_PATH_STREAMED).  The other entry points (hamiltonian, hpsi, the
+ * observables) take the streamed kernels. */
+enum { SALMON_PATH_AUTO = 0, SALMON_PATH_STREAMED = 1, SALMON_PATH_FUSED = 2, SALMON_PATH_SWEEP = 3 };
 int salmon_gpu_set_path(salmon_ctx *ctx, int path);
 int salmon_gpu_path(salmon_ctx *ctx);
 

@@ -658,8 +658,15 @@ __global__ void __launch_bounds__(NY * NZ * XS, WPE) k_fused_step(FusedArgs<T> a
             h[i].re = lr + d.re;
             h[i].im = li + d.im;
             if (n < 3) out[(x0 + i) * NYZ + yz] = h[i];  // the fourth result only enters the Taylor sum
-            acc[i].re += zr * h[i].re - zi * h[i].im;
-            acc[i].im += zr * h[i].im + zi * h[i].re;
+            if constexpr (sizeof(T) == 4) {
+              // one of zr, zi is zero: two FMAs per part, the zero one adding
+              // an exact zero (complex64; FP64 keeps the reference's rounding)
+              acc[i].re = fmaf(zr, h[i].re, fmaf(-zi, h[i].im, acc[i].re));
+              acc[i].im = fmaf(zr, h[i].im, fmaf(zi, h[i].re, acc[i].im));
+            } else {
+              acc[i].re += zr * h[i].re - zi * h[i].im;
+              acc[i].im += zr * h[i].im + zi * h[i].re;
+            }
             if constexpr (XS > 1) {
               if (i >= 4) win[H + i - 4] = h[i - 4];
             } else {

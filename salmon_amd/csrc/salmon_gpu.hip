@@ -103,6 +103,13 @@ struct salmon_ctx {
   int pc_max_sweeps = 0;
   double pc_stall = 0.3;
   RegionTimers* tm = nullptr;  // salmon_gpu_timer_enable
+  // the persistent sweep of rt_run off the fused shapes (SALMON_PATH_SWEEP):
+  // a batch of this context alone, on the context's stream, rebuilt when the
+  // buffer generation moves on; solo_fail marks a set-up that failed at
+  // generation solo_gen (SALMON_PATH_AUTO then takes the streamed kernels)
+  salmon_batch* solo = nullptr;
+  unsigned long solo_gen = 0;
+  bool solo_fail = false;
 };
 
 namespace {
@@ -680,6 +687,81 @@ int rt_run_scf(salmon_ctx* c, const double* ac, int nsteps, double dt, int reort
   }
   if (r == SALMON_OK && pending) r = fused_step<T>(c, nullptr, aobs, dt, d_res + 8 * (size_t)(nsteps - 1));
   return r;
+}
+
+// The persistent sweep kernel of the batches run for this context alone: the
+// midpoint propagator with the frozen potential, grids of at least 8 points
+// per axis.
+template <typename T>
+bool sweep_eligible(const salmon_ctx* c) {
+  if (c->scf || c->propagator != SALMON_PROP_MIDPOINT) return false;
+  return sweep_shape<T>(c->g)[0] > 0;
+}
+
+// The kernel path rt_run takes.
+int rt_path(const salmon_ctx* c) {
+  if (fused_ready(c)) return SALMON_PATH_FUSED;
+  if (c->scf || c->path == SALMON_PATH_STREAMED) return SALMON_PATH_STREAMED;
+  if (c->path == SALMON_PATH_SWEEP) return SALMON_PATH_SWEEP;
+  const bool ok = c->precision == SALMON_FP64 ? sweep_eligible<double>(c) : sweep_eligible<float>(c);
+  if (ok && !(c->solo_fail && c->solo_gen == c->gen)) return SALMON_PATH_SWEEP;
+  return SALMON_PATH_STREAMED;
+}
+
+// The context's one-system batch, built (descriptors, work buffers, sweep
+// plan) for the current buffers.  Runs no step.
+template <typename T>
+int solo_prepare(salmon_ctx* c) {
+  if (!sweep_eligible<T>(c))
+    return fail(c, SALMON_ERR_UNSUPPORTED,
+                "persistent sweep: needs the midpoint propagator, the frozen potential and >= 8 points per axis");
+  if (!c->solo) {
+    salmon_batch* b = new salmon_batch();
+    b->precision = c->precision;
+    b->stream = c->stream;  // not owned: released before the batch is deleted
+    b->kernel_req = SALMON_BATCH_SWEEP;
+    b->ctx.push_back(c);
+    c->solo = b;
+    c->solo_gen = c->gen;
+  }
+  salmon_batch* b = c->solo;
+  if (c->solo_gen != c->gen) {
+    (void)hipStreamSynchronize(c->stream);
+    batch_release(b);
+    b->tuned = false;
+    c->solo_gen = c->gen;
+    c->solo_fail = false;
+  }
+  if (!b->tuned) {
+    const int r = batch_tune<T>(b);
+    if (r) {
+      batch_release(b);  // frees what the failed set-up allocated
+      b->tuned = false;
+      c->solo_fail = true;
+      return fail(c, r, "persistent sweep: %s", b->err.c_str());
+    }
+  }
+  return SALMON_OK;
+}
+
+// nsteps steps through the persistent sweep, the observables closing each
+// run (and each stretch before a re-orthonormalisation) as in the batches;
+// the density of the last step's orbitals, taken (as psi_rho_RT follows the
+// Taylor step) before the re-orthonormalisation that may end the run.
+template <typename T>
+int rt_run_sweep(salmon_ctx* c, const double* ac, int nsteps, double dt, int reortho_every, double* jt, double* et,
+                 double* norm) {
+  salmon_batch* b = c->solo;
+  for (int n0 = 0; n0 < nsteps;) {
+    const int seg = reortho_every > 0 ? std::min(reortho_every - n0 % reortho_every, nsteps - n0) : nsteps - n0;
+    int r = batch_run<T>(b, ac + 3 * (size_t)n0, seg, dt, jt ? jt + 3 * (size_t)n0 : nullptr, et ? et + n0 : nullptr,
+                         norm ? norm + n0 : nullptr, false);
+    if (r) return fail(c, r, "persistent sweep: %s", b->err.c_str());
+    n0 += seg;
+    if (n0 == nsteps && (r = density_dev<T>(c))) return r;
+    if (reortho_every > 0 && n0 % reortho_every == 0 && (r = gram_schmidt_dev(c))) return r;
+  }
+  return SALMON_OK;
 }
 
 // Fixed device scratch of salmon_gpu_hartree_xc_noalloc: the DFT twiddles
@@ -1527,6 +1609,23 @@ int salmon_gpu_rt_run(salmon_ctx* c, const double* ac, int nsteps, double dt, in
   if (!ac || nsteps < 0) return fail(c, SALMON_ERR_ARG, "ac is NULL or nsteps < 0");
   Region reg(c, SALMON_LOG_DYNAMICS);
   if (nsteps == 0) return SALMON_OK;
+  if (rt_path(c) == SALMON_PATH_SWEEP) {
+    // set-up failures (grid, LDS or memory) before any step: SALMON_PATH_AUTO
+    // takes the streamed kernels instead
+    r = c->precision == SALMON_FP64 ? solo_prepare<double>(c) : solo_prepare<float>(c);
+    if (r == SALMON_OK) {
+      r = c->precision == SALMON_FP64 ? rt_run_sweep<double>(c, ac, nsteps, dt, reortho_every, jt, et, norm)
+                                      : rt_run_sweep<float>(c, ac, nsteps, dt, reortho_every, jt, et, norm);
+      if (r == SALMON_OK && rho) {
+        const hipError_t e = hipMemcpy(rho, c->d_rho, sizeof(double) * c->g.NL, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) r = fail(c, SALMON_ERR_HIP, "rt_run density: %s", hipGetErrorString(e));
+      }
+      (void)hipStreamSynchronize(c->stream);
+      return r;
+    }
+    if (c->path != SALMON_PATH_AUTO) return r;
+    r = SALMON_OK;
+  }
   // the per-step results stay allocated between runs, sized for at least
   // 4096 steps (256 KiB), so runs of the usual length allocate nothing
   if ((size_t)nsteps > c->res_cap) {
@@ -1543,7 +1642,7 @@ int salmon_gpu_rt_run(salmon_ctx* c, const double* ac, int nsteps, double dt, in
     r = c->precision == SALMON_FP64 ? rt_run_fused<double>(c, ac, nsteps, dt, reortho_every, d_res)
                                     : rt_run_fused<float>(c, ac, nsteps, dt, reortho_every, d_res);
   }
-  for (int n = 0; n < nsteps && r == SALMON_OK && !fused_ready(c) && !c->scf; ++n) {
+  for (int n = 0; n < nsteps && r == SALMON_OK && !fused_ready(c) && !c->scf; ++n) {  // streamed
     const double* a0 = ac + 3 * n;
     const double* a1 = ac + 3 * (n + 1);
     r = c->precision == SALMON_FP64 ? rt_step<double>(c, a0, a1, dt, d_res + 8 * (size_t)n)
@@ -1699,14 +1798,15 @@ int salmon_gpu_timer_get(salmon_ctx* c, int id, double* seconds, int* calls) {
 
 int salmon_gpu_set_path(salmon_ctx* c, int path) {
   if (!c) return SALMON_ERR_ARG;
-  if (path != SALMON_PATH_AUTO && path != SALMON_PATH_STREAMED) return fail(c, SALMON_ERR_ARG, "unknown path %d", path);
+  if (path != SALMON_PATH_AUTO && path != SALMON_PATH_STREAMED && path != SALMON_PATH_SWEEP)
+    return fail(c, SALMON_ERR_ARG, "unknown path %d", path);
   c->path = path;
   return SALMON_OK;
 }
 
 int salmon_gpu_path(salmon_ctx* c) {
   if (!c) return -1;
-  return fused_ready(c) ? SALMON_PATH_FUSED : SALMON_PATH_STREAMED;
+  return rt_path(c);
 }
 
 void* salmon_gpu_stream(salmon_ctx* c) { return c ? (void*)c->stream : nullptr; }
@@ -1714,6 +1814,11 @@ void* salmon_gpu_stream(salmon_ctx* c) { return c ? (void*)c->stream : nullptr; 
 int salmon_gpu_finalize(salmon_ctx* c) {
   if (!c) return SALMON_ERR_ARG;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  if (c->solo) {
+    c->solo->stream = nullptr;  // the context's own stream, destroyed below
+    (void)salmon_gpu_batch_finalize(c->solo);
+    c->solo = nullptr;
+  }
   fused_free(c);
   gs_free(c);
   dfree((void**)&c->d_vloc); dfree((void**)&c->d_vloc_f); dfree((void**)&c->d_rho); dfree((void**)&c->d_rho_part);

@@ -17,6 +17,7 @@ module salmon_gpu
   integer(c_int), parameter :: SALMON_PATH_AUTO = 0
   integer(c_int), parameter :: SALMON_PATH_STREAMED = 1
   integer(c_int), parameter :: SALMON_PATH_FUSED = 2
+  integer(c_int), parameter :: SALMON_PATH_SWEEP = 3
   integer(c_int), parameter :: SALMON_PC_NONE = 0
   integer(c_int), parameter :: SALMON_PC_KINETIC = 1
   integer(c_int), parameter :: SALMON_PC_LEARNED = 2

@@ -37,7 +37,7 @@ LIB_PATH = os.path.join(HERE, "lib", "libsalmon_gpu.so")
 
 FP64 = 0
 MIXED = 1
-PATH_AUTO, PATH_STREAMED, PATH_FUSED = 0, 1, 2
+PATH_AUTO, PATH_STREAMED, PATH_FUSED, PATH_SWEEP = 0, 1, 2, 3
 BATCH_AUTO, BATCH_PHASES, BATCH_SWEEP = 0, 1, 2
 PROP_MIDPOINT, PROP_ETRS = 0, 1
 XC_NONE, XC_PZ = 0, 1

@@ -76,21 +76,27 @@ def test_batch_every_shape_against_oracle(prec, kernel):
 
 
 @pytest.mark.parametrize("idx", [1, 2, 9])  # 20x20x28, 32^3, 24^3
-def test_streamed_context_other_shapes_against_oracle(idx):
-    """The single-context streamed path (every grid but 16^3) on the batch's
-    shapes, FP64 against the oracle and complex64 against the FP64 oracle."""
+def test_single_context_other_shapes_against_oracle(idx):
+    """A single context on the batch's shapes (every grid but 16^3), through
+    the persistent sweep (what SALMON_PATH_AUTO takes there) and through the
+    streamed kernels: FP64 against the oracle, complex64 against the FP64
+    oracle; J, E, norm, the last step's density and the orbitals."""
     nsteps = 3
     ac = _ac(nsteps)
     s = synth.config4_system(idx, scale=0.12)
-    o = Oracle(s).rt_run(ac, nsteps)
+    o = Oracle(s).rt_run(ac, nsteps, want_rho=True)
     for prec, tol, tolz in ((gpu.FP64, 1e-12, 1e-12), (gpu.MIXED, 1e-5, 1e-4)):
-        with gpu.Context(s, prec) as c:
-            assert c.path() == gpu.PATH_STREAMED
-            r = c.rt_run(ac, nsteps, dt=DT)
-            z = c.get_orbitals()
-        ej, ee, en, ez = _jerr(r["J"], o["J"]), rel(r["E"], o["E"]), rel(r["norm"], o["norm"]), rel(z, o["zu"])
-        print(f"[gpu] streamed {_shape(s)} prec={prec}: J {ej:.2e} E {ee:.2e} norm {en:.2e} zu {ez:.2e}", flush=True)
-        assert ej < tol and ee < tol and en < tol and ez < tolz
+        for path in (gpu.PATH_AUTO, gpu.PATH_STREAMED):
+            with gpu.Context(s, prec) as c:
+                c.set_path(path)
+                assert c.path() == (gpu.PATH_SWEEP if path == gpu.PATH_AUTO else gpu.PATH_STREAMED)
+                r = c.rt_run(ac, nsteps, dt=DT)
+                z = c.get_orbitals()
+            ej, ee, en, ez = _jerr(r["J"], o["J"]), rel(r["E"], o["E"]), rel(r["norm"], o["norm"]), rel(z, o["zu"])
+            er = rel(r["rho"], o["rho"])
+            print(f"[gpu] single context {_shape(s)} prec={prec} path={path}: J {ej:.2e} "
+                  f"E {ee:.2e} norm {en:.2e} rho {er:.2e} zu {ez:.2e}", flush=True)
+            assert ej < tol and ee < tol and en < tol and ez < tolz and er < tolz
 
 
 # one system of each shape, each with a modest orbital count, whose last k point is compared

@@ -1,0 +1,113 @@
+"""rt_run of a single context through the persistent sweep kernel (the path
+SALMON_PATH_AUTO takes off the fused 16^3 shape): against the streamed
+kernels and the fused step, with re-orthonormalisation, after new projector
+tables, and where the sweep does not apply (small grids, ETRS), where AUTO
+falls back and an explicit SALMON_PATH_SWEEP fails.
+
+Tolerance: FP64 1e-12 relative (J, E, norm, density, orbitals), the sweep's
+summation order against the streamed kernels' up to FMA contraction."""
+import dataclasses
+
+import numpy as np
+import pytest
+
+from helpers import load_case, rel
+from salmon_amd import gpu, synth
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-12
+
+
+def _run(s, path, ac, nsteps, prec=gpu.FP64, **kw):
+    with gpu.Context(s, prec) as c:
+        c.set_path(path)
+        p = c.path()
+        r = c.rt_run(ac, nsteps, **kw)
+        r["zu"] = c.get_orbitals()
+        return p, r
+
+
+def _close(a, b, keys=("J", "E", "norm", "rho", "zu"), tol=TOL):
+    for k in keys:
+        e = rel(a[k], b[k])
+        assert e < tol, (k, e)
+
+
+@pytest.mark.parametrize("name", ["small_k", "noncubic", "grid8"])
+def test_sweep_matches_streamed(name):
+    s, d = load_case(name)
+    n = d["ac"].shape[0] - 1
+    print(f"[gpu] single-context sweep {name} {n} steps", flush=True)
+    p1, r1 = _run(s, gpu.PATH_AUTO, d["ac"], n)
+    p2, r2 = _run(s, gpu.PATH_STREAMED, d["ac"], n)
+    assert p1 == gpu.PATH_SWEEP and p2 == gpu.PATH_STREAMED
+    _close(r1, r2)
+
+
+def test_sweep_with_reorthonormalisation():
+    s, d = load_case("small_k")
+    n = 6
+    print("[gpu] single-context sweep, Gram-Schmidt every 2 steps", flush=True)
+    _, r1 = _run(s, gpu.PATH_SWEEP, d["ac"], n, reortho_every=2)
+    _, r2 = _run(s, gpu.PATH_STREAMED, d["ac"], n, reortho_every=2)
+    _close(r1, r2)
+
+
+def test_sweep_on_the_fused_shape_matches_fused():
+    s, d = load_case("cfg1")
+    print("[gpu] single-context sweep on 16^3 against the fused step", flush=True)
+    p1, r1 = _run(s, gpu.PATH_SWEEP, d["ac"], 8)
+    p2, r2 = _run(s, gpu.PATH_AUTO, d["ac"], 8)
+    assert p1 == gpu.PATH_SWEEP and p2 == gpu.PATH_FUSED
+    _close(r1, r2)
+
+
+def test_sweep_after_new_projector_tables():
+    """Runs, new tables (an atom moved), runs again: the sweep rebuilds its
+    descriptors and follows the streamed kernels doing the same."""
+    s, d = load_case("noncubic")
+    coeff = np.where(s.iuV.reshape(s.NI, -1) > 0, 1.0, -1.0)
+    R = s.Rion.copy()
+    R[0] += [0.31, -0.17, 0.11]
+    t = synth.projector_tables(R, s.rc, (s.NLx, s.NLy, s.NLz), (s.aLx, s.aLy, s.aLz), 0.8, coeff)
+    s2 = dataclasses.replace(s, Rion=R, Mps=t[0], Jxyz=t[1], Jxx=t[2], Jyy=t[3], Jzz=t[4], uV=t[6])
+    print("[gpu] single-context sweep across a table change", flush=True)
+    out = {}
+    for path in (gpu.PATH_AUTO, gpu.PATH_STREAMED):
+        with gpu.Context(s, gpu.FP64) as c:
+            c.set_path(path)
+            a = c.rt_run(d["ac"][:3], 2)
+            c.set_projectors(s2)
+            b = c.rt_run(d["ac"][2:], 3)
+            out[path] = dict(J=np.concatenate([a["J"], b["J"]]), E=np.concatenate([a["E"], b["E"]]),
+                             norm=np.concatenate([a["norm"], b["norm"]]), rho=b["rho"], zu=c.get_orbitals())
+            assert c.path() == (gpu.PATH_SWEEP if path == gpu.PATH_AUTO else gpu.PATH_STREAMED)
+    _close(out[gpu.PATH_AUTO], out[gpu.PATH_STREAMED])
+
+
+def test_sweep_mixed_close_to_fp64():
+    s, d = load_case("small_k")
+    n = d["ac"].shape[0] - 1
+    _, r64 = _run(s, gpu.PATH_AUTO, d["ac"], n)
+    p, r32 = _run(s, gpu.PATH_AUTO, d["ac"], n, prec=gpu.MIXED)
+    assert p == gpu.PATH_SWEEP
+    _close(r32, r64, keys=("J", "E", "norm"), tol=1e-5)
+
+
+def test_where_the_sweep_does_not_apply():
+    """6^3 (under 8 points per axis) and ETRS: AUTO takes the streamed
+    kernels, an explicit SALMON_PATH_SWEEP fails with a message."""
+    small = synth.make_system(N=(6, 6, 6), L=6.0, nk3=(1, 1, 1), NB=2, seed=3)
+    ac = synth.acos2_pulse(2, 0.08, 0.05, 0.057, 826.8)
+    with gpu.Context(small, gpu.FP64) as c:
+        assert c.path() == gpu.PATH_STREAMED
+        c.set_path(gpu.PATH_SWEEP)
+        with pytest.raises(gpu.SalmonGPUError, match="persistent sweep"):
+            c.rt_run(ac, 2, dt=0.08)
+    s, d = load_case("small_k")
+    with gpu.Context(s, gpu.FP64) as c:
+        c.set_propagator(gpu.PROP_ETRS)
+        assert c.path() == gpu.PATH_STREAMED
+        c.set_path(gpu.PATH_SWEEP)
+        with pytest.raises(gpu.SalmonGPUError, match="midpoint propagator"):
+            c.rt_run(d["ac"], 2)
