@@ -1165,8 +1165,10 @@ static int fused_launch(salmon_ctx* c, const salmon::FusedArgs<T>& args, int nwg
 // of the entry orbitals.  want_rho false: the observables leave d_rho as it is
 // (a frozen-potential run sums the density of its last step only).
 template <typename T>
+// kbase: the device k points the launch adds amid / aobs to, kAc0 by default
+// (the hamiltonian entry passes the caller's kAc(NK,3) with amid = 0)
 static int fused_step(salmon_ctx* c, const double* amid, const double* aobs, double dt, double* res_dev,
-                      bool rho_post = false, bool want_rho = true) {
+                      bool rho_post = false, bool want_rho = true, const double* kbase = nullptr) {
   Region reg(c, SALMON_LOG_DT_EVOLVE);
   FusedState* f = (FusedState*)c->fz;
   if (!f || !f->tables) return fail(c, SALMON_ERR_UNSUPPORTED, "fused tables not built");
@@ -1199,7 +1201,7 @@ static int fused_step(salmon_ctx* c, const double* amid, const double* aobs, dou
   }
   salmon::FusedArgs<T> a{};
   a.zu = (salmon::cplx<T>*)c->d_zu;
-  a.kac0 = c->d_kac0;
+  a.kac0 = kbase ? kbase : c->d_kac0;
   a.occ = c->d_occ;
   a.vloc = vloc_ptr<T>(c);
   a.vloc64 = c->d_vloc;

@@ -1126,6 +1126,12 @@ int salmon_gpu_hamiltonian(salmon_ctx* c, const double* kAc, double dt) {
   int r = check_ready(c, true);
   if (r) return r;
   if ((r = upload_kac(c, kAc))) return r;
+  if (fused_ready(c)) {
+    // the fused step without observables or density: Taylor-4 at kAc
+    const double zero[3] = {0.0, 0.0, 0.0};
+    return c->precision == SALMON_FP64 ? fused_step<double>(c, zero, nullptr, dt, nullptr, false, false, c->d_kac)
+                                       : fused_step<float>(c, zero, nullptr, dt, nullptr, false, false, c->d_kac);
+  }
   if (c->precision == SALMON_FP64) return taylor_streamed<double>(c, c->d_kac, 0, 0, 0, dt);
   return taylor_streamed<float>(c, c->d_kac, 0, 0, 0, dt);
 }

@@ -42,6 +42,34 @@ def test_taylor_step_fp64_matches_reference(name):
     assert rel(z, d["taylor_zu"]) < TOL64
 
 
+@pytest.mark.parametrize("prec,tol", [(gpu.FP64, 1e-13), (gpu.MIXED, 1e-6)])
+def test_hamiltonian_entry_fused_matches_streamed(prec, tol):
+    """On 16^3 the hamiltonian entry (the drop-in's Taylor step at the
+    caller's kAc(NK,3)) runs the fused step without observables: three calls
+    at per-k wave vectors that are not kAc0 + one shift, against the streamed
+    kernels, and the FP64 orbitals against the oracle's Taylor step."""
+    s = synth.make_system(N=(16, 16, 16), L=10.26, nk3=(2, 2, 1), NB=4, seed=11)
+    rng = np.random.default_rng(5)
+    ks = [s.kAc0 + rng.uniform(-0.05, 0.05, size=s.kAc0.shape) for _ in range(3)]
+    out = {}
+    for path in (gpu.PATH_AUTO, gpu.PATH_STREAMED):
+        with _ctx(s, prec) as c:
+            c.set_path(path)
+            assert c.path() == (gpu.PATH_FUSED if path == gpu.PATH_AUTO else gpu.PATH_STREAMED)
+            for k in ks:
+                c.hamiltonian(k, s.dt)
+            out[path] = c.get_orbitals()
+    e = rel(out[gpu.PATH_AUTO], out[gpu.PATH_STREAMED])
+    print(f"[gpu] hamiltonian fused vs streamed prec={prec}: {e:.2e}", flush=True)
+    assert e < tol
+    if prec == gpu.FP64:
+        o = Oracle(s)
+        z = s.zu
+        for k in ks:
+            z = o.hamiltonian(k, z, s.dt)
+        assert rel(out[gpu.PATH_AUTO], z) < TOL64
+
+
 @pytest.mark.parametrize("name", CASES)
 def test_observables_fp64_match_reference(name):
     s, d = load_case(name)
