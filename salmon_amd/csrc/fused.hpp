@@ -1169,7 +1169,7 @@ template <typename T>
 // (the hamiltonian entry passes the caller's kAc(NK,3) with amid = 0)
 static int fused_step(salmon_ctx* c, const double* amid, const double* aobs, double dt, double* res_dev,
                       bool rho_post = false, bool want_rho = true, const double* kbase = nullptr) {
-  Region reg(c, SALMON_LOG_DT_EVOLVE);
+  Region reg(c, amid ? SALMON_LOG_DT_EVOLVE : SALMON_LOG_CURRENT);
   FusedState* f = (FusedState*)c->fz;
   if (!f || !f->tables) return fail(c, SALMON_ERR_UNSUPPORTED, "fused tables not built");
   int ncu = 256;

@@ -1136,16 +1136,33 @@ int salmon_gpu_hamiltonian(salmon_ctx* c, const double* kAc, double dt) {
   return taylor_streamed<float>(c, c->d_kac, 0, 0, 0, dt);
 }
 
+// Sums of the observables rows of the resident orbitals at k = kbase (J,
+// Ekin, Enl, Eloc, norm, as k_sum_rows lays them out): on 16^3 one
+// observables-only launch of the fused step, otherwise k_observe per orbital.
+static int observe_sums(salmon_ctx* c, const double* kbase, double out[8]) {
+  int r;
+  if (fused_ready(c)) {
+    const double zero[3] = {0.0, 0.0, 0.0};
+    r = c->precision == SALMON_FP64 ? fused_step<double>(c, nullptr, zero, 0.0, c->d_sum, false, false, kbase)
+                                    : fused_step<float>(c, nullptr, zero, 0.0, c->d_sum, false, false, kbase);
+    if (r) return r;
+    HIPCHK(c, hipMemcpyAsync(out, c->d_sum, sizeof(double) * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return SALMON_OK;
+  }
+  r = c->precision == SALMON_FP64 ? observe_rows<double>(c, kbase, 0, 0, 0) : observe_rows<float>(c, kbase, 0, 0, 0);
+  if (r) return r;
+  return read_sums(c, out);
+}
+
 int salmon_gpu_current(salmon_ctx* c, const double* kAc, double jav[3]) {
   int r = check_ready(c, true);
   if (r) return r;
   Region reg(c, SALMON_LOG_CURRENT);
   if (!jav) return fail(c, SALMON_ERR_ARG, "jav is NULL");
   if ((r = upload_kac(c, kAc))) return r;
-  r = c->precision == SALMON_FP64 ? observe_rows<double>(c, c->d_kac, 0, 0, 0) : observe_rows<float>(c, c->d_kac, 0, 0, 0);
-  if (r) return r;
   double s[8];
-  if ((r = read_sums(c, s))) return r;
+  if ((r = observe_sums(c, c->d_kac, s))) return r;
   jav[0] = s[0]; jav[1] = s[1]; jav[2] = s[2];
   return SALMON_OK;
 }
@@ -1155,10 +1172,8 @@ int salmon_gpu_energy(salmon_ctx* c, const double* kAc, double* ekin, double* en
   if (r) return r;
   Region reg(c, SALMON_LOG_TOTAL_ENERGY);
   if ((r = upload_kac(c, kAc))) return r;
-  r = c->precision == SALMON_FP64 ? observe_rows<double>(c, c->d_kac, 0, 0, 0) : observe_rows<float>(c, c->d_kac, 0, 0, 0);
-  if (r) return r;
   double s[8];
-  if ((r = read_sums(c, s))) return r;
+  if ((r = observe_sums(c, c->d_kac, s))) return r;
   if (ekin) *ekin = s[3];
   if (enl) *enl = s[4];
   if (eloc) *eloc = s[5];
@@ -1169,11 +1184,8 @@ int salmon_gpu_norm(salmon_ctx* c, double* norm) {
   int r = check_ready(c, true);
   if (r) return r;
   if (!norm) return fail(c, SALMON_ERR_ARG, "norm is NULL");
-  r = c->precision == SALMON_FP64 ? observe_rows<double>(c, c->d_kac0, 0, 0, 0)
-                                  : observe_rows<float>(c, c->d_kac0, 0, 0, 0);
-  if (r) return r;
   double s[8];
-  if ((r = read_sums(c, s))) return r;
+  if ((r = observe_sums(c, c->d_kac0, s))) return r;
   *norm = s[6];
   return SALMON_OK;
 }
