@@ -95,7 +95,11 @@ int salmon_gpu_hpsi(salmon_ctx *ctx, const double kAc[3], const double *in, doub
 int salmon_gpu_hamiltonian(salmon_ctx *ctx, const double *kAc, double dt);
 
 /* Current jav(3) at kAc(NK,3): current_RT -> current('RT'),
- * ARTED/RT/current.f90:65-292, including the summation (:276-291). */
+ * ARTED/RT/current.f90:65-292, including the summation (:276-291).  The
+ * current and the energies below come from one pass over the orbitals (on
+ * 16^3 one observables-only fused launch); a current or energy call at the
+ * same kAc with nothing changed in between (no step, orbitals, potential,
+ * tables or k points) reuses that pass. */
 int salmon_gpu_current(salmon_ctx *ctx, const double *kAc, double jav[3]);
 
 /* rho(NL) = sum occ |zu|^2: psi_rho_RT with Sym=1, ARTED/common/psi_rho.f90:40-206. */

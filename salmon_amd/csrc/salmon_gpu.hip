@@ -110,7 +110,19 @@ struct salmon_ctx {
   salmon_batch* solo = nullptr;
   unsigned long solo_gen = 0;
   bool solo_fail = false;
+  // bumped by every entry point that may change the orbitals, potential,
+  // tables, k points or kernel path; the observables sums of the last current
+  // or energy call are reused by the next one at the same kAc and epoch (the
+  // reference calls current_RT and Total_Energy_omp at one k after a step)
+  unsigned long epoch = 0;
+  unsigned long obs_epoch = ~0ul;
+  std::vector<double> obs_k;
+  double obs_sums[8] = {};
 };
+
+static inline void touch(salmon_ctx* c) {
+  if (c) ++c->epoch;
+}
 
 namespace {
 
@@ -891,6 +903,7 @@ int salmon_gpu_init(const salmon_grid_t* grid, int precision, salmon_ctx** out) 
 }
 
 int salmon_gpu_set_potential(salmon_ctx* c, const double* vloc) {
+  touch(c);
   if (!c) return SALMON_ERR_ARG;
   if (!vloc) return fail(c, SALMON_ERR_ARG, "vloc is NULL");
   const int NL = c->g.NL;
@@ -905,6 +918,7 @@ int salmon_gpu_set_potential(salmon_ctx* c, const double* vloc) {
 int salmon_gpu_set_projectors(salmon_ctx* c, int NI, int Nps, const int* Mps, const int* Jxyz, const int* Jxx,
                               const int* Jyy, const int* Jzz, int Nlma, const int* a_tbl, const double* uV,
                               const int* iuV) {
+  touch(c);
   if (!c) return SALMON_ERR_ARG;
   if (NI < 0 || Nps < 0 || Nlma < 0) return fail(c, SALMON_ERR_ARG, "negative table size");
   if ((NI > 0 && (!Mps || !Jxyz || !Jxx || !Jyy || !Jzz)) || (Nlma > 0 && (!a_tbl || !uV || !iuV)))
@@ -1002,6 +1016,7 @@ int salmon_gpu_set_projectors(salmon_ctx* c, int NI, int Nps, const int* Mps, co
 }
 
 int salmon_gpu_set_kpoints(salmon_ctx* c, int NK, int NB, const double* kAc0, const double* occ) {
+  touch(c);
   if (!c) return SALMON_ERR_ARG;
   if (NK < 1 || NB < 1 || !kAc0 || !occ) return fail(c, SALMON_ERR_ARG, "NK, NB must be >= 1 with kAc0, occ given");
   if ((size_t)NK * NB * c->g.NL > ((size_t)1 << 40)) return fail(c, SALMON_ERR_ARG, "orbital set too large");
@@ -1027,6 +1042,7 @@ int salmon_gpu_set_kpoints(salmon_ctx* c, int NK, int NB, const double* kAc0, co
 }
 
 int salmon_gpu_put_orbitals(salmon_ctx* c, const double* zu) {
+  touch(c);
   if (!c) return SALMON_ERR_ARG;
   if (!c->d_zu) return fail(c, SALMON_ERR_STATE, "salmon_gpu_set_kpoints not called");
   if (!zu) return fail(c, SALMON_ERR_ARG, "zu is NULL");
@@ -1123,6 +1139,7 @@ int salmon_gpu_hpsi(salmon_ctx* c, const double kAc[3], const double* in, double
 }
 
 int salmon_gpu_hamiltonian(salmon_ctx* c, const double* kAc, double dt) {
+  touch(c);
   int r = check_ready(c, true);
   if (r) return r;
   if ((r = upload_kac(c, kAc))) return r;
@@ -1155,14 +1172,30 @@ static int observe_sums(salmon_ctx* c, const double* kbase, double out[8]) {
   return read_sums(c, out);
 }
 
+// observe_sums at the caller's kAc(NK,3), or the sums the previous current or
+// energy call formed at the same kAc with nothing changed since
+static int observe_sums_at(salmon_ctx* c, const double* kAc, double out[8]) {
+  const size_t nk = 3 * (size_t)c->NK;
+  if (kAc && c->obs_epoch == c->epoch && c->obs_k.size() == nk && std::equal(kAc, kAc + nk, c->obs_k.begin())) {
+    std::copy(c->obs_sums, c->obs_sums + 8, out);
+    return SALMON_OK;
+  }
+  int r;
+  if ((r = upload_kac(c, kAc))) return r;
+  if ((r = observe_sums(c, c->d_kac, out))) return r;
+  c->obs_k.assign(kAc, kAc + nk);
+  std::copy(out, out + 8, c->obs_sums);
+  c->obs_epoch = c->epoch;
+  return SALMON_OK;
+}
+
 int salmon_gpu_current(salmon_ctx* c, const double* kAc, double jav[3]) {
   int r = check_ready(c, true);
   if (r) return r;
   Region reg(c, SALMON_LOG_CURRENT);
   if (!jav) return fail(c, SALMON_ERR_ARG, "jav is NULL");
-  if ((r = upload_kac(c, kAc))) return r;
   double s[8];
-  if ((r = observe_sums(c, c->d_kac, s))) return r;
+  if ((r = observe_sums_at(c, kAc, s))) return r;
   jav[0] = s[0]; jav[1] = s[1]; jav[2] = s[2];
   return SALMON_OK;
 }
@@ -1171,9 +1204,8 @@ int salmon_gpu_energy(salmon_ctx* c, const double* kAc, double* ekin, double* en
   int r = check_ready(c, true);
   if (r) return r;
   Region reg(c, SALMON_LOG_TOTAL_ENERGY);
-  if ((r = upload_kac(c, kAc))) return r;
   double s[8];
-  if ((r = observe_sums(c, c->d_kac, s))) return r;
+  if ((r = observe_sums_at(c, kAc, s))) return r;
   if (ekin) *ekin = s[3];
   if (enl) *enl = s[4];
   if (eloc) *eloc = s[5];
@@ -1202,6 +1234,7 @@ int salmon_gpu_density(salmon_ctx* c, double* rho) {
 }
 
 int salmon_gpu_gram_schmidt(salmon_ctx* c) {
+  touch(c);
   // needs the orbitals only (no potential or projectors)
   if (!c) return SALMON_ERR_ARG;
   if (c->NKB == 0) return fail(c, SALMON_ERR_STATE, "salmon_gpu_set_kpoints not called");
@@ -1273,6 +1306,7 @@ static int cg_results(salmon_ctx* c, double* esp_var, int* iters) {
 }
 
 int salmon_gpu_cg(salmon_ctx* c, const double* kAc, int iter_cg_max, int precond, double* esp_var, int* iters) {
+  touch(c);
   int r = gs_check(c, kAc);
   if (r) return r;
   Region reg(c, SALMON_LOG_CG);
@@ -1283,6 +1317,7 @@ int salmon_gpu_cg(salmon_ctx* c, const double* kAc, int iter_cg_max, int precond
 }
 
 int salmon_gpu_subspace_diag(salmon_ctx* c, const double* kAc, double* esp) {
+  touch(c);
   int r = gs_check(c, kAc);
   if (r) return r;
   Region reg(c, SALMON_LOG_DIAG);
@@ -1291,6 +1326,7 @@ int salmon_gpu_subspace_diag(salmon_ctx* c, const double* kAc, double* esp) {
 
 int salmon_gpu_ground_state(salmon_ctx* c, const double* kAc, int max_sweeps, int iter_cg_max, int precond,
                             double tol, double* esp, double* esp_var, int* sweeps, double* history) {
+  touch(c);
   int r = gs_check(c, kAc);
   if (r) return r;
   Region reg(c, SALMON_LOG_GROUND_STATE);
@@ -1392,6 +1428,7 @@ int salmon_gpu_pcnet_finalize(salmon_pcnet* n) {
 }
 
 int salmon_gpu_pcnet_apply(salmon_pcnet* n, salmon_ctx* c, const double* g, double* d) {
+  touch(c);
   if (!n || !g || !d) return c ? fail(c, SALMON_ERR_ARG, "NULL argument") : SALMON_ERR_ARG;
   int r = check_ready(c, false);
   if (r) return r;
@@ -1407,6 +1444,7 @@ int salmon_gpu_pcnet_apply(salmon_pcnet* n, salmon_ctx* c, const double* g, doub
 }
 
 int salmon_gpu_set_scf(salmon_ctx* c, const double* vpsl, int xc) {
+  touch(c);
   if (!c) return SALMON_ERR_ARG;
   if (!vpsl) {
     c->scf = false;
@@ -1432,6 +1470,7 @@ int salmon_gpu_set_scf(salmon_ctx* c, const double* vpsl, int xc) {
 }
 
 int salmon_gpu_hartree_xc(salmon_ctx* c, const double* rho, int xc, double* vh, double* vxc) {
+  touch(c);
   if (!c || !rho) return c ? fail(c, SALMON_ERR_ARG, "rho is NULL") : SALMON_ERR_ARG;
   if (xc != SALMON_XC_NONE && xc != SALMON_XC_PZ) return fail(c, SALMON_ERR_ARG, "xc %d is neither NONE nor PZ", xc);
   const size_t NL = c->g.NL;
@@ -1452,6 +1491,7 @@ int salmon_gpu_hartree_xc(salmon_ctx* c, const double* rho, int xc, double* vh, 
 }
 
 int salmon_gpu_hartree_xc_noalloc(salmon_ctx* c, const double* rho_dev, int xc, double* vh_dev, double* vxc_dev) {
+  touch(c);
   if (!c || !rho_dev) return c ? fail(c, SALMON_ERR_ARG, "rho_dev is NULL") : SALMON_ERR_ARG;
   if (xc != SALMON_XC_NONE && xc != SALMON_XC_PZ) return fail(c, SALMON_ERR_ARG, "xc %d is neither NONE nor PZ", xc);
   const Grid& g = c->g;
@@ -1474,6 +1514,7 @@ int salmon_gpu_hartree_xc_noalloc(salmon_ctx* c, const double* rho_dev, int xc, 
 }
 
 int salmon_gpu_set_propagator(salmon_ctx* c, int propagator) {
+  touch(c);
   if (!c) return SALMON_ERR_ARG;
   if (propagator != SALMON_PROP_MIDPOINT && propagator != SALMON_PROP_ETRS)
     return fail(c, SALMON_ERR_ARG, "propagator %d is neither SALMON_PROP_MIDPOINT nor SALMON_PROP_ETRS", propagator);
@@ -1489,6 +1530,7 @@ int salmon_gpu_set_graphs(salmon_ctx* c, int enable) {
 }
 
 int salmon_gpu_set_pcnet(salmon_ctx* c, salmon_pcnet* n) {
+  touch(c);
   if (!c) return SALMON_ERR_ARG;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   ++c->gen;  // a captured learned CG reads the previous network's weights
@@ -1506,6 +1548,7 @@ int salmon_gpu_set_pc_handover(salmon_ctx* c, int max_learned_sweeps, double sta
 }
 
 int salmon_gpu_pc_record(salmon_ctx* c, int max_samples, int stride) {
+  touch(c);
   int r = check_ready(c, true);
   if (r) return r;
   if (c->NK != 1) return fail(c, SALMON_ERR_UNSUPPORTED, "training samples are recorded for one k point");
@@ -1569,6 +1612,7 @@ int salmon_gpu_pc_get_sample(salmon_ctx* c, int i, double* g, double* x, int* ib
 }
 
 int salmon_gpu_pc_set_targets(salmon_ctx* c, const double* kAc) {
+  touch(c);
   int r = gs_check(c, kAc);
   if (r) return r;
   if (c->NK != 1) return fail(c, SALMON_ERR_UNSUPPORTED, "training targets are set for one k point");
@@ -1589,6 +1633,7 @@ static int pc_train_check(salmon_pcnet* n, salmon_ctx* c) {
 }
 
 int salmon_gpu_pcnet_gradient(salmon_pcnet* n, salmon_ctx* c, int nsamp, const int* idx, double* loss, double* grad) {
+  touch(c);
   int r = pc_train_check(n, c);
   if (r) return r;
   if (nsamp < 1 || !idx) return fail(c, SALMON_ERR_ARG, "no samples given");
@@ -1598,6 +1643,7 @@ int salmon_gpu_pcnet_gradient(salmon_pcnet* n, salmon_ctx* c, int nsamp, const i
 
 int salmon_gpu_pcnet_train(salmon_pcnet* n, salmon_ctx* c, int nsteps, int batch, double lr, unsigned seed,
                            double* mean_loss) {
+  touch(c);
   int r = pc_train_check(n, c);
   if (r) return r;
   if (nsteps < 0 || batch < 1) return fail(c, SALMON_ERR_ARG, "nsteps < 0 or batch < 1");
@@ -1622,6 +1668,7 @@ int salmon_gpu_pcnet_train(salmon_pcnet* n, salmon_ctx* c, int nsteps, int batch
 
 int salmon_gpu_rt_run(salmon_ctx* c, const double* ac, int nsteps, double dt, int reortho_every, double* jt, double* et,
                       double* norm, double* rho) {
+  touch(c);
   int r = check_ready(c, true);
   if (r) return r;
   if (!ac || nsteps < 0) return fail(c, SALMON_ERR_ARG, "ac is NULL or nsteps < 0");
@@ -1705,6 +1752,7 @@ int salmon_gpu_batch_create(int precision, salmon_batch** out) {
 }
 
 int salmon_gpu_batch_add(salmon_batch* b, salmon_ctx* c) {
+  touch(c);
   if (!b) return SALMON_ERR_ARG;
   if (!c) return bfail(b, SALMON_ERR_ARG, "context is NULL");
   int r = check_ready(c, true);
@@ -1735,6 +1783,8 @@ int salmon_gpu_batch_kernel(const salmon_batch* b) {
 }
 
 int salmon_gpu_batch_tune(salmon_batch* b) {
+  if (b)
+    for (salmon_ctx* c : b->ctx) touch(c);
   if (!b) return SALMON_ERR_ARG;
   if (b->ctx.empty()) return bfail(b, SALMON_ERR_STATE, "empty batch");
   return b->precision == SALMON_FP64 ? batch_tune<double>(b) : batch_tune<float>(b);
@@ -1747,6 +1797,8 @@ int salmon_gpu_batch_zblock(salmon_batch* b, int i) {
 
 int salmon_gpu_batch_rt_run(salmon_batch* b, const double* ac, int nsteps, double dt, double* jt, double* et,
                             double* norm) {
+  if (b)
+    for (salmon_ctx* c : b->ctx) touch(c);
   if (!b) return SALMON_ERR_ARG;
   if (b->ctx.empty()) return bfail(b, SALMON_ERR_STATE, "empty batch");
   if (!ac || nsteps < 0) return bfail(b, SALMON_ERR_ARG, "ac is NULL or nsteps < 0");
@@ -1757,6 +1809,8 @@ int salmon_gpu_batch_rt_run(salmon_batch* b, const double* ac, int nsteps, doubl
 
 int salmon_gpu_batch_rt_run_ms(salmon_batch* b, const double* ac, int nsteps, double dt, double* jt, double* et,
                                double* norm) {
+  if (b)
+    for (salmon_ctx* c : b->ctx) touch(c);
   if (!b) return SALMON_ERR_ARG;
   if (b->ctx.empty()) return bfail(b, SALMON_ERR_STATE, "empty batch");
   if (!ac || nsteps < 0) return bfail(b, SALMON_ERR_ARG, "ac is NULL or nsteps < 0");
@@ -1815,6 +1869,7 @@ int salmon_gpu_timer_get(salmon_ctx* c, int id, double* seconds, int* calls) {
 }
 
 int salmon_gpu_set_path(salmon_ctx* c, int path) {
+  touch(c);
   if (!c) return SALMON_ERR_ARG;
   if (path != SALMON_PATH_AUTO && path != SALMON_PATH_STREAMED && path != SALMON_PATH_SWEEP)
     return fail(c, SALMON_ERR_ARG, "unknown path %d", path);

@@ -183,6 +183,10 @@ class Context:
             msg = lib().salmon_gpu_last_error(self._h).decode() if self._h else ""
             raise SalmonGPUError(f"{what}: {lib().salmon_gpu_errstr(rc).decode()}: {msg}")
 
+    def set_potential(self, vloc):
+        """A new local potential Vloc(NL) (the Hartree/XC update of a step)."""
+        self._check(lib().salmon_gpu_set_potential(self._h, _p(_f64(vloc))), "set_potential")
+
     def set_projectors(self, sys):
         """Uploads the KB tables of sys (Mps, Jxyz, Jxx/Jyy/Jzz, a_tbl, uV, iuV);
         called again when the atoms move (prep_ps_periodic)."""

@@ -89,6 +89,41 @@ def test_observables_fp64_match_reference(name):
     assert rel(nrm, o.norm(z1)) < TOL64
 
 
+@pytest.mark.parametrize("name", ["cfg1", "small_k"])
+def test_observables_reused_only_while_nothing_changes(name):
+    """current and energy at one kAc share one pass over the orbitals; a
+    Taylor step, new orbitals or a new potential in between force a new one:
+    every value equals that of a fresh context holding the same state."""
+    s, d = load_case(name)
+    k = kac_at(s, d["ac"][1])
+
+    def fresh(zu, vloc):
+        with _ctx(s) as f:
+            if vloc is not None:
+                f.set_potential(vloc)
+            f.put_orbitals(zu)
+            return f.current(k), f.total_energy(k)
+
+    with _ctx(s) as c:
+        j0, e0 = c.current(k), c.total_energy(k)
+        jr, er = fresh(s.zu, None)
+        assert np.array_equal(j0, jr) and np.array_equal(e0, er)
+        c.hamiltonian(midpoint_kac(s, d["ac"], 0), s.dt)
+        z1 = c.get_orbitals()
+        e1, j1 = c.total_energy(k), c.current(k)
+        jr, er = fresh(z1, None)
+        assert np.array_equal(j1, jr) and np.array_equal(e1, er)
+        assert not np.array_equal(e1, e0)
+        v2 = s.vloc * 1.1
+        c.set_potential(v2)
+        e2 = c.total_energy(k)
+        _, er = fresh(z1, v2)
+        assert np.array_equal(e2, er) and not np.array_equal(e2, e1)
+        c.put_orbitals(s.zu)
+        j3 = c.current(k)
+        assert np.array_equal(j3, j0)
+
+
 @pytest.mark.parametrize("name", CASES)
 def test_ground_state_ops_fp64_match_reference(name):
     s, d = load_case(name)
