@@ -750,6 +750,7 @@ int solo_prepare(salmon_ctx* c) {
       batch_release(b);  // frees what the failed set-up allocated
       b->tuned = false;
       c->solo_fail = true;
+      (void)hipGetLastError();  // a failed allocation must not show up as a later launch's error
       return fail(c, r, "persistent sweep: %s", b->err.c_str());
     }
   }
