@@ -90,8 +90,10 @@ int salmon_gpu_hpsi(salmon_ctx *ctx, const double kAc[3], const double *in, doub
 
 /* One 4th-order Taylor step of every resident orbital at kAc(NK,3)
  * (the midpoint k+A): subroutine hamiltonian(zu, flag_current),
- * ARTED/RT/hamiltonian.f90:27-123.  On 16^3 grids (SALMON_PATH_AUTO) it runs
- * the fused step without observables, otherwise the streamed kernels. */
+ * ARTED/RT/hamiltonian.f90:27-123.  It takes rt_run's kernel path without
+ * observables: on 16^3 the fused step, on other grids of at least 8 points
+ * per axis the persistent sweep (SALMON_PATH_SWEEP), otherwise the streamed
+ * kernels. */
 int salmon_gpu_hamiltonian(salmon_ctx *ctx, const double *kAc, double dt);
 
 /* Current jav(3) at kAc(NK,3): current_RT -> current('RT'),

@@ -37,6 +37,12 @@ struct salmon_batch {
   int* d_next = nullptr;          // work counter of the sweep launch
   unsigned sweep_grid = 0;
   size_t sweep_lds = 0;
+  // one Taylor step at caller-given k points (batch_taylor): descriptors whose
+  // kac0 is the given table, a zero A(t) table and a zero step counter (in bufs)
+  void* d_sys_k = nullptr;
+  const double* const* sys_k_base = nullptr;
+  double* d_zero_ac = nullptr;
+  int* d_zero_step = nullptr;
   hipGraphExec_t exec = nullptr;
   double graph_dt = 0.0;
   long graph_stride = 0;          // the A(t) table stride the graph was captured with
@@ -64,6 +70,10 @@ static void batch_release(salmon_batch* b) {
   for (void* p : b->bufs) (void)hipFree(p);
   b->bufs.clear();
   b->d_sys = nullptr;
+  b->d_sys_k = nullptr;
+  b->sys_k_base = nullptr;
+  b->d_zero_ac = nullptr;
+  b->d_zero_step = nullptr;
   for (int ph = 0; ph < kPhases; ++ph) b->d_owner[ph] = nullptr;
   b->d_items = nullptr;
   b->nitems = 0;
@@ -333,6 +343,54 @@ static int batch_enqueue_step(salmon_batch* b, double dt, long ac_stride) {
   hipLaunchKernelGGL(k_b_observe<T>, dim3((unsigned)b->first[3][0]), dim3(256), 0, b->stream, S, n, st);
   hipLaunchKernelGGL(k_b_sum<T>, dim3(n), dim3(256), 0, b->stream, S, n, b->d_step, b->d_res, 0);
   hipLaunchKernelGGL(k_b_tick, dim3(1), dim3(64), 0, b->stream, b->d_step);
+  BHIP(b, hipGetLastError());
+  return SALMON_OK;
+}
+
+// One Taylor-4 step (hamiltonian.f90:27-123) of every orbital of the batch at
+// the k points kbase[i] (system i's device kAc(NK,3) table, in place of its
+// kAc0 + A) over dt, without observables: the persistent sweep with the step
+// counter at 0 (so no observables are formed) and A = 0.  The descriptors
+// with the replaced k tables are built on the first call for a kbase array
+// and kept with the work buffers.
+template <typename T>
+static int batch_taylor(salmon_batch* b, const double* const* kbase, double dt) {
+  const int n = (int)b->ctx.size();
+  int r;
+  if (!b->tuned && (r = batch_tune<T>(b))) return r;
+  if (!b->sweep) return bfail(b, SALMON_ERR_UNSUPPORTED, "Taylor step of a batch needs the persistent sweep");
+  if (!b->d_sys_k || b->sys_k_base != kbase) {
+    if (!b->d_sys_k) {
+      if (hipMalloc(&b->d_sys_k, sizeof(Sys<T>) * n) != hipSuccess ||
+          hipMalloc((void**)&b->d_zero_ac, sizeof(double) * 6) != hipSuccess ||
+          hipMalloc((void**)&b->d_zero_step, sizeof(int)) != hipSuccess)
+        return bfail(b, SALMON_ERR_NOMEM, "batch Taylor descriptors");
+      b->bufs.push_back(b->d_sys_k);
+      b->bufs.push_back(b->d_zero_ac);
+      b->bufs.push_back(b->d_zero_step);
+      BHIP(b, hipMemsetAsync(b->d_zero_ac, 0, sizeof(double) * 6, b->stream));
+      BHIP(b, hipMemsetAsync(b->d_zero_step, 0, sizeof(int), b->stream));
+    }
+    std::vector<Sys<T>> S(n);
+    BHIP(b, hipMemcpyAsync(S.data(), b->d_sys, sizeof(Sys<T>) * n, hipMemcpyDeviceToHost, b->stream));
+    BHIP(b, hipStreamSynchronize(b->stream));
+    for (int i = 0; i < n; ++i) S[i].kac0 = kbase[i];
+    BHIP(b, hipMemcpyAsync(b->d_sys_k, S.data(), sizeof(Sys<T>) * n, hipMemcpyHostToDevice, b->stream));
+    BHIP(b, hipStreamSynchronize(b->stream));
+    b->sys_k_base = kbase;
+  }
+  const Sys<T>* S = (const Sys<T>*)b->d_sys_k;
+  const Step st{b->d_zero_ac, b->d_zero_step, dt,
+                {b->d_owner[0], b->d_owner[1], b->d_owner[2], b->d_owner[3], b->d_owner[4]}, 0L};
+  if (b->max_ph > 0)
+    hipLaunchKernelGGL(k_b_phase<T>, dim3((unsigned)((b->max_ph + 255) / 256), n), dim3(256), 0, b->stream, S, st, 0);
+  hipLaunchKernelGGL(k_b_qreset, dim3(1), dim3(64), 0, b->stream, b->d_next);
+  if constexpr (sizeof(T) == 4)
+    hipLaunchKernelGGL((k_b_sweep<T, 4>), dim3(b->sweep_grid), dim3(sweep_waves<T>() * kWave), b->sweep_lds, b->stream,
+                       S, st, (const int2*)b->d_items, b->nitems, b->d_next);
+  else
+    hipLaunchKernelGGL((k_b_sweep<T, 2>), dim3(b->sweep_grid), dim3(sweep_waves<T>() * kWave), b->sweep_lds, b->stream,
+                       S, st, (const int2*)b->d_items, b->nitems, b->d_next);
   BHIP(b, hipGetLastError());
   return SALMON_OK;
 }

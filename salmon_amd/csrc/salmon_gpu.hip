@@ -1150,6 +1150,18 @@ int salmon_gpu_hamiltonian(salmon_ctx* c, const double* kAc, double dt) {
     return c->precision == SALMON_FP64 ? fused_step<double>(c, zero, nullptr, dt, nullptr, false, false, c->d_kac)
                                        : fused_step<float>(c, zero, nullptr, dt, nullptr, false, false, c->d_kac);
   }
+  if (rt_path(c) == SALMON_PATH_SWEEP) {
+    // the persistent sweep without observables at kAc; a set-up that fails
+    // leaves SALMON_PATH_AUTO with the streamed kernels
+    r = c->precision == SALMON_FP64 ? solo_prepare<double>(c) : solo_prepare<float>(c);
+    if (r == SALMON_OK) {
+      Region reg(c, SALMON_LOG_DT_EVOLVE);
+      r = c->precision == SALMON_FP64 ? batch_taylor<double>(c->solo, &c->d_kac, dt)
+                                      : batch_taylor<float>(c->solo, &c->d_kac, dt);
+      return r ? fail(c, r, "persistent sweep: %s", c->solo->err.c_str()) : SALMON_OK;
+    }
+    if (c->path != SALMON_PATH_AUTO) return r;
+  }
   if (c->precision == SALMON_FP64) return taylor_streamed<double>(c, c->d_kac, 0, 0, 0, dt);
   return taylor_streamed<float>(c, c->d_kac, 0, 0, 0, dt);
 }

@@ -111,3 +111,34 @@ def test_where_the_sweep_does_not_apply():
         c.set_path(gpu.PATH_SWEEP)
         with pytest.raises(gpu.SalmonGPUError, match="midpoint propagator"):
             c.rt_run(d["ac"], 2)
+
+
+@pytest.mark.parametrize("name", ["small_k", "noncubic", "grid8"])
+def test_hamiltonian_entry_through_the_sweep(name):
+    """The hamiltonian entry (the drop-in's Taylor step at the caller's
+    kAc(NK,3)) off 16^3 runs the persistent sweep without observables: three
+    calls at per-k wave vectors against the streamed kernels and the oracle,
+    then an rt_run continuing from those orbitals."""
+    from oracle.cpu import Oracle
+    s, d = load_case(name)
+    rng = np.random.default_rng(9)
+    ks = [s.kAc0 + rng.uniform(-0.05, 0.05, size=s.kAc0.shape) for _ in range(3)]
+    out = {}
+    for path in (gpu.PATH_AUTO, gpu.PATH_STREAMED):
+        with gpu.Context(s, gpu.FP64) as c:
+            c.set_path(path)
+            for k in ks:
+                c.hamiltonian(k, s.dt)
+            z = c.get_orbitals()
+            r = c.rt_run(d["ac"][:3], 2)
+            r["zu"] = c.get_orbitals()
+            out[path] = (z, r)
+    o = Oracle(s)
+    z = s.zu
+    for k in ks:
+        z = o.hamiltonian(k, z, s.dt)
+    print(f"[gpu] hamiltonian entry {name}: sweep vs streamed {rel(out[gpu.PATH_AUTO][0], out[gpu.PATH_STREAMED][0]):.2e}"
+          f" vs oracle {rel(out[gpu.PATH_AUTO][0], z):.2e}", flush=True)
+    assert rel(out[gpu.PATH_AUTO][0], z) < TOL
+    assert rel(out[gpu.PATH_AUTO][0], out[gpu.PATH_STREAMED][0]) < TOL
+    _close(out[gpu.PATH_AUTO][1], out[gpu.PATH_STREAMED][1])
