@@ -98,8 +98,10 @@ int salmon_gpu_hamiltonian(salmon_ctx *ctx, const double *kAc, double dt);
 
 /* Current jav(3) at kAc(NK,3): current_RT -> current('RT'),
  * ARTED/RT/current.f90:65-292, including the summation (:276-291).  The
- * current and the energies below come from one pass over the orbitals (on
- * 16^3 one observables-only fused launch); a current or energy call at the
+ * current and the energies below come from one pass over the orbitals (one
+ * observables-only launch of rt_run's kernel: the fused step on 16^3, the
+ * persistent sweep on other grids of at least 8 points per axis, else the
+ * streamed k_observe); a current or energy call at the
  * same kAc with nothing changed in between (no step, orbitals, potential,
  * tables or k points) reuses that pass. */
 int salmon_gpu_current(salmon_ctx *ctx, const double *kAc, double jav[3]);

@@ -347,51 +347,94 @@ static int batch_enqueue_step(salmon_batch* b, double dt, long ac_stride) {
   return SALMON_OK;
 }
 
-// One Taylor-4 step (hamiltonian.f90:27-123) of every orbital of the batch at
-// the k points kbase[i] (system i's device kAc(NK,3) table, in place of its
-// kAc0 + A) over dt, without observables: the persistent sweep with the step
-// counter at 0 (so no observables are formed) and A = 0.  The descriptors
-// with the replaced k tables are built on the first call for a kbase array
-// and kept with the work buffers.
+// Descriptors of the batch whose k tables are kbase[i] (system i's device
+// kAc(NK,3) table, in place of its kAc0 + A), a zero A(t) table and a zero
+// step counter, built on the first call for a kbase array and kept with the
+// work buffers.
 template <typename T>
-static int batch_taylor(salmon_batch* b, const double* const* kbase, double dt) {
+static int batch_k_descriptors(salmon_batch* b, const double* const* kbase) {
   const int n = (int)b->ctx.size();
   int r;
   if (!b->tuned && (r = batch_tune<T>(b))) return r;
-  if (!b->sweep) return bfail(b, SALMON_ERR_UNSUPPORTED, "Taylor step of a batch needs the persistent sweep");
-  if (!b->d_sys_k || b->sys_k_base != kbase) {
-    if (!b->d_sys_k) {
-      if (hipMalloc(&b->d_sys_k, sizeof(Sys<T>) * n) != hipSuccess ||
-          hipMalloc((void**)&b->d_zero_ac, sizeof(double) * 6) != hipSuccess ||
-          hipMalloc((void**)&b->d_zero_step, sizeof(int)) != hipSuccess)
-        return bfail(b, SALMON_ERR_NOMEM, "batch Taylor descriptors");
-      b->bufs.push_back(b->d_sys_k);
-      b->bufs.push_back(b->d_zero_ac);
-      b->bufs.push_back(b->d_zero_step);
-      BHIP(b, hipMemsetAsync(b->d_zero_ac, 0, sizeof(double) * 6, b->stream));
-      BHIP(b, hipMemsetAsync(b->d_zero_step, 0, sizeof(int), b->stream));
-    }
-    std::vector<Sys<T>> S(n);
-    BHIP(b, hipMemcpyAsync(S.data(), b->d_sys, sizeof(Sys<T>) * n, hipMemcpyDeviceToHost, b->stream));
-    BHIP(b, hipStreamSynchronize(b->stream));
-    for (int i = 0; i < n; ++i) S[i].kac0 = kbase[i];
-    BHIP(b, hipMemcpyAsync(b->d_sys_k, S.data(), sizeof(Sys<T>) * n, hipMemcpyHostToDevice, b->stream));
-    BHIP(b, hipStreamSynchronize(b->stream));
-    b->sys_k_base = kbase;
+  if (!b->sweep) return bfail(b, SALMON_ERR_UNSUPPORTED, "the persistent sweep is not set up");
+  if (b->d_sys_k && b->sys_k_base == kbase) return SALMON_OK;
+  if (!b->d_sys_k) {
+    if (hipMalloc(&b->d_sys_k, sizeof(Sys<T>) * n) != hipSuccess ||
+        hipMalloc((void**)&b->d_zero_ac, sizeof(double) * 6) != hipSuccess ||
+        hipMalloc((void**)&b->d_zero_step, sizeof(int)) != hipSuccess)
+      return bfail(b, SALMON_ERR_NOMEM, "batch k descriptors");
+    b->bufs.push_back(b->d_sys_k);
+    b->bufs.push_back(b->d_zero_ac);
+    b->bufs.push_back(b->d_zero_step);
+    BHIP(b, hipMemsetAsync(b->d_zero_ac, 0, sizeof(double) * 6, b->stream));
+    BHIP(b, hipMemsetAsync(b->d_zero_step, 0, sizeof(int), b->stream));
   }
-  const Sys<T>* S = (const Sys<T>*)b->d_sys_k;
-  const Step st{b->d_zero_ac, b->d_zero_step, dt,
-                {b->d_owner[0], b->d_owner[1], b->d_owner[2], b->d_owner[3], b->d_owner[4]}, 0L};
-  if (b->max_ph > 0)
-    hipLaunchKernelGGL(k_b_phase<T>, dim3((unsigned)((b->max_ph + 255) / 256), n), dim3(256), 0, b->stream, S, st, 0);
-  hipLaunchKernelGGL(k_b_qreset, dim3(1), dim3(64), 0, b->stream, b->d_next);
+  std::vector<Sys<T>> S(n);
+  BHIP(b, hipMemcpyAsync(S.data(), b->d_sys, sizeof(Sys<T>) * n, hipMemcpyDeviceToHost, b->stream));
+  BHIP(b, hipStreamSynchronize(b->stream));
+  for (int i = 0; i < n; ++i) S[i].kac0 = kbase[i];
+  BHIP(b, hipMemcpyAsync(b->d_sys_k, S.data(), sizeof(Sys<T>) * n, hipMemcpyHostToDevice, b->stream));
+  BHIP(b, hipStreamSynchronize(b->stream));
+  b->sys_k_base = kbase;
+  return SALMON_OK;
+}
+
+template <typename T>
+static void batch_sweep_launch(salmon_batch* b, const Sys<T>* S, const Step& st) {
   if constexpr (sizeof(T) == 4)
     hipLaunchKernelGGL((k_b_sweep<T, 4>), dim3(b->sweep_grid), dim3(sweep_waves<T>() * kWave), b->sweep_lds, b->stream,
                        S, st, (const int2*)b->d_items, b->nitems, b->d_next);
   else
     hipLaunchKernelGGL((k_b_sweep<T, 2>), dim3(b->sweep_grid), dim3(sweep_waves<T>() * kWave), b->sweep_lds, b->stream,
                        S, st, (const int2*)b->d_items, b->nitems, b->d_next);
+}
+
+// One Taylor-4 step (hamiltonian.f90:27-123) of every orbital of the batch at
+// the k points kbase[i] over dt, without observables: the persistent sweep
+// with the step counter at 0 (so no observables are formed) and A = 0.
+template <typename T>
+static int batch_taylor(salmon_batch* b, const double* const* kbase, double dt) {
+  const int n = (int)b->ctx.size();
+  int r;
+  if ((r = batch_k_descriptors<T>(b, kbase))) return r;
+  const Sys<T>* S = (const Sys<T>*)b->d_sys_k;
+  const Step st{b->d_zero_ac, b->d_zero_step, dt,
+                {b->d_owner[0], b->d_owner[1], b->d_owner[2], b->d_owner[3], b->d_owner[4]}, 0L};
+  if (b->max_ph > 0)
+    hipLaunchKernelGGL(k_b_phase<T>, dim3((unsigned)((b->max_ph + 255) / 256), n), dim3(256), 0, b->stream, S, st, 0);
+  hipLaunchKernelGGL(k_b_qreset, dim3(1), dim3(64), 0, b->stream, b->d_next);
+  batch_sweep_launch<T>(b, S, st);
   BHIP(b, hipGetLastError());
+  return SALMON_OK;
+}
+
+// The observables rows' sums of every system's orbitals at the k points
+// kbase[i] (current.f90, total_energy.f90 terms and the norm, as k_b_sum lays
+// them out) into out (nsys x 8): one observables-only sweep launch.
+template <typename T>
+static int batch_observe(salmon_batch* b, const double* const* kbase, double* out) {
+  const int n = (int)b->ctx.size();
+  int r;
+  if ((r = batch_k_descriptors<T>(b, kbase))) return r;
+  if (b->cap_steps < 1 || !b->d_res) {
+    if (b->d_res) (void)hipFree(b->d_res);
+    b->d_res = nullptr;
+    BHIP(b, hipMalloc((void**)&b->d_res, sizeof(double) * 8 * (size_t)n));
+    b->cap_steps = 1;
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);  // the step graph holds the old results pointer
+    b->exec = nullptr;
+  }
+  const Sys<T>* S = (const Sys<T>*)b->d_sys_k;
+  const Step st{b->d_zero_ac, b->d_zero_step, 0.0,
+                {b->d_owner[0], b->d_owner[1], b->d_owner[2], b->d_owner[3], b->d_owner[4]}, 0L, 1};
+  if (b->max_ph > 0)
+    hipLaunchKernelGGL(k_b_phase<T>, dim3((unsigned)((b->max_ph + 255) / 256), n), dim3(256), 0, b->stream, S, st, 2);
+  hipLaunchKernelGGL(k_b_qreset, dim3(1), dim3(64), 0, b->stream, b->d_next);
+  batch_sweep_launch<T>(b, S, st);
+  hipLaunchKernelGGL(k_b_sum<T>, dim3(n), dim3(256), 0, b->stream, S, n, b->d_zero_step, b->d_res, 0);
+  BHIP(b, hipGetLastError());
+  BHIP(b, hipMemcpyAsync(out, b->d_res, sizeof(double) * 8 * (size_t)n, hipMemcpyDeviceToHost, b->stream));
+  BHIP(b, hipStreamSynchronize(b->stream));
   return SALMON_OK;
 }
 

@@ -1167,8 +1167,9 @@ int salmon_gpu_hamiltonian(salmon_ctx* c, const double* kAc, double dt) {
 }
 
 // Sums of the observables rows of the resident orbitals at k = kbase (J,
-// Ekin, Enl, Eloc, norm, as k_sum_rows lays them out): on 16^3 one
-// observables-only launch of the fused step, otherwise k_observe per orbital.
+// Ekin, Enl, Eloc, norm, as k_sum_rows lays them out): one observables-only
+// launch of rt_run's kernel (the fused step on 16^3, the persistent sweep on
+// other grids of at least 8 points per axis), otherwise k_observe per orbital.
 static int observe_sums(salmon_ctx* c, const double* kbase, double out[8]) {
   int r;
   if (fused_ready(c)) {
@@ -1179,6 +1180,17 @@ static int observe_sums(salmon_ctx* c, const double* kbase, double out[8]) {
     HIPCHK(c, hipMemcpyAsync(out, c->d_sum, sizeof(double) * 8, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return SALMON_OK;
+  }
+  if (rt_path(c) == SALMON_PATH_SWEEP) {
+    // one observables-only persistent sweep launch; a set-up that fails
+    // leaves SALMON_PATH_AUTO with k_observe
+    r = c->precision == SALMON_FP64 ? solo_prepare<double>(c) : solo_prepare<float>(c);
+    if (r == SALMON_OK) {
+      const double* const* kb = kbase == c->d_kac0 ? &c->d_kac0 : &c->d_kac;
+      r = c->precision == SALMON_FP64 ? batch_observe<double>(c->solo, kb, out) : batch_observe<float>(c->solo, kb, out);
+      return r ? fail(c, r, "persistent sweep: %s", c->solo->err.c_str()) : SALMON_OK;
+    }
+    if (c->path != SALMON_PATH_AUTO) return r;
   }
   r = c->precision == SALMON_FP64 ? observe_rows<double>(c, kbase, 0, 0, 0) : observe_rows<float>(c, kbase, 0, 0, 0);
   if (r) return r;

@@ -142,3 +142,27 @@ def test_hamiltonian_entry_through_the_sweep(name):
     assert rel(out[gpu.PATH_AUTO][0], z) < TOL
     assert rel(out[gpu.PATH_AUTO][0], out[gpu.PATH_STREAMED][0]) < TOL
     _close(out[gpu.PATH_AUTO][1], out[gpu.PATH_STREAMED][1])
+
+
+@pytest.mark.parametrize("name", ["small_k", "noncubic", "grid8"])
+def test_observables_entries_through_the_sweep(name):
+    """current, energy and norm off 16^3 take one observables-only sweep
+    launch: against k_observe (the streamed path) at per-k wave vectors,
+    before and after a Taylor step."""
+    s, d = load_case(name)
+    rng = np.random.default_rng(13)
+    ks = [s.kAc0 + rng.uniform(-0.05, 0.05, size=s.kAc0.shape) for _ in range(2)]
+    out = {}
+    for path in (gpu.PATH_AUTO, gpu.PATH_STREAMED):
+        vals = []
+        with gpu.Context(s, gpu.FP64) as c:
+            c.set_path(path)
+            for k in ks:
+                vals.append((c.current(k), c.total_energy(k), c.norm()))
+                c.hamiltonian(k, s.dt)
+            vals.append((c.current(ks[0]), c.total_energy(ks[0]), c.norm()))
+        out[path] = vals
+    for a, b in zip(out[gpu.PATH_AUTO], out[gpu.PATH_STREAMED]):
+        assert rel(a[0], b[0]) < 1e-11
+        assert rel(a[1], b[1]) < TOL
+        assert rel(a[2], b[2]) < TOL
