@@ -114,6 +114,7 @@ struct salmon_ctx {
   // tables, k points or kernel path; the observables sums of the last current
   // or energy call are reused by the next one at the same kAc and epoch (the
   // reference calls current_RT and Total_Energy_omp at one k after a step)
+  std::vector<double> kac0_h;  // kAc0(NK,3) as set_kpoints gave it
   unsigned long epoch = 0;
   unsigned long obs_epoch = ~0ul;
   std::vector<double> obs_k;
@@ -701,20 +702,20 @@ int rt_run_scf(salmon_ctx* c, const double* ac, int nsteps, double dt, int reort
   return r;
 }
 
-// The persistent sweep kernel of the batches run for this context alone: the
-// midpoint propagator with the frozen potential, grids of at least 8 points
-// per axis.
+// The persistent sweep kernel of the batches run for this context alone:
+// the frozen potential, grids of at least 8 points per axis.
 template <typename T>
 bool sweep_eligible(const salmon_ctx* c) {
-  if (c->scf || c->propagator != SALMON_PROP_MIDPOINT) return false;
+  if (c->scf) return false;
   return sweep_shape<T>(c->g)[0] > 0;
 }
 
 // The kernel path rt_run takes.
 int rt_path(const salmon_ctx* c) {
   if (fused_ready(c)) return SALMON_PATH_FUSED;
-  if (c->scf || c->path == SALMON_PATH_STREAMED) return SALMON_PATH_STREAMED;
-  if (c->path == SALMON_PATH_SWEEP) return SALMON_PATH_SWEEP;
+  if (c->path == SALMON_PATH_STREAMED) return SALMON_PATH_STREAMED;
+  if (c->path == SALMON_PATH_SWEEP) return SALMON_PATH_SWEEP;  // fails at run time where it does not apply
+  if (c->scf) return SALMON_PATH_STREAMED;
   const bool ok = c->precision == SALMON_FP64 ? sweep_eligible<double>(c) : sweep_eligible<float>(c);
   if (ok && !(c->solo_fail && c->solo_gen == c->gen)) return SALMON_PATH_SWEEP;
   return SALMON_PATH_STREAMED;
@@ -725,8 +726,7 @@ int rt_path(const salmon_ctx* c) {
 template <typename T>
 int solo_prepare(salmon_ctx* c) {
   if (!sweep_eligible<T>(c))
-    return fail(c, SALMON_ERR_UNSUPPORTED,
-                "persistent sweep: needs the midpoint propagator, the frozen potential and >= 8 points per axis");
+    return fail(c, SALMON_ERR_UNSUPPORTED, "persistent sweep: needs the frozen potential and >= 8 points per axis");
   if (!c->solo) {
     salmon_batch* b = new salmon_batch();
     b->precision = c->precision;
@@ -761,9 +761,45 @@ int solo_prepare(salmon_ctx* c) {
 // run (and each stretch before a re-orthonormalisation) as in the batches;
 // the density of the last step's orbitals, taken (as psi_rho_RT follows the
 // Taylor step) before the re-orthonormalisation that may end the run.
+// ETRS on the persistent sweep (rt_step's order): per step the Taylor-only
+// launch over dt/2 at kAc0 + A_n, then at kAc0 + A_{n+1}, the density after
+// the last step, the observables-only launch at kAc0 + A_{n+1}.
+template <typename T>
+int rt_run_sweep_etrs(salmon_ctx* c, const double* ac, int nsteps, double dt, int reortho_every, double* jt,
+                      double* et, double* norm) {
+  salmon_batch* b = c->solo;
+  const size_t nk = (size_t)c->NK;
+  if (c->kac0_h.size() != 3 * nk) return fail(c, SALMON_ERR_STATE, "salmon_gpu_set_kpoints not called");
+  // one host table per half step: the step's observables call synchronises
+  // the stream before the tables are written again
+  std::vector<double> k[2] = {std::vector<double>(3 * nk), std::vector<double>(3 * nk)};
+  auto upload_at = [&](int half, const double* a) {
+    for (int d = 0; d < 3; ++d)
+      for (size_t i = 0; i < nk; ++i) k[half][d * nk + i] = c->kac0_h[d * nk + i] + a[d];
+    return upload_kac(c, k[half].data());
+  };
+  int r;
+  for (int n = 0; n < nsteps; ++n) {
+    for (int half = 0; half < 2; ++half) {
+      if ((r = upload_at(half, ac + 3 * (size_t)(n + half)))) return r;
+      if ((r = batch_taylor<T>(b, &c->d_kac, 0.5 * dt))) return fail(c, r, "persistent sweep: %s", b->err.c_str());
+    }
+    if (n == nsteps - 1 && (r = density_dev<T>(c))) return r;
+    double q[8];
+    if ((r = batch_observe<T>(b, &c->d_kac, q))) return fail(c, r, "persistent sweep: %s", b->err.c_str());
+    if (jt)
+      for (int d = 0; d < 3; ++d) jt[3 * (size_t)n + d] = q[d];
+    if (et) et[n] = q[3] + q[4] + q[5];
+    if (norm) norm[n] = q[6];
+    if (reortho_every > 0 && (n + 1) % reortho_every == 0 && (r = gram_schmidt_dev(c))) return r;
+  }
+  return SALMON_OK;
+}
+
 template <typename T>
 int rt_run_sweep(salmon_ctx* c, const double* ac, int nsteps, double dt, int reortho_every, double* jt, double* et,
                  double* norm) {
+  if (c->propagator == SALMON_PROP_ETRS) return rt_run_sweep_etrs<T>(c, ac, nsteps, dt, reortho_every, jt, et, norm);
   salmon_batch* b = c->solo;
   for (int n0 = 0; n0 < nsteps;) {
     const int seg = reortho_every > 0 ? std::min(reortho_every - n0 % reortho_every, nsteps - n0) : nsteps - n0;
@@ -1030,6 +1066,7 @@ int salmon_gpu_set_kpoints(salmon_ctx* c, int NK, int NB, const double* kAc0, co
     return r;
   HIPCHK(c, hipMemcpy(c->d_kac0, kAc0, sizeof(double) * 3 * NK, hipMemcpyHostToDevice));
   HIPCHK(c, hipMemcpy(c->d_occ, occ, sizeof(double) * c->NKB, hipMemcpyHostToDevice));
+  c->kac0_h.assign(kAc0, kAc0 + 3 * (size_t)NK);
   dfree(&c->d_zu);
   dfree((void**)&c->d_f64);
   dfree((void**)&c->d_rho_part);

@@ -95,8 +95,9 @@ def test_sweep_mixed_close_to_fp64():
 
 
 def test_where_the_sweep_does_not_apply():
-    """6^3 (under 8 points per axis) and ETRS: AUTO takes the streamed
-    kernels, an explicit SALMON_PATH_SWEEP fails with a message."""
+    """6^3 (under 8 points per axis) and the self-consistent potential: AUTO
+    takes the streamed kernels, an explicit SALMON_PATH_SWEEP fails with a
+    message."""
     small = synth.make_system(N=(6, 6, 6), L=6.0, nk3=(1, 1, 1), NB=2, seed=3)
     ac = synth.acos2_pulse(2, 0.08, 0.05, 0.057, 826.8)
     with gpu.Context(small, gpu.FP64) as c:
@@ -106,11 +107,34 @@ def test_where_the_sweep_does_not_apply():
             c.rt_run(ac, 2, dt=0.08)
     s, d = load_case("small_k")
     with gpu.Context(s, gpu.FP64) as c:
-        c.set_propagator(gpu.PROP_ETRS)
+        c.set_scf(s.vloc, gpu.XC_PZ)
         assert c.path() == gpu.PATH_STREAMED
         c.set_path(gpu.PATH_SWEEP)
-        with pytest.raises(gpu.SalmonGPUError, match="midpoint propagator"):
+        assert c.path() == gpu.PATH_SWEEP
+        with pytest.raises(gpu.SalmonGPUError, match="frozen potential"):
             c.rt_run(d["ac"], 2)
+
+
+@pytest.mark.parametrize("name", ["small_k", "noncubic"])
+def test_etrs_through_the_sweep(name):
+    """ETRS off 16^3: per step two Taylor-only sweep launches over dt/2 (at
+    kAc0 + A_n and kAc0 + A_{n+1}) and the observables-only launch, against
+    the streamed ETRS, also with Gram-Schmidt every 2 steps."""
+    s, d = load_case(name)
+    n = d["ac"].shape[0] - 1
+    for reortho in (0, 2):
+        out = {}
+        for path in (gpu.PATH_AUTO, gpu.PATH_STREAMED):
+            with gpu.Context(s, gpu.FP64) as c:
+                c.set_propagator(gpu.PROP_ETRS)
+                c.set_path(path)
+                assert c.path() == (gpu.PATH_SWEEP if path == gpu.PATH_AUTO else gpu.PATH_STREAMED)
+                r = c.rt_run(d["ac"], n, reortho_every=reortho)
+                r["zu"] = c.get_orbitals()
+                out[path] = r
+        print(f"[gpu] ETRS sweep {name} reortho={reortho}: J {rel(out[gpu.PATH_AUTO]['J'], out[gpu.PATH_STREAMED]['J']):.2e}",
+              flush=True)
+        _close(out[gpu.PATH_AUTO], out[gpu.PATH_STREAMED])
 
 
 @pytest.mark.parametrize("name", ["small_k", "noncubic", "grid8"])
