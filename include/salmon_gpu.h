@@ -286,9 +286,9 @@ int salmon_gpu_batch_finalize(salmon_batch *batch);
 
 /* Kernel path of the RT step.  SALMON_PATH_AUTO uses the LDS-resident fused
  * step where the grid has a compiled shape (16^3); on other grids of at least
- * 8 points per axis, with the frozen potential, the persistent sweep kernel
- * of the batches (one launch per step; ETRS: two Taylor-only launches over
- * dt/2 and one observables launch per step; the
+ * 8 points per axis the persistent sweep kernel of the batches (one launch
+ * per step; ETRS or the self-consistent potential: the Taylor-only
+ * launch(es), density and potential, one observables launch per step; the
  * context as a batch of one on its own stream, its work buffers four times
  * the orbitals' size; a set-up that fails falls back); the streamed kernels
  * otherwise.  SALMON_PATH_STREAMED forces the streamed kernels,

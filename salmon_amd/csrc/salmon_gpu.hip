@@ -703,10 +703,9 @@ int rt_run_scf(salmon_ctx* c, const double* ac, int nsteps, double dt, int reort
 }
 
 // The persistent sweep kernel of the batches run for this context alone:
-// the frozen potential, grids of at least 8 points per axis.
+// grids of at least 8 points per axis.
 template <typename T>
 bool sweep_eligible(const salmon_ctx* c) {
-  if (c->scf) return false;
   return sweep_shape<T>(c->g)[0] > 0;
 }
 
@@ -715,7 +714,6 @@ int rt_path(const salmon_ctx* c) {
   if (fused_ready(c)) return SALMON_PATH_FUSED;
   if (c->path == SALMON_PATH_STREAMED) return SALMON_PATH_STREAMED;
   if (c->path == SALMON_PATH_SWEEP) return SALMON_PATH_SWEEP;  // fails at run time where it does not apply
-  if (c->scf) return SALMON_PATH_STREAMED;
   const bool ok = c->precision == SALMON_FP64 ? sweep_eligible<double>(c) : sweep_eligible<float>(c);
   if (ok && !(c->solo_fail && c->solo_gen == c->gen)) return SALMON_PATH_SWEEP;
   return SALMON_PATH_STREAMED;
@@ -726,7 +724,7 @@ int rt_path(const salmon_ctx* c) {
 template <typename T>
 int solo_prepare(salmon_ctx* c) {
   if (!sweep_eligible<T>(c))
-    return fail(c, SALMON_ERR_UNSUPPORTED, "persistent sweep: needs the frozen potential and >= 8 points per axis");
+    return fail(c, SALMON_ERR_UNSUPPORTED, "persistent sweep: needs >= 8 points per axis");
   if (!c->solo) {
     salmon_batch* b = new salmon_batch();
     b->precision = c->precision;
@@ -761,30 +759,51 @@ int solo_prepare(salmon_ctx* c) {
 // run (and each stretch before a re-orthonormalisation) as in the batches;
 // the density of the last step's orbitals, taken (as psi_rho_RT follows the
 // Taylor step) before the re-orthonormalisation that may end the run.
-// ETRS on the persistent sweep (rt_step's order): per step the Taylor-only
-// launch over dt/2 at kAc0 + A_n, then at kAc0 + A_{n+1}, the density after
-// the last step, the observables-only launch at kAc0 + A_{n+1}.
+// Step by step on the persistent sweep, for ETRS and for the self-consistent
+// potential (the order of rt_step and rt_run_scf): the Taylor-only launch over
+// dt at kAc0 + (A_n + A_{n+1})/2, or ETRS's two over dt/2 at kAc0 + A_n and
+// kAc0 + A_{n+1} (with the self-consistent potential the extrapolated one
+// between them); the density (every step with the self-consistent
+// potential, else after the last); the potential update from it; the
+// observables-only launch at kAc0 + A_{n+1} with the potential then in place.
 template <typename T>
-int rt_run_sweep_etrs(salmon_ctx* c, const double* ac, int nsteps, double dt, int reortho_every, double* jt,
-                      double* et, double* norm) {
+int rt_run_sweep_steps(salmon_ctx* c, const double* ac, int nsteps, double dt, int reortho_every, double* jt,
+                       double* et, double* norm) {
   salmon_batch* b = c->solo;
+  const bool etrs = c->propagator == SALMON_PROP_ETRS;
   const size_t nk = (size_t)c->NK;
   if (c->kac0_h.size() != 3 * nk) return fail(c, SALMON_ERR_STATE, "salmon_gpu_set_kpoints not called");
-  // one host table per half step: the step's observables call synchronises
-  // the stream before the tables are written again
+  // one host table per upload of a step: the step's observables call
+  // synchronises the stream before the tables are written again
   std::vector<double> k[2] = {std::vector<double>(3 * nk), std::vector<double>(3 * nk)};
-  auto upload_at = [&](int half, const double* a) {
+  auto upload_at = [&](int slot, const double a[3]) {
     for (int d = 0; d < 3; ++d)
-      for (size_t i = 0; i < nk; ++i) k[half][d * nk + i] = c->kac0_h[d * nk + i] + a[d];
-    return upload_kac(c, k[half].data());
+      for (size_t i = 0; i < nk; ++i) k[slot][d * nk + i] = c->kac0_h[d * nk + i] + a[d];
+    return upload_kac(c, k[slot].data());
   };
+  auto taylor = [&](double h) {
+    const int q = batch_taylor<T>(b, &c->d_kac, h);
+    return q ? fail(c, q, "persistent sweep: %s", b->err.c_str()) : SALMON_OK;
+  };
+  const dim3 gr((c->g.NL + 255) / 256), bl(256);
   int r;
   for (int n = 0; n < nsteps; ++n) {
-    for (int half = 0; half < 2; ++half) {
-      if ((r = upload_at(half, ac + 3 * (size_t)(n + half)))) return r;
-      if ((r = batch_taylor<T>(b, &c->d_kac, 0.5 * dt))) return fail(c, r, "persistent sweep: %s", b->err.c_str());
+    const double* a0 = ac + 3 * (size_t)n;
+    const double* a1 = ac + 3 * (size_t)(n + 1);
+    if (etrs) {
+      if ((r = upload_at(0, a0)) || (r = taylor(0.5 * dt))) return r;
+      if (c->scf) {
+        hipLaunchKernelGGL(salmon::scf::k_vloc_extrapolate, gr, bl, 0, c->stream, c->d_vloc, c->d_vloc_f, c->d_vold,
+                           c->d_vold + c->g.NL, c->g.NL);
+        LAUNCHCHK(c);
+      }
+      if ((r = upload_at(1, a1)) || (r = taylor(0.5 * dt))) return r;
+    } else {
+      const double amid[3] = {0.5 * (a0[0] + a1[0]), 0.5 * (a0[1] + a1[1]), 0.5 * (a0[2] + a1[2])};
+      if ((r = upload_at(0, amid)) || (r = taylor(dt)) || (r = upload_at(1, a1))) return r;
     }
-    if (n == nsteps - 1 && (r = density_dev<T>(c))) return r;
+    if ((c->scf || n == nsteps - 1) && (r = density_dev<T>(c))) return r;
+    if (c->scf && (r = scf_potential(c, c->d_rho, true, c->scf_xc, nullptr, nullptr))) return r;
     double q[8];
     if ((r = batch_observe<T>(b, &c->d_kac, q))) return fail(c, r, "persistent sweep: %s", b->err.c_str());
     if (jt)
@@ -799,7 +818,8 @@ int rt_run_sweep_etrs(salmon_ctx* c, const double* ac, int nsteps, double dt, in
 template <typename T>
 int rt_run_sweep(salmon_ctx* c, const double* ac, int nsteps, double dt, int reortho_every, double* jt, double* et,
                  double* norm) {
-  if (c->propagator == SALMON_PROP_ETRS) return rt_run_sweep_etrs<T>(c, ac, nsteps, dt, reortho_every, jt, et, norm);
+  if (c->propagator == SALMON_PROP_ETRS || c->scf)
+    return rt_run_sweep_steps<T>(c, ac, nsteps, dt, reortho_every, jt, et, norm);
   salmon_batch* b = c->solo;
   for (int n0 = 0; n0 < nsteps;) {
     const int seg = reortho_every > 0 ? std::min(reortho_every - n0 % reortho_every, nsteps - n0) : nsteps - n0;

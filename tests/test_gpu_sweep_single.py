@@ -95,9 +95,8 @@ def test_sweep_mixed_close_to_fp64():
 
 
 def test_where_the_sweep_does_not_apply():
-    """6^3 (under 8 points per axis) and the self-consistent potential: AUTO
-    takes the streamed kernels, an explicit SALMON_PATH_SWEEP fails with a
-    message."""
+    """6^3 (under 8 points per axis): AUTO takes the streamed kernels, an
+    explicit SALMON_PATH_SWEEP fails with a message."""
     small = synth.make_system(N=(6, 6, 6), L=6.0, nk3=(1, 1, 1), NB=2, seed=3)
     ac = synth.acos2_pulse(2, 0.08, 0.05, 0.057, 826.8)
     with gpu.Context(small, gpu.FP64) as c:
@@ -105,14 +104,30 @@ def test_where_the_sweep_does_not_apply():
         c.set_path(gpu.PATH_SWEEP)
         with pytest.raises(gpu.SalmonGPUError, match="persistent sweep"):
             c.rt_run(ac, 2, dt=0.08)
-    s, d = load_case("small_k")
-    with gpu.Context(s, gpu.FP64) as c:
-        c.set_scf(s.vloc, gpu.XC_PZ)
-        assert c.path() == gpu.PATH_STREAMED
-        c.set_path(gpu.PATH_SWEEP)
-        assert c.path() == gpu.PATH_SWEEP
-        with pytest.raises(gpu.SalmonGPUError, match="frozen potential"):
-            c.rt_run(d["ac"], 2)
+
+
+@pytest.mark.parametrize("prop", [gpu.PROP_MIDPOINT, gpu.PROP_ETRS])
+def test_scf_through_the_sweep(prop):
+    """The self-consistent potential off 16^3: per step the Taylor-only sweep
+    launch(es) (ETRS: with the extrapolated potential between the halves),
+    the density, the Hartree + PZ potential from it and the observables-only
+    launch with that potential, against the streamed kernels in rt_run_scf's
+    order."""
+    s, d = load_case("noncubic")
+    n = d["ac"].shape[0] - 1
+    out = {}
+    for path in (gpu.PATH_AUTO, gpu.PATH_STREAMED):
+        with gpu.Context(s, gpu.FP64) as c:
+            c.set_propagator(prop)
+            c.set_scf(s.vloc, gpu.XC_PZ)
+            c.set_path(path)
+            assert c.path() == (gpu.PATH_SWEEP if path == gpu.PATH_AUTO else gpu.PATH_STREAMED)
+            r = c.rt_run(d["ac"], n, reortho_every=2)
+            r["zu"] = c.get_orbitals()
+            out[path] = r
+    print(f"[gpu] SCF sweep prop={prop}: J {rel(out[gpu.PATH_AUTO]['J'], out[gpu.PATH_STREAMED]['J']):.2e} "
+          f"rho {rel(out[gpu.PATH_AUTO]['rho'], out[gpu.PATH_STREAMED]['rho']):.2e}", flush=True)
+    _close(out[gpu.PATH_AUTO], out[gpu.PATH_STREAMED])
 
 
 @pytest.mark.parametrize("name", ["small_k", "noncubic"])
