@@ -135,6 +135,37 @@ def run_gpu(s, ac, prec, warmup, steps, ev, scf=False, etrs=False):
     return dict(ms=ms, wall=t1 - t0, J=r["J"], E=r["E"], norm=r["norm"])
 
 
+def other_grid(ev, steps=10):
+    """rt_run off the fused 16^3 shape: a 32^3 cell (L = 15 bohr, 8 atoms),
+    64 k x 32 bands, frozen potential, middlepoint, through the persistent
+    sweep (what the default path takes there) and through the streamed
+    kernels, FP64 and mixed; HIP events around the timed steps, and the same
+    12 S NL bytes per orbital-step against 8 TB/s."""
+    from salmon_amd import gpu, synth
+    s = synth.make_system(N=(32, 32, 32), L=15.0, nk3=(4, 4, 4), NB=32, seed=1, orbitals=False)
+    s.zu = synth.random_orbitals(np.random.default_rng(3), s.NK, s.NB, s.NL, s.Hxyz, orthonormalise=False)
+    ac = synth.acos2_pulse(steps + 3, 0.08, 0.05, 0.057, 826.8)
+    out = {"workload": f"32^3 cell, 64 k x 32 bands (NKB={s.NK * s.NB}), frozen potential, middlepoint, "
+                       f"{steps} steps after 2"}
+    for prec, tag in ((gpu.FP64, "fp64"), (gpu.MIXED, "mixed")):
+        for path, name in ((gpu.PATH_AUTO, "sweep"), (gpu.PATH_STREAMED, "streamed")):
+            with gpu.Context(s, prec) as c:
+                c.set_path(path)
+                c.rt_run(ac[:3], 2)
+                c.synchronize()
+                st = c.stream()
+                e0, e1 = ev.event(), ev.event()
+                ev.record(e0, st)
+                c.rt_run(ac[2:steps + 3], steps)
+                ev.record(e1, st)
+                c.synchronize()
+                ms = ev.elapsed_ms(e0, e1) / steps
+            B = 12 * (16 if prec == gpu.FP64 else 8) * s.NL * s.NK * s.NB
+            out[f"{tag}_{name}_ms_per_step"] = ms
+            out[f"{tag}_{name}_frac"] = B / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS
+    return out
+
+
 CPU_SAMPLE_SCRIPT = r"""
 import sys, time, json, os
 sys.path.insert(0, sys.argv[1])
@@ -282,6 +313,10 @@ def main():
     etrs = {"steps": nscf, "ms_per_step": e32["ms"] / nscf, "value": nkb * nscf / (e32["ms"] * 1e-3),
             "fp64": {"ms_per_step": e64["ms"] / nscf, "value": nkb * nscf / (e64["ms"] * 1e-3)},
             "J_rel_err": float(np.max(np.abs(e32["J"] - e64["J"])) / np.max(np.abs(e64["J"])))}
+    try:
+        og = other_grid(ev)
+    except Exception as e:  # reported, not fatal
+        og = {"error": str(e)[:300]}
     cpu = None
     if not args.no_cpu:
         try:
@@ -309,6 +344,7 @@ def main():
         "roofline": roof,
         "scf": scf,
         "etrs": etrs,
+        "other_grid": og,
         "flop_model": flop_model,
         "cpu_baseline": cpu,
     }
