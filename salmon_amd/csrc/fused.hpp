@@ -578,9 +578,8 @@ __global__ void __launch_bounds__(NY * NZ * XS, WPE) k_fused_step(FusedArgs<T> a
       // (finalize below).  The sweep loads its x window (own points and the
       // halo of 4) and forms its LDS bases itself, so nothing of it stays
       // live across the KB phase.
-      auto sweep = [&](auto obs_c, auto fold_c) __attribute__((always_inline)) {
+      auto sweep = [&](auto obs_c) __attribute__((always_inline)) {
         constexpr bool OBS = decltype(obs_c)::value;
-        constexpr bool FOLD = decltype(fold_c)::value;
         if constexpr (XS > 1) {
           // SEQ keeps its own points' inputs in the window from the previous
           // sweep (or the orbital's load) and reloads the halo only
@@ -634,15 +633,13 @@ __global__ void __launch_bounds__(NY * NZ * XS, WPE) k_fused_step(FusedArgs<T> a
             for (int d = 0; d < 3; ++d) js[d] += er * (double)gi[d] - ei * (double)gr[d];
             lr = vAi * e0.re + Lr + (kmx * gi[0] + kmy * gi[1] + kmz * gi[2]);
             li = vAi * e0.im + Li - (kmx * gr[0] + kmy * gr[1] + kmz * gr[2]);
-          } else if constexpr (!DEFER || FOLD) {
-            // k folded into the gradient coefficients, D(a, n) = k_a nab_a(n)
+          } else if constexpr (!DEFER) {
             lr = vAi * e0.re;
             li = vAi * e0.im;
             nb_sums_k<T, NX, NYZ, H, XS, W>(a, D, zb, yb, win, i, lr, li);
           } else {
-            // the observables sweep's arithmetic without the observables: the
-            // first H application of every step takes it, so that a run split
-            // into two calls gives the orbitals of one call to the last bit
+            // the same arithmetic as the observables sweep, so that all four
+            // H applications of a step use one Hamiltonian to the last bit
             T Lr = 0, Li = 0;
             T gr[3] = {0, 0, 0}, gi[3] = {0, 0, 0};
             nb_sums<T, NX, NYZ, H, XS, W>(a, zb, yb, win, i, Lr, Li, gr, gi);
@@ -709,13 +706,10 @@ __global__ void __launch_bounds__(NY * NZ * XS, WPE) k_fused_step(FusedArgs<T> a
       };
       auto stencil_phase = [&]() __attribute__((always_inline)) {
         if constexpr (DEFER) {
-          // the first H application in the observables' arithmetic (their
-          // neighbour sums L and G_a), the other three with k folded in
-          if (obs) sweep(BoolC<true>{}, BoolC<false>{});
-          else if (n == 0) sweep(BoolC<false>{}, BoolC<false>{});
-          else sweep(BoolC<false>{}, BoolC<true>{});
+          if (obs) sweep(BoolC<true>{});
+          else sweep(BoolC<false>{});
         } else {
-          sweep(BoolC<false>{}, BoolC<false>{});
+          sweep(BoolC<false>{});
         }
       };
       if constexpr (AB) {
