@@ -291,7 +291,8 @@ int salmon_gpu_batch_finalize(salmon_batch *batch);
  * launch(es), density and potential, one observables launch per step; the
  * context as a batch of one on its own stream, its work buffers four times
  * the orbitals' size; a set-up that fails falls back); the streamed kernels
- * otherwise.  SALMON_PATH_STREAMED forces the streamed kernels,
+ * otherwise.  SALMON_PATH_STREAMED forces the streamed kernels (and frees
+ * the sweep's work buffers),
  * SALMON_PATH_SWEEP the persistent sweep (rt_run then fails with
  * SALMON_ERR_UNSUPPORTED where it does not apply).  Like the reference's
  * build-time stencil selection (ARTED/stencil/CMakeLists.txt:1-52) the choice

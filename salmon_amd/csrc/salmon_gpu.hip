@@ -1956,6 +1956,12 @@ int salmon_gpu_set_path(salmon_ctx* c, int path) {
   if (path != SALMON_PATH_AUTO && path != SALMON_PATH_STREAMED && path != SALMON_PATH_SWEEP)
     return fail(c, SALMON_ERR_ARG, "unknown path %d", path);
   c->path = path;
+  if (path == SALMON_PATH_STREAMED && c->solo) {
+    // the persistent sweep's work buffers (four times the orbitals) go
+    (void)hipStreamSynchronize(c->stream);
+    batch_release(c->solo);
+    c->solo->tuned = false;
+  }
   return SALMON_OK;
 }
 

@@ -205,3 +205,19 @@ def test_observables_entries_through_the_sweep(name):
         assert rel(a[0], b[0]) < 1e-11
         assert rel(a[1], b[1]) < TOL
         assert rel(a[2], b[2]) < TOL
+
+
+def test_switching_paths_mid_run():
+    """AUTO (sweep), then STREAMED (the sweep's work buffers freed), then AUTO
+    again (rebuilt): the same trajectory as the streamed kernels throughout."""
+    s, d = load_case("small_k")
+    seq = [(gpu.PATH_AUTO, 0, 2), (gpu.PATH_STREAMED, 2, 4), (gpu.PATH_AUTO, 4, 6)]
+    out = {}
+    for mode in ("switch", "streamed"):
+        J = []
+        with gpu.Context(s, gpu.FP64) as c:
+            for path, a, b in seq:
+                c.set_path(path if mode == "switch" else gpu.PATH_STREAMED)
+                J.append(c.rt_run(d["ac"][a:b + 1], b - a)["J"])
+            out[mode] = dict(J=np.concatenate(J), zu=c.get_orbitals())
+    _close(out["switch"], out["streamed"], keys=("J", "zu"))
