@@ -104,6 +104,10 @@ def ref_flop_per_orbital_step(s):
     return 4 * 158 * s.NL + 4 * (4 + 21 * sum_mps) + 4 * 8 * s.NL
 
 
+# seconds of steps before each timed run, so that the GPU clocks are steady
+WARM_S = 0.3
+
+
 def run_gpu(s, ac, prec, warmup, steps, ev, scf=False, etrs=False):
     from salmon_amd import gpu
     with gpu.Context(s, prec) as c:
@@ -115,7 +119,7 @@ def run_gpu(s, ac, prec, warmup, steps, ev, scf=False, etrs=False):
         # orbitals (and the self-consistent potential) start over, so every
         # precision times the same trajectory
         t_warm = time.perf_counter()
-        while time.perf_counter() - t_warm < 0.3:
+        while time.perf_counter() - t_warm < WARM_S:
             c.rt_run(ac[: max(warmup, 1) + 1], max(warmup, 1))
             c.synchronize()
         c.put_orbitals(s.zu)
