@@ -66,6 +66,8 @@ struct FusedArgs {
   Grid g;
   double* wg_rows;       // (nwg*8)
   double* rho_part;      // (nwg*NL)
+  double* res;           // the rows' sums (8), formed by the workgroup that finishes last; or null
+  unsigned* cnt;         // workgroups finished (zero between launches)
 };
 
 template <typename T>
@@ -955,6 +957,37 @@ __global__ void __launch_bounds__(NY * NZ * XS, WPE) k_fused_step(FusedArgs<T> a
       for (int i = 0; i < 8; ++i) r[i] = __builtin_nan("");
     }
   }
+  if (a.res) {
+    // the workgroup that finishes last sums every workgroup's row in row
+    // order (the same order whichever it is): no separate reduction launch
+    __shared__ int last;
+    if (t == 0) {
+      const unsigned prev = __hip_atomic_fetch_add(a.cnt, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+      last = prev == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (last) {
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+      double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+      for (int q = t; q < (int)gridDim.x; q += NT) {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) v[i] += a.wg_rows[(size_t)q * 8 + i];
+      }
+      wave_sums(v);
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        const double x = lane_read(v[0], wave_sums_lane(i));
+        if (lane == 0) red[wv * 8 + i] = x;
+      }
+      __syncthreads();
+      if (t < 8) {
+        double acc = 0.0;
+        for (int w = 0; w < NW; ++w) acc += red[w * 8 + t];
+        a.res[t] = acc;
+      }
+      if (t == 0) __hip_atomic_store(a.cnt, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+  }
 }
 
 }  // namespace salmon
@@ -993,6 +1026,7 @@ struct FusedState {
   const double *sgnp = nullptr, *uvp64 = nullptr;
   const float* uvp32 = nullptr;
   double* wg_rows = nullptr;
+  unsigned* cnt = nullptr;  // finished workgroups of the launch (the last sums the rows)
   double* rho_part = nullptr;
   int nwg_alloc = 0;
   bool tables = false;
@@ -1024,6 +1058,7 @@ static void fused_free(salmon_ctx* c) {
   fused_free_tables(f);
   if (f->wg_rows) (void)hipFree(f->wg_rows);
   if (f->rho_part) (void)hipFree(f->rho_part);
+  if (f->cnt) (void)hipFree(f->cnt);
   delete f;
   c->fz = nullptr;
 }
@@ -1230,6 +1265,12 @@ static int fused_step(salmon_ctx* c, const double* amid, const double* aobs, dou
   a.g = c->g;
   a.wg_rows = f->wg_rows;
   a.rho_part = f->rho_part;
+  if (aobs && !f->cnt) {
+    if (hipMalloc((void**)&f->cnt, sizeof(unsigned)) != hipSuccess) return fail(c, SALMON_ERR_NOMEM, "fused counter");
+    HIPCHK(c, hipMemsetAsync(f->cnt, 0, sizeof(unsigned), c->stream));
+  }
+  a.res = aobs ? res_dev : nullptr;
+  a.cnt = f->cnt;
   int r;
   // the overlap fold is compiled only into the instantiation for cells whose
   // atom spheres share grid points
@@ -1241,10 +1282,6 @@ static int fused_step(salmon_ctx* c, const double* amid, const double* aobs, dou
                   : fused_launch<T, 16, 16, 16, kFusedXS32, kFusedWPE32, false>(c, a, nwg);
   }
   if (r || !(aobs || a.rho_post)) return r;
-  if (aobs) {
-    hipLaunchKernelGGL(salmon::k_sum_rows, dim3(1), dim3(256), 0, c->stream, f->wg_rows, nwg, res_dev);
-    LAUNCHCHK(c);
-  }
   // the FP64 observables-only launch sums no density (d_rho keeps rho(t+dt))
   if (sizeof(T) == 8 && !a.prop) return SALMON_OK;
   if (!a.rho && !a.rho_post) return SALMON_OK;
