@@ -221,3 +221,21 @@ def test_switching_paths_mid_run():
                 J.append(c.rt_run(d["ac"][a:b + 1], b - a)["J"])
             out[mode] = dict(J=np.concatenate(J), zu=c.get_orbitals())
     _close(out["switch"], out["streamed"], keys=("J", "zu"))
+
+
+@pytest.mark.parametrize("scf,prop", [(False, gpu.PROP_ETRS), (True, gpu.PROP_MIDPOINT), (True, gpu.PROP_ETRS)])
+def test_mixed_step_by_step_sweep_close_to_fp64(scf, prop):
+    """complex64 ETRS and self-consistent runs on the sweep (step by step)
+    within 1e-5 of FP64 on the same path (J, E, norm)."""
+    s, d = load_case("noncubic")
+    n = d["ac"].shape[0] - 1
+    out = {}
+    for prec in (gpu.FP64, gpu.MIXED):
+        with gpu.Context(s, prec) as c:
+            c.set_propagator(prop)
+            if scf:
+                c.set_scf(s.vloc, gpu.XC_PZ)
+            assert c.path() == gpu.PATH_SWEEP
+            out[prec] = c.rt_run(d["ac"], n)
+    print(f"[gpu] mixed sweep scf={scf} prop={prop}: J {rel(out[gpu.MIXED]['J'], out[gpu.FP64]['J']):.2e}", flush=True)
+    _close(out[gpu.MIXED], out[gpu.FP64], keys=("J", "E", "norm"), tol=1e-5)
